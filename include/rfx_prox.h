@@ -1,0 +1,182 @@
+/*
+ * rfx_prox.h -- C ABI of the MI355X random-forest proximity path.
+ *
+ * Plain pointers, sizes and status codes; no C++ or torch types.  Every
+ * device buffer is owned by the caller (allocated with hipMalloc or taken from
+ * any HIP allocator); every GPU entry point enqueues work on the stream it is
+ * given and returns without synchronising unless stated.  Errors are returned
+ * as negative status codes, never swallowed.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository).
+ */
+#ifndef RFX_PROX_H
+#define RFX_PROX_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define RFX_OK 0
+#define RFX_ERR_INVALID (-1)     /* bad argument, shape or forest            */
+#define RFX_ERR_ALLOC (-2)       /* host allocation failed                   */
+#define RFX_ERR_HIP (-3)         /* a HIP call failed                        */
+#define RFX_ERR_WORKSPACE (-4)   /* workspace too small                      */
+#define RFX_ERR_UNSUPPORTED (-5) /* combination not implemented             */
+
+/* ---- limits ------------------------------------------------------------- */
+#define RFX_MAX_SAMPLES 4000000LL
+#define RFX_MAX_FEATURES 1023
+#define RFX_MAX_NODES_PER_TREE 4194303LL /* left child index fits 22 bits   */
+#define RFX_MAX_TOTAL_NODES 1073741823LL /* (global node id << 1) | bag fits int32 */
+
+/* ---- synthetic datasets (rfx_make_dataset) ------------------------------ */
+#define RFX_DATA_GAUSS_MIXTURE 0    /* class means ~ N(0, 4I), x ~ N(mu, I)  */
+#define RFX_DATA_COVTYPE_LIKE 1     /* p = 54, 7 imbalanced classes          */
+#define RFX_DATA_SUBSPACE_MIXTURE 2 /* 16 comps in 8-dim subspace + noise    */
+#define RFX_DATA_OVERLAP_MIXTURE 3  /* class means ~ N(0, 0.35^2 I), x ~ N(mu, I) */
+
+/* ---- grower flags -------------------------------------------------------- */
+#define RFX_GROW_NO_FMA 1 /* Gini pno sum without fused multiply-add        */
+
+/* ---- proximity variants -------------------------------------------------
+ * ALL       C[i,j] = #{t : leaf_t(i) = leaf_t(j)}                (symmetric)
+ * OOB_ASYM  P[i,j] = #{t : i out-of-bag, j in-bag, same leaf}   (reference
+ *           src/rf_proximity.cpp:179-230, non-case-wise)
+ * OOB_UPPER U[i,j] = P[i,j] + P[j,i]                            (reference
+ *           cuda/rf_proximity_upper_triangle.cu:149-306 fold, diagonal 0)
+ * A sample whose traversal ends on a non-terminal node takes part in no pair.
+ */
+#define RFX_PROX_ALL 0
+#define RFX_PROX_OOB_ASYM 1
+#define RFX_PROX_OOB_UPPER 2
+
+/* layouts: FULL = n x n, element (i,j) at i + j*n (column-major, as the
+ * reference's prox[n + kk*nsample]); UPPER = packed upper triangle with the
+ * diagonal, row-major: (i,j), i <= j, at i*n - i*(i-1)/2 + (j-i), 64-bit. */
+#define RFX_LAYOUT_FULL 0
+#define RFX_LAYOUT_UPPER 1
+
+/* accumulation algorithms */
+#define RFX_ALGO_AUTO 0   /* choose from the forest's leaf-size statistics  */
+#define RFX_ALGO_DENSE 1  /* LDS-staged key tiles, register-tiled compare   */
+#define RFX_ALGO_BUCKET 2 /* leaf buckets, integer atomics per matching pair */
+
+/* ========================================================================
+ * Host side: forest input contract (not timed)
+ * ======================================================================== */
+typedef struct rfx_forest rfx_forest;
+
+/* MT19937 output #index (0-based) after init_genrand(seed); the stream of
+ * include/rf_utils.hpp:74-141. */
+uint32_t rfx_mt19937_u32(uint32_t seed, int64_t index);
+
+/* Bootstrap multiplicities of one tree; replaces rf::cpu_bootstrap's nin
+ * output (src/rf_bootstrap.cpp:14-82) for a given seed. */
+int rfx_bootstrap(int64_t n, uint32_t seed, int32_t* nin);
+
+/* Deterministic synthetic data, x row-major n x p float32, y int32 classes. */
+int rfx_make_dataset(int32_t kind, int64_t n, int32_t p, int32_t nclass, uint64_t seed, float* x, int32_t* y);
+
+/* Grow a classification forest exactly as the reference CPU path would:
+ * bootstrap seed boot_seed + t (src/rf_bootstrap.cpp:24), grower seed
+ * iseed + t (src/rf_random_forest.cpp:515), rf::cpu_growtree semantics
+ * (src/rf_growtree.cpp:382-942).  maxnode <= 0 means 2n+1.  nthreads <= 0
+ * means all hardware threads (trees are independent). */
+int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int32_t nclass, int32_t ntree,
+                    int32_t mtry, int32_t minndsize, int32_t maxnode, int32_t iseed, int32_t boot_seed,
+                    int32_t flags, int32_t nthreads, rfx_forest** out);
+void rfx_forest_free(rfx_forest* f);
+int32_t rfx_forest_ntree(const rfx_forest* f);
+int32_t rfx_forest_nnode(const rfx_forest* f, int32_t t);
+/* Export the reference's per-tree arrays flattened by node_offset[T+1]:
+ * nodestatus/bestvar/xbestsplit/nodeclass [total], treemap [2*total],
+ * nin [T*n].  Any output pointer except node_offset may be NULL. */
+int rfx_forest_export(const rfx_forest* f, int64_t* node_offset, int32_t* nodestatus, int32_t* bestvar,
+                      float* xbestsplit, int32_t* treemap, int32_t* nodeclass, int32_t* nin);
+/* Pack reference node arrays into the device node records (2 x uint32 per
+ * node) read by rfx_leaf_assign. */
+int rfx_pack_nodes(const int64_t* node_offset, int32_t ntree, const int32_t* nodestatus, const int32_t* bestvar,
+                   const float* xbestsplit, const int32_t* treemap, int32_t p, uint32_t* packed);
+/* nout[i] = mult * #{t : nin_t(i) = 0}; the reference CPU path has mult = 2
+ * (src/rf_bootstrap.cpp:61 and src/rf_random_forest.cpp:1108-1110). */
+int rfx_oob_counts(const int32_t* nin, int64_t n, int32_t ntree, int32_t mult, int32_t* nout);
+/* In-bag bitmask: bit (i % 32) of word [t * ceil(n/32) + i/32] = (nin_t(i) > 0). */
+int rfx_pack_inbag_bits(const int32_t* nin, int64_t n, int32_t ntree, uint32_t* bits);
+
+/* ========================================================================
+ * GPU side
+ * ======================================================================== */
+
+/* A1: leaf assignment of every sample through every tree.  Replaces
+ * rf::cpu_testreebag (src/rf_testreebag.cpp:13-127, out-of-bag samples) and
+ * RandomForest::find_terminal_node (src/rf_random_forest.cpp:2194-2244,
+ * in-bag samples), as called per tree at src/rf_random_forest.cpp:1043 and
+ * 1295-1303; GPU twin gpu_compute_nodextr_inbag_kernel
+ * (cuda/rf_growtree.cu:1763-1850).
+ *   x_col      device, p x n float32 column-major (feature m of sample i at m*n+i)
+ *   nodes      device, packed node records (rfx_pack_nodes)
+ *   node_off   device, int64 [T+1]
+ *   inbag      device, in-bag bitmask (rfx_pack_inbag_bits)
+ *   cat        device int32 [p] category counts (1 = numeric) or NULL (all numeric)
+ *   nodextr    device int32 [T][n] node reached (reference semantics) or NULL
+ *   keys       device int32 [T][n]: (global node id << 1 | in-bag) when the
+ *              node is terminal, else -1; or NULL */
+int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n, int32_t p, const uint32_t* nodes,
+                    const int64_t* node_off, int32_t ntree, const uint32_t* inbag, const int32_t* cat,
+                    int32_t* nodextr, int32_t* keys);
+
+/* Workspace bytes rfx_prox_accumulate needs for (n, T, total_nodes, algo). */
+size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo);
+
+/* A2+A3: co-occurrence counts from leaf keys.  Replaces the per-tree
+ * rf::cpu_proximity (include/rf_proximity.hpp:10-21, src/rf_proximity.cpp:
+ * 99-231) and rf::cuda::gpu_proximity_upper_triangle_fp16
+ * (include/rf_proximity_upper_triangle.hpp:42-55) with one call over all
+ * trees; counts are int32 and exact.
+ *   keys        device int32 [T][n] from rfx_leaf_assign
+ *   out         device int32, FULL (n*n) or UPPER (n(n+1)/2)
+ *   accumulate  0: out is overwritten; 1: counts are added to out
+ *   workspace   device, >= rfx_prox_workspace_bytes(...) (may be NULL for DENSE) */
+int rfx_prox_accumulate(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                        int32_t variant, int32_t layout, int32_t algo, int32_t accumulate, int32_t* out,
+                        void* workspace, size_t workspace_bytes);
+
+/* ALL and OOB_UPPER in one pass (same layout for both outputs). */
+int rfx_prox_accumulate_pair(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree,
+                             int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,
+                             int32_t* out_all, int32_t* out_oob, void* workspace, size_t workspace_bytes);
+
+/* Rows [row_begin, row_end) of the FULL matrix of one variant, written
+ * densely as (row_end-row_begin) x n int32 row-major (row r, column j at
+ * (r-row_begin)*n + j).  Used to check large problems block by block. */
+int rfx_prox_rows(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int32_t variant,
+                  int64_t row_begin, int64_t row_end, int32_t* out);
+
+/* Leaf-size statistics of a key array: sum over trees and leaves of
+ * |leaf|^2 (matching ordered pairs incl. self), as a double.  Feeds the
+ * AUTO choice and the bench's work accounting. */
+int rfx_prox_pair_stats(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                        void* workspace, size_t workspace_bytes, double* sum_sq_host);
+
+/* A5: finishprox.  Replaces rf::cpu_finishprox (src/rf_finishprox.cpp:
+ * 12-74): S_ii = 1; for i != j, a = (P_ij + P_ji)/2, S_ij = 0.5*(a*r_i + a*r_j)
+ * with r_k = 1/nout_k (r_k = 1 when nout_k = 0), in float64, operation for
+ * operation as the reference, so the result is bit-identical.
+ *   counts      device int32, OOB_ASYM FULL (in_layout FULL) or OOB_UPPER
+ *               UPPER (in_layout UPPER)
+ *   nout        device int32 [n]
+ *   proxsym     device float64, FULL column-major (out_layout FULL) or UPPER */
+int rfx_finishprox(hipStream_t stream, const int32_t* counts, int32_t in_layout, const int32_t* nout, int64_t n,
+                   double* proxsym, int32_t out_layout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RFX_PROX_H */

@@ -1,0 +1,319 @@
+/*
+ * oracle.c -- CPU restatement of the reference proximity path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (rfx_amd/) loads this
+ * library: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * use it as the checker and as the timed CPU port of the reference's OpenMP
+ * path.  Built by oracle/Makefile with -ffp-contract=off.
+ *
+ * Forests are given as the reference's own per-node arrays, flattened over
+ * trees by node_offset[T+1] (nodestatus, bestvar, xbestsplit, treemap with
+ * two entries per node), plus nin[T][n] bootstrap multiplicities and the
+ * row-major sample matrix x[i*p + m].
+ *
+ * Functions and the reference code each follows:
+ *   orc_leaf_assign     src/rf_testreebag.cpp:13-127 (out-of-bag samples) and
+ *                       src/rf_random_forest.cpp:2194-2244 (in-bag samples),
+ *                       combined as at src/rf_random_forest.cpp:1043, 1295-1303
+ *   orc_prox_v1         src/rf_proximity.cpp:99-231, called once per tree as at
+ *                       src/rf_random_forest.cpp:1305-1308 (OpenMP parallel-for
+ *                       over out-of-bag rows with an atomic FP64 +=)
+ *   orc_counts_bucket   the ALL / OOB_ASYM / OOB_UPPER integer counts by leaf
+ *                       buckets (ALL has no reference implementation; it is
+ *                       pinned by brute force and closed-form invariants)
+ *   orc_counts_brute    the same by the O(T n^2) definition (tiny n only)
+ *   orc_finishprox      src/rf_finishprox.cpp:12-74
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#pragma STDC FP_CONTRACT OFF
+
+#define ORC_ALL 0
+#define ORC_OOB_ASYM 1
+#define ORC_OOB_UPPER 2
+
+/* Terminal node reached by sample i in tree t (node id local to the tree). */
+static int32_t orc_traverse(const float* xi, int32_t inbag, const int32_t* st, const int32_t* bv, const float* xs,
+                            const int32_t* tm, int32_t nnode, const int32_t* cat) {
+  if (inbag) {
+    /* find_terminal_node: loop bounded by maxnode; categorical (int)(x+0.5)
+       in [1, ncat] -> catgoleft (all zero) -> right, else root. */
+    int32_t k = 0;
+    for (int64_t guard = 0; guard < 2 * (int64_t)nnode + 2; ++guard) {
+      if (st[k] == -1) return k;
+      int32_t v = bv[k];
+      float f = xi[v];
+      if (!cat || cat[v] == 1) {
+        k = (f <= xs[k]) ? tm[2 * k] : tm[2 * k + 1];
+      } else {
+        int32_t jcat = (int32_t)((double)f + 0.5);
+        if (jcat >= 1 && jcat <= cat[v])
+          k = tm[2 * k + 1];
+        else
+          return 0;
+      }
+    }
+    return 0;
+  } else {
+    /* cpu_testreebag: descend while split, count < 1000 guard. */
+    int32_t kt = 0;
+    int32_t count = 0;
+    while (st[kt] == 1 && count < 1000) {
+      count++;
+      if (count >= 1000) break;
+      int32_t v = bv[kt];
+      float f = xi[v];
+      if (!cat || cat[v] == 1) {
+        kt = (f <= xs[kt]) ? tm[2 * kt] : tm[2 * kt + 1];
+      } else {
+        int32_t cv = (int32_t)f;
+        /* catgoleft is never written by the grower: every category goes right */
+        (void)cv;
+        kt = tm[2 * kt + 1];
+      }
+      if (kt < 0 || kt >= nnode) {
+        kt = 0;
+        break;
+      }
+      if (st[kt] != 1) break;
+    }
+    if (st[kt] == -1 || st[kt] == 2) return kt;
+    return 0;
+  }
+}
+
+int orc_leaf_assign(const float* x, int64_t n, int32_t p, int32_t ntree, const int64_t* node_offset,
+                    const int32_t* nodestatus, const int32_t* bestvar, const float* xbestsplit, const int32_t* treemap,
+                    const int32_t* nin, const int32_t* cat, int32_t* nodextr) {
+  if (!x || !node_offset || !nodestatus || !bestvar || !xbestsplit || !treemap || !nin || !nodextr) return -1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int64_t b = node_offset[t];
+    const int32_t nn = (int32_t)(node_offset[t + 1] - b);
+    for (int64_t i = 0; i < n; ++i) {
+      nodextr[(int64_t)t * n + i] = orc_traverse(x + i * p, nin[(int64_t)t * n + i] > 0, nodestatus + b, bestvar + b,
+                                                 xbestsplit + b, treemap + 2 * b, nn, cat);
+    }
+  }
+  return 0;
+}
+
+/* One tree of the reference's accumulating dense branch. */
+static void v1_one_tree(const int32_t* st, const int32_t* ndx, const int32_t* nin, int64_t n, int32_t nnode,
+                        double* prox, int32_t* nod, int32_t* ncount, int32_t* ncn, int32_t* nodexb, int32_t* ndbegin,
+                        int32_t* npcase) {
+  int32_t nterm = 0;
+  for (int32_t k = 0; k < nnode; ++k) nod[k] = -1;
+  for (int32_t k = 0; k < nnode; ++k)
+    if (st[k] == -1) nod[k] = nterm++;
+  for (int64_t i = 0; i < n; ++i) {
+    ncount[i] = 0;
+    ncn[i] = 0;
+    nodexb[i] = -1;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t node = ndx[i];
+    if (node >= 0 && node < nnode) {
+      int32_t k = nod[node];
+      if (k >= 0 && k < nterm) {
+        ncount[k]++;
+        ncn[i] = ncount[k];
+        nodexb[i] = k;
+      }
+    }
+  }
+  ndbegin[0] = 0;
+  for (int32_t k = 1; k <= nterm; ++k) ndbegin[k] = ndbegin[k - 1] + ncount[k - 1];
+  for (int64_t i = 0; i < n; ++i) npcase[i] = -1;
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t k = nodexb[i];
+    if (k >= 0 && k < nterm) npcase[ndbegin[k] + ncn[i] - 1] = (int32_t)i;
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    if (nin[i] != 0) continue;
+    int32_t k = nodexb[i];
+    if (k < 0 || k >= nterm) continue;
+    for (int32_t j = ndbegin[k]; j < ndbegin[k + 1]; ++j) {
+      int32_t kk = npcase[j];
+      if (kk >= 0 && nin[kk] > 0) {
+#pragma omp atomic
+        prox[i + (int64_t)kk * n] += 1.0;
+      }
+    }
+  }
+}
+
+/* Accumulate all trees into prox (column-major n x n FP64, caller-zeroed). */
+int orc_prox_v1(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr, const int32_t* nin,
+                int64_t n, int32_t ntree, double* prox) {
+  int32_t maxnn = 0;
+  for (int32_t t = 0; t < ntree; ++t) {
+    int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    if (nn > maxnn) maxnn = nn;
+  }
+  int32_t* nod = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
+  int32_t* ws = (int32_t*)malloc(sizeof(int32_t) * (size_t)(4 * n));
+  int32_t* ndbegin = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 2));
+  if (!nod || !ws || !ndbegin) {
+    free(nod);
+    free(ws);
+    free(ndbegin);
+    return -2;
+  }
+  for (int32_t t = 0; t < ntree; ++t) {
+    v1_one_tree(nodestatus + node_offset[t], nodextr + (int64_t)t * n, nin + (int64_t)t * n, n,
+                (int32_t)(node_offset[t + 1] - node_offset[t]), prox, nod, ws, ws + n, ws + 2 * n, ndbegin, ws + 3 * n);
+  }
+  free(nod);
+  free(ws);
+  free(ndbegin);
+  return 0;
+}
+
+/* Output index helpers: FULL column-major, UPPER packed row-major. */
+static inline int64_t up_idx(int64_t i, int64_t j, int64_t n) { return i * n - (i * (i - 1)) / 2 + (j - i); }
+
+static int valid_leaf(const int32_t* st, int32_t node, int32_t nnode) {
+  return node >= 0 && node < nnode && st[node] == -1;
+}
+
+/* Integer counts by leaf buckets.  layout 0 = FULL, 1 = UPPER. */
+int orc_counts_bucket(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr,
+                      const int32_t* nin, int64_t n, int32_t ntree, int32_t variant, int32_t layout, int32_t* out) {
+  if (variant == ORC_OOB_ASYM && layout != 0) return -1;
+  int32_t maxnn = 0;
+  for (int32_t t = 0; t < ntree; ++t) {
+    int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    if (nn > maxnn) maxnn = nn;
+  }
+  int32_t* head = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
+  int32_t* memb = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* cnt = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
+  if (!head || !memb || !cnt) {
+    free(head);
+    free(memb);
+    free(cnt);
+    return -2;
+  }
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int32_t* st = nodestatus + node_offset[t];
+    const int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    const int32_t* ndx = nodextr + (int64_t)t * n;
+    const int32_t* bag = nin + (int64_t)t * n;
+    memset(cnt, 0, sizeof(int32_t) * (size_t)(nn + 1));
+    for (int64_t i = 0; i < n; ++i)
+      if (valid_leaf(st, ndx[i], nn)) cnt[ndx[i] + 1]++;
+    head[0] = 0;
+    for (int32_t k = 0; k < nn; ++k) head[k + 1] = head[k] + cnt[k + 1];
+    memset(cnt, 0, sizeof(int32_t) * (size_t)(nn + 1));
+    for (int64_t i = 0; i < n; ++i)
+      if (valid_leaf(st, ndx[i], nn)) memb[head[ndx[i]] + cnt[ndx[i]]++] = (int32_t)i; /* ascending */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int32_t k = 0; k < nn; ++k) {
+      for (int32_t a = head[k]; a < head[k + 1]; ++a) {
+        const int64_t i = memb[a];
+        if (variant == ORC_ALL) {
+          /* diagonal and pairs j > i; rows are owned by distinct i within a leaf,
+             but different leaves never share (i, j), so no race */
+          int64_t o = layout ? up_idx(i, i, n) : i + i * n;
+          out[o] += 1;
+        }
+        for (int32_t b2 = a + 1; b2 < head[k + 1]; ++b2) {
+          const int64_t j = memb[b2];
+          const int bi = bag[i] > 0, bj = bag[j] > 0;
+          if (variant == ORC_ALL) {
+            if (layout) {
+              out[up_idx(i, j, n)] += 1;
+            } else {
+              out[i + j * n] += 1;
+              out[j + i * n] += 1;
+            }
+          } else if (bi != bj) {
+            if (variant == ORC_OOB_UPPER) {
+              if (layout) {
+                out[up_idx(i, j, n)] += 1;
+              } else {
+                out[i + j * n] += 1;
+                out[j + i * n] += 1;
+              }
+            } else { /* OOB_ASYM: row = out-of-bag sample */
+              if (!bi)
+                out[i + j * n] += 1;
+              else
+                out[j + i * n] += 1;
+            }
+          }
+        }
+      }
+    }
+  }
+  free(head);
+  free(memb);
+  free(cnt);
+  return 0;
+}
+
+/* The definition, O(T n^2): out FULL column-major. */
+int orc_counts_brute(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr, const int32_t* nin,
+                     int64_t n, int32_t ntree, int32_t variant, int32_t* out) {
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int32_t* st = nodestatus + node_offset[t];
+    const int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    const int32_t* ndx = nodextr + (int64_t)t * n;
+    const int32_t* bag = nin + (int64_t)t * n;
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < n; ++i) {
+        if (!valid_leaf(st, ndx[i], nn) || !valid_leaf(st, ndx[j], nn) || ndx[i] != ndx[j]) continue;
+        int hit;
+        if (variant == ORC_ALL)
+          hit = 1;
+        else if (variant == ORC_OOB_ASYM)
+          hit = bag[i] == 0 && bag[j] > 0;
+        else
+          hit = (bag[i] == 0) != (bag[j] == 0);
+        out[i + j * n] += hit;
+      }
+  }
+  return 0;
+}
+
+/* rf::cpu_finishprox, step for step (column-major n x n FP64). */
+int orc_finishprox(int64_t n, const int32_t* nout, const double* prox, double* proxsym) {
+  for (int64_t q = 0; q < n * n; ++q) proxsym[q] = prox[q];
+  for (int64_t i = 0; i < n; ++i) proxsym[i + i * n] = (double)nout[i];
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = i + 1; j < n; ++j) {
+      double a = (prox[i + j * n] + prox[j + i * n]) / 2.0;
+      proxsym[i + j * n] = a;
+      proxsym[j + i * n] = a;
+    }
+  for (int64_t i = 0; i < n; ++i)
+    if (nout[i] > 0) {
+      double r = 1.0 / (double)nout[i];
+      for (int64_t j = 0; j < n; ++j) proxsym[i + j * n] *= r;
+    }
+  for (int64_t i = 0; i < n; ++i) {
+    proxsym[i + i * n] = 1.0;
+    for (int64_t j = i + 1; j < n; ++j) {
+      double a = 0.5 * (proxsym[i + j * n] + proxsym[j + i * n]);
+      proxsym[i + j * n] = a;
+      proxsym[j + i * n] = a;
+    }
+  }
+  return 0;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

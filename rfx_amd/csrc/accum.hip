@@ -1,0 +1,612 @@
+// A2 + A3: same-leaf co-occurrence counts over all trees, bit-exact int32.
+//
+// Input is the key array of rfx_leaf_assign: keys[t][i] = (global node id << 1)
+// | in-bag, or -1 when sample i ends on a non-terminal node of tree t.  Two
+// algorithms compute the same integers:
+//
+// DENSE  (tile compare).  A workgroup owns a 128 x 128 tile of the output;
+//        the row and column keys of 32 trees at a time are staged in LDS
+//        after a per-variant transform, and each thread compares an 8 x 8
+//        register tile (v_cmp_eq + v_addc per tree-pair), so the count
+//        matrix is written exactly once, with no atomics at all.  Cost is
+//        T*n^2 compares: VALU-bound, independent of the leaf structure.
+//
+// BUCKET (leaf buckets).  A counting sort groups the (tree, sample) slots by
+//        terminal node (histogram, scan, scatter), then every thread walks
+//        the bucket of one slot and adds 1 with an integer atomic for each
+//        partner of larger sample index.  Cost is sum over trees and leaves
+//        of |leaf|^2 plus one pass over the output, so for fully grown trees
+//        on large n (expected count per pair << 1) it is far below T*n^2.
+//
+// The per-variant transform maps keys to a row key R and column key S with
+// R_i == S_j exactly when the pair counts, so one compare kernel serves
+//   ALL        R = k >> 1 (invalid -1),            S = k >> 1 (invalid -2)
+//   OOB_UPPER  R = k,                              S = k ^ 1  (invalid -2)
+//   OOB_ASYM   R = k if out-of-bag else -1,        S = k ^ 1 if in-bag else -2
+// and the fused PAIR mode packs ALL (low 16 bits) and OOB_UPPER (high 16
+// bits) counters in one register: x = R ^ S, ALL += x < 2, OOB += x == 1.
+//
+// Reference semantics: src/rf_proximity.cpp:179-230 (OOB_ASYM, accumulated
+// per tree), cuda/rf_proximity_upper_triangle.cu:238-248 (OOB_UPPER fold).
+
+#include "rfx_common.hpp"
+
+namespace {
+
+constexpr int kMode_ALL = RFX_PROX_ALL;
+constexpr int kMode_OOB_ASYM = RFX_PROX_OOB_ASYM;
+constexpr int kMode_OOB_UPPER = RFX_PROX_OOB_UPPER;
+constexpr int kMode_PAIR = 3;
+
+constexpr int kTile = 128;
+constexpr int kSub = 8;
+constexpr int kTreeChunk = 32;
+constexpr int kThreads = 256;
+
+template <int MODE>
+__device__ __forceinline__ int32_t row_key(int32_t k) {
+  if (MODE == kMode_ALL) return k >> 1;
+  if (MODE == kMode_OOB_ASYM) return (k >= 0 && !(k & 1)) ? k : -1;
+  return k;  // OOB_UPPER, PAIR (invalid stays -1)
+}
+template <int MODE>
+__device__ __forceinline__ int32_t col_key(int32_t k) {
+  if (MODE == kMode_ALL) return k < 0 ? -2 : (k >> 1);
+  if (MODE == kMode_OOB_ASYM) return (k >= 0 && (k & 1)) ? (k ^ 1) : -2;
+  if (MODE == kMode_OOB_UPPER) return k < 0 ? -2 : (k ^ 1);
+  return k < 0 ? 0x40000000 : k;  // PAIR
+}
+template <int MODE>
+__device__ __forceinline__ int32_t row_pad() {
+  return -1;
+}
+template <int MODE>
+__device__ __forceinline__ int32_t col_pad() {
+  return MODE == kMode_PAIR ? 0x40000000 : -2;
+}
+
+// Upper-tile decode: linear k -> (bi, bj), bi <= bj, row-major over the
+// upper triangle of an nb x nb tile grid.
+__device__ __forceinline__ void upper_tile(int64_t k, int64_t nb, int64_t& bi, int64_t& bj) {
+  const double b = 2.0 * nb + 1.0;
+  int64_t r = static_cast<int64_t>((b - sqrt(b * b - 8.0 * static_cast<double>(k))) * 0.5);
+  if (r < 0) r = 0;
+  if (r >= nb) r = nb - 1;
+  auto start = [nb](int64_t q) { return q * nb - (q * (q - 1)) / 2; };
+  while (r > 0 && start(r) > k) --r;
+  while (r + 1 < nb && start(r + 1) <= k) ++r;
+  bi = r;
+  bj = r + (k - start(r));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so a
+// contiguous run of tiles (sharing row keys) lands on one XCD and its L2.
+__device__ __forceinline__ int64_t xcd_linear(int64_t b, int64_t nblocks) {
+  const int64_t q = nblocks / 8, r = nblocks % 8;
+  const int64_t x = b % 8, local = b / 8;
+  return x * q + (x < r ? x : r) + local;
+}
+
+template <int MODE, int LAYOUT>
+__global__ __launch_bounds__(kThreads) void dense_tile_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                              int32_t ntree, int64_t nb, int32_t accumulate,
+                                                              int32_t* __restrict__ out,
+                                                              int32_t* __restrict__ out2) {
+  __shared__ int32_t sR[kTreeChunk][kTile];
+  __shared__ int32_t sS[kTreeChunk][kTile];
+  const int64_t nblocks = gridDim.x;
+  const int64_t lin = xcd_linear(blockIdx.x, nblocks);
+  int64_t bi, bj;
+  if (LAYOUT == RFX_LAYOUT_UPPER) {
+    upper_tile(lin, nb, bi, bj);
+  } else {
+    bi = lin % nb;
+    bj = lin / nb;
+  }
+  const int64_t i0 = bi * kTile, j0 = bj * kTile;
+  const int tid = threadIdx.x;
+  const int fast = tid & 15, slow = tid >> 4;
+  // Row-major packed output wants columns contiguous per thread; the
+  // column-major full matrix wants rows contiguous.
+  const int rb = (LAYOUT == RFX_LAYOUT_UPPER ? slow : fast) * kSub;
+  const int cb = (LAYOUT == RFX_LAYOUT_UPPER ? fast : slow) * kSub;
+
+  uint32_t cnt[kSub][kSub];
+#pragma unroll
+  for (int r = 0; r < kSub; ++r)
+#pragma unroll
+    for (int c = 0; c < kSub; ++c) cnt[r][c] = 0;
+
+  for (int32_t t0 = 0; t0 < ntree; t0 += kTreeChunk) {
+    const int32_t tc = min(kTreeChunk, ntree - t0);
+    __syncthreads();
+    for (int e = tid; e < kTreeChunk * kTile; e += kThreads) {
+      const int tt = e / kTile, q = e % kTile;
+      const int64_t i = i0 + q, j = j0 + q;
+      const bool tv = tt < tc;
+      const int64_t trow = static_cast<int64_t>(t0 + tt) * n;
+      sR[tt][q] = (tv && i < n) ? row_key<MODE>(keys[trow + i]) : row_pad<MODE>();
+      sS[tt][q] = (tv && j < n) ? col_key<MODE>(keys[trow + j]) : col_pad<MODE>();
+    }
+    __syncthreads();
+    for (int tt = 0; tt < tc; ++tt) {
+      int32_t a[kSub], b[kSub];
+      const int4* pa = reinterpret_cast<const int4*>(&sR[tt][rb]);
+      const int4* pb = reinterpret_cast<const int4*>(&sS[tt][cb]);
+      const int4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+      a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
+      a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
+      b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+      b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+#pragma unroll
+      for (int r = 0; r < kSub; ++r)
+#pragma unroll
+        for (int c = 0; c < kSub; ++c) {
+          if (MODE == kMode_PAIR) {
+            const uint32_t xr = static_cast<uint32_t>(a[r] ^ b[c]);
+            cnt[r][c] += xr < 2u ? ((xr << 16) | 1u) : 0u;
+          } else {
+            cnt[r][c] += (a[r] == b[c]) ? 1u : 0u;
+          }
+        }
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < kSub; ++r) {
+    const int64_t i = i0 + rb + r;
+    if (i >= n) continue;
+#pragma unroll
+    for (int c = 0; c < kSub; ++c) {
+      const int64_t j = j0 + cb + c;
+      if (j >= n) continue;
+      int64_t o;
+      if (LAYOUT == RFX_LAYOUT_UPPER) {
+        if (j < i) continue;
+        o = rfx_upper_index(i, j, n);
+      } else {
+        o = i + j * n;
+      }
+      if (MODE == kMode_PAIR) {
+        const int32_t va = static_cast<int32_t>(cnt[r][c] & 0xffffu);
+        const int32_t vb = static_cast<int32_t>(cnt[r][c] >> 16);
+        out[o] = accumulate ? out[o] + va : va;
+        out2[o] = accumulate ? out2[o] + vb : vb;
+      } else {
+        const int32_t v = static_cast<int32_t>(cnt[r][c]);
+        out[o] = accumulate ? out[o] + v : v;
+      }
+    }
+  }
+}
+
+// Rows [r0, r1) of the full matrix, written row-major (r - r0) * n + j.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void dense_rows_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                              int32_t ntree, int64_t r0, int64_t r1,
+                                                              int32_t* __restrict__ out) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t i = r0 + blockIdx.y;
+  if (j >= n || i >= r1) return;
+  uint32_t c = 0;
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int64_t tr = static_cast<int64_t>(t) * n;
+    c += (row_key<MODE>(keys[tr + i]) == col_key<MODE>(keys[tr + j])) ? 1u : 0u;
+  }
+  out[(i - r0) * n + j] = static_cast<int32_t>(c);
+}
+
+// ---------------------------------------------------------------------------
+// BUCKET algorithm
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__global__ void bucket_hist_kernel(const int32_t* __restrict__ keys, int64_t total, uint32_t* __restrict__ cnt) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t k = keys[e];
+    if (k >= 0) atomicAdd(&cnt[k >> 1], 1u);
+  }
+}
+
+// Block-wide exclusive scan of kScanItems per thread; returns the block total.
+__device__ uint32_t block_exclusive_scan(uint32_t (&v)[kScanItems], uint32_t* lds_waves) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const uint32_t x = v[q];
+    v[q] = s;
+    s += x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) lds_waves[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < (kScanThreads / 64) ? lds_waves[lane] : 0u;
+    uint32_t wi = w;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(wi, d, 64);
+      if (lane >= d) wi += y;
+    }
+    if (lane < (kScanThreads / 64)) lds_waves[lane] = wi - w;
+    if (lane == (kScanThreads / 64) - 1) lds_waves[kScanThreads / 64] = wi;
+  }
+  __syncthreads();
+  const uint32_t base = lds_waves[wave] + (incl - s);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) v[q] += base;
+  const uint32_t total = lds_waves[kScanThreads / 64];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t len,
+                                                                   uint32_t* __restrict__ part) {
+  __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) v[q] = base + q < len ? in[base + q] : 0u;
+  const uint32_t tot = block_exclusive_scan(v, lw);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ part, int64_t nparts) {
+  __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nparts; b0 += kScanChunk) {
+    const int64_t base = b0 + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) v[q] = base + q < nparts ? part[base + q] : 0u;
+    const uint32_t tot = block_exclusive_scan(v, lw);
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+      if (base + q < nparts) part[base + q] = v[q] + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) part[nparts] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t len,
+                                                                  const uint32_t* __restrict__ part,
+                                                                  uint32_t* __restrict__ out) {
+  __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) v[q] = base + q < len ? in[base + q] : 0u;
+  block_exclusive_scan(v, lw);
+  const uint32_t off = part[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q)
+    if (base + q < len) out[base + q] = v[q] + off;
+}
+
+// Scatter (tree, sample) slots into their buckets.  cursor starts as the
+// exclusive scan and ends as the bucket end offsets.
+__global__ void bucket_scatter_kernel(const int32_t* __restrict__ keys, int64_t n, int64_t total,
+                                      uint32_t* __restrict__ cursor, uint32_t* __restrict__ members,
+                                      uint32_t* __restrict__ mbucket) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t k = keys[e];
+    if (k < 0) continue;
+    const uint32_t b = static_cast<uint32_t>(k) >> 1;
+    const uint32_t pos = atomicAdd(&cursor[b], 1u);
+    const uint32_t i = static_cast<uint32_t>(e % n);
+    members[pos] = (i << 1) | (static_cast<uint32_t>(k) & 1u);
+    mbucket[pos] = b;
+  }
+}
+
+// One thread per bucket slot: walk the bucket, count each unordered pair
+// once (from its smaller sample index).
+template <int MODE, int LAYOUT>
+__global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __restrict__ members,
+                                                          const uint32_t* __restrict__ mbucket,
+                                                          const uint32_t* __restrict__ cnt,
+                                                          const uint32_t* __restrict__ bend,
+                                                          const uint32_t* __restrict__ nvalid_p, int64_t n,
+                                                          int32_t* __restrict__ out, int32_t* __restrict__ out2) {
+  const uint32_t nvalid = *nvalid_p;
+  for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < nvalid;
+       m += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t me = members[m];
+    const uint32_t b = mbucket[m];
+    const uint32_t e = bend[b], s = e - cnt[b];
+    const int64_t i = me >> 1;
+    const uint32_t bi = me & 1u;
+    for (uint32_t q = s; q < e; ++q) {
+      const uint32_t o = members[q];
+      const int64_t j = o >> 1;
+      if (j <= i) continue;
+      const uint32_t bj = o & 1u;
+      if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+        if (LAYOUT == RFX_LAYOUT_UPPER) {
+          atomicAdd(&out[rfx_upper_index(i, j, n)], 1);
+        } else {
+          atomicAdd(&out[i + j * n], 1);
+          atomicAdd(&out[j + i * n], 1);
+        }
+      }
+      if (bi != bj) {
+        int32_t* o2 = MODE == kMode_PAIR ? out2 : out;
+        if (MODE == kMode_OOB_UPPER || MODE == kMode_PAIR) {
+          if (LAYOUT == RFX_LAYOUT_UPPER) {
+            atomicAdd(&o2[rfx_upper_index(i, j, n)], 1);
+          } else {
+            atomicAdd(&o2[i + j * n], 1);
+            atomicAdd(&o2[j + i * n], 1);
+          }
+        } else if (MODE == kMode_OOB_ASYM) {
+          // row = the out-of-bag sample, column = the in-bag one
+          if (bi == 0)
+            atomicAdd(&out[i + j * n], 1);
+          else
+            atomicAdd(&out[j + i * n], 1);
+        }
+      }
+    }
+  }
+}
+
+// Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
+__global__ void all_diag_kernel(const int32_t* __restrict__ keys, int64_t n, int32_t ntree, int32_t layout,
+                                int32_t* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = 0;
+  for (int32_t t = 0; t < ntree; ++t) c += keys[static_cast<int64_t>(t) * n + i] >= 0 ? 1 : 0;
+  const int64_t o = layout == RFX_LAYOUT_UPPER ? rfx_upper_index(i, i, n) : i + i * n;
+  out[o] += c;
+}
+
+__global__ void sum_squares_kernel(const uint32_t* __restrict__ cnt, int64_t len,
+                                   unsigned long long* __restrict__ acc) {
+  unsigned long long s = 0;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < len;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const unsigned long long c = cnt[e];
+    s += c * c;
+  }
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+}
+
+struct BucketWs {
+  uint32_t* cnt;
+  uint32_t* bend;
+  uint32_t* part;
+  uint32_t* members;
+  uint32_t* mbucket;
+  unsigned long long* stat;
+  int64_t nparts;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
+  const int64_t nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  return align256(4 * total_nodes) * 2 + align256(4 * (nparts + 1)) + align256(4 * slots) * 2 + 256;
+}
+
+BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes) {
+  BucketWs w;
+  char* p = static_cast<char*>(ws);
+  w.nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  w.cnt = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * total_nodes);
+  w.bend = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * total_nodes);
+  w.part = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * (w.nparts + 1));
+  w.members = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * slots);
+  w.mbucket = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * slots);
+  w.stat = reinterpret_cast<unsigned long long*>(p);
+  return w;
+}
+
+unsigned grid_for(int64_t work, int threads) {
+  int64_t g = (work + threads - 1) / threads;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+// Histogram + scan: cnt and bend (as bucket start) ready, part[nparts] = total.
+int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                 const BucketWs& w) {
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  RFX_HIP_TRY(hipMemsetAsync(w.cnt, 0, 4 * total_nodes, st));
+  hipLaunchKernelGGL(bucket_hist_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, slots, w.cnt);
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
+                     total_nodes, w.part);
+  hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
+                     total_nodes, w.part, w.bend);
+  return rfx_launch_status();
+}
+
+template <int MODE, int LAYOUT>
+int run_dense(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
+              int32_t* out2) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  const int64_t tiles = LAYOUT == RFX_LAYOUT_UPPER ? nb * (nb + 1) / 2 : nb * nb;
+  if (tiles > 0x7fffffffLL) return RFX_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((dense_tile_kernel<MODE, LAYOUT>), dim3(static_cast<unsigned>(tiles)), dim3(kThreads), 0, st,
+                     keys, n, ntree, nb, accumulate, out, out2);
+  return rfx_launch_status();
+}
+
+template <int MODE, int LAYOUT>
+int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+               int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  const int64_t elems = LAYOUT == RFX_LAYOUT_UPPER ? n * (n + 1) / 2 : n * n;
+  if (!accumulate) {
+    RFX_HIP_TRY(hipMemsetAsync(out, 0, 4 * elems, st));
+    if (out2) RFX_HIP_TRY(hipMemsetAsync(out2, 0, 4 * elems, st));
+  }
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, n, slots, w.bend,
+                     w.members, w.mbucket);
+  hipLaunchKernelGGL((bucket_pair_kernel<MODE, LAYOUT>), dim3(grid_for(slots, 256)), dim3(256), 0, st, w.members,
+                     w.mbucket, w.cnt, w.bend, w.part + w.nparts, n, out, out2);
+  if (MODE == kMode_ALL || MODE == kMode_PAIR)
+    hipLaunchKernelGGL(all_diag_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, keys, n, ntree, LAYOUT, out);
+  return rfx_launch_status();
+}
+
+template <int MODE>
+int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout,
+             int32_t algo, int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+  if (algo == RFX_ALGO_DENSE) {
+    return layout == RFX_LAYOUT_UPPER ? run_dense<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, accumulate, out, out2)
+                                      : run_dense<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, accumulate, out, out2);
+  }
+  return layout == RFX_LAYOUT_UPPER
+             ? run_bucket<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws)
+             : run_bucket<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws);
+}
+
+// AUTO: compare the compare-kernel cost T*n^2/2 against the bucket cost
+// sum |leaf|^2 (+ one pass over the output), with the per-item cost ratio
+// measured on MI355X (an atomic pair update costs about as much as
+// kBucketPerPairVsCompare tile compares).
+constexpr double kBucketPerPairVsCompare = 24.0;
+
+int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, void* ws,
+                size_t ws_bytes, int32_t* algo) {
+  if (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) {
+    *algo = RFX_ALGO_DENSE;
+    return RFX_OK;
+  }
+  double sq = 0;
+  int rc = rfx_prox_pair_stats(st, keys, n, ntree, total_nodes, ws, ws_bytes, &sq);
+  if (rc) return rc;
+  const double dense_cost = 0.5 * static_cast<double>(ntree) * static_cast<double>(n) * static_cast<double>(n);
+  const double bucket_cost = 0.5 * sq * kBucketPerPairVsCompare + 0.5 * static_cast<double>(n) * n * 4.0;
+  *algo = bucket_cost < dense_cost ? RFX_ALGO_BUCKET : RFX_ALGO_DENSE;
+  return RFX_OK;
+}
+
+int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout, int32_t algo) {
+  if (!keys || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || ntree > 65535) return RFX_ERR_INVALID;
+  if (layout != RFX_LAYOUT_FULL && layout != RFX_LAYOUT_UPPER) return RFX_ERR_INVALID;
+  if (algo < RFX_ALGO_AUTO || algo > RFX_ALGO_BUCKET) return RFX_ERR_INVALID;
+  if (algo != RFX_ALGO_DENSE && (total_nodes <= 0 || total_nodes > RFX_MAX_TOTAL_NODES)) return RFX_ERR_INVALID;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 32)) return RFX_ERR_UNSUPPORTED;
+  return RFX_OK;
+}
+
+}  // namespace
+
+extern "C" size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo) {
+  if (algo == RFX_ALGO_DENSE) return 0;
+  if (n <= 0 || ntree <= 0 || total_nodes <= 0) return 0;
+  return bucket_ws_bytes(n, ntree, total_nodes);
+}
+
+extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                   int64_t total_nodes, void* ws, size_t ws_bytes, double* sum_sq_host) {
+  if (!keys || !ws || !sum_sq_host || n <= 0 || ntree <= 0 || total_nodes <= 0 ||
+      total_nodes > RFX_MAX_TOTAL_NODES)
+    return RFX_ERR_INVALID;
+  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  RFX_HIP_TRY(hipMemsetAsync(w.cnt, 0, 4 * total_nodes, st));
+  RFX_HIP_TRY(hipMemsetAsync(w.stat, 0, 8, st));
+  hipLaunchKernelGGL(bucket_hist_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, slots, w.cnt);
+  hipLaunchKernelGGL(sum_squares_kernel, dim3(grid_for(total_nodes, 256)), dim3(256), 0, st, w.cnt, total_nodes,
+                     w.stat);
+  int rc = rfx_launch_status();
+  if (rc) return rc;
+  unsigned long long h = 0;
+  RFX_HIP_TRY(hipMemcpyAsync(&h, w.stat, 8, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  *sum_sq_host = static_cast<double>(h);
+  return RFX_OK;
+}
+
+extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                   int64_t total_nodes, int32_t variant, int32_t layout, int32_t algo,
+                                   int32_t accumulate, int32_t* out, void* ws, size_t ws_bytes) {
+  int rc = check_common(keys, n, ntree, total_nodes, layout, algo);
+  if (rc) return rc;
+  if (!out) return RFX_ERR_INVALID;
+  if (variant == RFX_PROX_OOB_ASYM && layout != RFX_LAYOUT_FULL) return RFX_ERR_UNSUPPORTED;
+  if (algo == RFX_ALGO_AUTO) {
+    rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
+    if (rc) return rc;
+  }
+  if (algo == RFX_ALGO_BUCKET && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+    return RFX_ERR_WORKSPACE;
+  switch (variant) {
+    case RFX_PROX_ALL:
+      return dispatch<kMode_ALL>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr, ws);
+    case RFX_PROX_OOB_ASYM:
+      return dispatch<kMode_OOB_ASYM>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr, ws);
+    case RFX_PROX_OOB_UPPER:
+      return dispatch<kMode_OOB_UPPER>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr,
+                                       ws);
+    default:
+      return RFX_ERR_INVALID;
+  }
+}
+
+extern "C" int rfx_prox_accumulate_pair(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                        int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,
+                                        int32_t* out_all, int32_t* out_oob, void* ws, size_t ws_bytes) {
+  int rc = check_common(keys, n, ntree, total_nodes, layout, algo);
+  if (rc) return rc;
+  if (!out_all || !out_oob) return RFX_ERR_INVALID;
+  if (algo == RFX_ALGO_AUTO) {
+    rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
+    if (rc) return rc;
+  }
+  if (algo == RFX_ALGO_BUCKET && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+    return RFX_ERR_WORKSPACE;
+  return dispatch<kMode_PAIR>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out_all, out_oob, ws);
+}
+
+extern "C" int rfx_prox_rows(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int32_t variant,
+                             int64_t row_begin, int64_t row_end, int32_t* out) {
+  if (!keys || !out || n <= 0 || ntree <= 0 || row_begin < 0 || row_end > n || row_end <= row_begin ||
+      row_end - row_begin > 65535)
+    return RFX_ERR_INVALID;
+  dim3 grid(rfx_ceil_div(n, kThreads), static_cast<unsigned>(row_end - row_begin));
+  switch (variant) {
+    case RFX_PROX_ALL:
+      hipLaunchKernelGGL(dense_rows_kernel<kMode_ALL>, grid, dim3(kThreads), 0, st, keys, n, ntree, row_begin,
+                         row_end, out);
+      break;
+    case RFX_PROX_OOB_ASYM:
+      hipLaunchKernelGGL(dense_rows_kernel<kMode_OOB_ASYM>, grid, dim3(kThreads), 0, st, keys, n, ntree,
+                         row_begin, row_end, out);
+      break;
+    case RFX_PROX_OOB_UPPER:
+      hipLaunchKernelGGL(dense_rows_kernel<kMode_OOB_UPPER>, grid, dim3(kThreads), 0, st, keys, n, ntree,
+                         row_begin, row_end, out);
+      break;
+    default:
+      return RFX_ERR_INVALID;
+  }
+  return rfx_launch_status();
+}

@@ -1,0 +1,112 @@
+// A1: leaf assignment of every sample through every tree on gfx950.
+//
+// One thread follows one sample down TREES_PER_BLOCK consecutive trees.
+// Node records are 8 bytes (threshold bits, left << 10 | feature; -1 marks a
+// terminal node) so a traversal step is one 8-byte gather plus one 4-byte
+// gather of the sample's feature from the column-major matrix; the top
+// levels of every tree are shared by the whole wavefront and stay in L1/L2.
+//
+// Semantics follow the reference CPU path exactly:
+//  * out-of-bag samples (src/rf_testreebag.cpp:51-122): descend while the node
+//    is split, at most 999 descents, a node that is not terminal afterwards
+//    is reported as node 0; categorical features use (int)x with a 0-based
+//    catgoleft lookup, which the reference grower never fills, so they always
+//    go right;
+//  * in-bag samples (src/rf_random_forest.cpp:2194-2244): categorical
+//    features use (int)(x + 0.5); a value in [1, ncat] goes right (catgoleft
+//    is all zero) and anything else stops at node 0.
+// With all features numeric both rules reduce to x <= threshold -> left.
+
+#include "rfx_common.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTreesPerBlock = 8;
+constexpr int kMaxOobDescents = 999;
+
+__global__ __launch_bounds__(kBlock) void leaf_assign_kernel(const float* __restrict__ x, int64_t n,
+                                                             const uint2* __restrict__ nodes,
+                                                             const int64_t* __restrict__ node_off, int32_t ntree,
+                                                             const uint32_t* __restrict__ inbag,
+                                                             const int32_t* __restrict__ cat,
+                                                             int32_t* __restrict__ nodextr,
+                                                             int32_t* __restrict__ keys) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t words = (n + 31) / 32;
+  const int32_t t0 = blockIdx.y * kTreesPerBlock;
+  const int32_t t1 = min(t0 + kTreesPerBlock, ntree);
+  for (int32_t t = t0; t < t1; ++t) {
+    const int64_t base = node_off[t];
+    const uint2* tn = nodes + base;
+    const bool bag = (inbag[t * words + (i >> 5)] >> (i & 31)) & 1u;
+    uint32_t kt = 0;
+    uint2 nd = tn[0];
+    if (cat == nullptr) {
+      // All-numeric fast path: both traversal rules are x <= thr -> left.
+      int32_t steps = 0;
+      while (nd.y != 0xffffffffu && (bag || steps < kMaxOobDescents)) {
+        const uint32_t f = nd.y & 1023u, l = nd.y >> 10;
+        const float v = x[static_cast<int64_t>(f) * n + i];
+        kt = v <= __uint_as_float(nd.x) ? l : l + 1;
+        nd = tn[kt];
+        ++steps;
+      }
+      if (nd.y != 0xffffffffu) kt = 0;  // out-of-bag depth cap reached
+    } else if (bag) {
+      while (nd.y != 0xffffffffu) {
+        const uint32_t f = nd.y & 1023u, l = nd.y >> 10;
+        const float v = x[static_cast<int64_t>(f) * n + i];
+        const int32_t nc = cat[f];
+        if (nc == 1) {
+          kt = v <= __uint_as_float(nd.x) ? l : l + 1;
+        } else {
+          const int32_t jcat = static_cast<int32_t>(static_cast<double>(v) + 0.5);
+          if (jcat >= 1 && jcat <= nc) {
+            kt = l + 1;
+          } else {
+            kt = 0;
+            nd = tn[0];
+            break;
+          }
+        }
+        nd = tn[kt];
+      }
+    } else {
+      int32_t steps = 0;
+      while (nd.y != 0xffffffffu && steps < kMaxOobDescents) {
+        const uint32_t f = nd.y & 1023u, l = nd.y >> 10;
+        const float v = x[static_cast<int64_t>(f) * n + i];
+        kt = (cat[f] != 1 || v > __uint_as_float(nd.x)) ? l + 1 : l;
+        nd = tn[kt];
+        ++steps;
+      }
+      if (nd.y != 0xffffffffu) {
+        kt = 0;
+        nd = tn[0];
+      }
+    }
+    const int64_t o = static_cast<int64_t>(t) * n + i;
+    if (nodextr) nodextr[o] = static_cast<int32_t>(kt);
+    if (keys) {
+      const bool terminal = nd.y == 0xffffffffu;
+      keys[o] = terminal ? static_cast<int32_t>(((base + kt) << 1) | (bag ? 1 : 0)) : -1;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n, int32_t p, const uint32_t* nodes,
+                               const int64_t* node_off, int32_t ntree, const uint32_t* inbag, const int32_t* cat,
+                               int32_t* nodextr, int32_t* keys) {
+  if (!x_col || !nodes || !node_off || !inbag || n <= 0 || n > RFX_MAX_SAMPLES || p <= 0 || p > RFX_MAX_FEATURES ||
+      ntree <= 0)
+    return RFX_ERR_INVALID;
+  if (!nodextr && !keys) return RFX_OK;
+  dim3 grid(rfx_ceil_div(n, kBlock), rfx_ceil_div(ntree, kTreesPerBlock));
+  hipLaunchKernelGGL(leaf_assign_kernel, grid, dim3(kBlock), 0, stream, x_col, n,
+                     reinterpret_cast<const uint2*>(nodes), node_off, ntree, inbag, cat, nodextr, keys);
+  return rfx_launch_status();
+}

@@ -1,0 +1,26 @@
+// Shared helpers of the HIP translation units: status mapping and the
+// packed upper-triangle index.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "rfx_prox.h"
+
+#define RFX_HIP_TRY(expr)                     \
+  do {                                        \
+    hipError_t rfx_e_ = (expr);               \
+    if (rfx_e_ != hipSuccess) return RFX_ERR_HIP; \
+  } while (0)
+
+// Launch status: a failed launch is reported, never ignored.
+inline int rfx_launch_status() { return hipGetLastError() == hipSuccess ? RFX_OK : RFX_ERR_HIP; }
+
+// Offset of row i in the packed row-major upper triangle with diagonal.
+__host__ __device__ inline int64_t rfx_upper_row(int64_t i, int64_t n) { return i * n - (i * (i - 1)) / 2; }
+
+__host__ __device__ inline int64_t rfx_upper_index(int64_t i, int64_t j, int64_t n) {
+  return rfx_upper_row(i, n) + (j - i);
+}
+
+inline unsigned rfx_ceil_div(int64_t a, int64_t b) { return static_cast<unsigned>((a + b - 1) / b); }

@@ -1,0 +1,141 @@
+"""GPU proximity path over the C ABI: leaf assignment, co-occurrence counts,
+finishprox.  torch provides device memory and the current HIP stream; all
+arithmetic runs in librfx's HIP kernels.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .forest import Forest
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the rfx_amd proximity path runs on an MI355X GPU; no GPU is visible")
+
+
+def _dptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def upper_size(n: int) -> int:
+    return n * (n + 1) // 2
+
+
+class DeviceForest:
+    """A forest and its sample matrix resident in HBM.
+
+    x is stored column-major (feature m of sample i at m*n + i) so that a
+    traversal step of 64 consecutive samples reads one feature column; nodes
+    are the packed 8-byte records; the bootstrap is an in-bag bitmask.
+    """
+
+    def __init__(self, forest: Forest, x: np.ndarray, cat: np.ndarray | None = None, device: str = "cuda:0"):
+        _require_gpu()
+        if forest.total_nodes > _lib.MAX_TOTAL_NODES:
+            raise ValueError("forest too large for 32-bit leaf keys")
+        x = np.asarray(x, np.float32)
+        if x.shape != (forest.n, forest.p):
+            raise ValueError(f"x shape {x.shape} does not match forest ({forest.n}, {forest.p})")
+        self.forest = forest
+        self.n, self.p, self.ntree = forest.n, forest.p, forest.ntree
+        self.total_nodes = forest.total_nodes
+        dev = torch.device(device)
+        self.device = dev
+        self.x_col = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+        self.nodes = torch.from_numpy(forest.packed_nodes().view(np.int32)).to(dev)
+        self.node_off = torch.from_numpy(forest.node_offset).to(dev)
+        self.inbag = torch.from_numpy(forest.inbag_bits().view(np.int32)).to(dev)
+        self.cat = None
+        if cat is not None and np.any(np.asarray(cat) != 1):
+            self.cat = torch.from_numpy(np.ascontiguousarray(cat, np.int32)).to(dev)
+
+    def leaf_assign(self, keys: torch.Tensor | None = None, nodextr: torch.Tensor | None = None,
+                    want_nodextr: bool = False) -> tuple[torch.Tensor, torch.Tensor | None]:
+        """keys [T, n] int32 ((global node << 1) | in-bag, -1 off-leaf) and,
+        optionally, nodextr [T, n] (the reference's node ids)."""
+        if keys is None:
+            keys = torch.empty((self.ntree, self.n), dtype=torch.int32, device=self.device)
+        if want_nodextr and nodextr is None:
+            nodextr = torch.empty((self.ntree, self.n), dtype=torch.int32, device=self.device)
+        check(lib().rfx_leaf_assign(_stream(), _dptr(self.x_col), self.n, self.p, _dptr(self.nodes),
+                                    _dptr(self.node_off), self.ntree, _dptr(self.inbag), _dptr(self.cat),
+                                    _dptr(nodextr), _dptr(keys)), "rfx_leaf_assign")
+        return keys, nodextr
+
+
+class Accumulator:
+    """Co-occurrence counts from leaf keys, with its workspace held in HBM."""
+
+    def __init__(self, n: int, ntree: int, total_nodes: int, device: str = "cuda:0"):
+        _require_gpu()
+        self.n, self.ntree, self.total_nodes = n, ntree, total_nodes
+        self.device = torch.device(device)
+        nb = lib().rfx_prox_workspace_bytes(n, ntree, total_nodes, _lib.ALGO_BUCKET)
+        self.ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=self.device)
+
+    def out_elems(self, layout: int) -> int:
+        return upper_size(self.n) if layout == _lib.LAYOUT_UPPER else self.n * self.n
+
+    def alloc(self, layout: int) -> torch.Tensor:
+        return torch.empty(self.out_elems(layout), dtype=torch.int32, device=self.device)
+
+    def accumulate(self, keys: torch.Tensor, variant: int, layout: int, algo: int = _lib.ALGO_AUTO,
+                   out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+        if out is None:
+            out = self.alloc(layout)
+            accumulate = False
+        check(lib().rfx_prox_accumulate(_stream(), _dptr(keys), self.n, self.ntree, self.total_nodes, variant, layout,
+                                        algo, int(accumulate), _dptr(out), _dptr(self.ws), self.ws.numel()),
+              "rfx_prox_accumulate")
+        return out
+
+    def accumulate_pair(self, keys: torch.Tensor, layout: int, algo: int = _lib.ALGO_AUTO,
+                        out_all: torch.Tensor | None = None, out_oob: torch.Tensor | None = None,
+                        accumulate: bool = False) -> tuple[torch.Tensor, torch.Tensor]:
+        if out_all is None or out_oob is None:
+            out_all, out_oob = self.alloc(layout), self.alloc(layout)
+            accumulate = False
+        check(lib().rfx_prox_accumulate_pair(_stream(), _dptr(keys), self.n, self.ntree, self.total_nodes, layout,
+                                             algo, int(accumulate), _dptr(out_all), _dptr(out_oob), _dptr(self.ws),
+                                             self.ws.numel()), "rfx_prox_accumulate_pair")
+        return out_all, out_oob
+
+    def rows(self, keys: torch.Tensor, variant: int, r0: int, r1: int) -> torch.Tensor:
+        out = torch.empty((r1 - r0, self.n), dtype=torch.int32, device=self.device)
+        check(lib().rfx_prox_rows(_stream(), _dptr(keys), self.n, self.ntree, variant, r0, r1, _dptr(out)),
+              "rfx_prox_rows")
+        return out
+
+    def pair_stats(self, keys: torch.Tensor) -> float:
+        import ctypes as C
+        v = C.c_double()
+        check(lib().rfx_prox_pair_stats(_stream(), _dptr(keys), self.n, self.ntree, self.total_nodes, _dptr(self.ws),
+                                        self.ws.numel(), C.byref(v)), "rfx_prox_pair_stats")
+        return v.value
+
+
+def finishprox(counts: torch.Tensor, in_layout: int, nout: torch.Tensor, n: int,
+               out_layout: int = _lib.LAYOUT_FULL) -> torch.Tensor:
+    """rf::cpu_finishprox on the GPU (bit-identical float64)."""
+    elems = upper_size(n) if out_layout == _lib.LAYOUT_UPPER else n * n
+    out = torch.empty(elems, dtype=torch.float64, device=counts.device)
+    check(lib().rfx_finishprox(_stream(), _dptr(counts), in_layout, _dptr(nout), n, _dptr(out), out_layout),
+          "rfx_finishprox")
+    return out
+
+
+def unpack_upper(u: np.ndarray, n: int) -> np.ndarray:
+    """Packed row-major upper triangle -> symmetric dense (host helper)."""
+    m = np.zeros((n, n), dtype=u.dtype)
+    iu = np.triu_indices(n)
+    m[iu] = u
+    m.T[iu] = u
+    return m

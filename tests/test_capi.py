@@ -1,0 +1,65 @@
+"""CPU: the C ABI library loads, exports every declared entry point, and
+rejects bad arguments with status codes (no GPU work is launched)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from rfx_amd import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "rfx_prox.h"
+
+
+def declared_symbols() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(rfx_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in sorted(syms):
+        assert hasattr(L, s), f"{s} declared in rfx_prox.h but not exported"
+    assert syms <= set(_lib.exported_symbols()), "binding table misses a declared symbol"
+
+
+def test_invalid_arguments_return_status():
+    L = _lib.lib()
+    assert L.rfx_leaf_assign(None, None, 0, 1, None, None, 1, None, None, None, None) == -1
+    assert L.rfx_prox_accumulate(None, None, 0, 1, 1, 0, 0, 0, 0, None, None, 0) == -1
+    assert L.rfx_prox_accumulate(None, None, 10, 1, 1, 9, 0, 1, 0, None, None, 0) == -1
+    assert L.rfx_finishprox(None, None, 0, None, 10, None, 0) == -1
+    assert L.rfx_prox_rows(None, None, 10, 1, 0, 5, 3, None) == -1
+    h = C.c_void_p()
+    x = np.zeros((4, 2), np.float32)
+    y = np.zeros(4, np.int32)
+    assert L.rfx_forest_grow(x.ctypes.data, y.ctypes.data, 4, 2, 1, 1, 1, 1, 0, 0, 0, 0, 1, C.byref(h)) == -1
+
+
+def test_pack_nodes_rejects_malformed_forest():
+    L = _lib.lib()
+    off = np.array([0, 3], np.int64)
+    st = np.array([1, -1, -1], np.int32)
+    bv = np.array([0, 0, 0], np.int32)
+    xs = np.zeros(3, np.float32)
+    tm = np.array([1, 2, 0, 0, 0, 0], np.int32)
+    out = np.empty(6, np.uint32)
+    a = lambda v: v.ctypes.data  # noqa: E731
+    assert L.rfx_pack_nodes(a(off), 1, a(st), a(bv), a(xs), a(tm), 1, a(out)) == 0
+    assert out[1] == (1 << 10) and out[3] == 0xFFFFFFFF
+    tm_bad = np.array([1, 1, 0, 0, 0, 0], np.int32)  # right child must be left + 1
+    assert L.rfx_pack_nodes(a(off), 1, a(st), a(bv), a(xs), a(tm_bad), 1, a(out)) == -1
+    st_bad = np.array([2, -1, -1], np.int32)  # unfinished node
+    assert L.rfx_pack_nodes(a(off), 1, a(st_bad), a(bv), a(xs), a(tm), 1, a(out)) == -1
+
+
+def test_workspace_size_formula():
+    L = _lib.lib()
+    assert L.rfx_prox_workspace_bytes(1000, 10, 500, _lib.ALGO_DENSE) == 0
+    b = L.rfx_prox_workspace_bytes(1000, 10, 500, _lib.ALGO_BUCKET)
+    assert b >= 8 * 500 + 8 * 1000 * 10

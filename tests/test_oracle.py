@@ -1,0 +1,175 @@
+"""CPU: the oracle and the host-side forest contract, pinned against the
+reference's golden vectors, published known answers and closed forms."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rfx_amd import _lib
+from rfx_amd.classifier import detect_categorical
+from rfx_amd.forest import Forest, make_dataset
+
+from helpers import load_golden, small_forest
+
+GOLDEN = ["gauss_n200.npz", "covtype_cat_n240.npz"]
+
+
+def test_mt19937_known_answers():
+    L = _lib.lib()
+    # mt19937ar.c, init_genrand(5489): first outputs; std::mt19937 10000th output
+    first = [L.rfx_mt19937_u32(5489, i) for i in range(5)]
+    assert first == [3499211612, 581869302, 3890346734, 3586334585, 545404204]
+    assert L.rfx_mt19937_u32(5489, 9999) == 4123659995
+
+
+def test_bootstrap_closed_form():
+    # nin_i = #{draws d : int(float(u_d) * n) == i}, u_d = MT19937 output / (2^32 - 1)
+    L = _lib.lib()
+    n, seed = 97, 12345 + 3
+    nin = np.zeros(n, np.int32)
+    assert L.rfx_bootstrap(n, seed, nin.ctypes.data) == 0
+    u = np.array([L.rfx_mt19937_u32(seed, d) for d in range(n)], np.float64) / 4294967295.0
+    idx = np.minimum((u.astype(np.float32) * np.float32(n)).astype(np.int32), n - 1)
+    assert np.array_equal(nin, np.bincount(idx, minlength=n))
+    assert nin.sum() == n
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_grower_matches_reference_forest(name):
+    g = load_golden(name)
+    x, y = g["x"], g["y"]
+    f = Forest.grow(x, y, int(g["ntree"]), mtry=int(g["mtry"]), iseed=int(g["iseed"]))
+    ref = g["forest"]
+    assert np.array_equal(f.node_offset, ref.node_offset)
+    assert np.array_equal(f.nodestatus, ref.nodestatus)
+    assert np.array_equal(f.bestvar, ref.bestvar)
+    assert np.array_equal(f.xbestsplit.view(np.uint32), ref.xbestsplit.view(np.uint32))
+    assert np.array_equal(f.treemap, ref.treemap)
+    assert np.array_equal(f.nin, ref.nin)
+    assert np.array_equal(f.nout(2), g["nout"])
+
+
+def test_categorical_detection_matches_fixture():
+    g = load_golden("covtype_cat_n240.npz")
+    cat = detect_categorical(g["x"], int(g["maxcat"]))
+    assert np.array_equal(cat, g["cat"])
+    assert (cat[:10] == 1).all() and (cat[10:] == 2).all()
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_oracle_leaf_assign_matches_reference(name):
+    g = load_golden(name)
+    cat = g["cat"] if int(g["maxcat"]) > 1 else None
+    nd = O.leaf_assign(g["forest"], g["x"], cat)
+    assert np.array_equal(nd, g["nodextr"])
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_oracle_v1_and_counts_match_reference(name):
+    g = load_golden(name)
+    f = g["forest"]
+    prox = O.prox_v1(f, g["nodextr"])
+    assert np.array_equal(prox, g["prox"])
+    P = O.counts(f, g["nodextr"], O.OOB_ASYM, 0)
+    assert np.array_equal(P.astype(np.float64), g["prox"])
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_oracle_finishprox_bitwise(name):
+    g = load_golden(name)
+    n = g["x"].shape[0]
+    S = O.finishprox(n, g["nout"], g["prox"])
+    assert np.array_equal(S.view(np.uint64), g["proxsym"].view(np.uint64))
+
+
+def test_finishprox_closed_form():
+    g = load_golden("gauss_n200.npz")
+    n = g["x"].shape[0]
+    P = g["prox"].reshape(n, n, order="F")
+    nout = g["nout"].astype(np.float64)
+    r = np.where(nout > 0, 1.0 / np.where(nout > 0, nout, 1.0), 1.0)
+    a = (P + P.T) / 2.0
+    S = 0.5 * (a * r[:, None] + a * r[None, :])
+    np.fill_diagonal(S, 1.0)
+    assert np.allclose(S, g["proxsym"].reshape(n, n, order="F"), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("variant", [O.ALL, O.OOB_ASYM, O.OOB_UPPER])
+def test_bucket_counts_equal_brute_force(variant):
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 150, 5, 3, 7, seed=2)
+    nd = O.leaf_assign(f, x)
+    a = O.counts(f, nd, variant, 0)
+    b = O.counts_brute(f, nd, variant)
+    assert np.array_equal(a, b)
+    n = f.n
+    if variant != O.OOB_ASYM:
+        u = O.counts(f, nd, variant, 1)
+        iu = np.triu_indices(n)
+        assert np.array_equal(u, b.reshape(n, n, order="F")[iu])
+
+
+def test_count_invariants():
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 180, 6, 4, 9, seed=4)
+    n, T = f.n, f.ntree
+    nd = O.leaf_assign(f, x)
+    C = O.counts(f, nd, O.ALL, 0).reshape(n, n, order="F").astype(np.int64)
+    P = O.counts(f, nd, O.OOB_ASYM, 0).reshape(n, n, order="F").astype(np.int64)
+    U = O.counts(f, nd, O.OOB_UPPER, 0).reshape(n, n, order="F").astype(np.int64)
+    # numeric features: every sample reaches a leaf in every tree
+    assert (np.diag(C) == T).all()
+    assert (C == C.T).all()
+    sizes = np.zeros(n, np.int64)
+    total = 0
+    oob_row = np.zeros(n, np.int64)
+    for t in range(T):
+        _, inv, cnt = np.unique(nd[t], return_inverse=True, return_counts=True)
+        sizes += cnt[inv]
+        total += int((cnt.astype(np.int64) ** 2).sum())
+        inb = f.nin[t] > 0
+        inb_per_leaf = np.bincount(inv, weights=inb, minlength=len(cnt))
+        oob_row += np.where(~inb, inb_per_leaf[inv], 0).astype(np.int64)
+    assert np.array_equal(C.sum(axis=1), sizes)
+    assert C.sum() == total
+    assert (np.diag(P) == 0).all()
+    assert np.array_equal(P.sum(axis=1), oob_row)
+    assert np.array_equal(U, P + P.T)
+
+
+def test_oracle_leaf_assign_against_reference_fresh():
+    if not O.ref_available():
+        pytest.skip("reference build oracle/_ref not present")
+    x, y = make_dataset(_lib.DATA_OVERLAP_MIXTURE, 300, 7, 3, 9)
+    r = O.ref_run_forest(x, y, 6, 2, iseed=3, do_prox=True)
+    f = Forest.from_arrays(300, 7, r["nnode"], r["nodestatus"], r["bestvar"], r["xbestsplit"], r["treemap"], r["nin"],
+                           maxnode=r["maxnode"])
+    nd = O.leaf_assign(f, x)
+    assert np.array_equal(nd, r["nodextr"])
+    assert np.array_equal(O.prox_v1(f, nd), r["prox"])
+    g = Forest.grow(x, y, 6, mtry=2, iseed=3)
+    assert np.array_equal(g.nodestatus, f.nodestatus) and np.array_equal(g.treemap, f.treemap)
+
+
+def test_mds_oracle_recovers_euclidean_configuration():
+    # a proximity built from a Euclidean 3-D configuration: P = max - D  ->
+    # classical MDS recovers the centred points up to an orthogonal map
+    rng = np.random.default_rng(0)
+    Z = rng.normal(size=(60, 3)) * np.array([5.0, 3.0, 1.5])
+    Z -= Z.mean(axis=0)
+    D = np.sqrt(((Z[:, None, :] - Z[None, :, :]) ** 2).sum(-1))
+    P = D.max() - D
+    X, w = O.classical_mds(P, 3)
+    Dx = np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1))
+    assert np.allclose(Dx, D, atol=1e-8)
+    G = Z @ Z.T
+    assert np.allclose(np.sort(np.linalg.eigvalsh(G))[::-1][:3], w[:3], rtol=1e-10)
+
+
+def test_embedding_error_metrics():
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(50, 3))
+    e = O.embedding_errors(X * np.array([1, -1, 1]), X)
+    assert e["rel_frobenius"] < 1e-14 and e["max_principal_angle"] < 1e-6
+    Y = X.copy()
+    Y[:, 2] = rng.normal(size=50)
+    assert O.embedding_errors(Y, X)["max_principal_angle"] > 0.1
