@@ -77,7 +77,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket"])
+    ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket", "gather"])
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -93,6 +93,7 @@ def main() -> int:
     from rfx_amd.forest import Forest, make_dataset
     from rfx_amd.proximity import Accumulator, DeviceForest
 
+    NAMES = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket", _lib.ALGO_GATHER: "gather"}
     kind, n, p, ncl, T, mtry, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     x, y = make_dataset(kind, n, p, ncl, 0)
@@ -107,11 +108,12 @@ def main() -> int:
     out_oob = acc.alloc(_lib.LAYOUT_UPPER)
     df.leaf_assign(keys=keys)
     sq = acc.pair_stats(keys)
-    algo = {"auto": _lib.ALGO_AUTO, "dense": _lib.ALGO_DENSE, "bucket": _lib.ALGO_BUCKET}[args.algo]
+    algo = {"auto": _lib.ALGO_AUTO, "dense": _lib.ALGO_DENSE, "bucket": _lib.ALGO_BUCKET,
+            "gather": _lib.ALGO_GATHER}[args.algo]
     if algo == _lib.ALGO_AUTO:
-        # resolve AUTO once, outside the timed region, by timing both kernels
+        # resolve AUTO once, outside the timed region, by timing each algorithm
         cand = {}
-        for a in (_lib.ALGO_DENSE, _lib.ALGO_BUCKET):
+        for a in (_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER):
             acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, a, out_all, out_oob)
             torch.cuda.synchronize()
             s = time.perf_counter()
@@ -120,8 +122,8 @@ def main() -> int:
             torch.cuda.synchronize()
             cand[a] = (time.perf_counter() - s) / 3
         algo = min(cand, key=cand.get)
-        log(f"algo timings (s): dense {cand[_lib.ALGO_DENSE]:.5f} bucket {cand[_lib.ALGO_BUCKET]:.5f}")
-    algo_name = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket"}[algo]
+        log("algo timings (s): " + " ".join(f"{NAMES[a]} {v:.5f}" for a, v in cand.items()))
+    algo_name = NAMES[algo]
 
     stream = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),

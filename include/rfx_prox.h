@@ -66,6 +66,7 @@ extern "C" {
 #define RFX_ALGO_AUTO 0   /* choose from the forest's leaf-size statistics  */
 #define RFX_ALGO_DENSE 1  /* LDS-staged key tiles, register-tiled compare   */
 #define RFX_ALGO_BUCKET 2 /* leaf buckets, integer atomics per matching pair */
+#define RFX_ALGO_GATHER 3 /* leaf buckets gathered per output line in LDS  */
 
 /* ========================================================================
  * Host side: forest input contract (not timed)
@@ -174,6 +175,71 @@ int rfx_prox_pair_stats(hipStream_t stream, const int32_t* keys, int64_t n, int3
  *   proxsym     device float64, FULL column-major (out_layout FULL) or UPPER */
 int rfx_finishprox(hipStream_t stream, const int32_t* counts, int32_t in_layout, const int32_t* nout, int64_t n,
                    double* proxsym, int32_t out_layout);
+
+/* ---- MDS ---------------------------------------------------------------- */
+#define RFX_MDS_COUNTS 0    /* M_ij = c_ij * scale (e.g. ALL counts / T)       */
+#define RFX_MDS_REFERENCE 1 /* OOB_UPPER counts -> finishprox -> float32 -> nout renormalisation */
+
+#define RFX_PREC_FP32 0
+#define RFX_PREC_FP64 1
+#define RFX_PREC_BF16 2
+#define RFX_PREC_INT8 3
+#define RFX_PREC_FP8 4 /* FP8 e4m3 (OCP) */
+
+/* A6+A7: classical 3-D MDS of a count-matrix proximity.  Replaces
+ * rf::compute_mds_3d_cpu (include/rf_mds_cpu.hpp:27-33, src/rf_mds_cpu.cpp:
+ * 88-341) as called by the classifier's compute_mds_3d_cpu
+ * (python/randomforest_py.cpp:1221-1252): D = max(M) - M (zero diagonal),
+ * B = -1/2 J D^2 J with FP64 row means, top three eigenpairs with
+ * lambda > 1e-10 in descending order, X = V sqrt(lambda).  Eigenpairs come
+ * from block subspace iteration (16 vectors) with FP64 CholeskyQR2 and
+ * Rayleigh-Ritz; the D^2 operator runs in `precision` (FP32 or FP64).
+ *   counts      device int32, symmetric counts (COUNTS; UPPER or FULL) or
+ *               OOB_UPPER counts (REFERENCE; UPPER)
+ *   scale       COUNTS: M_ij = c_ij * scale
+ *   nout        device int32 [n] (REFERENCE), the reference's doubled nout
+ *   coords      device float64 n x 3 row-major (reference output layout)
+ *   eigvals_host host float64 [16] Ritz values (descending), may be NULL
+ *   iters_host  host int32, iterations taken, may be NULL
+ * Synchronises the stream (the k x k problems run on the host).  Returns
+ * RFX_ERR_INVALID when fewer than three eigenvalues exceed 1e-10, as the
+ * reference throws. */
+size_t rfx_mds_dense_workspace_bytes(int64_t n, int32_t precision);
+int rfx_mds_dense(hipStream_t stream, const int32_t* counts, int32_t layout, int64_t n, int32_t mode, double scale,
+                  const int32_t* nout, int32_t precision, int32_t max_iter, double tol, void* workspace,
+                  size_t workspace_bytes, double* coords, double* eigvals_host, int32_t* iters_host);
+
+/* A8: low-rank factors of the ALL-count proximity P = C/T (PSD): U (n x rank,
+ * FP32 row-major) with P ~ U U^T from randomized subspace iteration whose
+ * products C V are matrix-free through the leaf buckets (nothing n x n is
+ * formed).  Replaces the factor accumulation of
+ * rf::cuda::LowRankProximityMatrix (include/rf_proximity_lowrank.hpp:27-420,
+ * cuda/rf_proximity_lowrank.cu:863-1299) and get_factors (1753-1779).
+ *   node_offset_host  host int64 [T+1] (tree chunking of the leaf sums)
+ *   eigvals_host      host float64 [rank]: top eigenvalues of C/T, may be NULL
+ * Synchronises the stream. */
+size_t rfx_lowrank_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t rank, int32_t oversample);
+int rfx_lowrank_factors(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                        const int64_t* node_offset_host, int32_t rank, int32_t oversample, int32_t power_iters,
+                        uint64_t seed, void* workspace, size_t workspace_bytes, float* U, double* eigvals_host);
+
+/* Factor storage: format FP32 (copy), BF16, INT8 (q = round(u/s_c), |q| <= 127,
+ * s_c = max|u_c|/127) or FP8 (e4m3, s_c = max|u_c|/448); scales FP32 [rank]
+ * (1 for FP32/BF16).  Replaces quantize_factors
+ * (cuda/rf_proximity_lowrank.cu:1412-1553). */
+int rfx_quantize_factors(hipStream_t stream, const float* U, int64_t n, int32_t rank, int32_t format, void* Uq,
+                         float* scales);
+
+/* A9: classical 3-D MDS from factors: E_ij = (1 - u_i.u_j)^2 off the diagonal,
+ * 0 on it, double-centred with FP64 row means; top three eigenpairs by the
+ * same subspace iteration as rfx_mds_dense, operator arithmetic in
+ * `precision` (FP32 or FP64) over factors stored in `format`.  Replaces
+ * LowRankProximityMatrix::compute_mds_from_factors
+ * (cuda/rf_proximity_lowrank.cu:1935-2531).  rank <= 64.  Synchronises. */
+size_t rfx_mds_factors_workspace_bytes(int64_t n, int32_t rank, int32_t precision);
+int rfx_mds_factors(hipStream_t stream, const void* Uq, const float* scales, int32_t format, int64_t n, int32_t rank,
+                    int32_t precision, int32_t max_iter, double tol, void* workspace, size_t workspace_bytes,
+                    double* coords, double* eigvals_host, int32_t* iters_host);
 
 #ifdef __cplusplus
 }

@@ -37,8 +37,18 @@ LAYOUT_UPPER = 1
 ALGO_AUTO = 0
 ALGO_DENSE = 1
 ALGO_BUCKET = 2
+ALGO_GATHER = 3
 
 MAX_TOTAL_NODES = 1073741823
+
+MDS_COUNTS = 0
+MDS_REFERENCE = 1
+
+PREC_FP32 = 0
+PREC_FP64 = 1
+PREC_BF16 = 2
+PREC_INT8 = 3
+PREC_FP8 = 4
 
 _p = C.c_void_p
 _i32 = C.c_int32
@@ -69,6 +79,14 @@ _SIGS = {
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),
     "rfx_prox_pair_stats": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz, C.POINTER(C.c_double)]),
     "rfx_finishprox": (_i32, [_p, _p, _i32, _p, _i64, _p, _i32]),
+    "rfx_mds_dense_workspace_bytes": (_sz, [_i64, _i32]),
+    "rfx_lowrank_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32, _i32]),
+    "rfx_lowrank_factors": (_i32, [_p, _p, _i64, _i32, _i64, _p, _i32, _i32, _i32, _u64, _p, _sz, _p, _p]),
+    "rfx_quantize_factors": (_i32, [_p, _p, _i64, _i32, _i32, _p, _p]),
+    "rfx_mds_factors_workspace_bytes": (_sz, [_i64, _i32, _i32]),
+    "rfx_mds_factors": (_i32, [_p, _p, _p, _i32, _i64, _i32, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_mds_dense": (_i32, [_p, _p, _i32, _i64, _i32, C.c_double, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p,
+                             _p]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "librfx.so"
 
-HIP_SOURCES = ["leaf.hip", "accum.hip", "finish.hip", "mds.hip", "batch.hip"]
+HIP_SOURCES = ["leaf.hip", "accum.hip", "finish.hip", "mds.hip", "factor.hip", "batch.hip"]
 CXX_SOURCES = ["forest.cpp"]
 # Per-file extra flags: finishprox must not contract a*b+c into an FMA.
 HIP_EXTRA = {"finish.hip": ["-ffp-contract=off"]}

@@ -1,8 +1,9 @@
 // A2 + A3: same-leaf co-occurrence counts over all trees, bit-exact int32.
 //
 // Input is the key array of rfx_leaf_assign: keys[t][i] = (global node id << 1)
-// | in-bag, or -1 when sample i ends on a non-terminal node of tree t.  Two
-// algorithms compute the same integers:
+// | in-bag, or -1 when sample i ends on a non-terminal node of tree t.
+// Three algorithms compute the same integers (GATHER is described at its
+// kernel below):
 //
 // DENSE  (tile compare).  A workgroup owns a 128 x 128 tile of the output;
 //        the row and column keys of 32 trees at a time are staged in LDS
@@ -30,6 +31,7 @@
 // per tree), cuda/rf_proximity_upper_triangle.cu:238-248 (OOB_UPPER fold).
 
 #include "rfx_common.hpp"
+#include "bucket.hpp"
 
 namespace {
 
@@ -196,119 +198,6 @@ __global__ __launch_bounds__(kThreads) void dense_rows_kernel(const int32_t* __r
   out[(i - r0) * n + j] = static_cast<int32_t>(c);
 }
 
-// ---------------------------------------------------------------------------
-// BUCKET algorithm
-// ---------------------------------------------------------------------------
-constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 4;
-constexpr int kScanChunk = kScanThreads * kScanItems;
-
-__global__ void bucket_hist_kernel(const int32_t* __restrict__ keys, int64_t total, uint32_t* __restrict__ cnt) {
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int32_t k = keys[e];
-    if (k >= 0) atomicAdd(&cnt[k >> 1], 1u);
-  }
-}
-
-// Block-wide exclusive scan of kScanItems per thread; returns the block total.
-__device__ uint32_t block_exclusive_scan(uint32_t (&v)[kScanItems], uint32_t* lds_waves) {
-  uint32_t s = 0;
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    const uint32_t x = v[q];
-    v[q] = s;
-    s += x;
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t incl = s;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
-  if (lane == 63) lds_waves[wave] = incl;
-  __syncthreads();
-  if (wave == 0) {
-    uint32_t w = lane < (kScanThreads / 64) ? lds_waves[lane] : 0u;
-    uint32_t wi = w;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(wi, d, 64);
-      if (lane >= d) wi += y;
-    }
-    if (lane < (kScanThreads / 64)) lds_waves[lane] = wi - w;
-    if (lane == (kScanThreads / 64) - 1) lds_waves[kScanThreads / 64] = wi;
-  }
-  __syncthreads();
-  const uint32_t base = lds_waves[wave] + (incl - s);
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) v[q] += base;
-  const uint32_t total = lds_waves[kScanThreads / 64];
-  __syncthreads();
-  return total;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t len,
-                                                                   uint32_t* __restrict__ part) {
-  __shared__ uint32_t lw[kScanThreads / 64 + 1];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) v[q] = base + q < len ? in[base + q] : 0u;
-  const uint32_t tot = block_exclusive_scan(v, lw);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ part, int64_t nparts) {
-  __shared__ uint32_t lw[kScanThreads / 64 + 1];
-  uint32_t carry = 0;
-  for (int64_t b0 = 0; b0 < nparts; b0 += kScanChunk) {
-    const int64_t base = b0 + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q) v[q] = base + q < nparts ? part[base + q] : 0u;
-    const uint32_t tot = block_exclusive_scan(v, lw);
-#pragma unroll
-    for (int q = 0; q < kScanItems; ++q)
-      if (base + q < nparts) part[base + q] = v[q] + carry;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) part[nparts] = carry;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t len,
-                                                                  const uint32_t* __restrict__ part,
-                                                                  uint32_t* __restrict__ out) {
-  __shared__ uint32_t lw[kScanThreads / 64 + 1];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q) v[q] = base + q < len ? in[base + q] : 0u;
-  block_exclusive_scan(v, lw);
-  const uint32_t off = part[blockIdx.x];
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q)
-    if (base + q < len) out[base + q] = v[q] + off;
-}
-
-// Scatter (tree, sample) slots into their buckets.  cursor starts as the
-// exclusive scan and ends as the bucket end offsets.
-__global__ void bucket_scatter_kernel(const int32_t* __restrict__ keys, int64_t n, int64_t total,
-                                      uint32_t* __restrict__ cursor, uint32_t* __restrict__ members,
-                                      uint32_t* __restrict__ mbucket) {
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int32_t k = keys[e];
-    if (k < 0) continue;
-    const uint32_t b = static_cast<uint32_t>(k) >> 1;
-    const uint32_t pos = atomicAdd(&cursor[b], 1u);
-    const uint32_t i = static_cast<uint32_t>(e % n);
-    members[pos] = (i << 1) | (static_cast<uint32_t>(k) & 1u);
-    mbucket[pos] = b;
-  }
-}
-
 // One thread per bucket slot: walk the bucket, count each unordered pair
 // once (from its smaller sample index).
 template <int MODE, int LAYOUT>
@@ -360,6 +249,116 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// GATHER algorithm: one workgroup owns one output line segment (a row of the
+// packed upper triangle, or a column of the column-major full matrix) and
+// collects that line's partners from the leaf buckets into LDS counters, so
+// every output element is written exactly once, coalesced, with no global
+// atomics and no separate zero fill.  Each wavefront takes 64 trees at a
+// time (lane = tree), prefix-sums their bucket lengths, and walks the union
+// of the 64 buckets 64 members per step (owner lane found by a binary search
+// over the prefix), so a bucket of 1000 members and one of 2 cost the same
+// per member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
+// ---------------------------------------------------------------------------
+constexpr int kGatherChunk = 16384;  // LDS words per workgroup (64 KiB)
+
+template <int MODE, int LAYOUT>
+__global__ __launch_bounds__(256) void gather_line_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                          int32_t ntree, const uint32_t* __restrict__ cnt,
+                                                          const uint32_t* __restrict__ bend,
+                                                          const uint32_t* __restrict__ members, int32_t accumulate,
+                                                          int32_t* __restrict__ out, int32_t* __restrict__ out2) {
+  __shared__ uint32_t acc[kGatherChunk];
+  __shared__ uint32_t diag_valid;
+  const int64_t line = blockIdx.x;
+  const int64_t c0 = static_cast<int64_t>(blockIdx.y) * kGatherChunk;
+  int64_t lo = c0;
+  const int64_t hi = min(n, c0 + kGatherChunk);
+  if (LAYOUT == RFX_LAYOUT_UPPER) {
+    if (hi <= line) return;
+    lo = max(lo, line);
+  }
+  const int len_chunk = static_cast<int>(hi - lo);
+  for (int q = threadIdx.x; q < len_chunk; q += 256) acc[q] = 0u;
+  if (threadIdx.x == 0) diag_valid = 0u;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t valid = 0;
+  for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
+    const int32_t t = tb + lane;
+    uint32_t s = 0, len = 0, bag = 0;
+    if (t < ntree) {
+      const int32_t k = keys[static_cast<int64_t>(t) * n + line];
+      if (k >= 0) {
+        const uint32_t b = static_cast<uint32_t>(k) >> 1;
+        const uint32_t e = bend[b];
+        len = cnt[b];
+        s = e - len;
+        bag = static_cast<uint32_t>(k) & 1u;
+        ++valid;
+      }
+    }
+    if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
+    uint32_t pre = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(pre, d, 64);
+      if (lane >= d) pre += y;
+    }
+    const uint32_t excl = pre - len;
+    const uint32_t total = __shfl(pre, 63, 64);
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+      const uint32_t g = g0 + lane;
+      // owner = number of lanes whose inclusive prefix is <= g
+      int o = 0;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t v = __shfl(pre, o + d - 1, 64);
+        if (v <= g) o += d;
+      }
+      o = min(o, 63);
+      const uint32_t so = __shfl(s, o, 64);
+      const uint32_t eo = __shfl(excl, o, 64);
+      const uint32_t bo = __shfl(bag, o, 64);
+      if (g < total) {
+        const uint32_t m = members[so + (g - eo)];
+        const int64_t q = m >> 1;
+        const uint32_t bq = m & 1u;
+        bool in_range = q >= lo && q < hi && q != line;
+        if (LAYOUT == RFX_LAYOUT_UPPER) in_range = in_range && q > line;
+        uint32_t inc = 0;
+        if (MODE == kMode_ALL) inc = 1u;
+        if (MODE == kMode_OOB_UPPER) inc = bq != bo ? 1u : 0u;
+        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+        if (MODE == kMode_PAIR) inc = 1u | (bq != bo ? 0x10000u : 0u);
+        if (in_range && inc) atomicAdd(&acc[q - lo], inc);
+      }
+    }
+  }
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  __syncthreads();
+  if ((MODE == kMode_ALL || MODE == kMode_PAIR) && threadIdx.x == 0 && line >= lo && line < hi)
+    acc[line - lo] = diag_valid;
+  __syncthreads();
+
+  const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
+  for (int q = threadIdx.x; q < len_chunk; q += 256) {
+    const uint32_t v = acc[q];
+    if (MODE == kMode_PAIR) {
+      const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+      out[base + q] = accumulate ? out[base + q] + va : va;
+      out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+    } else {
+      out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+    }
+  }
+}
+
 // Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
 __global__ void all_diag_kernel(const int32_t* __restrict__ keys, int64_t n, int32_t ntree, int32_t layout,
                                 int32_t* __restrict__ out) {
@@ -381,64 +380,6 @@ __global__ void sum_squares_kernel(const uint32_t* __restrict__ cnt, int64_t len
   }
   for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
-}
-
-struct BucketWs {
-  uint32_t* cnt;
-  uint32_t* bend;
-  uint32_t* part;
-  uint32_t* members;
-  uint32_t* mbucket;
-  unsigned long long* stat;
-  int64_t nparts;
-};
-
-size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-
-size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
-  const int64_t nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  return align256(4 * total_nodes) * 2 + align256(4 * (nparts + 1)) + align256(4 * slots) * 2 + 256;
-}
-
-BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes) {
-  BucketWs w;
-  char* p = static_cast<char*>(ws);
-  w.nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  w.cnt = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * total_nodes);
-  w.bend = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * total_nodes);
-  w.part = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * (w.nparts + 1));
-  w.members = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * slots);
-  w.mbucket = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * slots);
-  w.stat = reinterpret_cast<unsigned long long*>(p);
-  return w;
-}
-
-unsigned grid_for(int64_t work, int threads) {
-  int64_t g = (work + threads - 1) / threads;
-  if (g > 65536) g = 65536;
-  if (g < 1) g = 1;
-  return static_cast<unsigned>(g);
-}
-
-// Histogram + scan: cnt and bend (as bucket start) ready, part[nparts] = total.
-int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-                 const BucketWs& w) {
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  RFX_HIP_TRY(hipMemsetAsync(w.cnt, 0, 4 * total_nodes, st));
-  hipLaunchKernelGGL(bucket_hist_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, slots, w.cnt);
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
-                     total_nodes, w.part);
-  hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts);
-  hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
-                     total_nodes, w.part, w.bend);
-  return rfx_launch_status();
 }
 
 template <int MODE, int LAYOUT>
@@ -464,12 +405,27 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   }
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, n, slots, w.bend,
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
                      w.members, w.mbucket);
   hipLaunchKernelGGL((bucket_pair_kernel<MODE, LAYOUT>), dim3(grid_for(slots, 256)), dim3(256), 0, st, w.members,
                      w.mbucket, w.cnt, w.bend, w.part + w.nparts, n, out, out2);
   if (MODE == kMode_ALL || MODE == kMode_PAIR)
     hipLaunchKernelGGL(all_diag_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, keys, n, ntree, LAYOUT, out);
+  return rfx_launch_status();
+}
+
+template <int MODE, int LAYOUT>
+int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+               int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
+                     w.members, w.mbucket);
+  dim3 grid(static_cast<unsigned>(n), rfx_ceil_div(n, kGatherChunk));
+  hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, keys, n, ntree, w.cnt, w.bend,
+                     w.members, accumulate, out, out2);
   return rfx_launch_status();
 }
 
@@ -480,16 +436,22 @@ int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int6
     return layout == RFX_LAYOUT_UPPER ? run_dense<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, accumulate, out, out2)
                                       : run_dense<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, accumulate, out, out2);
   }
+  if (algo == RFX_ALGO_GATHER) {
+    return layout == RFX_LAYOUT_UPPER
+               ? run_gather<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws)
+               : run_gather<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws);
+  }
   return layout == RFX_LAYOUT_UPPER
              ? run_bucket<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws)
              : run_bucket<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws);
 }
 
-// AUTO: compare the compare-kernel cost T*n^2/2 against the bucket cost
-// sum |leaf|^2 (+ one pass over the output), with the per-item cost ratio
-// measured on MI355X (an atomic pair update costs about as much as
-// kBucketPerPairVsCompare tile compares).
-constexpr double kBucketPerPairVsCompare = 24.0;
+// AUTO: compare the compare-kernel cost T*n^2/2 (one tile compare per
+// tree-pair) with the gather cost: sum over trees and leaves of |leaf|^2
+// bucket visits plus one pass over the output.  The per-item weights are
+// MI355X measurements on the covtype-like workload (n = 20,000, T = 500).
+constexpr double kGatherVisitVsCompare = 8.0;
+constexpr double kGatherWriteVsCompare = 4.0;
 
 int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, void* ws,
                 size_t ws_bytes, int32_t* algo) {
@@ -500,16 +462,17 @@ int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, i
   double sq = 0;
   int rc = rfx_prox_pair_stats(st, keys, n, ntree, total_nodes, ws, ws_bytes, &sq);
   if (rc) return rc;
-  const double dense_cost = 0.5 * static_cast<double>(ntree) * static_cast<double>(n) * static_cast<double>(n);
-  const double bucket_cost = 0.5 * sq * kBucketPerPairVsCompare + 0.5 * static_cast<double>(n) * n * 4.0;
-  *algo = bucket_cost < dense_cost ? RFX_ALGO_BUCKET : RFX_ALGO_DENSE;
+  const double nn = static_cast<double>(n) * static_cast<double>(n);
+  const double dense_cost = 0.5 * static_cast<double>(ntree) * nn;
+  const double gather_cost = sq * kGatherVisitVsCompare + 0.5 * nn * kGatherWriteVsCompare;
+  *algo = gather_cost < dense_cost ? RFX_ALGO_GATHER : RFX_ALGO_DENSE;
   return RFX_OK;
 }
 
 int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout, int32_t algo) {
   if (!keys || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || ntree > 65535) return RFX_ERR_INVALID;
   if (layout != RFX_LAYOUT_FULL && layout != RFX_LAYOUT_UPPER) return RFX_ERR_INVALID;
-  if (algo < RFX_ALGO_AUTO || algo > RFX_ALGO_BUCKET) return RFX_ERR_INVALID;
+  if (algo < RFX_ALGO_AUTO || algo > RFX_ALGO_GATHER) return RFX_ERR_INVALID;
   if (algo != RFX_ALGO_DENSE && (total_nodes <= 0 || total_nodes > RFX_MAX_TOTAL_NODES)) return RFX_ERR_INVALID;
   if (n * static_cast<int64_t>(ntree) >= (1LL << 32)) return RFX_ERR_UNSUPPORTED;
   return RFX_OK;
@@ -556,7 +519,7 @@ extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t 
     rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
     if (rc) return rc;
   }
-  if (algo == RFX_ALGO_BUCKET && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+  if ((algo == RFX_ALGO_BUCKET || algo == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
     return RFX_ERR_WORKSPACE;
   switch (variant) {
     case RFX_PROX_ALL:
@@ -581,7 +544,7 @@ extern "C" int rfx_prox_accumulate_pair(hipStream_t st, const int32_t* keys, int
     rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
     if (rc) return rc;
   }
-  if (algo == RFX_ALGO_BUCKET && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+  if ((algo == RFX_ALGO_BUCKET || algo == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
     return RFX_ERR_WORKSPACE;
   return dispatch<kMode_PAIR>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out_all, out_oob, ws);
 }

@@ -1,0 +1,608 @@
+// A8 + A9: low-rank proximity factors and 3-D MDS from the factors.
+//
+// Factors.  P = C / T with C = sum_t Z_t Z_t^T (Z_t the sample-to-leaf
+// indicator of tree t) is positive semidefinite, so P ~ U U^T with U the top
+// eigenvectors scaled by sqrt(eigenvalue).  They come from randomized
+// subspace iteration (Halko, Martinsson & Tropp 2011, Alg. 4.4 with a final
+// Rayleigh-Ritz step) where every product C V is matrix-free through the
+// leaf buckets: per tree chunk, leaf sums S_L = sum_{j in L} V_j
+// (bucket-major, members in ascending order so the float sums are
+// deterministic), then Y_i += sum_t S_{leaf_t(i)} (sample-major gather).
+// No n x n array exists at any point, so the same path serves n = 10^6.
+// The reference's low-rank object (cuda/rf_proximity_lowrank.cu:414-1779)
+// builds rank-1 power-iteration columns per tree; this path approximates
+// the accumulated proximity itself and is judged against FP64.
+//
+// Storage.  U is kept in FP32, BF16, INT8 or FP8-e4m3 (OCP), the last two
+// with per-column FP32 scales (u = q * s_c).
+//
+// Factor MDS.  With P~ = U U^T and the exact unit diagonal, the squared
+// distances are E_ij = (1 - P~_ij)^2 (i != j), E_ii = 0, and
+//   E v = (1^T v) 1 - 2 U (U^T v) + q(v) - (1 - |u_i|^2)^2 v_i,
+//   q_i(v) = sum_j (u_i . u_j)^2 v_j = x_i^T diag(w) Z,  Z = X^T v,
+// where x_i collects u_ia u_ib (a <= b), u_ia, and 1, and w holds the
+// symmetric multiplicities (1 or 2), -2 and 1.  X is never stored: both
+// products regenerate it from U row by row.  The double-centred operator,
+// FP64 row means, CholeskyQR2 and Rayleigh-Ritz are shared with the dense
+// path (subspace.hpp).  Replaces LowRankProximityMatrix::compute_mds_from_factors
+// (cuda/rf_proximity_lowrank.cu:1935-2531).
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bucket.hpp"
+#include "rfx_common.hpp"
+#include "subspace.hpp"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// storage formats
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+  uint32_t b = static_cast<uint32_t>(h) << 16;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+  b += 0x7fffu + ((b >> 16) & 1u);                      // round to nearest even
+  return static_cast<uint16_t>(b >> 16);
+}
+// FP8 e4m3 (OCP): bias 7, no infinities, max 448, subnormal step 2^-9.
+__host__ __device__ inline float e4m3_to_f32(uint8_t c) {
+  const int e = (c >> 3) & 15, m = c & 7;
+  float v = e == 0 ? ldexpf(static_cast<float>(m), -9) : ldexpf(1.0f + static_cast<float>(m) * 0.125f, e - 7);
+  return (c & 0x80) ? -v : v;
+}
+__host__ __device__ inline uint8_t f32_to_e4m3(float f) {
+  const uint8_t sign = f < 0.0f ? 0x80 : 0x00;
+  float a = fabsf(f);
+  if (!(a == a)) return 0x7f;
+  if (a >= 448.0f) return sign | 0x7e;
+  if (a < 0.015625f) {  // subnormal range: multiples of 2^-9, ties to even
+    const int m = static_cast<int>(rintf(a * 512.0f));
+    return sign | static_cast<uint8_t>(m);  // m == 8 is the smallest normal
+  }
+  int e;
+  frexpf(a, &e);
+  e -= 1;  // a = 1.f * 2^e
+  int q = static_cast<int>(rintf((ldexpf(a, -e) - 1.0f) * 8.0f));
+  if (q == 8) {
+    q = 0;
+    e += 1;
+  }
+  int code = ((e + 7) << 3) | q;
+  if (code >= 0x7f) code = 0x7e;
+  return sign | static_cast<uint8_t>(code);
+}
+
+template <int FMT>
+struct Store;
+template <>
+struct Store<RFX_PREC_FP32> {
+  using T = float;
+  __device__ static float load(const float* U, int64_t idx, const float*, int) { return U[idx]; }
+};
+template <>
+struct Store<RFX_PREC_BF16> {
+  using T = uint16_t;
+  __device__ static float load(const uint16_t* U, int64_t idx, const float* s, int c) {
+    return bf16_to_f32(U[idx]) * s[c];
+  }
+};
+template <>
+struct Store<RFX_PREC_INT8> {
+  using T = int8_t;
+  __device__ static float load(const int8_t* U, int64_t idx, const float* s, int c) {
+    return static_cast<float>(U[idx]) * s[c];
+  }
+};
+template <>
+struct Store<RFX_PREC_FP8> {
+  using T = uint8_t;
+  __device__ static float load(const uint8_t* U, int64_t idx, const float* s, int c) {
+    return e4m3_to_f32(U[idx]) * s[c];
+  }
+};
+
+size_t fmt_bytes(int fmt) {
+  switch (fmt) {
+    case RFX_PREC_FP32: return 4;
+    case RFX_PREC_BF16: return 2;
+    case RFX_PREC_INT8:
+    case RFX_PREC_FP8: return 1;
+    default: return 0;
+  }
+}
+
+// stride is a multiple of r, so each thread stays on one column
+__global__ void colabsmax_kernel(const float* __restrict__ U, int64_t n, int r, int64_t stride,
+                                 unsigned* __restrict__ amax) {
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e0 >= stride) return;
+  float m = 0.0f;
+  for (int64_t e = e0; e < n * r; e += stride) m = fmaxf(m, fabsf(U[e]));
+  if (e0 < n * r) atomicMax(&amax[e0 % r], __float_as_uint(m));
+}
+
+__global__ void scales_kernel(const unsigned* __restrict__ amax, int r, int fmt, float* __restrict__ scale) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= r) return;
+  const float a = __uint_as_float(amax[c]);
+  float s = 1.0f;
+  if (fmt == RFX_PREC_INT8) s = a > 0.0f ? a / 127.0f : 1.0f;
+  if (fmt == RFX_PREC_FP8) s = a > 0.0f ? a / 448.0f : 1.0f;
+  scale[c] = s;
+}
+
+__global__ void quantize_kernel(const float* __restrict__ U, int64_t n, int r, int fmt,
+                                const float* __restrict__ scale, void* __restrict__ out) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * r) return;
+  const int c = static_cast<int>(e % r);
+  const float v = U[e] / scale[c];
+  if (fmt == RFX_PREC_FP32) {
+    static_cast<float*>(out)[e] = U[e];
+  } else if (fmt == RFX_PREC_BF16) {
+    static_cast<uint16_t*>(out)[e] = f32_to_bf16(v);
+  } else if (fmt == RFX_PREC_INT8) {
+    const float q = fminf(127.0f, fmaxf(-127.0f, rintf(v)));
+    static_cast<int8_t*>(out)[e] = static_cast<int8_t>(q);
+  } else {
+    static_cast<uint8_t*>(out)[e] = f32_to_e4m3(v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// matrix-free C V through the leaf buckets
+// ---------------------------------------------------------------------------
+// S[b - lo][c] = sum over members of bucket b (ascending) of V[m][c]
+__global__ void leafsum_kernel(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ bend,
+                               const uint32_t* __restrict__ members, int64_t lo, int64_t hi,
+                               const float* __restrict__ V, int L, float* __restrict__ S) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= (hi - lo) * L) return;
+  const int64_t b = lo + e / L;
+  const int c = static_cast<int>(e % L);
+  const uint32_t k = cnt[b];
+  float s = 0.0f;
+  if (k) {
+    const uint32_t q1 = bend[b];
+    for (uint32_t q = q1 - k; q < q1; ++q) s += V[static_cast<int64_t>(members[q] >> 1) * L + c];
+  }
+  S[e] = s;
+}
+
+// Y[i][c] (+)= sum over trees t in [t0, t1) of S[leaf_t(i) - lo][c]
+__global__ void leafgather_kernel(const int32_t* __restrict__ keys, int64_t n, int32_t t0, int32_t t1, int64_t lo,
+                                  const float* __restrict__ S, int L, int32_t accumulate, float* __restrict__ Y) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * L) return;
+  const int64_t i = e / L;
+  const int c = static_cast<int>(e % L);
+  float s = accumulate ? Y[e] : 0.0f;
+  for (int32_t t = t0; t < t1; ++t) {
+    const int32_t k = keys[static_cast<int64_t>(t) * n + i];
+    if (k >= 0) s += S[((static_cast<int64_t>(k) >> 1) - lo) * L + c];
+  }
+  Y[e] = s;
+}
+
+__global__ void scale_cols_kernel(float* __restrict__ Y, int64_t count, float a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < count) Y[e] *= a;
+}
+
+struct CvPlan {
+  std::vector<int32_t> t0;  // tree chunk boundaries
+  int64_t max_nodes = 0;    // largest node range of a chunk
+};
+
+CvPlan plan_chunks(const int64_t* node_off, int32_t ntree, int L, size_t s_budget_bytes) {
+  CvPlan p;
+  const int64_t max_nodes = static_cast<int64_t>(s_budget_bytes / (4 * static_cast<size_t>(L)));
+  int32_t t = 0;
+  p.t0.push_back(0);
+  while (t < ntree) {
+    int32_t u = t + 1;
+    while (u < ntree && node_off[u + 1] - node_off[t] <= max_nodes) ++u;
+    p.max_nodes = std::max(p.max_nodes, node_off[u] - node_off[t]);
+    p.t0.push_back(u);
+    t = u;
+  }
+  return p;
+}
+
+constexpr size_t kLeafSumBudget = size_t(2) << 30;  // bytes of leaf sums per tree chunk
+
+// ---------------------------------------------------------------------------
+// factor MDS operator
+// ---------------------------------------------------------------------------
+constexpr int kPairChunk = 256;
+
+struct PairTable {
+  const uint8_t* a;  // first index (r = "one")
+  const uint8_t* b;  // second index
+  const float* w;    // multiplicity
+  int count;
+};
+
+// partial[rb][p][c] = sum over rows of row block rb of x_ip V[i][c]
+template <int FMT, typename Real, int K>
+__global__ __launch_bounds__(256) void fac_reduce_kernel(const typename Store<FMT>::T* __restrict__ U,
+                                                         const float* __restrict__ scale, int64_t n, int r,
+                                                         PairTable pt, const Real* __restrict__ V, int64_t rows_per_blk,
+                                                         Real* __restrict__ partial) {
+  __shared__ float Us[32][65];
+  __shared__ Real Vs[32][K];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rows_per_blk;
+  const int64_t r1 = min(n, r0 + rows_per_blk);
+  const int a = p < pt.count ? pt.a[p] : r, b = p < pt.count ? pt.b[p] : r;
+  Real acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = Real(0);
+  for (int64_t i0 = r0; i0 < r1; i0 += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * (r + 1); e += 256) {
+      const int q = e / (r + 1), col = e % (r + 1);
+      const int64_t i = i0 + q;
+      float v = 0.0f;
+      if (i < r1) v = col < r ? Store<FMT>::load(U, i * r + col, scale, col) : 1.0f;
+      Us[q][col] = v;
+    }
+    for (int e = threadIdx.x; e < 32 * K; e += 256) {
+      const int q = e / K, c = e % K;
+      const int64_t i = i0 + q;
+      Vs[q][c] = i < r1 ? V[i * K + c] : Real(0);
+    }
+    __syncthreads();
+    const int rows = static_cast<int>(min<int64_t>(32, r1 - i0));
+    for (int q = 0; q < rows; ++q) {
+      const Real x = static_cast<Real>(Us[q][a]) * static_cast<Real>(Us[q][b]);
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] += x * Vs[q][c];
+    }
+  }
+  if (p < pt.count) {
+    Real* out = partial + (static_cast<int64_t>(blockIdx.y) * pt.count + p) * K;
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[c] = acc[c];
+  }
+}
+
+// Z[p][c] = w_p * sum over row blocks of partial (FP64 sum, fixed order)
+template <typename Real, int K>
+__global__ void fac_reduce_partials_kernel(const Real* __restrict__ partial, int nrb, PairTable pt,
+                                           Real* __restrict__ Z) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pt.count * K) return;
+  double s = 0.0;
+  for (int q = 0; q < nrb; ++q) s += static_cast<double>(partial[static_cast<int64_t>(q) * pt.count * K + e]);
+  Z[e] = static_cast<Real>(s * static_cast<double>(pt.w[e / K]));
+}
+
+// Y[i][c] = x_i^T Z[:, c] - (1 - |u_i|^2)^2 V[i][c]
+template <int FMT, typename Real, int K>
+__global__ __launch_bounds__(256) void fac_apply_kernel(const typename Store<FMT>::T* __restrict__ U,
+                                                        const float* __restrict__ scale, int64_t n, int r,
+                                                        PairTable pt, const Real* __restrict__ Z,
+                                                        const Real* __restrict__ V, Real* __restrict__ Y) {
+  __shared__ float Us[65][256];
+  __shared__ Real Zs[kPairChunk][K];
+  __shared__ uint8_t As[kPairChunk], Bs[kPairChunk];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * 256;
+  const int tid = threadIdx.x;
+  const int64_t i = i0 + tid;
+  for (int e = tid; e < (r + 1) * 256; e += 256) {
+    const int col = e / 256, q = e % 256;
+    const int64_t ii = i0 + q;
+    float v = 0.0f;
+    if (ii < n) v = col < r ? Store<FMT>::load(U, ii * r + col, scale, col) : 1.0f;
+    Us[col][q] = v;
+  }
+  Real acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = Real(0);
+  for (int p0 = 0; p0 < pt.count; p0 += kPairChunk) {
+    const int pc = min(kPairChunk, pt.count - p0);
+    __syncthreads();
+    for (int e = tid; e < pc * K; e += 256) Zs[e / K][e % K] = Z[static_cast<int64_t>(p0) * K + e];
+    for (int e = tid; e < pc; e += 256) {
+      As[e] = pt.a[p0 + e];
+      Bs[e] = pt.b[p0 + e];
+    }
+    __syncthreads();
+    for (int q = 0; q < pc; ++q) {
+      const Real x = static_cast<Real>(Us[As[q]][tid]) * static_cast<Real>(Us[Bs[q]][tid]);
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] += x * Zs[q][c];
+    }
+  }
+  if (i < n) {
+    double nrm = 0.0;
+    for (int col = 0; col < r; ++col) nrm += static_cast<double>(Us[col][tid]) * Us[col][tid];
+    const double d = (1.0 - nrm) * (1.0 - nrm);
+#pragma unroll
+    for (int c = 0; c < K; ++c)
+      Y[i * K + c] = static_cast<Real>(static_cast<double>(acc[c]) - d * static_cast<double>(V[i * K + c]));
+  }
+}
+
+size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+constexpr int kMdsK = 16;
+
+int64_t fac_rows_per_blk(int64_t n) {
+  int64_t rb = (n + 127) / 128;  // about 128 row blocks
+  rb = (rb + 31) / 32 * 32;
+  return rb < 32 ? 32 : rb;
+}
+
+struct PairHost {
+  std::vector<uint8_t> a, b;
+  std::vector<float> w;
+};
+
+PairHost make_pairs(int r) {
+  PairHost h;
+  for (int a = 0; a < r; ++a)
+    for (int b = a; b < r; ++b) {
+      h.a.push_back(static_cast<uint8_t>(a));
+      h.b.push_back(static_cast<uint8_t>(b));
+      h.w.push_back(a == b ? 1.0f : 2.0f);
+    }
+  for (int a = 0; a < r; ++a) {
+    h.a.push_back(static_cast<uint8_t>(a));
+    h.b.push_back(static_cast<uint8_t>(r));
+    h.w.push_back(-2.0f);
+  }
+  h.a.push_back(static_cast<uint8_t>(r));
+  h.b.push_back(static_cast<uint8_t>(r));
+  h.w.push_back(1.0f);
+  return h;
+}
+
+template <typename Real>
+size_t fac_mds_ws(int64_t n, int r) {
+  const int P = r * (r + 1) / 2 + r + 1;
+  const int64_t nrb = (n + fac_rows_per_blk(n) - 1) / fac_rows_per_blk(n);
+  return al(8 * n) + al(P * 6) + al(sizeof(Real) * nrb * P * kMdsK) + al(8 * nrb * P) + al(sizeof(Real) * P * kMdsK) +
+         al(8 * P) + al(8 * n) + rfx::SubspaceSolver<Real>::workspace_bytes(n, kMdsK);
+}
+
+template <int FMT, typename Real>
+int fac_mds_run(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, int max_iter, double tol,
+                char* p, double* coords, rfx::SubspaceResult& res, double* r_dev_keep, const Real* V0,
+                const PairTable& pt, int64_t nrb, Real* partial, double* partial64, Real* Z, double* Z64,
+                double* ones) {
+  using ST = typename Store<FMT>::T;
+  const ST* U = static_cast<const ST*>(Uq);
+  const int64_t rpb = fac_rows_per_blk(n);
+  dim3 gr(rfx_ceil_div(pt.count, 256), static_cast<unsigned>(nrb));
+  // FP64 row means of E: E 1 / n
+  hipLaunchKernelGGL((fac_reduce_kernel<FMT, double, 1>), gr, dim3(256), 0, st, U, scale, n, r, pt, ones, rpb,
+                     partial64);
+  hipLaunchKernelGGL((fac_reduce_partials_kernel<double, 1>), dim3(rfx_ceil_div(pt.count, 256)), dim3(256), 0, st,
+                     partial64, static_cast<int>(nrb), pt, Z64);
+  hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
+                     r, pt, Z64, ones, r_dev_keep);
+  std::vector<double> rh(n);
+  RFX_HIP_TRY(hipMemcpyAsync(rh.data(), r_dev_keep, 8 * n, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  double g = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    rh[i] /= static_cast<double>(n);
+    g += rh[i];
+  }
+  g /= static_cast<double>(n);
+  RFX_HIP_TRY(hipMemcpyAsync(r_dev_keep, rh.data(), 8 * n, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+
+  rfx::SubspaceSolver<Real> solver(st, n, kMdsK);
+  solver.carve(p);
+  auto apply = [&](const Real* V, Real* W, int) -> int {
+    hipLaunchKernelGGL((fac_reduce_kernel<FMT, Real, kMdsK>), gr, dim3(256), 0, st, U, scale, n, r, pt, V, rpb,
+                       partial);
+    hipLaunchKernelGGL((fac_reduce_partials_kernel<Real, kMdsK>), dim3(rfx_ceil_div(pt.count * kMdsK, 256)),
+                       dim3(256), 0, st, partial, static_cast<int>(nrb), pt, Z);
+    hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale,
+                       n, r, pt, Z, V, W);
+    return rfx_launch_status();
+  };
+  (void)V0;
+  return solver.run(apply, r_dev_keep, g, 3, max_iter, tol, 0x5eedull, coords, res);
+}
+
+template <typename Real>
+int fac_mds_dispatch(hipStream_t st, const void* Uq, const float* scale, int fmt, int64_t n, int r, int max_iter,
+                     double tol, void* ws, double* coords, rfx::SubspaceResult& res) {
+  const int P = r * (r + 1) / 2 + r + 1;
+  const int64_t rpb = fac_rows_per_blk(n);
+  const int64_t nrb = (n + rpb - 1) / rpb;
+  char* p = static_cast<char*>(ws);
+  double* rdev = reinterpret_cast<double*>(p);
+  p += al(8 * n);
+  uint8_t* pa = reinterpret_cast<uint8_t*>(p);
+  uint8_t* pb = pa + P;
+  float* pw = reinterpret_cast<float*>(p + ((2 * P + 15) / 16) * 16);
+  p += al(P * 6 + 16);
+  Real* partial = reinterpret_cast<Real*>(p);
+  p += al(sizeof(Real) * nrb * P * kMdsK);
+  double* partial64 = reinterpret_cast<double*>(p);
+  p += al(8 * nrb * P);
+  Real* Z = reinterpret_cast<Real*>(p);
+  p += al(sizeof(Real) * P * kMdsK);
+  double* Z64 = reinterpret_cast<double*>(p);
+  p += al(8 * P);
+  double* ones = reinterpret_cast<double*>(p);
+  p += al(8 * n);
+  PairHost h = make_pairs(r);
+  RFX_HIP_TRY(hipMemcpyAsync(pa, h.a.data(), P, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(pb, h.b.data(), P, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(pw, h.w.data(), 4 * P, hipMemcpyHostToDevice, st));
+  std::vector<double> onesh(n, 1.0);
+  RFX_HIP_TRY(hipMemcpyAsync(ones, onesh.data(), 8 * n, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  PairTable pt{pa, pb, pw, P};
+  switch (fmt) {
+    case RFX_PREC_FP32:
+      return fac_mds_run<RFX_PREC_FP32, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
+                                              nrb, partial, partial64, Z, Z64, ones);
+    case RFX_PREC_BF16:
+      return fac_mds_run<RFX_PREC_BF16, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
+                                              nrb, partial, partial64, Z, Z64, ones);
+    case RFX_PREC_INT8:
+      return fac_mds_run<RFX_PREC_INT8, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
+                                              nrb, partial, partial64, Z, Z64, ones);
+    case RFX_PREC_FP8:
+      return fac_mds_run<RFX_PREC_FP8, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
+                                             nrb, partial, partial64, Z, Z64, ones);
+    default:
+      return RFX_ERR_INVALID;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" size_t rfx_lowrank_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t rank,
+                                              int32_t oversample) {
+  if (n <= 0 || ntree <= 0 || total_nodes <= 0 || rank <= 0 || oversample < 0) return 0;
+  const int L = rank + oversample;
+  const size_t sbytes = std::min(kLeafSumBudget, static_cast<size_t>(4) * L * static_cast<size_t>(total_nodes));
+  return bucket_ws_bytes(n, ntree, total_nodes) + al(sbytes + 4 * static_cast<size_t>(L) * 1024 * 1024) +
+         rfx::SubspaceSolver<float>::workspace_bytes(n, L);
+}
+
+extern "C" int rfx_lowrank_factors(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                   int64_t total_nodes, const int64_t* node_offset_host, int32_t rank,
+                                   int32_t oversample, int32_t power_iters, uint64_t seed, void* ws, size_t ws_bytes,
+                                   float* U, double* eigvals_host) {
+  if (!keys || !node_offset_host || !ws || !U || n < 2 || ntree <= 0 || total_nodes <= 0 ||
+      total_nodes > RFX_MAX_TOTAL_NODES || rank < 1 || rank > 128 || oversample < 0 || power_iters < 0)
+    return RFX_ERR_INVALID;
+  const int L = rank + oversample;
+  if (L > n) return RFX_ERR_INVALID;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 32)) return RFX_ERR_UNSUPPORTED;
+  if (ws_bytes < rfx_lowrank_workspace_bytes(n, ntree, total_nodes, rank, oversample)) return RFX_ERR_WORKSPACE;
+  char* p = static_cast<char*>(ws);
+  const BucketWs w = bucket_ws_carve(p, n, ntree, total_nodes);
+  p += bucket_ws_bytes(n, ntree, total_nodes);
+  const size_t sbytes = std::min(kLeafSumBudget, static_cast<size_t>(4) * L * static_cast<size_t>(total_nodes));
+  const CvPlan plan = plan_chunks(node_offset_host, ntree, L, sbytes);
+  if (static_cast<size_t>(plan.max_nodes) * 4 * L > sbytes + 4 * static_cast<size_t>(L) * 1024 * 1024)
+    return RFX_ERR_UNSUPPORTED;  // a single tree larger than the leaf-sum budget
+  float* S = reinterpret_cast<float*>(p);
+  p += al(sbytes + 4 * static_cast<size_t>(L) * 1024 * 1024);
+  rfx::SubspaceSolver<float> solver(st, n, L);
+  solver.carve(p);
+  float* V = solver.block_v();
+  float* Y = solver.block_w();
+
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
+                     w.members, w.mbucket);
+
+  auto cv = [&](const float* Vin, float* Yout) -> int {
+    for (size_t q = 0; q + 1 < plan.t0.size(); ++q) {
+      const int32_t t0 = plan.t0[q], t1 = plan.t0[q + 1];
+      const int64_t lo = node_offset_host[t0], hi = node_offset_host[t1];
+      hipLaunchKernelGGL(leafsum_kernel, dim3(rfx_ceil_div((hi - lo) * L, 256)), dim3(256), 0, st, w.cnt, w.bend,
+                         w.members, lo, hi, Vin, L, S);
+      hipLaunchKernelGGL(leafgather_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, keys, n, t0, t1, lo, S,
+                         L, static_cast<int32_t>(q > 0), Yout);
+    }
+    return rfx_launch_status();
+  };
+
+  hipLaunchKernelGGL(rfx::ss_init_kernel<float>, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, V, n, L, seed);
+  rc = cv(V, Y);
+  if (rc) return rc;
+  // scale by 1/T so the block stays O(1) (C/T has unit diagonal)
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, Y, n * L,
+                     1.0f / static_cast<float>(ntree));
+  RFX_HIP_TRY(hipMemcpyAsync(V, Y, 4 * n * L, hipMemcpyDeviceToDevice, st));
+  rc = solver.orthonormalize(V);
+  if (rc) return rc;
+  for (int it = 0; it < power_iters; ++it) {
+    rc = cv(V, Y);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, Y, n * L,
+                       1.0f / static_cast<float>(ntree));
+    RFX_HIP_TRY(hipMemcpyAsync(V, Y, 4 * n * L, hipMemcpyDeviceToDevice, st));
+    rc = solver.orthonormalize(V);
+    if (rc) return rc;
+  }
+  // Rayleigh-Ritz: H = Q^T (C/T) Q, U = Q Z_r sqrt(theta_r)
+  rc = cv(V, Y);
+  if (rc) return rc;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, Y, n * L,
+                     1.0f / static_cast<float>(ntree));
+  std::vector<double> H, wv, Z;
+  rc = solver.gram(V, L, Y, L, H);
+  if (rc) return rc;
+  for (int a = 0; a < L; ++a)
+    for (int c = a + 1; c < L; ++c) {
+      const double m = 0.5 * (H[a * L + c] + H[c * L + a]);
+      H[a * L + c] = H[c * L + a] = m;
+    }
+  rfx::jacobi_eigh(H, L, wv, Z);
+  std::vector<double> M(static_cast<size_t>(L) * rank);
+  for (int a = 0; a < L; ++a)
+    for (int c = 0; c < rank; ++c) M[a * rank + c] = Z[a * L + c] * std::sqrt(std::max(wv[c], 0.0));
+  rc = solver.upload(solver.small_matrix(), M.data(), M.size());
+  if (rc) return rc;
+  hipLaunchKernelGGL(rfx::ss_rotate_kernel<float>, dim3(rfx_ceil_div(n * rank, 256)), dim3(256), 0, st, V, L,
+                     solver.small_matrix(), rank, n, U);
+  if (eigvals_host)
+    for (int c = 0; c < rank; ++c) eigvals_host[c] = wv[c];
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  return rfx_launch_status();
+}
+
+extern "C" int rfx_quantize_factors(hipStream_t st, const float* U, int64_t n, int32_t rank, int32_t format,
+                                    void* Uq, float* scales) {
+  if (!U || !Uq || !scales || n <= 0 || rank <= 0 || rank > 128 || fmt_bytes(format) == 0) return RFX_ERR_INVALID;
+  unsigned* amax = reinterpret_cast<unsigned*>(scales);  // reuse the scale array as the max buffer
+  RFX_HIP_TRY(hipMemsetAsync(amax, 0, 4 * rank, st));
+  const int64_t threads = 256LL * 1024;
+  const int64_t stride = (threads / rank) * rank;
+  hipLaunchKernelGGL(colabsmax_kernel, dim3(rfx_ceil_div(stride, 256)), dim3(256), 0, st, U, n, rank, stride, amax);
+  hipLaunchKernelGGL(scales_kernel, dim3(rfx_ceil_div(rank, 64)), dim3(64), 0, st, amax, rank, format, scales);
+  hipLaunchKernelGGL(quantize_kernel, dim3(rfx_ceil_div(n * rank, 256)), dim3(256), 0, st, U, n, rank, format,
+                     scales, Uq);
+  return rfx_launch_status();
+}
+
+extern "C" size_t rfx_mds_factors_workspace_bytes(int64_t n, int32_t rank, int32_t precision) {
+  if (n <= 0 || rank <= 0) return 0;
+  return precision == RFX_PREC_FP64 ? fac_mds_ws<double>(n, rank) : fac_mds_ws<float>(n, rank);
+}
+
+extern "C" int rfx_mds_factors(hipStream_t st, const void* Uq, const float* scales, int32_t format, int64_t n,
+                               int32_t rank, int32_t precision, int32_t max_iter, double tol, void* ws,
+                               size_t ws_bytes, double* coords, double* eigvals_host, int32_t* iters_host) {
+  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > 64 || fmt_bytes(format) == 0 || max_iter < 1)
+    return RFX_ERR_INVALID;
+  if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
+  if (ws_bytes < rfx_mds_factors_workspace_bytes(n, rank, precision)) return RFX_ERR_WORKSPACE;
+  rfx::SubspaceResult res;
+  int rc = precision == RFX_PREC_FP64
+               ? fac_mds_dispatch<double>(st, Uq, scales, format, n, rank, max_iter, tol, ws, coords, res)
+               : fac_mds_dispatch<float>(st, Uq, scales, format, n, rank, max_iter, tol, ws, coords, res);
+  if (rc) return rc;
+  if (eigvals_host)
+    for (int c = 0; c < kMdsK; ++c) eigvals_host[c] = res.eigvals[c];
+  if (iters_host) *iters_host = res.iterations;
+  if (!(res.eigvals[2] > 1e-10)) return RFX_ERR_INVALID;
+  return RFX_OK;
+}

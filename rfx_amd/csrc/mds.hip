@@ -1,0 +1,269 @@
+// A6 + A7: classical (Torgerson) 3-D MDS of the forest proximity on the GPU.
+//
+// The reference (src/rf_mds_cpu.cpp:88-341) builds D = max(M) - M with a zero
+// diagonal, squares it, double-centres it in FP64 and hands the n x n matrix
+// to LAPACK dsyevd for all n eigenpairs, keeping the top three with
+// lambda > 1e-10.  Here nothing n x n is materialised besides the int32
+// counts: the squared distances E = D o D are formed on the fly from the
+// counts, tile by tile, inside the operator kernel, and the top eigenpairs
+// come from block subspace iteration (subspace.hpp) with FP64 row means,
+// FP64 CholeskyQR2 and FP64 Rayleigh-Ritz.  The operator runs in FP32 or FP64.
+//
+// Proximity forms (the MDS input M):
+//   COUNTS     M_ij = c_ij * scale (e.g. ALL counts / T), any symmetric count
+//              matrix, UPPER or FULL layout
+//   REFERENCE  the pybind path's input (python/randomforest_py.cpp:1138-1146,
+//              1221-1252; src/rf_mds_cpu.cpp:154-186): OOB_UPPER counts ->
+//              finishprox S (FP64) -> float32 -> M_ij = 1/2 (S_ij/nout_i +
+//              S_ij/nout_j), M_ii = 1 (rows with nout_i = 0 are not divided)
+
+#include <cstring>
+#include <vector>
+
+#include "rfx_common.hpp"
+#include "subspace.hpp"
+
+namespace {
+
+constexpr int kRows = 256;  // rows per workgroup = threads
+constexpr int kCols = 32;   // columns per LDS tile
+constexpr int kBlockK = 16; // subspace block size
+
+struct ProxForm {
+  int32_t mode;    // RFX_MDS_COUNTS / RFX_MDS_REFERENCE
+  int32_t layout;  // RFX_LAYOUT_UPPER / RFX_LAYOUT_FULL
+  double scale;
+  double mx;       // max of M
+  const int32_t* nout;
+};
+
+__device__ __forceinline__ int64_t src_index(int32_t layout, int64_t i, int64_t j, int64_t n) {
+  if (layout == RFX_LAYOUT_FULL) return i + j * n;
+  const int64_t a = i < j ? i : j, b = i < j ? j : i;
+  return rfx_upper_index(a, b, n);
+}
+
+__device__ __forceinline__ double recip_or_one(int32_t c) { return c > 0 ? 1.0 / static_cast<double>(c) : 1.0; }
+
+// M_ij (i != j) from the stored count c_ij
+__device__ __forceinline__ double prox_value(const ProxForm& f, int32_t c, int64_t i, int64_t j) {
+  if (f.mode == RFX_MDS_COUNTS) return static_cast<double>(c) * f.scale;
+  const int32_t ni = f.nout[i], nj = f.nout[j];
+  const double a = static_cast<double>(c) / 2.0;
+  const double S = 0.5 * (a * recip_or_one(ni) + a * recip_or_one(nj));
+  const double Sf = static_cast<double>(static_cast<float>(S));
+  const double ti = ni > 0 ? Sf / static_cast<double>(ni) : Sf;
+  const double tj = nj > 0 ? Sf / static_cast<double>(nj) : Sf;
+  return 0.5 * (ti + tj);
+}
+
+__device__ __forceinline__ double diag_value(const ProxForm& f, int32_t c) {
+  return f.mode == RFX_MDS_COUNTS ? static_cast<double>(c) * f.scale : 1.0;
+}
+
+// Max of M over all entries (diagonal included), FP64 bits via atomicMax.
+__global__ void mds_max_kernel(const int32_t* __restrict__ counts, int64_t n, ProxForm f,
+                               unsigned long long* __restrict__ out) {
+  const int64_t i = blockIdx.x;
+  double m = 0.0;
+  const int64_t j0 = f.layout == RFX_LAYOUT_UPPER ? i : 0;
+  for (int64_t j = j0 + threadIdx.x; j < n; j += blockDim.x) {
+    // row i of the symmetric matrix, read along the contiguous direction
+    const int32_t c = counts[f.layout == RFX_LAYOUT_UPPER ? src_index(f.layout, i, j, n) : j + i * n];
+    const double v = (i == j) ? diag_value(f, c) : prox_value(f, c, i, j);
+    m = v > m ? v : m;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    const double o = __shfl_down(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// Y_part[s][i][c] = sum over j in split s of E_ij * V[j][c]  (ROWSUM: V = 1, K = 1)
+template <typename Real, int K, bool ROWSUM>
+__global__ __launch_bounds__(kRows) void mds_apply_kernel(const int32_t* __restrict__ counts, int64_t n, ProxForm f,
+                                                          const Real* __restrict__ V, int64_t jlen,
+                                                          Real* __restrict__ Ypart) {
+  __shared__ Real Et[kCols][kRows];
+  __shared__ Real Vt[kCols][K];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kRows;
+  const int64_t jb = static_cast<int64_t>(blockIdx.y) * jlen;
+  const int64_t je = min(n, jb + jlen);
+  const int tid = threadIdx.x;
+  const int64_t i = i0 + tid;
+  Real acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = Real(0);
+
+  for (int64_t j0 = jb; j0 < je; j0 += kCols) {
+    __syncthreads();
+    // Stage E for rows [i0, i0+256) x columns [j0, j0+32); choose the thread
+    // mapping that reads the packed source contiguously.
+    const bool right = f.layout == RFX_LAYOUT_UPPER && j0 >= i0 + kRows;
+    for (int e = tid; e < kRows * kCols; e += kRows) {
+      int r, q;
+      if (right) {
+        r = e / kCols;
+        q = e % kCols;
+      } else {
+        q = e / kRows;
+        r = e % kRows;
+      }
+      const int64_t ii = i0 + r, jj = j0 + q;
+      Real v = Real(0);
+      if (ii < n && jj < je && ii != jj) {
+        const int32_t c = counts[src_index(f.layout, ii, jj, n)];
+        const double d = f.mx - prox_value(f, c, ii, jj);
+        v = static_cast<Real>(d * d);
+      }
+      Et[q][r] = v;
+    }
+    if (!ROWSUM) {
+      for (int e = tid; e < kCols * K; e += kRows) {
+        const int q = e / K, c = e % K;
+        const int64_t jj = j0 + q;
+        Vt[q][c] = jj < je ? V[jj * K + c] : Real(0);
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int q = 0; q < kCols; ++q) {
+      const Real e = Et[q][tid];
+      if (ROWSUM) {
+        acc[0] += e;
+      } else {
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] += e * Vt[q][c];
+      }
+    }
+  }
+  if (i < n) {
+    Real* out = Ypart + (static_cast<int64_t>(blockIdx.y) * n + i) * K;
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[c] = acc[c];
+  }
+}
+
+template <typename Real, typename Out>
+__global__ void mds_sum_splits_kernel(const Real* __restrict__ Ypart, int splits, int64_t nk, Out* __restrict__ Y) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nk) return;
+  double s = 0.0;
+  for (int q = 0; q < splits; ++q) s += static_cast<double>(Ypart[static_cast<int64_t>(q) * nk + e]);
+  Y[e] = static_cast<Out>(s);
+}
+
+__global__ void mds_scale_kernel(double* __restrict__ r, int64_t n, double inv) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) r[i] *= inv;
+}
+
+size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+int splits_for(int64_t n) {
+  // enough workgroups to cover 256 CUs a few times over
+  const int64_t row_blocks = (n + kRows - 1) / kRows;
+  int64_t s = (2048 + row_blocks - 1) / row_blocks;
+  const int64_t max_s = (n + kCols - 1) / kCols;
+  if (s > max_s) s = max_s;
+  if (s > 64) s = 64;
+  return static_cast<int>(s < 1 ? 1 : s);
+}
+
+template <typename Real>
+size_t dense_ws(int64_t n) {
+  const int sp = splits_for(n);
+  return al(8 * n) + al(8 * 4) + al(sizeof(Real) * sp * n * kBlockK) + al(8 * sp * n) +
+         rfx::SubspaceSolver<Real>::workspace_bytes(n, kBlockK);
+}
+
+template <typename Real>
+int run_dense(hipStream_t st, const int32_t* counts, int32_t layout, int64_t n, int32_t mode, double scale,
+              const int32_t* nout, int32_t max_iter, double tol, void* ws, double* coords, double* eigvals_host,
+              int32_t* iters_host) {
+  char* p = static_cast<char*>(ws);
+  double* r = reinterpret_cast<double*>(p);
+  p += al(8 * n);
+  unsigned long long* mx_bits = reinterpret_cast<unsigned long long*>(p);
+  p += al(8 * 4);
+  const int sp = splits_for(n);
+  Real* Ypart = reinterpret_cast<Real*>(p);
+  p += al(sizeof(Real) * sp * n * kBlockK);
+  double* rpart = reinterpret_cast<double*>(p);
+  p += al(8 * sp * n);
+  ProxForm f{mode, layout, scale, 0.0, nout};
+
+  RFX_HIP_TRY(hipMemsetAsync(mx_bits, 0, 8, st));
+  hipLaunchKernelGGL(mds_max_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, counts, n, f, mx_bits);
+  unsigned long long bits = 0;
+  RFX_HIP_TRY(hipMemcpyAsync(&bits, mx_bits, 8, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  std::memcpy(&f.mx, &bits, 8);
+
+  const int64_t jlen = ((n + sp - 1) / sp + kCols - 1) / kCols * kCols;
+  dim3 grid(rfx_ceil_div(n, kRows), static_cast<unsigned>(sp));
+  // FP64 row means of E (double-centring)
+  hipLaunchKernelGGL((mds_apply_kernel<double, 1, true>), grid, dim3(kRows), 0, st, counts, n, f,
+                     static_cast<const double*>(nullptr), jlen, rpart);
+  hipLaunchKernelGGL((mds_sum_splits_kernel<double, double>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, rpart,
+                     sp, n, r);
+  hipLaunchKernelGGL(mds_scale_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, r, n,
+                     1.0 / static_cast<double>(n));
+  std::vector<double> rh(n);
+  RFX_HIP_TRY(hipMemcpyAsync(rh.data(), r, 8 * n, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  double g = 0.0;
+  for (int64_t i = 0; i < n; ++i) g += rh[i];
+  g /= static_cast<double>(n);
+
+  rfx::SubspaceSolver<Real> solver(st, n, kBlockK);
+  solver.carve(p);
+  auto apply = [&](const Real* V, Real* W, int k) -> int {
+    (void)k;
+    hipLaunchKernelGGL((mds_apply_kernel<Real, kBlockK, false>), grid, dim3(kRows), 0, st, counts, n, f, V, jlen,
+                       Ypart);
+    hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
+                       Ypart, sp, n * kBlockK, W);
+    return rfx_launch_status();
+  };
+  rfx::SubspaceResult res;
+  int rc = solver.run(apply, r, g, 3, max_iter, tol, 0x5eedull, coords, res);
+  if (rc) return rc;
+  // Fewer than three positive Ritz values while large negative ones fill the
+  // block: rerun on B + sigma I with sigma = |most negative Ritz value|.
+  if (res.eigvals.size() >= 3 && !(res.eigvals[2] > 1e-10) && res.eigvals.back() < 0.0) {
+    const double sigma = -res.eigvals.back();
+    rc = solver.run(apply, r, g, 3, max_iter, tol, 0x5eedull, coords, res, sigma);
+    if (rc) return rc;
+  }
+  if (eigvals_host)
+    for (int c = 0; c < kBlockK; ++c) eigvals_host[c] = res.eigvals[c];
+  if (iters_host) *iters_host = res.iterations;
+  if (res.eigvals.size() < 3 || !(res.eigvals[2] > 1e-10)) return RFX_ERR_INVALID;
+  return RFX_OK;
+}
+
+}  // namespace
+
+extern "C" size_t rfx_mds_dense_workspace_bytes(int64_t n, int32_t precision) {
+  if (n <= 0) return 0;
+  return precision == RFX_PREC_FP64 ? dense_ws<double>(n) : dense_ws<float>(n);
+}
+
+extern "C" int rfx_mds_dense(hipStream_t st, const int32_t* counts, int32_t layout, int64_t n, int32_t mode,
+                             double scale, const int32_t* nout, int32_t precision, int32_t max_iter, double tol,
+                             void* workspace, size_t workspace_bytes, double* coords, double* eigvals_host,
+                             int32_t* iters_host) {
+  if (!counts || !coords || !workspace || n < 4 || n > RFX_MAX_SAMPLES || max_iter < 1) return RFX_ERR_INVALID;
+  if (layout != RFX_LAYOUT_UPPER && layout != RFX_LAYOUT_FULL) return RFX_ERR_INVALID;
+  if (mode != RFX_MDS_COUNTS && mode != RFX_MDS_REFERENCE) return RFX_ERR_INVALID;
+  if (mode == RFX_MDS_REFERENCE && (!nout || layout != RFX_LAYOUT_UPPER)) return RFX_ERR_INVALID;
+  if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
+  if (workspace_bytes < rfx_mds_dense_workspace_bytes(n, precision)) return RFX_ERR_WORKSPACE;
+  if (precision == RFX_PREC_FP64)
+    return run_dense<double>(st, counts, layout, n, mode, scale, nout, max_iter, tol, workspace, coords, eigvals_host,
+                             iters_host);
+  return run_dense<float>(st, counts, layout, n, mode, scale, nout, max_iter, tol, workspace, coords, eigvals_host,
+                          iters_host);
+}

@@ -139,3 +139,67 @@ def unpack_upper(u: np.ndarray, n: int) -> np.ndarray:
     m[iu] = u
     m.T[iu] = u
     return m
+
+
+def mds_dense(counts: torch.Tensor, layout: int, n: int, mode: int = _lib.MDS_COUNTS, scale: float = 1.0,
+              nout: torch.Tensor | None = None, precision: int = _lib.PREC_FP64, max_iter: int = 500,
+              tol: float | None = None) -> tuple[torch.Tensor, np.ndarray, int]:
+    """Classical 3-D MDS on the GPU from a count matrix; returns (coords n x 3
+    float64 on the device, the 16 Ritz values, iterations)."""
+    import ctypes as C
+
+    if tol is None:
+        tol = 1e-14 if precision == _lib.PREC_FP64 else 1e-7
+    nb = lib().rfx_mds_dense_workspace_bytes(n, precision)
+    ws = torch.empty(nb, dtype=torch.uint8, device=counts.device)
+    coords = torch.empty((n, 3), dtype=torch.float64, device=counts.device)
+    eig = np.zeros(16, np.float64)
+    it = C.c_int32(0)
+    check(lib().rfx_mds_dense(_stream(), _dptr(counts), layout, n, mode, scale, _dptr(nout), precision, max_iter, tol,
+                              _dptr(ws), nb, _dptr(coords), eig.ctypes.data, C.addressof(it)), "rfx_mds_dense")
+    return coords, eig, it.value
+
+
+_FMT_DTYPE = {_lib.PREC_FP32: torch.float32, _lib.PREC_BF16: torch.int16, _lib.PREC_INT8: torch.int8,
+              _lib.PREC_FP8: torch.uint8}
+
+
+def lowrank_factors(keys: torch.Tensor, forest: Forest, rank: int = 64, oversample: int = 16, power_iters: int = 2,
+                    seed: int = 1) -> tuple[torch.Tensor, np.ndarray]:
+    """U (n x rank, float32, device) with C/T ~ U U^T, and the top eigenvalues of C/T."""
+    n, T = forest.n, forest.ntree
+    nb = lib().rfx_lowrank_workspace_bytes(n, T, forest.total_nodes, rank, oversample)
+    ws = torch.empty(nb, dtype=torch.uint8, device=keys.device)
+    U = torch.empty((n, rank), dtype=torch.float32, device=keys.device)
+    eig = np.zeros(rank, np.float64)
+    off = np.ascontiguousarray(forest.node_offset, np.int64)
+    check(lib().rfx_lowrank_factors(_stream(), _dptr(keys), n, T, forest.total_nodes, off.ctypes.data, rank,
+                                    oversample, power_iters, seed, _dptr(ws), nb, _dptr(U), eig.ctypes.data),
+          "rfx_lowrank_factors")
+    return U, eig
+
+
+def quantize_factors(U: torch.Tensor, fmt: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """Storage copy of the factors in `fmt` plus per-column float32 scales."""
+    n, r = U.shape
+    Uq = torch.empty((n, r), dtype=_FMT_DTYPE[fmt], device=U.device)
+    sc = torch.empty(r, dtype=torch.float32, device=U.device)
+    check(lib().rfx_quantize_factors(_stream(), _dptr(U), n, r, fmt, _dptr(Uq), _dptr(sc)), "rfx_quantize_factors")
+    return Uq, sc
+
+
+def mds_factors(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, precision: int = _lib.PREC_FP32,
+                max_iter: int = 500, tol: float | None = None) -> tuple[torch.Tensor, np.ndarray, int]:
+    import ctypes as C
+
+    n, r = Uq.shape
+    if tol is None:
+        tol = 1e-14 if precision == _lib.PREC_FP64 else 1e-7
+    nb = lib().rfx_mds_factors_workspace_bytes(n, r, precision)
+    ws = torch.empty(nb, dtype=torch.uint8, device=Uq.device)
+    coords = torch.empty((n, 3), dtype=torch.float64, device=Uq.device)
+    eig = np.zeros(16, np.float64)
+    it = C.c_int32(0)
+    check(lib().rfx_mds_factors(_stream(), _dptr(Uq), _dptr(scales), fmt, n, r, precision, max_iter, tol, _dptr(ws),
+                                nb, _dptr(coords), eig.ctypes.data, C.addressof(it)), "rfx_mds_factors")
+    return coords, eig, it.value

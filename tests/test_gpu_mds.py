@@ -1,0 +1,89 @@
+"""GPU MDS against the FP64 classical-MDS oracle (sign-aligned relative
+Frobenius error and largest principal angle; the eigengap is asserted so the
+comparison is well posed)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rfx_amd import _lib
+from rfx_amd.proximity import Accumulator, DeviceForest, mds_dense, unpack_upper
+
+from helpers import small_forest
+
+pytestmark = pytest.mark.gpu
+
+
+def _counts(kind, n, p, ncl, T, seed, variant, mtry=0):
+    x, y, f = small_forest(kind, n, p, ncl, T, seed, mtry=mtry)
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(f.n, f.ntree, f.total_nodes)
+    U = acc.accumulate(keys, variant, _lib.LAYOUT_UPPER, _lib.ALGO_DENSE)
+    return f, U
+
+
+@pytest.mark.parametrize("precision,tol_rel", [(_lib.PREC_FP64, 1e-6), (_lib.PREC_FP32, 2e-3)])
+def test_mds_counts_over_T(gpu, precision, tol_rel):
+    print(f"mds counts/T precision {precision}", flush=True)
+    f, U = _counts(_lib.DATA_COVTYPE_LIKE, 1800, 54, 7, 60, 11, _lib.PROX_ALL, mtry=7)
+    n, T = f.n, f.ntree
+    X, eig, it = mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_COUNTS, 1.0 / T, precision=precision)
+    C = unpack_upper(U.cpu().numpy(), n).astype(np.float64) / T
+    Xref, w = O.classical_mds(C, 3)
+    assert w[2] / w[3] > 1.02, f"eigengap too small for a stable comparison: {w[:5]}"
+    e = O.embedding_errors(X.cpu().numpy(), Xref)
+    print(it, eig[:4], w[:4], e, flush=True)
+    assert e["rel_frobenius"] < tol_rel
+    assert np.allclose(eig[:3], w[:3], rtol=tol_rel)
+
+
+def centred(M: np.ndarray) -> np.ndarray:
+    D = M.max() - M
+    np.fill_diagonal(D, 0.0)
+    D2 = D * D
+    r = D2.mean(axis=1)
+    return -0.5 * (D2 - r[:, None] - r[None, :] + r.mean())
+
+
+def eig_residuals(B: np.ndarray, X: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """||B v - lambda v|| / (lambda ||v||) per embedding column (v = x / sqrt(lambda))."""
+    V = X / np.sqrt(lam[None, :])
+    return np.linalg.norm(B @ V - V * lam[None, :], axis=0) / (np.abs(lam) * np.linalg.norm(V, axis=0))
+
+
+@pytest.mark.parametrize("kind,n,p,ncl,T,seed", [(_lib.DATA_COVTYPE_LIKE, 1500, 54, 7, 100, 12),
+                                                 (_lib.DATA_OVERLAP_MIXTURE, 1200, 8, 3, 80, 12)])
+def test_mds_reference_chain(gpu, kind, n, p, ncl, T, seed):
+    print(f"mds reference chain kind {kind}", flush=True)
+    f, U = _counts(kind, n, p, ncl, T, seed, _lib.PROX_OOB_UPPER)
+    n = f.n
+    nout = f.nout(2)
+    X, eig, it = mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_REFERENCE, nout=torch.from_numpy(nout).to(gpu),
+                           precision=_lib.PREC_FP64)
+    Uh = unpack_upper(U.cpu().numpy(), n).astype(np.float64)
+    # finishprox from the symmetric pair sums: P_ij + P_ji = U_ij
+    S = O.finishprox(n, nout, (Uh / 2.0).reshape(-1, order="F"))  # P = U/2 gives the same pair sums
+    M = O.mds_input_reference(S.reshape(n, n, order="F"), nout)
+    Xref, w = O.classical_mds(M, 3)
+    Xg = X.cpu().numpy()
+    # eigenvalues and eigen-residuals pin the result whatever the eigengap
+    assert np.allclose(eig[:3], w[:3], rtol=1e-9)
+    res = eig_residuals(centred(M), Xg, eig[:3])
+    print(it, eig[:5], w[:5], res, flush=True)
+    assert res.max() < 1e-5
+    if w[2] / w[3] > 1.02:  # a well-separated third eigenvalue: compare the embedding itself
+        e = O.embedding_errors(Xg, Xref)
+        print(e, flush=True)
+        assert e["rel_frobenius"] < 1e-5
+
+
+def test_mds_full_layout_matches_upper(gpu):
+    f, U = _counts(_lib.DATA_OVERLAP_MIXTURE, 700, 6, 4, 40, 13, _lib.PROX_ALL)
+    n = f.n
+    full = torch.from_numpy(unpack_upper(U.cpu().numpy(), n).reshape(-1)).to(gpu)
+    Xu, eu, _ = mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_COUNTS, 1.0 / f.ntree)
+    Xf, ef, _ = mds_dense(full, _lib.LAYOUT_FULL, n, _lib.MDS_COUNTS, 1.0 / f.ntree)
+    assert O.embedding_errors(Xf.cpu().numpy(), Xu.cpu().numpy())["rel_frobenius"] < 1e-8

@@ -15,7 +15,7 @@ from helpers import keys_from_nodextr, load_golden, small_forest
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = [_lib.ALGO_DENSE, _lib.ALGO_BUCKET]
+ALGOS = [_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER]
 
 
 def _say(msg: str) -> None:
@@ -183,6 +183,10 @@ def test_full_size_properties_c2_shape(gpu):
     a_b, o_b = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET)
     a_d, o_d = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_DENSE)
     assert torch.equal(a_b, a_d) and torch.equal(o_b, o_d)
+    del a_d, o_d
+    a_g, o_g = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+    assert torch.equal(a_b, a_g) and torch.equal(o_b, o_g)
+    del a_g, o_g
     n = f.n
     diag = torch.from_numpy(np.array([n * i - i * (i - 1) // 2 for i in range(n)], np.int64)).to(gpu)
     assert bool((a_b[diag] == f.ntree).all())
