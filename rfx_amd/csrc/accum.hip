@@ -198,12 +198,12 @@ __global__ __launch_bounds__(kThreads) void dense_rows_kernel(const int32_t* __r
   out[(i - r0) * n + j] = static_cast<int32_t>(c);
 }
 
-// One thread per bucket slot: walk the bucket, count each unordered pair
-// once (from its smaller sample index).
+// One thread per sorted slot: its partners of larger sample index are the
+// rest of its leaf's member list (ascending); each pair is added once with a
+// global integer atomic.
 template <int MODE, int LAYOUT>
 __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __restrict__ members,
                                                           const uint32_t* __restrict__ mbucket,
-                                                          const uint32_t* __restrict__ cnt,
                                                           const uint32_t* __restrict__ bend,
                                                           const uint32_t* __restrict__ nvalid_p, int64_t n,
                                                           int32_t* __restrict__ out, int32_t* __restrict__ out2) {
@@ -211,14 +211,12 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
   for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < nvalid;
        m += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const uint32_t me = members[m];
-    const uint32_t b = mbucket[m];
-    const uint32_t e = bend[b], s = e - cnt[b];
+    const uint32_t e = bend[mbucket[m]];
     const int64_t i = me >> 1;
     const uint32_t bi = me & 1u;
-    for (uint32_t q = s; q < e; ++q) {
+    for (uint32_t q = static_cast<uint32_t>(m) + 1; q < e; ++q) {
       const uint32_t o = members[q];
       const int64_t j = o >> 1;
-      if (j <= i) continue;
       const uint32_t bj = o & 1u;
       if (MODE == kMode_ALL || MODE == kMode_PAIR) {
         if (LAYOUT == RFX_LAYOUT_UPPER) {
@@ -255,29 +253,51 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
 // collects that line's partners from the leaf buckets into LDS counters, so
 // every output element is written exactly once, coalesced, with no global
 // atomics and no separate zero fill.  Each wavefront takes 64 trees at a
-// time (lane = tree), prefix-sums their bucket lengths, and walks the union
-// of the 64 buckets 64 members per step (owner lane found by a binary search
-// over the prefix), so a bucket of 1000 members and one of 2 cost the same
-// per member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
+// time (lane = tree): one load of pe gives the line's position in its leaf
+// and the leaf's end, so the partners of larger index are a contiguous run
+// (members are sorted); binary searches trim the run to the column chunk.
+// The wavefront then walks the union of its 64 runs 64 members per step
+// (owner lane by a binary search over the prefix sum of run lengths), four
+// steps in flight, so a leaf of 1000 members and one of 2 cost the same per
+// member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
 // ---------------------------------------------------------------------------
-constexpr int kGatherChunk = 16384;  // LDS words per workgroup (64 KiB)
+#ifndef RFX_GATHER_CHUNK
+#define RFX_GATHER_CHUNK 8192
+#endif
+#ifndef RFX_GATHER_UNROLL
+#define RFX_GATHER_UNROLL 8
+#endif
+constexpr int kGatherChunk = RFX_GATHER_CHUNK;  // LDS words per workgroup
+constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
+
+// first position in members[lo, hi) whose sample index is >= key
+__device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
+                                              uint32_t key) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((members[mid] >> 1) < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 template <int MODE, int LAYOUT>
-__global__ __launch_bounds__(256) void gather_line_kernel(const int32_t* __restrict__ keys, int64_t n,
-                                                          int32_t ntree, const uint32_t* __restrict__ cnt,
-                                                          const uint32_t* __restrict__ bend,
-                                                          const uint32_t* __restrict__ members, int32_t accumulate,
-                                                          int32_t* __restrict__ out, int32_t* __restrict__ out2) {
+__global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restrict__ pe,
+                                                          const uint32_t* __restrict__ bstart,
+                                                          const uint32_t* __restrict__ mbucket, int64_t n,
+                                                          int32_t ntree, const uint32_t* __restrict__ members,
+                                                          int32_t accumulate, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ out2) {
   __shared__ uint32_t acc[kGatherChunk];
   __shared__ uint32_t diag_valid;
   const int64_t line = blockIdx.x;
-  const int64_t c0 = static_cast<int64_t>(blockIdx.y) * kGatherChunk;
-  int64_t lo = c0;
-  const int64_t hi = min(n, c0 + kGatherChunk);
-  if (LAYOUT == RFX_LAYOUT_UPPER) {
-    if (hi <= line) return;
-    lo = max(lo, line);
-  }
+  // UPPER rows are chunked from the diagonal: chunk y of row i covers
+  // [i + y*CH, i + (y+1)*CH); FULL columns in aligned chunks.
+  const int64_t lo = (LAYOUT == RFX_LAYOUT_UPPER ? line : 0) + static_cast<int64_t>(blockIdx.y) * kGatherChunk;
+  if (lo >= n) return;
+  const int64_t hi = min(n, lo + kGatherChunk);
   const int len_chunk = static_cast<int>(hi - lo);
   for (int q = threadIdx.x; q < len_chunk; q += 256) acc[q] = 0u;
   if (threadIdx.x == 0) diag_valid = 0u;
@@ -289,14 +309,22 @@ __global__ __launch_bounds__(256) void gather_line_kernel(const int32_t* __restr
     const int32_t t = tb + lane;
     uint32_t s = 0, len = 0, bag = 0;
     if (t < ntree) {
-      const int32_t k = keys[static_cast<int64_t>(t) * n + line];
-      if (k >= 0) {
-        const uint32_t b = static_cast<uint32_t>(k) >> 1;
-        const uint32_t e = bend[b];
-        len = cnt[b];
-        s = e - len;
-        bag = static_cast<uint32_t>(k) & 1u;
+      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
+      const uint32_t end = q.y & 0x7fffffffu;
+      if (end > 0) {
         ++valid;
+        bag = q.y >> 31;
+        uint32_t b0, b1;
+        if (LAYOUT == RFX_LAYOUT_UPPER) {
+          b0 = q.x + 1;
+          if (lo > line) b0 = member_lb(members, b0, end, static_cast<uint32_t>(lo));
+        } else {
+          const uint32_t bs = bstart[mbucket[q.x]];
+          b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
+        }
+        b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
+        s = b0;
+        len = b1 > b0 ? b1 - b0 : 0u;
       }
     }
     if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
@@ -308,31 +336,37 @@ __global__ __launch_bounds__(256) void gather_line_kernel(const int32_t* __restr
     }
     const uint32_t excl = pre - len;
     const uint32_t total = __shfl(pre, 63, 64);
-    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
-      const uint32_t g = g0 + lane;
-      // owner = number of lanes whose inclusive prefix is <= g
-      int o = 0;
+    for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
+      uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
+      bool ok[kGatherUnroll];
 #pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        const uint32_t v = __shfl(pre, o + d - 1, 64);
-        if (v <= g) o += d;
+      for (int u = 0; u < kGatherUnroll; ++u) {
+        const uint32_t g = g0 + u * 64 + lane;
+        int o = 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          const uint32_t v = __shfl(pre, o + d - 1, 64);
+          if (v <= g) o += d;
+        }
+        o = min(o, 63);
+        const uint32_t so = __shfl(s, o, 64);
+        const uint32_t eo = __shfl(excl, o, 64);
+        bo[u] = __shfl(bag, o, 64);
+        ok[u] = g < total;
+        mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
       }
-      o = min(o, 63);
-      const uint32_t so = __shfl(s, o, 64);
-      const uint32_t eo = __shfl(excl, o, 64);
-      const uint32_t bo = __shfl(bag, o, 64);
-      if (g < total) {
-        const uint32_t m = members[so + (g - eo)];
-        const int64_t q = m >> 1;
-        const uint32_t bq = m & 1u;
-        bool in_range = q >= lo && q < hi && q != line;
-        if (LAYOUT == RFX_LAYOUT_UPPER) in_range = in_range && q > line;
+#pragma unroll
+      for (int u = 0; u < kGatherUnroll; ++u) {
+        if (!ok[u]) continue;
+        const int64_t q = mv[u] >> 1;
+        const uint32_t bq = mv[u] & 1u;
+        if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
         uint32_t inc = 0;
         if (MODE == kMode_ALL) inc = 1u;
-        if (MODE == kMode_OOB_UPPER) inc = bq != bo ? 1u : 0u;
+        if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
         if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
-        if (MODE == kMode_PAIR) inc = 1u | (bq != bo ? 0x10000u : 0u);
-        if (in_range && inc) atomicAdd(&acc[q - lo], inc);
+        if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
+        if (inc) atomicAdd(&acc[q - lo], inc);
       }
     }
   }
@@ -370,12 +404,12 @@ __global__ void all_diag_kernel(const int32_t* __restrict__ keys, int64_t n, int
   out[o] += c;
 }
 
-__global__ void sum_squares_kernel(const uint32_t* __restrict__ cnt, int64_t len,
-                                   unsigned long long* __restrict__ acc) {
+__global__ void sum_squares_kernel(const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+                                   int64_t len, unsigned long long* __restrict__ acc) {
   unsigned long long s = 0;
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < len;
        e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const unsigned long long c = cnt[e];
+    const unsigned long long c = bend[e] - bstart[e];
     s += c * c;
   }
   for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
@@ -405,10 +439,8 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   }
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
-                     w.members, w.mbucket);
   hipLaunchKernelGGL((bucket_pair_kernel<MODE, LAYOUT>), dim3(grid_for(slots, 256)), dim3(256), 0, st, w.members,
-                     w.mbucket, w.cnt, w.bend, w.part + w.nparts, n, out, out2);
+                     w.mbucket, w.bend, w.nvalid, n, out, out2);
   if (MODE == kMode_ALL || MODE == kMode_PAIR)
     hipLaunchKernelGGL(all_diag_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, keys, n, ntree, LAYOUT, out);
   return rfx_launch_status();
@@ -418,13 +450,10 @@ template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  const int64_t slots = n * static_cast<int64_t>(ntree);
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
-                     w.members, w.mbucket);
   dim3 grid(static_cast<unsigned>(n), rfx_ceil_div(n, kGatherChunk));
-  hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, keys, n, ntree, w.cnt, w.bend,
+  hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
                      w.members, accumulate, out, out2);
   return rfx_launch_status();
 }
@@ -474,7 +503,7 @@ int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_no
   if (layout != RFX_LAYOUT_FULL && layout != RFX_LAYOUT_UPPER) return RFX_ERR_INVALID;
   if (algo < RFX_ALGO_AUTO || algo > RFX_ALGO_GATHER) return RFX_ERR_INVALID;
   if (algo != RFX_ALGO_DENSE && (total_nodes <= 0 || total_nodes > RFX_MAX_TOTAL_NODES)) return RFX_ERR_INVALID;
-  if (n * static_cast<int64_t>(ntree) >= (1LL << 32)) return RFX_ERR_UNSUPPORTED;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
   return RFX_OK;
 }
 
@@ -491,15 +520,15 @@ extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t 
   if (!keys || !ws || !sum_sq_host || n <= 0 || ntree <= 0 || total_nodes <= 0 ||
       total_nodes > RFX_MAX_TOTAL_NODES)
     return RFX_ERR_INVALID;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
   if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  RFX_HIP_TRY(hipMemsetAsync(w.cnt, 0, 4 * total_nodes, st));
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (rc) return rc;
   RFX_HIP_TRY(hipMemsetAsync(w.stat, 0, 8, st));
-  hipLaunchKernelGGL(bucket_hist_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, slots, w.cnt);
-  hipLaunchKernelGGL(sum_squares_kernel, dim3(grid_for(total_nodes, 256)), dim3(256), 0, st, w.cnt, total_nodes,
-                     w.stat);
-  int rc = rfx_launch_status();
+  hipLaunchKernelGGL(sum_squares_kernel, dim3(grid_for(total_nodes, 256)), dim3(256), 0, st, w.bstart, w.bend,
+                     total_nodes, w.stat);
+  rc = rfx_launch_status();
   if (rc) return rc;
   unsigned long long h = 0;
   RFX_HIP_TRY(hipMemcpyAsync(&h, w.stat, 8, hipMemcpyDeviceToHost, st));

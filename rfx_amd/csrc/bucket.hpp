@@ -1,26 +1,32 @@
-// Leaf buckets of the (tree, sample) slots: histogram over terminal nodes,
-// exclusive scan, stable scatter.  Shared by the co-occurrence counts and
-// the matrix-free proximity products of the factor path.
+// Leaf buckets of the (tree, sample) slots, shared by the co-occurrence
+// counts and the matrix-free proximity products of the factor path.
+//
+// Every slot s = t*n + i with a terminal leaf gets the sort key "global node
+// id" (off-leaf slots get total_nodes, which sorts last) and the value
+// (s << 1) | in-bag.  A stable LSD radix sort (8-bit digits, 4096-key tiles:
+// tile histograms, one exclusive scan over the digit-major histogram, and a
+// scatter whose in-tile ranks come from wavefront ballots on the digit bits)
+// groups the slots by leaf with the samples of a leaf in ascending order.
+// From the sorted arrays:
+//   bstart[b], bend[b]  the member range of leaf b (0, 0 when empty)
+//   members[p]          (i << 1) | in-bag of the p-th sorted slot
+//   mbucket[p]          its leaf (the sorted keys)
+//   pe[s]               (position of slot s in members, end of its leaf |
+//                       in-bag << 31); (0, 0) off-leaf
+// so a sample finds the partners that follow it in its leaf with one load.
 #pragma once
 
 #include "rfx_common.hpp"
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// BUCKET algorithm
-// ---------------------------------------------------------------------------
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanChunk = kScanThreads * kScanItems;
 
-__global__ void bucket_hist_kernel(const int32_t* __restrict__ keys, int64_t total, uint32_t* __restrict__ cnt) {
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int32_t k = keys[e];
-    if (k >= 0) atomicAdd(&cnt[k >> 1], 1u);
-  }
-}
+constexpr int kRsThreads = 256;
+constexpr int kRsRounds = 16;
+constexpr int kRsTile = kRsThreads * kRsRounds;  // keys per tile
 
 // Block-wide exclusive scan of kScanItems per thread; returns the block total.
 __device__ uint32_t block_exclusive_scan(uint32_t (&v)[kScanItems], uint32_t* lds_waves) {
@@ -88,9 +94,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* _
   if (threadIdx.x == 0) part[nparts] = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t len,
-                                                                  const uint32_t* __restrict__ part,
-                                                                  uint32_t* __restrict__ out) {
+// Exclusive scan of in[] into out[] (in-place allowed: each element is read
+// and written by the same thread).
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* in, int64_t len,
+                                                                  const uint32_t* __restrict__ part, uint32_t* out) {
   __shared__ uint32_t lw[kScanThreads / 64 + 1];
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
@@ -103,81 +110,178 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
     if (base + q < len) out[base + q] = v[q] + off;
 }
 
-// Stable scatter of the (tree, sample) slots into their buckets: one
-// workgroup per tree walks its samples in chunks of 256 in index order; a
-// sample's position is the bucket cursor plus the number of earlier samples
-// of the chunk in the same bucket, so every bucket lists its members in
-// ascending sample order and the layout is deterministic.  cursor starts as
-// the exclusive scan and ends as the bucket end offsets.
-__global__ __launch_bounds__(256) void bucket_scatter_kernel(const int32_t* __restrict__ keys, int64_t n,
-                                                             uint32_t* __restrict__ cursor,
-                                                             uint32_t* __restrict__ members,
-                                                             uint32_t* __restrict__ mbucket) {
-  __shared__ uint32_t sb[256];
-  const int64_t t = blockIdx.x;
-  const int tid = threadIdx.x;
-  for (int64_t i0 = 0; i0 < n; i0 += 256) {
-    const int64_t i = i0 + tid;
-    const int32_t k = i < n ? keys[t * n + i] : -1;
-    const uint32_t b = k >= 0 ? static_cast<uint32_t>(k) >> 1 : 0xffffffffu;
-    sb[tid] = b;
+// Sort input: key = leaf (global node id) or total_nodes, value = slot<<1|bag.
+__global__ void rs_prep_kernel(const int32_t* __restrict__ keys, int64_t slots, uint32_t total_nodes,
+                               uint32_t* __restrict__ K, uint32_t* __restrict__ V) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= slots) return;
+  const int32_t k = keys[s];
+  K[s] = k >= 0 ? static_cast<uint32_t>(k) >> 1 : total_nodes;
+  V[s] = (static_cast<uint32_t>(s) << 1) | (k >= 0 ? (static_cast<uint32_t>(k) & 1u) : 0u);
+}
+
+// Digit histogram of one tile, written digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __restrict__ K, int64_t N, int shift,
+                                                             int64_t ntiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  for (int q = 0; q < kRsRounds; ++q) {
+    const int64_t g = base + q * kRsThreads + threadIdx.x;
+    if (g < N) atomicAdd(&cnt[(K[g] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter of one tile.  Keys are visited round by round (256
+// consecutive keys per round, in index order); within a wavefront the lanes
+// holding the same digit are found with 8 ballots, across the 4 wavefronts
+// through per-wave digit counts in LDS.
+__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* __restrict__ Kin,
+                                                                const uint32_t* __restrict__ Vin, int64_t N, int shift,
+                                                                int64_t ntiles, const uint32_t* __restrict__ offs,
+                                                                uint32_t* __restrict__ Kout,
+                                                                uint32_t* __restrict__ Vout) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcnt[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  running[tid] = offs[static_cast<int64_t>(tid) * ntiles + blockIdx.x];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int q = 0; q < kRsRounds; ++q) {
+    wcnt[0][tid] = 0;
+    wcnt[1][tid] = 0;
+    wcnt[2][tid] = 0;
+    wcnt[3][tid] = 0;
     __syncthreads();
-    uint32_t rank = 0;
-    bool last = true;
-    for (int q = 0; q < 256; ++q) {
-      const uint32_t bq = sb[q];
-      if (bq == b) {
-        rank += q < tid ? 1u : 0u;
-        last = last && !(q > tid);
-      }
+    const int64_t g = base + q * kRsThreads + tid;
+    const bool valid = g < N;
+    uint32_t k = 0, v = 0;
+    if (valid) {
+      k = Kin[g];
+      v = Vin[g];
     }
-    uint32_t base = 0;
-    if (k >= 0) {
-      base = cursor[b];
-      const uint32_t pos = base + rank;
-      members[pos] = (static_cast<uint32_t>(i) << 1) | (static_cast<uint32_t>(k) & 1u);
-      mbucket[pos] = b;
+    const uint32_t d = (k >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t m = __ballot(on);
+      peers &= on ? m : ~m;
+    }
+    const uint32_t rank_w = __popcll(peers & lt_mask);
+    if (valid && rank_w == 0) wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank_w;
+      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+      Kout[pos] = k;
+      Vout[pos] = v;
     }
     __syncthreads();
-    if (k >= 0 && last) cursor[b] = base + rank + 1;
+    running[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
     __syncthreads();
   }
 }
 
+// Leaf ranges from the sorted keys; nvalid = first off-leaf position.
+__global__ void rs_bounds_kernel(const uint32_t* __restrict__ K, int64_t N, uint32_t total_nodes,
+                                 uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend,
+                                 uint32_t* __restrict__ nvalid) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t k = K[p];
+  const uint32_t prev = p > 0 ? K[p - 1] : 0xffffffffu;
+  if (k == total_nodes) {
+    if (prev != total_nodes) *nvalid = static_cast<uint32_t>(p);
+    return;
+  }
+  if (prev != k) bstart[k] = static_cast<uint32_t>(p);
+  const uint32_t next = p + 1 < N ? K[p + 1] : 0xffffffffu;
+  if (next != k) bend[k] = static_cast<uint32_t>(p + 1);
+  if (p + 1 == N) *nvalid = static_cast<uint32_t>(N);
+}
+
+// members[p] = (i << 1) | bag; pe[s] = (p, end of its leaf | bag << 31),
+// (0, 0) off-leaf.
+__global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N, uint32_t n,
+                                  uint32_t total_nodes, const uint32_t* __restrict__ bend,
+                                  uint32_t* __restrict__ members, uint2* __restrict__ pe) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t v = V[p];
+  const uint32_t s = v >> 1, bag = v & 1u;
+  const uint32_t i = s % n;
+  members[p] = (i << 1) | bag;
+  const uint32_t k = K[p];
+  pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
+}
+
 struct BucketWs {
-  uint32_t* cnt;
+  uint32_t* bstart;
   uint32_t* bend;
-  uint32_t* part;
   uint32_t* members;
-  uint32_t* mbucket;
+  uint32_t* mbucket;  // sorted keys (one of the two key buffers)
+  uint2* pe;
+  uint32_t* nvalid;
+  uint32_t* K[2];
+  uint32_t* V[2];
+  uint32_t* hist;
+  uint32_t* part;
   unsigned long long* stat;
+  int64_t ntiles;
   int64_t nparts;
+  int passes;
 };
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+int radix_passes(int64_t total_nodes) {
+  int bits = 0;
+  while ((1LL << bits) <= total_nodes) ++bits;  // keys range over [0, total_nodes]
+  return (bits + 7) / 8;
+}
+
 size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
-  const int64_t nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
   const int64_t slots = n * static_cast<int64_t>(ntree);
-  return align256(4 * total_nodes) * 2 + align256(4 * (nparts + 1)) + align256(4 * slots) * 2 + 256;
+  const int64_t ntiles = (slots + kRsTile - 1) / kRsTile;
+  const int64_t hlen = 256 * ntiles;
+  const int64_t nparts = (hlen + kScanChunk - 1) / kScanChunk;
+  return align256(4 * total_nodes) * 2 + align256(4 * slots) * 5 + align256(8 * slots) + align256(4 * hlen) +
+         align256(4 * (nparts + 1)) + 512;
 }
 
 BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes) {
   BucketWs w;
   char* p = static_cast<char*>(ws);
-  w.nparts = (total_nodes + kScanChunk - 1) / kScanChunk;
   const int64_t slots = n * static_cast<int64_t>(ntree);
-  w.cnt = reinterpret_cast<uint32_t*>(p);
+  w.ntiles = (slots + kRsTile - 1) / kRsTile;
+  const int64_t hlen = 256 * w.ntiles;
+  w.nparts = (hlen + kScanChunk - 1) / kScanChunk;
+  w.passes = radix_passes(total_nodes);
+  w.bstart = reinterpret_cast<uint32_t*>(p);
   p += align256(4 * total_nodes);
   w.bend = reinterpret_cast<uint32_t*>(p);
   p += align256(4 * total_nodes);
-  w.part = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * (w.nparts + 1));
   w.members = reinterpret_cast<uint32_t*>(p);
   p += align256(4 * slots);
-  w.mbucket = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * slots);
-  w.stat = reinterpret_cast<unsigned long long*>(p);
+  for (int b = 0; b < 2; ++b) {
+    w.K[b] = reinterpret_cast<uint32_t*>(p);
+    p += align256(4 * slots);
+    w.V[b] = reinterpret_cast<uint32_t*>(p);
+    p += align256(4 * slots);
+  }
+  w.pe = reinterpret_cast<uint2*>(p);
+  p += align256(8 * slots);
+  w.hist = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * hlen);
+  w.part = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * (w.nparts + 1));
+  w.nvalid = reinterpret_cast<uint32_t*>(p);
+  w.stat = reinterpret_cast<unsigned long long*>(p + 256);
+  w.mbucket = w.K[w.passes & 1];
   return w;
 }
 
@@ -188,17 +292,33 @@ unsigned grid_for(int64_t work, int threads) {
   return static_cast<unsigned>(g);
 }
 
-// Histogram + scan: cnt and bend (as bucket start) ready, part[nparts] = total.
+// Full bucket build from the leaf keys (see the file comment).
 int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                  const BucketWs& w) {
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  RFX_HIP_TRY(hipMemsetAsync(w.cnt, 0, 4 * total_nodes, st));
-  hipLaunchKernelGGL(bucket_hist_kernel, dim3(grid_for(slots, 256)), dim3(256), 0, st, keys, slots, w.cnt);
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
-                     total_nodes, w.part);
-  hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts);
-  hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.cnt,
-                     total_nodes, w.part, w.bend);
+  const int64_t N = n * static_cast<int64_t>(ntree);
+  const uint32_t tn = static_cast<uint32_t>(total_nodes);
+  RFX_HIP_TRY(hipMemsetAsync(w.bstart, 0, 4 * total_nodes, st));
+  RFX_HIP_TRY(hipMemsetAsync(w.bend, 0, 4 * total_nodes, st));
+  hipLaunchKernelGGL(rs_prep_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, N, tn, w.K[0], w.V[0]);
+  const int64_t hlen = 256 * w.ntiles;
+  for (int pass = 0; pass < w.passes; ++pass) {
+    const int src = pass & 1, dst = src ^ 1;
+    hipLaunchKernelGGL(rs_hist_kernel, dim3(static_cast<unsigned>(w.ntiles)), dim3(kRsThreads), 0, st, w.K[src], N,
+                       8 * pass, w.ntiles, w.hist);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
+                       hlen, w.part);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
+                       hlen, w.part, w.hist);
+    hipLaunchKernelGGL(rs_scatter_kernel, dim3(static_cast<unsigned>(w.ntiles)), dim3(kRsThreads), 0, st, w.K[src],
+                       w.V[src], N, 8 * pass, w.ntiles, w.hist, w.K[dst], w.V[dst]);
+  }
+  const int fin = w.passes & 1;
+  hipLaunchKernelGGL(rs_bounds_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], N, tn, w.bstart, w.bend,
+                     w.nvalid);
+  hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
+                     static_cast<uint32_t>(n), tn,
+                     w.bend, w.members, w.pe);
   return rfx_launch_status();
 }
 

@@ -162,19 +162,16 @@ __global__ void quantize_kernel(const float* __restrict__ U, int64_t n, int r, i
 // matrix-free C V through the leaf buckets
 // ---------------------------------------------------------------------------
 // S[b - lo][c] = sum over members of bucket b (ascending) of V[m][c]
-__global__ void leafsum_kernel(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ bend,
+__global__ void leafsum_kernel(const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
                                const uint32_t* __restrict__ members, int64_t lo, int64_t hi,
                                const float* __restrict__ V, int L, float* __restrict__ S) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= (hi - lo) * L) return;
   const int64_t b = lo + e / L;
   const int c = static_cast<int>(e % L);
-  const uint32_t k = cnt[b];
   float s = 0.0f;
-  if (k) {
-    const uint32_t q1 = bend[b];
-    for (uint32_t q = q1 - k; q < q1; ++q) s += V[static_cast<int64_t>(members[q] >> 1) * L + c];
-  }
+  const uint32_t q1 = bend[b];
+  for (uint32_t q = bstart[b]; q < q1; ++q) s += V[static_cast<int64_t>(members[q] >> 1) * L + c];
   S[e] = s;
 }
 
@@ -491,7 +488,7 @@ extern "C" int rfx_lowrank_factors(hipStream_t st, const int32_t* keys, int64_t 
     return RFX_ERR_INVALID;
   const int L = rank + oversample;
   if (L > n) return RFX_ERR_INVALID;
-  if (n * static_cast<int64_t>(ntree) >= (1LL << 32)) return RFX_ERR_UNSUPPORTED;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
   if (ws_bytes < rfx_lowrank_workspace_bytes(n, ntree, total_nodes, rank, oversample)) return RFX_ERR_WORKSPACE;
   char* p = static_cast<char*>(ws);
   const BucketWs w = bucket_ws_carve(p, n, ntree, total_nodes);
@@ -509,14 +506,12 @@ extern "C" int rfx_lowrank_factors(hipStream_t st, const int32_t* keys, int64_t 
 
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.bend,
-                     w.members, w.mbucket);
 
   auto cv = [&](const float* Vin, float* Yout) -> int {
     for (size_t q = 0; q + 1 < plan.t0.size(); ++q) {
       const int32_t t0 = plan.t0[q], t1 = plan.t0[q + 1];
       const int64_t lo = node_offset_host[t0], hi = node_offset_host[t1];
-      hipLaunchKernelGGL(leafsum_kernel, dim3(rfx_ceil_div((hi - lo) * L, 256)), dim3(256), 0, st, w.cnt, w.bend,
+      hipLaunchKernelGGL(leafsum_kernel, dim3(rfx_ceil_div((hi - lo) * L, 256)), dim3(256), 0, st, w.bstart, w.bend,
                          w.members, lo, hi, Vin, L, S);
       hipLaunchKernelGGL(leafgather_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, keys, n, t0, t1, lo, S,
                          L, static_cast<int32_t>(q > 0), Yout);
