@@ -37,6 +37,9 @@ WORKLOADS = {
                                     "upper-triangle int32 counts"),
     "c1": (0, 2000, 20, 3, 100, 4, "3-class Gaussian mixture n=2000 p=20 T=100 mtry=4: leaf assignment + ALL and "
                                    "OOB upper-triangle int32 counts"),
+    "c3": (1, 200000, 54, 7, 1000, 7, "covtype-like n=200000 p=54 T=1000 mtry=7: leaf assignment + ALL and OOB "
+                                      "upper-triangle int32 counts; rank-64 factors; factor 3-D MDS per storage "
+                                      "format"),
 }
 
 
@@ -77,6 +80,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--rank", type=int, default=64, help="factor rank (c3)")
     ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket", "gather"])
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,14 +117,19 @@ def main() -> int:
     if algo == _lib.ALGO_AUTO:
         # resolve AUTO once, outside the timed region, by timing each algorithm
         cand = {}
-        for a in (_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER):
+        # row-segment GATHER and the T*n^2 DENSE compare are only candidates
+        # at moderate n (see the AUTO rule in accum.hip)
+        small = n <= 65536
+        algos = (_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER) if small else (_lib.ALGO_GATHER, _lib.ALGO_BUCKET)
+        reps = 3 if small else 1
+        for a in algos:
             acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, a, out_all, out_oob)
             torch.cuda.synchronize()
             s = time.perf_counter()
-            for _ in range(3):
+            for _ in range(reps):
                 acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, a, out_all, out_oob)
             torch.cuda.synchronize()
-            cand[a] = (time.perf_counter() - s) / 3
+            cand[a] = (time.perf_counter() - s) / reps
         algo = min(cand, key=cand.get)
         log("algo timings (s): " + " ".join(f"{NAMES[a]} {v:.5f}" for a, v in cand.items()))
     algo_name = NAMES[algo]
@@ -171,13 +180,92 @@ def main() -> int:
                    "sum_leaf_sq": sq, "pair_density": sq / (T * float(n) * n)},
         "roofline": roofline,
     }
-    if not args.no_cpu:
+    if not args.no_cpu and n > 50000:
+        # the reference CPU path holds an n x n float64 matrix (320 GB at n = 200,000)
+        line["cpu_baseline"] = None
+    elif not args.no_cpu:
         trees = args.cpu_trees or T
         cb = cpu_baseline(forest, x, trees)
         cb.pop("seconds")
         line["cpu_baseline"] = cb
+    if args.workload == "c3":
+        del keys
+        out_all = out_oob = None
+        torch.cuda.empty_cache()
+        line["mds"] = factor_mds_section(df, forest, args.rank)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def factor_mds_section(df, forest, rank: int) -> dict:
+    """Rank-r factors of P = C/T and the 3-D MDS from them in each storage
+    format, timed, with the error of each against the FP64 operator on the
+    same factors; plus the truncation check on a seeded 20k-sample problem
+    (same generator) against the exact dense FP64 MDS."""
+    import numpy as np
+    import torch
+
+    from rfx_amd import _lib
+    from rfx_amd.forest import Forest, make_dataset
+    from rfx_amd.proximity import (Accumulator, DeviceForest, lowrank_factors, mds_dense, mds_factors,
+                                   quantize_factors)
+
+    def errs(X, Xref):
+        X = X.cpu().numpy()
+        s = np.sign(np.sum(X * Xref, axis=0))
+        s[s == 0] = 1
+        rel = float(np.linalg.norm(X * s - Xref) / np.linalg.norm(Xref))
+        qa, _ = np.linalg.qr(X)
+        qb, _ = np.linalg.qr(Xref)
+        ang = float(np.arccos(np.clip(np.linalg.svd(qa.T @ qb, compute_uv=False).min(), -1, 1)))
+        return rel, ang
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    out = {"rank": rank}
+    keys, _ = df.leaf_assign()
+    (U, eig), t_fac = timed(lambda: lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2))
+    del keys
+    torch.cuda.empty_cache()
+    out["factor_build_s"] = round(t_fac, 4)
+    out["factor_top_eigs"] = [round(float(v), 4) for v in eig[:6]]
+    log(f"factors: {t_fac:.3f} s")
+    U32, s32 = quantize_factors(U, _lib.PREC_FP32)
+    (X64, e64, it64), t64 = timed(lambda: mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP64))
+    Xref = X64.cpu().numpy()
+    out["fp64_s"] = round(t64, 4)
+    out["eigvals_fp64"] = [float(v) for v in e64[:5]]
+    out["eigengap_l3_l4"] = float(e64[2] - e64[3])
+    for name, fmt in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16), ("int8", _lib.PREC_INT8),
+                      ("fp8", _lib.PREC_FP8)):
+        Uq, sc = quantize_factors(U, fmt)
+        mds_factors(Uq, sc, fmt, precision=_lib.PREC_FP32)  # warm-up
+        (X, ev, it), t = timed(lambda: mds_factors(Uq, sc, fmt, precision=_lib.PREC_FP32))
+        rel, ang = errs(X, Xref)
+        out[name] = {"mds_s": round(t, 4), "iters": it, "rel_err_vs_fp64": rel, "max_angle_rad": ang,
+                     "bytes_per_factor_row": rank * {"fp32": 4, "bf16": 2, "int8": 1, "fp8": 1}[name]}
+        log(f"factor mds {name}: {t:.4f} s rel {rel:.2e}")
+    # truncation: the exact dense FP64 MDS of a seeded 20k problem vs its factor MDS
+    xs, ys = make_dataset(_lib.DATA_COVTYPE_LIKE, 20000, 54, 7, 7)
+    fs = Forest.grow(xs, ys, forest.ntree, mtry=7, iseed=0)
+    dfs = DeviceForest(fs, xs)
+    ks, _ = dfs.leaf_assign()
+    acc = Accumulator(fs.n, fs.ntree, fs.total_nodes)
+    Cs = acc.accumulate(ks, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_AUTO)
+    Xd, ed, _ = mds_dense(Cs, _lib.LAYOUT_UPPER, fs.n, _lib.MDS_COUNTS, 1.0 / fs.ntree, precision=_lib.PREC_FP64)
+    del Cs
+    Us, _ = lowrank_factors(ks, fs, rank=rank, oversample=16, power_iters=2)
+    Uq, sc = quantize_factors(Us, _lib.PREC_BF16)
+    Xs, es, _ = mds_factors(Uq, sc, _lib.PREC_BF16)
+    rel, ang = errs(Xs, Xd.cpu().numpy())
+    out["subset_20k"] = {"dense_fp64_eigs": [float(v) for v in ed[:4]], "factor_bf16_eigs": [float(v) for v in es[:4]],
+                         "rel_err_vs_dense_fp64": rel, "max_angle_rad": ang}
+    return out
 
 
 if __name__ == "__main__":

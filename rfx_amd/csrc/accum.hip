@@ -267,8 +267,13 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
 #ifndef RFX_GATHER_UNROLL
 #define RFX_GATHER_UNROLL 8
 #endif
+#ifndef RFX_GATHER_LINE_MAX_N
+#define RFX_GATHER_LINE_MAX_N 32768
+#endif
 constexpr int kGatherChunk = RFX_GATHER_CHUNK;  // LDS words per workgroup
 constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
+// lines up to this length use the flattened line kernel, longer ones the walk
+constexpr int64_t kGatherLineMaxN = RFX_GATHER_LINE_MAX_N;
 
 // first position in members[lo, hi) whose sample index is >= key
 __device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
@@ -393,6 +398,98 @@ __global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// GATHER, long rows: one workgroup walks a whole output line chunk by chunk
+// (LDS counters of kWalkChunk words).  Each thread owns TPT trees of the line
+// and keeps, per tree, a cursor into the tree's member run with the next
+// member in a register; members are sorted, so for every chunk a thread
+// advances each cursor while the next member's column is below the chunk
+// end.  No search and no flattening: a run is streamed once across all the
+// chunks of the line, and an empty (tree, chunk) costs one register compare.
+// ---------------------------------------------------------------------------
+constexpr int kWalkChunk = 8192;
+
+template <int MODE, int LAYOUT, int TPT>
+__global__ __launch_bounds__(256) void gather_walk_kernel(const uint2* __restrict__ pe,
+                                                          const uint32_t* __restrict__ bstart,
+                                                          const uint32_t* __restrict__ mbucket, int64_t n,
+                                                          int32_t ntree, const uint32_t* __restrict__ members,
+                                                          int32_t accumulate, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ out2) {
+  __shared__ uint32_t acc[kWalkChunk];
+  __shared__ uint32_t diag_valid;
+  const int64_t line = blockIdx.x;
+  const int tid = threadIdx.x;
+  uint32_t cur[TPT], end[TPT], nv[TPT];
+  uint32_t bagm = 0, valid = 0;
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const int32_t t = tid + u * 256;
+    cur[u] = 0;
+    end[u] = 0;
+    nv[u] = 0xffffffffu;
+    if (t < ntree) {
+      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
+      const uint32_t e = q.y & 0x7fffffffu;
+      if (e > 0) {
+        ++valid;
+        const uint32_t bag = q.y >> 31;
+        bagm |= bag << u;
+        cur[u] = LAYOUT == RFX_LAYOUT_UPPER ? q.x + 1 : bstart[mbucket[q.x]];
+        end[u] = (MODE == kMode_OOB_ASYM && !bag) ? cur[u] : e;
+        if (cur[u] < end[u]) nv[u] = members[cur[u]];
+      }
+    }
+  }
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+    if (tid == 0) diag_valid = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if ((tid & 63) == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  const int64_t first = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
+  for (int64_t lo = first; lo < n; lo += kWalkChunk) {
+    const int64_t hi = min(n, lo + kWalkChunk);
+    const int len_chunk = static_cast<int>(hi - lo);
+    __syncthreads();
+    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const uint32_t bu = (bagm >> u) & 1u;
+      while ((nv[u] >> 1) < static_cast<uint32_t>(hi)) {
+        const uint32_t m = nv[u];
+        const int64_t j = m >> 1;
+        const uint32_t bq = m & 1u;
+        ++cur[u];
+        nv[u] = cur[u] < end[u] ? members[cur[u]] : 0xffffffffu;
+        if (LAYOUT == RFX_LAYOUT_FULL && j == line) continue;
+        uint32_t inc = 0;
+        if (MODE == kMode_ALL) inc = 1u;
+        if (MODE == kMode_OOB_UPPER) inc = bq != bu ? 1u : 0u;
+        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+        if (MODE == kMode_PAIR) inc = 1u | (bq != bu ? 0x10000u : 0u);
+        if (inc) atomicAdd(&acc[j - lo], inc);
+      }
+    }
+    __syncthreads();
+    if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
+    __syncthreads();
+    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
+    for (int q = tid; q < len_chunk; q += 256) {
+      const uint32_t v = acc[q];
+      if (MODE == kMode_PAIR) {
+        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+        out[base + q] = accumulate ? out[base + q] + va : va;
+        out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+      } else {
+        out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+      }
+    }
+  }
+}
+
 // Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
 __global__ void all_diag_kernel(const int32_t* __restrict__ keys, int64_t n, int32_t ntree, int32_t layout,
                                 int32_t* __restrict__ out) {
@@ -452,9 +549,30 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  dim3 grid(static_cast<unsigned>(n), rfx_ceil_div(n, kGatherChunk));
-  hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                     w.members, accumulate, out, out2);
+  if (n <= kGatherLineMaxN) {
+    dim3 grid(static_cast<unsigned>(n), rfx_ceil_div(n, kGatherChunk));
+    hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2);
+    return rfx_launch_status();
+  }
+  const dim3 grid(static_cast<unsigned>(n)), blk(256);
+  if (ntree <= 256)
+    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 1>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members, accumulate, out, out2);
+  else if (ntree <= 512)
+    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 2>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members, accumulate, out, out2);
+  else if (ntree <= 1024)
+    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 4>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members, accumulate, out, out2);
+  else if (ntree <= 2048)
+    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 8>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members, accumulate, out, out2);
+  else if (ntree <= 4096)
+    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 16>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members, accumulate, out, out2);
+  else
+    return RFX_ERR_UNSUPPORTED;
   return rfx_launch_status();
 }
 
@@ -476,11 +594,15 @@ int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int6
 }
 
 // AUTO: compare the compare-kernel cost T*n^2/2 (one tile compare per
-// tree-pair) with the gather cost: sum over trees and leaves of |leaf|^2
-// bucket visits plus one pass over the output.  The per-item weights are
-// MI355X measurements on the covtype-like workload (n = 20,000, T = 500).
-constexpr double kGatherVisitVsCompare = 8.0;
-constexpr double kGatherWriteVsCompare = 4.0;
+// tree-pair) with the leaf-bucket cost: sum over trees and leaves of |leaf|^2
+// partner visits plus one pass over the output.  GATHER (LDS row segments)
+// serves T <= 4096 (the walk kernel keeps per-tree cursors in registers);
+// beyond that the global-atomic BUCKET form is used.  The per-item weights
+// are MI355X
+// measurements on the covtype-like workload (n = 20,000, T = 500).
+constexpr double kVisitVsCompare = 8.0;
+constexpr double kWriteVsCompare = 4.0;
+constexpr double kAtomicVsCompare = 24.0;
 
 int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, void* ws,
                 size_t ws_bytes, int32_t* algo) {
@@ -493,8 +615,10 @@ int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, i
   if (rc) return rc;
   const double nn = static_cast<double>(n) * static_cast<double>(n);
   const double dense_cost = 0.5 * static_cast<double>(ntree) * nn;
-  const double gather_cost = sq * kGatherVisitVsCompare + 0.5 * nn * kGatherWriteVsCompare;
-  *algo = gather_cost < dense_cost ? RFX_ALGO_GATHER : RFX_ALGO_DENSE;
+  const bool gather = ntree <= 4096;
+  const double sparse_cost =
+      sq * (gather ? kVisitVsCompare : 0.5 * kAtomicVsCompare) + 0.5 * nn * kWriteVsCompare;
+  *algo = sparse_cost < dense_cost ? (gather ? RFX_ALGO_GATHER : RFX_ALGO_BUCKET) : RFX_ALGO_DENSE;
   return RFX_OK;
 }
 

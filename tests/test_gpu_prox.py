@@ -200,3 +200,24 @@ def test_full_size_properties_c2_shape(gpu):
         b = i * n - i * (i - 1) // 2
         assert np.array_equal(sub[b:b + n - i], rows[k, i:])
         assert np.array_equal(o_b[b:b + n - i].cpu().numpy(), rows[k, i:])
+
+
+def test_long_rows_walk_kernel(gpu):
+    """n above the line-kernel limit (32,768): the row-walk gather streams
+    each row through LDS chunks; UPPER ALL/OOB and the FULL OOB_ASYM layout
+    against the oracle, and the same through the global-atomic form."""
+    _say("walk kernel n=33000")
+    x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, 33000, 54, 7, 12, 8, mtry=7)
+    df, keys, _ = run_keys(f, x)
+    nd = O.leaf_assign(f, x)
+    acc = Accumulator(f.n, f.ntree, f.total_nodes)
+    a, o = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+    assert np.array_equal(a.cpu().numpy(), expect(f, nd, _lib.PROX_ALL, _lib.LAYOUT_UPPER))
+    want_o = expect(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
+    assert np.array_equal(o.cpu().numpy(), want_o)
+    del a
+    b = acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET)
+    assert torch.equal(b, o)
+    del b, o, want_o
+    P = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
+    assert np.array_equal(P.cpu().numpy(), expect(f, nd, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL))
