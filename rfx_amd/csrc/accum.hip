@@ -455,13 +455,18 @@ __global__ __launch_bounds__(256) void gather_walk_kernel(const uint2* __restric
     __syncthreads();
     for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
     __syncthreads();
+    // the thread's TPT runs advance one member per step, interleaved, so
+    // TPT independent loads are in flight
+    bool more = true;
+    while (more) {
+      more = false;
 #pragma unroll
-    for (int u = 0; u < TPT; ++u) {
-      const uint32_t bu = (bagm >> u) & 1u;
-      while ((nv[u] >> 1) < static_cast<uint32_t>(hi)) {
+      for (int u = 0; u < TPT; ++u) {
+        if ((nv[u] >> 1) >= static_cast<uint32_t>(hi)) continue;
+        more = true;
         const uint32_t m = nv[u];
         const int64_t j = m >> 1;
-        const uint32_t bq = m & 1u;
+        const uint32_t bq = m & 1u, bu = (bagm >> u) & 1u;
         ++cur[u];
         nv[u] = cur[u] < end[u] ? members[cur[u]] : 0xffffffffu;
         if (LAYOUT == RFX_LAYOUT_FULL && j == line) continue;

@@ -2,16 +2,23 @@
 // double-centred squared-distance operator B = -1/2 J E J, shared by the
 // dense (count-matrix) and low-rank (factor) MDS paths.
 //
-// The caller supplies E V (the un-centred product) through an operator
-// callback, plus the FP64 row means r of E and their mean g.  Then
-//   B V = -1/2 (E V - r s^T - 1 w^T + g 1 s^T),  s = 1^T V,  w = r^T V
-// (E symmetric).  Orthonormalisation is CholeskyQR2 and the k x k
-// Rayleigh-Ritz problem is solved in FP64 on the host, whatever the storage
-// precision of V (float or double).
+// The caller supplies W = E V (the un-centred product) through an operator
+// callback, plus the FP64 row means r of E and their mean g.  With s = 1^T V,
+// w = r^T V and q = w - g s the centred product is
+//   B V = Wc = -1/2 (W - r s^T - 1 q^T)          (E symmetric)
+// and is never formed: one fused pass over (V, W) per iteration gathers in
+// FP64 the Grams V^T V, V^T W, W^T W and the sums 1^T V, r^T V, 1^T W, r^T W,
+// from which the host builds H = V^T Wc and Wc^T Wc exactly.  The k x k
+// work is then FP64 on the host: generalised Rayleigh-Ritz H z = theta
+// (V^T V) z (so a slightly non-orthonormal V costs nothing), the Ritz
+// residual norms ||Wc z - theta V z||^2 = z^T (Wc^T Wc) z - theta^2 as the
+// stopping test, and the next block V <- Wc Z R^{-1} (Cholesky of
+// Z^T Wc^T Wc Z) written by one kernel.  The iteration runs on B + shift I.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <functional>
@@ -34,35 +41,112 @@ __global__ void ss_init_kernel(Real* V, int64_t n, int k, uint64_t seed) {
   V[e] = static_cast<Real>((static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0);
 }
 
-// partial[b][a*kb + c] = sum over rows of block b of A[i,a] * B[i,c]
+constexpr int kGramRows = 32;    // rows per LDS tile
+constexpr int kGramBlocks = 240; // row-range partials (about one per CU)
+
+// partial[b][a*kb + c] = sum over the rows of block b of A[i,a] * B[i,c],
+// rows staged through LDS in tiles, FP64 accumulation.
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void ss_gram_kernel(const TA* __restrict__ A, int ka, const TB* __restrict__ B,
                                                       int kb, int64_t n, int64_t rows_per_block,
                                                       double* __restrict__ partial) {
-  extern __shared__ double red[];
+  extern __shared__ double tile[];  // [kGramRows][ka + kb]
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
   const int64_t r1 = min(n, r0 + rows_per_block);
-  const int pairs = ka * kb;
-  for (int pc = threadIdx.x; pc < pairs; pc += blockDim.x) red[pc] = 0.0;
-  __syncthreads();
-  // each thread owns (a, c) pairs and walks the rows of the block
-  for (int pc = threadIdx.x; pc < pairs; pc += blockDim.x) {
-    const int a = pc / kb, c = pc % kb;
-    double s = 0.0;
-    for (int64_t i = r0; i < r1; ++i) s += static_cast<double>(A[i * ka + a]) * static_cast<double>(B[i * kb + c]);
-    red[pc] = s;
+  const int pairs = ka * kb, w = ka + kb;
+  constexpr int kMaxPerThread = 32;
+  double acc[kMaxPerThread];
+#pragma unroll
+  for (int q = 0; q < kMaxPerThread; ++q) acc[q] = 0.0;
+  for (int64_t i0 = r0; i0 < r1; i0 += kGramRows) {
+    const int rows = static_cast<int>(min<int64_t>(kGramRows, r1 - i0));
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * w; e += 256) {
+      const int q = e / w, c = e % w;
+      const int64_t i = i0 + q;
+      tile[q * w + c] = c < ka ? static_cast<double>(A[i * ka + c]) : static_cast<double>(B[i * kb + (c - ka)]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kMaxPerThread; ++s) {
+      const int pc = threadIdx.x + s * 256;
+      if (pc < pairs) {
+        const int a = pc / kb, c = ka + pc % kb;
+        double t = 0.0;
+        for (int q = 0; q < rows; ++q) t += tile[q * w + a] * tile[q * w + c];
+        acc[s] += t;
+      }
+    }
   }
-  __syncthreads();
-  for (int pc = threadIdx.x; pc < pairs; pc += blockDim.x) partial[static_cast<int64_t>(blockIdx.x) * pairs + pc] = red[pc];
+#pragma unroll
+  for (int s = 0; s < kMaxPerThread; ++s) {
+    const int pc = threadIdx.x + s * 256;
+    if (pc < pairs) partial[static_cast<int64_t>(blockIdx.x) * pairs + pc] = acc[s];
+  }
 }
 
+// out[pc] = sum over blocks of partial[b][pc]: one wavefront per entry,
+// lanes stride the blocks, fixed-order tree reduction (deterministic).
 static __global__ void ss_reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int pairs,
-                                          double* __restrict__ out) {
-  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 double* __restrict__ out) {
+  const int pc = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (pc >= pairs) return;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partial[static_cast<int64_t>(b) * pairs + pc];
-  out[pc] = s;
+  for (int b = lane; b < nblocks; b += 64) s += partial[static_cast<int64_t>(b) * pairs + pc];
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_down(s, d, 64);
+  if (lane == 0) out[pc] = s;
+}
+
+// Fused per-iteration sums over rows of [1, r, V, W]^T [V, W]: entry
+// (a, c), a < 2 + 2k (0 = ones, 1 = r, then V, then W), c < 2k (V then W).
+constexpr int kFusedPerThread = 20;
+
+template <typename Real>
+__global__ __launch_bounds__(256) void ss_fused_gram_kernel(const Real* __restrict__ V, const Real* __restrict__ W,
+                                                            const double* __restrict__ r, int k, int64_t n,
+                                                            int64_t rows_per_block, double* __restrict__ partial) {
+  extern __shared__ double tile[];  // [kGramRows][2 + 2k]
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+  const int64_t r1 = min(n, r0 + rows_per_block);
+  const int wdt = 2 + 2 * k, nc = 2 * k, pairs = wdt * nc;
+  double acc[kFusedPerThread];
+#pragma unroll
+  for (int q = 0; q < kFusedPerThread; ++q) acc[q] = 0.0;
+  for (int64_t i0 = r0; i0 < r1; i0 += kGramRows) {
+    const int rows = static_cast<int>(min<int64_t>(kGramRows, r1 - i0));
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * wdt; e += 256) {
+      const int q = e / wdt, c = e % wdt;
+      const int64_t i = i0 + q;
+      double v;
+      if (c == 0)
+        v = 1.0;
+      else if (c == 1)
+        v = r[i];
+      else if (c < 2 + k)
+        v = static_cast<double>(V[i * k + (c - 2)]);
+      else
+        v = static_cast<double>(W[i * k + (c - 2 - k)]);
+      tile[q * wdt + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kFusedPerThread; ++s) {
+      const int pc = threadIdx.x + s * 256;
+      if (pc < pairs) {
+        const int a = pc / nc, c = 2 + pc % nc;
+        double t = 0.0;
+        for (int q = 0; q < rows; ++q) t += tile[q * wdt + a] * tile[q * wdt + c];
+        acc[s] += t;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kFusedPerThread; ++s) {
+    const int pc = threadIdx.x + s * 256;
+    if (pc < pairs) partial[static_cast<int64_t>(blockIdx.x) * pairs + pc] = acc[s];
+  }
 }
 
 // out[i, :] = V[i, :] * M   (M k x kout, FP64, row-major)
@@ -78,25 +162,26 @@ __global__ void ss_rotate_kernel(const Real* __restrict__ V, int k, const double
   out[e] = static_cast<Real>(s);
 }
 
-// W <- -1/2 (W - r s^T - 1 w^T + g 1 s^T) + shift V; sw = [s; w] (2 x k, FP64)
+// Next block from the centred product without forming it:
+//   V[i, c] = sum_a W[i,a] M[a,c] + r_i ar[c] + ao[c] + sum_a Vold[i,a] Ms[a,c]
+// with prm = [M (k*k), ar (k), ao (k), Ms (k*k)]; one thread per (i, c).
 template <typename Real>
-__global__ void ss_center_kernel(Real* __restrict__ W, const Real* __restrict__ V, const double* __restrict__ r,
-                                 const double* __restrict__ sw, double g, double shift, int64_t n, int k) {
+__global__ void ss_update_kernel(const Real* __restrict__ W, const Real* __restrict__ Vold, Real* __restrict__ Vnew,
+                                 const double* __restrict__ r, const double* __restrict__ prm, int k, int64_t n,
+                                 int with_shift) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n * k) return;
   const int64_t i = e / k;
   const int c = static_cast<int>(e % k);
-  const double s = sw[c], w = sw[k + c];
-  const double v = static_cast<double>(W[e]) - r[i] * s - w + g * s;
-  W[e] = static_cast<Real>(-0.5 * v + shift * static_cast<double>(V[e]));
-}
-
-// [1, r_i] rows for the s and w column sums
-static __global__ void ss_ones_r_kernel(const double* __restrict__ r, int64_t n, double* __restrict__ out) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[2 * i] = 1.0;
-  out[2 * i + 1] = r[i];
+  const double* M = prm;
+  const double* ar = prm + k * k;
+  const double* ao = ar + k;
+  const double* Ms = ao + k;
+  double s = r[i] * ar[c] + ao[c];
+  for (int a = 0; a < k; ++a) s += static_cast<double>(W[i * k + a]) * M[a * k + c];
+  if (with_shift)
+    for (int a = 0; a < k; ++a) s += static_cast<double>(Vold[i * k + a]) * Ms[a * k + c];
+  Vnew[e] = static_cast<Real>(s);
 }
 
 // X[i, c] = Vtop[i, c] * sqrt(lambda_c) in FP64, row-major n x kout
@@ -113,10 +198,9 @@ __global__ void ss_coords_kernel(const Real* __restrict__ V, int k, const double
 struct SubspaceResult {
   std::vector<double> eigvals;  // Ritz values of B, descending
   int iterations = 0;
-  double residual = 0.0;  // max relative Ritz-value change of the wanted ones at exit
+  double residual = 0.0;  // max relative Ritz residual of the wanted pairs at exit
 };
 
-// Device scratch for the solver (all allocated by the caller's workspace).
 template <typename Real>
 class SubspaceSolver {
  public:
@@ -124,10 +208,15 @@ class SubspaceSolver {
 
   SubspaceSolver(hipStream_t st, int64_t n, int k) : st_(st), n_(n), k_(k) {}
 
+  static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+  static int pairs_max(int k) {
+    const int fused = (2 + 2 * k) * 2 * k;
+    return fused > k * k ? fused : k * k;
+  }
+
   static size_t workspace_bytes(int64_t n, int k) {
-    const int64_t nb = gram_blocks(n);
-    return al(sizeof(Real) * n * k) * 3 + al(8 * 2 * n) + al(8 * nb * k * k) + al(8 * k * k) * 2 + al(8 * 2 * k) +
-           al(8 * k);
+    return al(sizeof(Real) * n * k) * 3 + al(8 * static_cast<size_t>(kGramBlocks) * pairs_max(k)) +
+           al(8 * static_cast<size_t>(pairs_max(k))) + al(8 * static_cast<size_t>(2 * k * k + 2 * k)) + al(8 * k);
   }
 
   void carve(void* ws) {
@@ -138,91 +227,19 @@ class SubspaceSolver {
     p += al(sizeof(Real) * n_ * k_);
     T_ = reinterpret_cast<Real*>(p);
     p += al(sizeof(Real) * n_ * k_);
-    onesr_ = reinterpret_cast<double*>(p);
-    p += al(8 * 2 * n_);
     partial_ = reinterpret_cast<double*>(p);
-    p += al(8 * gram_blocks(n_) * k_ * k_);
+    p += al(8 * static_cast<size_t>(kGramBlocks) * pairs_max(k_));
     kk_ = reinterpret_cast<double*>(p);
-    p += al(8 * k_ * k_);
+    p += al(8 * static_cast<size_t>(pairs_max(k_)));
     M_ = reinterpret_cast<double*>(p);
-    p += al(8 * k_ * k_);
-    sw_ = reinterpret_cast<double*>(p);
-    p += al(8 * 2 * k_);
+    p += al(8 * static_cast<size_t>(2 * k_ * k_ + 2 * k_));
     scale_ = reinterpret_cast<double*>(p);
   }
 
-  // Top `want` eigenpairs of B; coords (device, FP64 n x want) = V sqrt(lambda).
-  // The iteration runs on B + shift I (a positive shift moves large negative
-  // eigenvalues of a non-Euclidean B out of the way of the positive ones);
-  // reported eigenvalues are those of B.
-  int run(const Apply& apply, const double* r_dev, double g, int want, int max_iter, double tol, uint64_t seed,
-          double* coords, SubspaceResult& res, double shift = 0.0) {
-    shift_ = shift;
-    const unsigned gb = rfx_ceil_div(n_ * k_, 256);
-    hipLaunchKernelGGL(ss_init_kernel<Real>, dim3(gb), dim3(256), 0, st_, V_, n_, k_, seed);
-    hipLaunchKernelGGL(ss_ones_r_kernel, dim3(rfx_ceil_div(n_, 256)), dim3(256), 0, st_, r_dev, n_, onesr_);
-    int rc = orthonormalize(V_);
-    if (rc) return rc;
-    std::vector<double> prev(want, 0.0), H, w, Z;
-    res.residual = 1.0;
-    int it = 0;
-    for (; it < max_iter; ++it) {
-      rc = apply_b(apply, r_dev, g, V_, W_);
-      if (rc) return rc;
-      rc = gram(V_, k_, W_, k_, H);  // H = V^T B V
-      if (rc) return rc;
-      for (int a = 0; a < k_; ++a)
-        for (int c = a + 1; c < k_; ++c) {
-          const double m = 0.5 * (H[a * k_ + c] + H[c * k_ + a]);
-          H[a * k_ + c] = H[c * k_ + a] = m;
-        }
-      jacobi_eigh(H, k_, w, Z);
-      // V <- orth(W Z): the rotated block B V Z spans the next subspace
-      rc = upload(M_, Z.data(), static_cast<size_t>(k_) * k_);
-      if (rc) return rc;
-      hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(gb), dim3(256), 0, st_, W_, k_, M_, k_, n_, V_);
-      rc = orthonormalize(V_);
-      if (rc) return rc;
-      double change = 0.0;
-      for (int c = 0; c < want; ++c) change = std::max(change, std::fabs(w[c] - prev[c]) / std::max(std::fabs(w[c]), 1e-300));
-      prev.assign(w.begin(), w.begin() + want);
-      res.residual = change;
-      if (it >= 2 && change < tol) {
-        ++it;
-        break;
-      }
-    }
-    // final Rayleigh-Ritz on the converged block
-    rc = apply_b(apply, r_dev, g, V_, W_);
-    if (rc) return rc;
-    rc = gram(V_, k_, W_, k_, H);
-    if (rc) return rc;
-    for (int a = 0; a < k_; ++a)
-      for (int c = a + 1; c < k_; ++c) {
-        const double m = 0.5 * (H[a * k_ + c] + H[c * k_ + a]);
-        H[a * k_ + c] = H[c * k_ + a] = m;
-      }
-    jacobi_eigh(H, k_, w, Z);
-    for (double& v : w) v -= shift_;
-    res.eigvals = w;
-    res.iterations = it;
-    std::vector<double> Zw(static_cast<size_t>(k_) * want), sc(want);
-    for (int a = 0; a < k_; ++a)
-      for (int c = 0; c < want; ++c) Zw[a * want + c] = Z[a * k_ + c];
-    for (int c = 0; c < want; ++c) sc[c] = w[c] > 0 ? std::sqrt(w[c]) : 0.0;
-    rc = upload(M_, Zw.data(), static_cast<size_t>(k_) * want);
-    if (rc) return rc;
-    rc = upload(scale_, sc.data(), want);
-    if (rc) return rc;
-    hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, V_, k_, M_,
-                       want, n_, T_);
-    hipLaunchKernelGGL(ss_coords_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, T_, want,
-                       scale_, want, n_, coords);
-    RFX_HIP_TRY(hipStreamSynchronize(st_));
-    return rfx_launch_status();
-  }
-
-  static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+  Real* block_v() { return V_; }
+  Real* block_w() { return W_; }
+  Real* block_t() { return T_; }
+  double* small_matrix() { return M_; }
 
   // Small host -> device upload; waits so the host buffer may be reused.
   int upload(double* dst, const double* src, size_t count) {
@@ -231,19 +248,16 @@ class SubspaceSolver {
     return RFX_OK;
   }
 
-  static int64_t gram_blocks(int64_t n) {
-    int64_t b = (n + 511) / 512;
-    return b < 1 ? 1 : (b > 2048 ? 2048 : b);
-  }
-
   template <typename TA, typename TB>
   int gram(const TA* A, int ka, const TB* B, int kb, std::vector<double>& out) {
-    const int64_t nb = gram_blocks(n_);
-    const int64_t rpb = (n_ + nb - 1) / nb;
     const int pairs = ka * kb;
-    hipLaunchKernelGGL((ss_gram_kernel<TA, TB>), dim3(static_cast<unsigned>(nb)), dim3(256), 8 * pairs, st_, A, ka,
-                       B, kb, n_, rpb, partial_);
-    hipLaunchKernelGGL(ss_reduce_partials_kernel, dim3(rfx_ceil_div(pairs, 256)), dim3(256), 0, st_, partial_,
+    if (pairs > 32 * 256 || (ka + kb) * kGramRows * 8 > 64 * 1024 || pairs > pairs_max(k_))
+      return RFX_ERR_UNSUPPORTED;
+    const int64_t nb = std::min<int64_t>(kGramBlocks, (n_ + kGramRows - 1) / kGramRows);
+    const int64_t rpb = (n_ + nb - 1) / nb;
+    hipLaunchKernelGGL((ss_gram_kernel<TA, TB>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                       8 * kGramRows * (ka + kb), st_, A, ka, B, kb, n_, rpb, partial_);
+    hipLaunchKernelGGL(ss_reduce_partials_kernel, dim3(rfx_ceil_div(pairs, 4)), dim3(256), 0, st_, partial_,
                        static_cast<int>(nb), pairs, kk_);
     out.resize(pairs);
     RFX_HIP_TRY(hipMemcpyAsync(out.data(), kk_, 8 * pairs, hipMemcpyDeviceToHost, st_));
@@ -251,21 +265,16 @@ class SubspaceSolver {
     return rfx_launch_status();
   }
 
-  Real* block_v() { return V_; }
-  Real* block_w() { return W_; }
-  Real* block_t() { return T_; }
-  double* small_matrix() { return M_; }
-
   // CholeskyQR2 in FP64: twice V <- V R^{-1} with V^T V = R^T R.
   int orthonormalize(Real* V) {
     std::vector<double> G;
     for (int pass = 0; pass < 2; ++pass) {
       int rc = gram(V, k_, V, k_, G);
       if (rc) return rc;
+      double tr = 0;
+      for (int a = 0; a < k_; ++a) tr += G[a * k_ + a];
       if (!cholesky_lower(G, k_)) {
         // rank-deficient block: shift the Gram diagonal and retry once
-        double tr = 0;
-        for (int a = 0; a < k_; ++a) tr += G[a * k_ + a];
         rc = gram(V, k_, V, k_, G);
         if (rc) return rc;
         for (int a = 0; a < k_; ++a) G[a * k_ + a] += 1e-12 * (tr > 0 ? tr : 1.0);
@@ -281,26 +290,190 @@ class SubspaceSolver {
     return rfx_launch_status();
   }
 
- private:
-  int apply_b(const Apply& apply, const double* r_dev, double g, const Real* V, Real* W) {
-    int rc = apply(V, W, k_);
+  // Top `want` eigenpairs of B; coords (device, FP64 n x want) = V sqrt(lambda).
+  // The iteration runs on B + shift I (a positive shift moves large negative
+  // eigenvalues of a non-Euclidean B out of the way of the positive ones);
+  // reported eigenvalues are those of B.  With start_from_v the block already
+  // in block_v() is the starting subspace (a refinement sweep).
+  int run(const Apply& apply, const double* r_dev, double g, int want, int max_iter, double tol, uint64_t seed,
+          double* coords, SubspaceResult& res, double shift = 0.0, bool start_from_v = false) {
+    const int k = k_;
+    if (k > 32 || want > k) return RFX_ERR_UNSUPPORTED;
+    if (!start_from_v)
+      hipLaunchKernelGGL(ss_init_kernel<Real>, dim3(rfx_ceil_div(n_ * k, 256)), dim3(256), 0, st_, V_, n_, k, seed);
+    int rc = orthonormalize(V_);
     if (rc) return rc;
-    std::vector<double> sw;
-    rc = gram(onesr_, 2, V, k_, sw);  // rows: s (ones^T V), w (r^T V)
-    if (rc) return rc;
-    rc = upload(sw_, sw.data(), 2 * static_cast<size_t>(k_));
-    if (rc) return rc;
-    hipLaunchKernelGGL(ss_center_kernel<Real>, dim3(rfx_ceil_div(n_ * k_, 256)), dim3(256), 0, st_, W, V, r_dev, sw_,
-                       g, shift_, n_, k_);
-    return rfx_launch_status();
+    // constants of the centring algebra: r^T r and 1^T r
+    double rr0 = 0.0, rr1 = 0.0;
+    {
+      std::vector<double> rh(n_);
+      RFX_HIP_TRY(hipMemcpyAsync(rh.data(), r_dev, 8 * n_, hipMemcpyDeviceToHost, st_));
+      RFX_HIP_TRY(hipStreamSynchronize(st_));
+      for (int64_t i = 0; i < n_; ++i) {
+        rr0 += rh[i] * rh[i];
+        rr1 += rh[i];
+      }
+    }
+    const int wdt = 2 + 2 * k, nc = 2 * k, pairs = wdt * nc;
+    const int64_t nb = std::min<int64_t>(kGramBlocks, (n_ + kGramRows - 1) / kGramRows);
+    const int64_t rpb = (n_ + nb - 1) / nb;
+    const size_t kk = static_cast<size_t>(k) * k;
+    std::vector<double> S(pairs), Gvv(kk), Gvw(kk), Gww(kk), sV(k), wV(k), sW(k), wW(k), q(k);
+    std::vector<double> H(kk), Gc(kk), theta, Y, Z(kk), prm(2 * kk + 2 * k);
+    const double sh = shift;
+    res.residual = 1.0;
+    for (int it = 0;; ++it) {
+      rc = apply(V_, W_, k);
+      if (rc) return rc;
+      hipLaunchKernelGGL((ss_fused_gram_kernel<Real>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                         8 * kGramRows * wdt, st_, V_, W_, r_dev, k, n_, rpb, partial_);
+      hipLaunchKernelGGL(ss_reduce_partials_kernel, dim3(rfx_ceil_div(pairs, 4)), dim3(256), 0, st_, partial_,
+                         static_cast<int>(nb), pairs, kk_);
+      RFX_HIP_TRY(hipMemcpyAsync(S.data(), kk_, 8 * pairs, hipMemcpyDeviceToHost, st_));
+      RFX_HIP_TRY(hipStreamSynchronize(st_));
+      rc = rfx_launch_status();
+      if (rc) return rc;
+      for (int c = 0; c < k; ++c) {
+        sV[c] = S[c];
+        sW[c] = S[k + c];
+        wV[c] = S[nc + c];
+        wW[c] = S[nc + k + c];
+      }
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          Gvv[a * k + c] = S[(2 + a) * nc + c];
+          Gvw[a * k + c] = S[(2 + a) * nc + k + c];
+          Gww[a * k + c] = S[(2 + k + a) * nc + k + c];
+        }
+      for (int c = 0; c < k; ++c) q[c] = wV[c] - g * sV[c];
+      // H = V^T (Wc + sh V), Wc = -1/2 (W - r s^T - 1 q^T)
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c)
+          H[a * k + c] = -0.5 * (Gvw[a * k + c] - wV[a] * sV[c] - sV[a] * q[c]) + sh * Gvv[a * k + c];
+      for (int a = 0; a < k; ++a)
+        for (int c = a + 1; c < k; ++c) H[a * k + c] = H[c * k + a] = 0.5 * (H[a * k + c] + H[c * k + a]);
+      // Gc = (Wc + sh V)^T (Wc + sh V)
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double v = Gww[a * k + c] - wW[a] * sV[c] - sW[a] * q[c] - sV[a] * wW[c] + sV[a] * rr0 * sV[c] +
+                     sV[a] * rr1 * q[c] - q[a] * sW[c] + q[a] * rr1 * sV[c] +
+                     q[a] * static_cast<double>(n_) * q[c];
+          v *= 0.25;
+          if (sh != 0.0) {
+            const double vwc_ac = -0.5 * (Gvw[a * k + c] - wV[a] * sV[c] - sV[a] * q[c]);
+            const double vwc_ca = -0.5 * (Gvw[c * k + a] - wV[c] * sV[a] - sV[c] * q[a]);
+            v += sh * (vwc_ac + vwc_ca) + sh * sh * Gvv[a * k + c];
+          }
+          Gc[a * k + c] = v;
+        }
+      // generalised Rayleigh-Ritz: L L^T = V^T V, eig((L^T)^-T H (L^T)^-1)
+      std::vector<double> L = Gvv;
+      if (!cholesky_lower(L, k)) return RFX_ERR_INVALID;
+      const std::vector<double> Li = inv_upper_from_lower(L, k);
+      std::vector<double> tmp(kk, 0.0), Ht(kk, 0.0);
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += H[a * k + b] * Li[b * k + c];
+          tmp[a * k + c] = s;
+        }
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += Li[b * k + a] * tmp[b * k + c];
+          Ht[a * k + c] = s;
+        }
+      jacobi_eigh(Ht, k, theta, Y);
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += Li[a * k + b] * Y[b * k + c];
+          Z[a * k + c] = s;
+        }
+      // Ritz residuals ||(B + sh) x - theta x||^2 = z^T Gc z - theta^2
+      double worst = 0.0;
+      for (int c = 0; c < want; ++c) {
+        double zgz = 0;
+        for (int a = 0; a < k; ++a)
+          for (int b = 0; b < k; ++b) zgz += Z[a * k + c] * Gc[a * k + b] * Z[b * k + c];
+        const double r2 = std::max(zgz - theta[c] * theta[c], 0.0);
+        worst = std::max(worst, std::sqrt(r2) / std::max(std::fabs(theta[c] - sh), 1e-300));
+      }
+      res.residual = worst;
+      if ((it >= 1 && worst < tol) || it + 1 >= max_iter) {
+        // Ritz vectors X = V Z (orthonormal: Z^T Gvv Z = I), scaled by sqrt(lambda)
+        std::vector<double> Zw(static_cast<size_t>(k) * want), sc(want);
+        for (int a = 0; a < k; ++a)
+          for (int c = 0; c < want; ++c) Zw[a * want + c] = Z[a * k + c];
+        res.eigvals.resize(k);
+        for (int c = 0; c < k; ++c) res.eigvals[c] = theta[c] - sh;
+        for (int c = 0; c < want; ++c) sc[c] = res.eigvals[c] > 0 ? std::sqrt(res.eigvals[c]) : 0.0;
+        rc = upload(M_, Zw.data(), static_cast<size_t>(k) * want);
+        if (rc) return rc;
+        rc = upload(scale_, sc.data(), want);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, V_, k, M_,
+                           want, n_, T_);
+        hipLaunchKernelGGL(ss_coords_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, T_, want,
+                           scale_, want, n_, coords);
+        RFX_HIP_TRY(hipStreamSynchronize(st_));
+        res.iterations = it + 1;
+        return rfx_launch_status();
+      }
+      // next block: (Wc + sh V) Z R^{-1}, R^T R = Z^T Gc Z
+      std::vector<double> t2(kk, 0.0), Gz(kk, 0.0);
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += Gc[a * k + b] * Z[b * k + c];
+          t2[a * k + c] = s;
+        }
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += Z[b * k + a] * t2[b * k + c];
+          Gz[a * k + c] = s;
+        }
+      for (int a = 0; a < k; ++a)
+        for (int c = a + 1; c < k; ++c) Gz[a * k + c] = Gz[c * k + a] = 0.5 * (Gz[a * k + c] + Gz[c * k + a]);
+      if (!cholesky_lower(Gz, k)) return RFX_ERR_INVALID;
+      const std::vector<double> Ri = inv_upper_from_lower(Gz, k);
+      std::vector<double> Mz(kk, 0.0);
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          double s = 0;
+          for (int b = 0; b < k; ++b) s += Z[a * k + b] * Ri[b * k + c];
+          Mz[a * k + c] = s;
+        }
+      // V' = -1/2 W Mz + 1/2 r (s^T Mz) + 1/2 1 (q^T Mz) + sh V Mz
+      for (int a = 0; a < k; ++a)
+        for (int c = 0; c < k; ++c) {
+          prm[a * k + c] = -0.5 * Mz[a * k + c];
+          prm[kk + 2 * k + a * k + c] = sh * Mz[a * k + c];
+        }
+      for (int c = 0; c < k; ++c) {
+        double sm = 0, qm = 0;
+        for (int a = 0; a < k; ++a) {
+          sm += sV[a] * Mz[a * k + c];
+          qm += q[a] * Mz[a * k + c];
+        }
+        prm[kk + c] = 0.5 * sm;
+        prm[kk + k + c] = 0.5 * qm;
+      }
+      rc = upload(M_, prm.data(), prm.size());
+      if (rc) return rc;
+      hipLaunchKernelGGL(ss_update_kernel<Real>, dim3(rfx_ceil_div(n_ * k, 256)), dim3(256), 0, st_, W_, V_, T_,
+                         r_dev, M_, k, n_, sh != 0.0 ? 1 : 0);
+      RFX_HIP_TRY(hipMemcpyAsync(V_, T_, sizeof(Real) * n_ * k, hipMemcpyDeviceToDevice, st_));
+    }
   }
 
+ private:
   hipStream_t st_;
   int64_t n_;
   int k_;
-  double shift_ = 0.0;
   Real *V_ = nullptr, *W_ = nullptr, *T_ = nullptr;
-  double *onesr_ = nullptr, *partial_ = nullptr, *kk_ = nullptr, *M_ = nullptr, *sw_ = nullptr, *scale_ = nullptr;
+  double *partial_ = nullptr, *kk_ = nullptr, *M_ = nullptr, *scale_ = nullptr;
 };
 
 }  // namespace rfx

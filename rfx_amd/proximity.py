@@ -149,7 +149,7 @@ def mds_dense(counts: torch.Tensor, layout: int, n: int, mode: int = _lib.MDS_CO
     import ctypes as C
 
     if tol is None:
-        tol = 1e-14 if precision == _lib.PREC_FP64 else 1e-7
+        tol = 1e-10 if precision == _lib.PREC_FP64 else 1e-5
     nb = lib().rfx_mds_dense_workspace_bytes(n, precision)
     ws = torch.empty(nb, dtype=torch.uint8, device=counts.device)
     coords = torch.empty((n, 3), dtype=torch.float64, device=counts.device)
@@ -194,7 +194,7 @@ def mds_factors(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, precision: int
 
     n, r = Uq.shape
     if tol is None:
-        tol = 1e-14 if precision == _lib.PREC_FP64 else 1e-7
+        tol = 1e-10 if precision == _lib.PREC_FP64 else 1e-5
     nb = lib().rfx_mds_factors_workspace_bytes(n, r, precision)
     ws = torch.empty(nb, dtype=torch.uint8, device=Uq.device)
     coords = torch.empty((n, 3), dtype=torch.float64, device=Uq.device)
