@@ -8,7 +8,7 @@ problem: n = 20,000 samples, p = 54 features, 7 classes, T = 500 trees,
 mtry = 7, forest grown on the host by the restated reference grower (growth
 is setup, not timed).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1] [--algo auto|dense|bucket]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--algo auto|dense|bucket|gather]
 
 value = T * n(n+1)/2 tree-pair updates per second (each tree-pair slot counts
 once although it feeds both matrices).  The line carries the roofline of the
@@ -40,6 +40,10 @@ WORKLOADS = {
     "c3": (1, 200000, 54, 7, 1000, 7, "covtype-like n=200000 p=54 T=1000 mtry=7: leaf assignment + ALL and OOB "
                                       "upper-triangle int32 counts; rank-64 factors; factor 3-D MDS per storage "
                                       "format"),
+    "c5": (0, 0, 0, 0, 0, 0, "64 independent forests (instance i: seed i, n log-uniform in [1000, 50000], p "
+                             "uniform in [8, 64], T uniform in [50, 500], 2-10 classes, Gaussian mixture, "
+                             "mtry = floor(sqrt(p))): leaf assignment + OOB upper-triangle int32 counts of all 64 "
+                             "as one batch"),
 }
 
 
@@ -84,6 +88,7 @@ def main() -> int:
     ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket", "gather"])
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mds", action="store_true", help="c5: also time the 3-D MDS of every instance")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the proximity path runs on a single MI355X here "
@@ -97,6 +102,8 @@ def main() -> int:
     from rfx_amd.forest import Forest, make_dataset
     from rfx_amd.proximity import Accumulator, DeviceForest
 
+    if args.workload == "c5":
+        return bench_c5(args)
     NAMES = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket", _lib.ALGO_GATHER: "gather"}
     kind, n, p, ncl, T, mtry, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -266,6 +273,159 @@ def factor_mds_section(df, forest, rank: int) -> dict:
     out["subset_20k"] = {"dense_fp64_eigs": [float(v) for v in ed[:4]], "factor_bf16_eigs": [float(v) for v in es[:4]],
                          "rel_err_vs_dense_fp64": rel, "max_angle_rad": ang}
     return out
+
+
+def c5_instances(count: int = 64):
+    """Instance i of the 64-forest workload, from seed i."""
+    import numpy as np
+
+    out = []
+    for i in range(count):
+        r = np.random.default_rng(i)
+        n = int(round(np.exp(r.uniform(np.log(1000), np.log(50000)))))
+        p = int(r.integers(8, 65))
+        T = int(r.integers(50, 501))
+        ncl = int(r.integers(2, 11))
+        out.append((i, n, p, T, ncl))
+    return out
+
+
+def bench_c5(args) -> int:
+    import numpy as np
+    import torch
+
+    from rfx_amd import _lib
+    from rfx_amd.batch import ForestBatch
+    from rfx_amd.forest import Forest, default_mtry, make_dataset
+    from rfx_amd.proximity import Accumulator, DeviceForest
+
+    inst = c5_instances()
+    t0 = time.perf_counter()
+    forests, xs, dfs = [], [], []
+    for i, n, p, T, ncl in inst:
+        x, y = make_dataset(_lib.DATA_GAUSS_MIXTURE, n, p, ncl, i)
+        f = Forest.grow(x, y, T, mtry=default_mtry(p), iseed=i)
+        forests.append(f)
+        xs.append(x)
+        dfs.append(DeviceForest(f, x))
+        if i % 16 == 15:
+            log(f"{i + 1} forests grown, {time.perf_counter() - t0:.1f} s")
+    log(f"64 forests grown in {time.perf_counter() - t0:.1f} s")
+    work = sum(f.ntree * (f.n * (f.n + 1) // 2) for f in forests)
+
+    batch = ForestBatch(dfs, "oob_upper")
+    best_ms = batch.autotune(reps=1)
+    lpi, bpc = batch.tuning()
+    log(f"autotune: {lpi} rows per item, {bpc} workgroups per CU, {best_ms:.2f} ms")
+
+    # 64 sequential launches of the one-problem path (GATHER), same outputs
+    seq = []
+    for k, (df, f) in enumerate(zip(dfs, forests)):
+        keys = torch.empty((f.ntree, f.n), dtype=torch.int32, device=batch.device)
+        seq.append((df, Accumulator(f.n, f.ntree, f.total_nodes), keys, batch.out[k]))
+
+    def run_seq():
+        for df, acc, keys, out in seq:
+            df.leaf_assign(keys=keys)
+            acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER, out)
+
+    run_seq()
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in batch.out[:4]]
+    s = time.perf_counter()
+    run_seq()
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - s
+    del seq
+    torch.cuda.empty_cache()
+
+    for _ in range(args.warmup):
+        batch.run(use_graph=True)
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert torch.equal(ref[k], batch.out[k]), "batched and sequential counts differ"
+    del ref
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_start = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        batch.run(use_graph=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_step = wall * 1e3 / args.steps
+    ev_ms = e0.elapsed_time(e1) / args.steps
+    value = work / (ms_step / 1e3)
+    P_total = sum(f.n * (f.n + 1) // 2 for f in forests)
+    nT_total = sum(f.n * f.ntree for f in forests)
+    step_bytes = 4 * P_total + 4 * nT_total
+    achieved = step_bytes / (ev_ms / 1e3) / 1e9
+    line = {
+        "metric": METRIC, "value": value, "unit": "tree·pair updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["c5"][6], "instances": len(forests), "tree_pair_updates": work,
+                   "sum_n": int(sum(f.n for f in forests)), "layout": "upper", "graph": True,
+                   "rows_per_item": lpi, "workgroups_per_cu": bpc,
+                   "sequential_s": round(seq_s, 5), "batched_s": round(ms_step / 1e3, 5),
+                   "speedup_vs_sequential": round(seq_s / (ms_step / 1e3), 3),
+                   "sequential_value": work / seq_s},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched step (leaf + segmented sort + persistent gather), HIP events",
+                     "algorithmic_bytes": step_bytes, "avg_ms": round(ev_ms, 4)},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = c5_cpu_baseline(forests, xs)
+    if args.mds:
+        line["mds"] = c5_mds(batch, forests)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def c5_cpu_baseline(forests, xs, budget_s: float = 15.0) -> dict:
+    """The oracle port of the reference CPU path on whole instances, smallest
+    first, until the time budget is spent."""
+    from oracle import oracle as O
+
+    order = sorted(range(len(forests)), key=lambda k: forests[k].n)
+    done, work, dt = [], 0, 0.0
+    for k in order:
+        f = forests[k]
+        if dt > budget_s:
+            break
+        t0 = time.perf_counter()
+        nd = O.leaf_assign(f, xs[k])
+        prox = O.prox_v1(f, nd)
+        dt += time.perf_counter() - t0
+        del prox, nd
+        work += f.ntree * (f.n * (f.n + 1) // 2)
+        done.append(k)
+    return {"value": work / dt, "unit": "tree·pair updates/s", "cores": int(O.num_threads()), "kind": "port",
+            "sample": f"{len(done)} smallest instances whole (n {forests[done[0]].n}..{forests[done[-1]].n}): oracle "
+                      f"leaf assignment + rf::cpu_proximity dense OpenMP branch (src/rf_proximity.cpp:99-231), "
+                      f"one instance after another; {dt:.2f} s"}
+
+
+def c5_mds(batch, forests) -> dict:
+    """The reference chain's 3-D MDS (OOB counts -> finishprox -> float32 ->
+    renormalisation -> classical MDS) of every instance, FP32 operator."""
+    import torch
+
+    from rfx_amd import _lib
+    from rfx_amd.proximity import mds_dense
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for f, out in zip(forests, batch.out):
+        nout = torch.from_numpy(f.nout(2)).to(out.device)
+        _, _, it = mds_dense(out, _lib.LAYOUT_UPPER, f.n, _lib.MDS_REFERENCE, 1.0, nout, precision=_lib.PREC_FP32)
+        iters.append(it)
+    torch.cuda.synchronize()
+    return {"total_s": round(time.perf_counter() - t0, 4), "mean_iters": float(sum(iters) / len(iters)),
+            "mode": "reference chain, FP32 operator, FP64 Rayleigh-Ritz"}
 
 
 if __name__ == "__main__":

@@ -176,15 +176,23 @@ int rfx_prox_pair_stats(hipStream_t stream, const int32_t* keys, int64_t n, int3
 int rfx_finishprox(hipStream_t stream, const int32_t* counts, int32_t in_layout, const int32_t* nout, int64_t n,
                    double* proxsym, int32_t out_layout);
 
+/* A5 on an FP64 accumulated matrix, as rf::cpu_finishprox receives it
+ * (include/rf_finishprox.hpp:9-12): prox FULL column-major float64 (any
+ * values, e.g. case-wise weights), same arithmetic, bit-identical. */
+int rfx_finishprox_f64(hipStream_t stream, const double* prox, const int32_t* nout, int64_t n, double* proxsym,
+                       int32_t out_layout);
+
 /* ---- MDS ---------------------------------------------------------------- */
 #define RFX_MDS_COUNTS 0    /* M_ij = c_ij * scale (e.g. ALL counts / T)       */
 #define RFX_MDS_REFERENCE 1 /* OOB_UPPER counts -> finishprox -> float32 -> nout renormalisation */
+#define RFX_MDS_MATRIX 2    /* a given FP64 proximity matrix (rfx_mds_dense_matrix) */
 
 #define RFX_PREC_FP32 0
 #define RFX_PREC_FP64 1
 #define RFX_PREC_BF16 2
 #define RFX_PREC_INT8 3
 #define RFX_PREC_FP8 4 /* FP8 e4m3 (OCP) */
+#define RFX_PREC_FP16 5 /* IEEE half, only as a count-conversion target */
 
 /* A6+A7: classical 3-D MDS of a count-matrix proximity.  Replaces
  * rf::compute_mds_3d_cpu (include/rf_mds_cpu.hpp:27-33, src/rf_mds_cpu.cpp:
@@ -208,6 +216,23 @@ size_t rfx_mds_dense_workspace_bytes(int64_t n, int32_t precision);
 int rfx_mds_dense(hipStream_t stream, const int32_t* counts, int32_t layout, int64_t n, int32_t mode, double scale,
                   const int32_t* nout, int32_t precision, int32_t max_iter, double tol, void* workspace,
                   size_t workspace_bytes, double* coords, double* eigvals_host, int32_t* iters_host);
+
+/* A6+A7 on a given FP64 proximity: replaces rf::compute_mds_3d_cpu
+ * (include/rf_mds_cpu.hpp:27-33) for its non-RF-GAP inputs.  prox is a
+ * device FP64 n x n column-major matrix (the finishprox output S, or any
+ * symmetric proximity).  With nout (device int32 [n]) the reference's
+ * renormalisation applies (src/rf_mds_cpu.cpp:154-186): M_ij = 1/2
+ * (S_ij/nout_i + S_ji/nout_j), rows with nout_i = 0 undivided, M_ii = 1;
+ * with nout = NULL, M = S.  Then as rfx_mds_dense (same workspace size). */
+int rfx_mds_dense_matrix(hipStream_t stream, const double* prox, int64_t n, const int32_t* nout, int32_t precision,
+                         int32_t max_iter, double tol, void* workspace, size_t workspace_bytes, double* coords,
+                         double* eigvals_host, int32_t* iters_host);
+
+/* Element-wise conversion of int32 counts to FP64, FP32 or FP16 (IEEE
+ * round-to-nearest-even; exact up to 2048 for FP16).  The reference keeps
+ * counts as FP64 (src/rf_proximity.cpp:179-230) or FP16
+ * (cuda/rf_proximity_upper_triangle.cu:149-306). */
+int rfx_counts_convert(hipStream_t stream, const int32_t* counts, int64_t len, int32_t format, void* out);
 
 /* A8: low-rank factors of the ALL-count proximity P = C/T (PSD): U (n x rank,
  * FP32 row-major) with P ~ U U^T from randomized subspace iteration whose
@@ -240,6 +265,54 @@ size_t rfx_mds_factors_workspace_bytes(int64_t n, int32_t rank, int32_t precisio
 int rfx_mds_factors(hipStream_t stream, const void* Uq, const float* scales, int32_t format, int64_t n, int32_t rank,
                     int32_t precision, int32_t max_iter, double tol, void* workspace, size_t workspace_bytes,
                     double* coords, double* eigvals_host, int32_t* iters_host);
+
+/* ========================================================================
+ * A10: batches of independent forests
+ * ======================================================================== */
+/* Batch-only variant: ALL counts into out and OOB_UPPER counts into out2. */
+#define RFX_PROX_PAIR 3
+
+/* One problem of a batch: an all-numeric forest (categorical features are
+ * not supported here) and its UPPER int32 output(s), all device pointers. */
+typedef struct rfx_batch_problem {
+  const float* x_col;      /* p x n float32 column-major */
+  int64_t n;
+  int32_t p;
+  int32_t ntree;
+  const uint32_t* nodes;   /* packed node records (rfx_pack_nodes) */
+  const int64_t* node_off; /* int64 [T+1], offsets within this forest */
+  int64_t total_nodes;     /* node_off[T] */
+  const uint32_t* inbag;   /* in-bag bitmask (rfx_pack_inbag_bits) */
+  int32_t* out;            /* UPPER n(n+1)/2 int32, overwritten */
+  int32_t* out2;           /* UPPER, OOB_UPPER counts for RFX_PROX_PAIR, else NULL */
+} rfx_batch_problem;
+
+typedef struct rfx_batch rfx_batch;
+
+/* Plan a batch: leaf assignment and co-occurrence counts (variant ALL,
+ * OOB_UPPER or PAIR; UPPER layout) of `count` forests in one leaf launch,
+ * one leaf-bucket sort over all of them and one persistent gather launch
+ * fed by a device work queue (largest problems first).  The reference has
+ * no multi-forest batching; it runs each forest tree by tree from the host
+ * (fit_batch_gpu, src/rf_random_forest.cpp:1364-1570; batch sizing
+ * get_recommended_batch_size, src/rf_cuda_config.cpp:225-295).  The plan
+ * copies the descriptors; the device buffers they name must outlive it.
+ * Needs the current device to be the one the batch runs on. */
+int rfx_batch_create(const rfx_batch_problem* problems, int32_t count, int32_t variant, rfx_batch** out);
+size_t rfx_batch_workspace_bytes(const rfx_batch* b);
+/* Enqueue the batch on `stream`.  use_graph = 1 records the launch sequence
+ * into a hipGraph on the first call (re-recorded when the workspace or the
+ * tuning changes) and replays it.  The first call with a workspace uploads
+ * the descriptors and synchronises; later calls do not.  Two runs of one
+ * plan must not overlap (they share the workspace). */
+int rfx_batch_run(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t use_graph);
+/* Time every (rows per work item, workgroups per CU) candidate on the
+ * batch's own data (reps eager runs each), keep the fastest; synchronises. */
+int rfx_batch_autotune(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t reps,
+                       double* best_ms);
+int rfx_batch_set_tuning(rfx_batch* b, int32_t lines_per_item, int32_t blocks_per_cu);
+int rfx_batch_get_tuning(const rfx_batch* b, int32_t* lines_per_item, int32_t* blocks_per_cu);
+void rfx_batch_free(rfx_batch* b);
 
 #ifdef __cplusplus
 }

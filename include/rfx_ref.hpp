@@ -1,0 +1,130 @@
+// rfx_ref.hpp -- C++ entry points with the reference's signatures, running
+// on the MI355X through the C ABI of rfx_prox.h.
+//
+// A host program written against the reference's headers switches by
+// including this header and calling rfx::ref:: instead of rf:: /
+// rf::cuda::.  Argument meaning, array layouts and ownership follow the
+// reference; where the reference is silent on failure (void functions) these
+// throw std::runtime_error instead of returning quietly, and the helper
+// functions that the reference documents as returning false / an empty
+// vector do exactly that.  Every call synchronises (the reference calls are
+// synchronous) and uses the null stream of the current device.
+//
+// The per-tree functions exist for drop-in compatibility with callers that
+// loop over trees; the all-trees path (rfx_leaf_assign +
+// rfx_prox_accumulate over every tree in one call) is the fast one.
+#ifndef RFX_REF_HPP
+#define RFX_REF_HPP
+
+#include <cstdint>
+#include <vector>
+
+#include "rfx_prox.h"
+
+namespace rfx {
+namespace ref {
+
+using integer_t = int32_t;
+using dp_t = double;
+
+// Per-tree OOB proximity, accumulated into the caller's column-major n x n
+// float64 prox: prox[i + j*n] += 1 for i out-of-bag, j in-bag, same terminal
+// node (nodestatus -1).  Replaces rf::cpu_proximity / rf::gpu_proximity
+// (include/rf_proximity.hpp:10-21, src/rf_proximity.cpp:99-231,
+// non-case-wise).  nod/ncount/ncn/nodexb/ndbegin/npcase are the reference's
+// caller-owned scratch arrays; they are accepted and not touched.
+void cpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
+                   integer_t nnode, dp_t* prox, integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
+                   integer_t* ndbegin, integer_t* npcase);
+void gpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
+                   integer_t nnode, dp_t* prox, integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
+                   integer_t* ndbegin, integer_t* npcase);
+
+// One tree's OOB fold U[i<=j] = P_ij + P_ji into a device packed upper
+// triangle of n(n+1)/2 IEEE half values (bits as uint16_t), overwritten.
+// Replaces rf::cuda::gpu_proximity_upper_triangle_fp16
+// (include/rf_proximity_upper_triangle.hpp:42-55,
+// cuda/rf_proximity_upper_triangle.cu:149-306, zeroed at 948).
+void gpu_proximity_upper_triangle_fp16(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin,
+                                       integer_t nsample, integer_t nnode, uint16_t* prox_upper_fp16,
+                                       integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
+                                       integer_t* ndbegin, integer_t* npcase);
+
+// Replaces rf::cpu_finishprox / rf::gpu_finishprox
+// (include/rf_finishprox.hpp:9-12, src/rf_finishprox.cpp:12-74): host
+// float64 n x n column-major prox in, proxsym out, bit-identical.
+void cpu_finishprox(integer_t nsample, const integer_t* nout, dp_t* prox, dp_t* proxsym);
+void gpu_finishprox(integer_t nsample, const integer_t* nout, dp_t* prox, dp_t* proxsym);
+
+// Replaces rf::compute_mds_3d_cpu (include/rf_mds_cpu.hpp:27-33,
+// src/rf_mds_cpu.cpp:88-341): n x 3 row-major coordinates of the classical
+// MDS of the host float64 column-major proximity.  With oob_counts and
+// use_rfgap == false the reference's nout renormalisation is applied
+// (src/rf_mds_cpu.cpp:154-186).  Throws std::runtime_error when fewer than
+// three eigenvalues exceed 1e-10 (as the reference) and for use_rfgap ==
+// true (RF-GAP is not part of this path).  memory_check is accepted; the
+// GPU form needs O(n) memory besides the input.
+std::vector<double> compute_mds_3d_cpu(const dp_t* proximity_matrix, integer_t n_samples, bool memory_check = true,
+                                       const integer_t* oob_counts_rfgap = nullptr, bool use_rfgap = false);
+
+// Storage levels of the factors (the reference's QuantizationLevel names
+// FP32/FP16/INT8/NF4; here the stored formats of rfx_quantize_factors).
+enum class FactorFormat : int32_t { FP32 = RFX_PREC_FP32, BF16 = RFX_PREC_BF16, INT8 = RFX_PREC_INT8,
+                                    FP8 = RFX_PREC_FP8 };
+
+// Low-rank proximity factors, the interface of
+// rf::cuda::LowRankProximityMatrix (include/rf_proximity_lowrank.hpp:27-420)
+// over this build's factor path: trees are added as the reference's per-tree
+// arrays, finalize_accumulation forms rank-r factors U of P = C/T (ALL
+// counts / trees, PSD) matrix-free from the leaf buckets, and get_factors
+// returns A = B = U (device float64, n x r column-major, object-owned), as the
+// reference's get_factors (cuda/rf_proximity_lowrank.cu:1753-1779).
+class LowRankProximityMatrix {
+ public:
+  LowRankProximityMatrix(integer_t n_samples, integer_t initial_rank, FactorFormat format = FactorFormat::BF16,
+                         integer_t max_rank = 0);
+  ~LowRankProximityMatrix();
+  LowRankProximityMatrix(const LowRankProximityMatrix&) = delete;
+  LowRankProximityMatrix& operator=(const LowRankProximityMatrix&) = delete;
+
+  bool initialize();
+  // One tree (host arrays: nodestatus[nnode], nodextr[n], nin[n]).
+  void add_tree_contribution(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin,
+                             integer_t nnode);
+  // Builds the factors once all trees are in (final_call true).
+  void finalize_accumulation(integer_t n_trees_processed, bool final_call = true);
+  void get_factors(dp_t** A_dev, dp_t** B_dev, integer_t* rank);
+  // n x k row-major MDS coordinates from the factors (k <= 3).
+  std::vector<double> compute_mds_from_factors(integer_t k = 3);
+  integer_t get_rank() const { return rank_; }
+  integer_t get_n_samples() const { return n_; }
+  std::vector<double> top_eigenvalues() const { return eig_; }
+
+ private:
+  void reserve_trees(int64_t trees);
+  integer_t n_, rank_;
+  FactorFormat format_;
+  int64_t trees_ = 0, cap_trees_ = 0, total_nodes_ = 0;
+  std::vector<int64_t> node_offset_{0};
+  int32_t* keys_ = nullptr;      // device [T][n]
+  float* U_ = nullptr;           // device n x r float32 row-major
+  void* Uq_ = nullptr;           // device stored factors
+  float* scales_ = nullptr;      // device [r]
+  double* A_ = nullptr;          // device n x r float64 column-major
+  std::vector<double> eig_;
+  bool ready_ = false;
+};
+
+// Helpers of include/rf_lowrank_helpers.hpp:12-24: host copies allocated
+// with new[] (the caller delete[]s them); false / an empty vector on failure.
+bool get_lowrank_factors_host(void* lowrank_proximity_ptr, dp_t** A_host, dp_t** B_host, integer_t* rank,
+                              integer_t nsamples);
+std::vector<double> compute_mds_from_factors_host(void* lowrank_proximity_ptr, integer_t k);
+inline std::vector<double> compute_mds_3d_from_factors_host(void* lowrank_proximity_ptr) {
+  return compute_mds_from_factors_host(lowrank_proximity_ptr, 3);
+}
+
+}  // namespace ref
+}  // namespace rfx
+
+#endif  // RFX_REF_HPP

@@ -30,6 +30,7 @@ GROW_NO_FMA = 1
 PROX_ALL = 0
 PROX_OOB_ASYM = 1
 PROX_OOB_UPPER = 2
+PROX_PAIR = 3  # batch: ALL into out, OOB_UPPER into out2
 
 LAYOUT_FULL = 0
 LAYOUT_UPPER = 1
@@ -43,12 +44,14 @@ MAX_TOTAL_NODES = 1073741823
 
 MDS_COUNTS = 0
 MDS_REFERENCE = 1
+MDS_MATRIX = 2
 
 PREC_FP32 = 0
 PREC_FP64 = 1
 PREC_BF16 = 2
 PREC_INT8 = 3
 PREC_FP8 = 4
+PREC_FP16 = 5
 
 _p = C.c_void_p
 _i32 = C.c_int32
@@ -79,15 +82,32 @@ _SIGS = {
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),
     "rfx_prox_pair_stats": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz, C.POINTER(C.c_double)]),
     "rfx_finishprox": (_i32, [_p, _p, _i32, _p, _i64, _p, _i32]),
+    "rfx_finishprox_f64": (_i32, [_p, _p, _p, _i64, _p, _i32]),
     "rfx_mds_dense_workspace_bytes": (_sz, [_i64, _i32]),
     "rfx_lowrank_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32, _i32]),
     "rfx_lowrank_factors": (_i32, [_p, _p, _i64, _i32, _i64, _p, _i32, _i32, _i32, _u64, _p, _sz, _p, _p]),
     "rfx_quantize_factors": (_i32, [_p, _p, _i64, _i32, _i32, _p, _p]),
     "rfx_mds_factors_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "rfx_mds_factors": (_i32, [_p, _p, _p, _i32, _i64, _i32, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_mds_dense_matrix": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_counts_convert": (_i32, [_p, _p, _i64, _i32, _p]),
+    "rfx_batch_create": (_i32, [_p, _i32, _i32, C.POINTER(_p)]),
+    "rfx_batch_workspace_bytes": (_sz, [_p]),
+    "rfx_batch_run": (_i32, [_p, _p, _p, _sz, _i32]),
+    "rfx_batch_autotune": (_i32, [_p, _p, _p, _sz, _i32, C.POINTER(C.c_double)]),
+    "rfx_batch_set_tuning": (_i32, [_p, _i32, _i32]),
+    "rfx_batch_get_tuning": (_i32, [_p, C.POINTER(_i32), C.POINTER(_i32)]),
+    "rfx_batch_free": (None, [_p]),
     "rfx_mds_dense": (_i32, [_p, _p, _i32, _i64, _i32, C.c_double, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p,
                              _p]),
 }
+
+
+class BatchProblem(C.Structure):
+    """rfx_batch_problem (include/rfx_prox.h)."""
+    _fields_ = [("x_col", _p), ("n", _i64), ("p", _i32), ("ntree", _i32), ("nodes", _p), ("node_off", _p),
+                ("total_nodes", _i64), ("inbag", _p), ("out", _p), ("out2", _p)]
+
 
 _lib = None
 

@@ -32,13 +32,10 @@
 
 #include "rfx_common.hpp"
 #include "bucket.hpp"
+#include "gather.hpp"
 
 namespace {
 
-constexpr int kMode_ALL = RFX_PROX_ALL;
-constexpr int kMode_OOB_ASYM = RFX_PROX_OOB_ASYM;
-constexpr int kMode_OOB_UPPER = RFX_PROX_OOB_UPPER;
-constexpr int kMode_PAIR = 3;
 
 constexpr int kTile = 128;
 constexpr int kSub = 8;
@@ -261,33 +258,6 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
 // steps in flight, so a leaf of 1000 members and one of 2 cost the same per
 // member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
 // ---------------------------------------------------------------------------
-#ifndef RFX_GATHER_CHUNK
-#define RFX_GATHER_CHUNK 8192
-#endif
-#ifndef RFX_GATHER_UNROLL
-#define RFX_GATHER_UNROLL 8
-#endif
-#ifndef RFX_GATHER_LINE_MAX_N
-#define RFX_GATHER_LINE_MAX_N 32768
-#endif
-constexpr int kGatherChunk = RFX_GATHER_CHUNK;  // LDS words per workgroup
-constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
-// lines up to this length use the flattened line kernel, longer ones the walk
-constexpr int64_t kGatherLineMaxN = RFX_GATHER_LINE_MAX_N;
-
-// first position in members[lo, hi) whose sample index is >= key
-__device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
-                                              uint32_t key) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if ((members[mid] >> 1) < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 template <int MODE, int LAYOUT>
 __global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restrict__ pe,
                                                           const uint32_t* __restrict__ bstart,
@@ -297,105 +267,8 @@ __global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restric
                                                           int32_t* __restrict__ out2) {
   __shared__ uint32_t acc[kGatherChunk];
   __shared__ uint32_t diag_valid;
-  const int64_t line = blockIdx.x;
-  // UPPER rows are chunked from the diagonal: chunk y of row i covers
-  // [i + y*CH, i + (y+1)*CH); FULL columns in aligned chunks.
-  const int64_t lo = (LAYOUT == RFX_LAYOUT_UPPER ? line : 0) + static_cast<int64_t>(blockIdx.y) * kGatherChunk;
-  if (lo >= n) return;
-  const int64_t hi = min(n, lo + kGatherChunk);
-  const int len_chunk = static_cast<int>(hi - lo);
-  for (int q = threadIdx.x; q < len_chunk; q += 256) acc[q] = 0u;
-  if (threadIdx.x == 0) diag_valid = 0u;
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t valid = 0;
-  for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
-    const int32_t t = tb + lane;
-    uint32_t s = 0, len = 0, bag = 0;
-    if (t < ntree) {
-      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
-      const uint32_t end = q.y & 0x7fffffffu;
-      if (end > 0) {
-        ++valid;
-        bag = q.y >> 31;
-        uint32_t b0, b1;
-        if (LAYOUT == RFX_LAYOUT_UPPER) {
-          b0 = q.x + 1;
-          if (lo > line) b0 = member_lb(members, b0, end, static_cast<uint32_t>(lo));
-        } else {
-          const uint32_t bs = bstart[mbucket[q.x]];
-          b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
-        }
-        b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
-        s = b0;
-        len = b1 > b0 ? b1 - b0 : 0u;
-      }
-    }
-    if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
-    uint32_t pre = len;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(pre, d, 64);
-      if (lane >= d) pre += y;
-    }
-    const uint32_t excl = pre - len;
-    const uint32_t total = __shfl(pre, 63, 64);
-    for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
-      uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
-      bool ok[kGatherUnroll];
-#pragma unroll
-      for (int u = 0; u < kGatherUnroll; ++u) {
-        const uint32_t g = g0 + u * 64 + lane;
-        int o = 0;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-          const uint32_t v = __shfl(pre, o + d - 1, 64);
-          if (v <= g) o += d;
-        }
-        o = min(o, 63);
-        const uint32_t so = __shfl(s, o, 64);
-        const uint32_t eo = __shfl(excl, o, 64);
-        bo[u] = __shfl(bag, o, 64);
-        ok[u] = g < total;
-        mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < kGatherUnroll; ++u) {
-        if (!ok[u]) continue;
-        const int64_t q = mv[u] >> 1;
-        const uint32_t bq = mv[u] & 1u;
-        if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
-        uint32_t inc = 0;
-        if (MODE == kMode_ALL) inc = 1u;
-        if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
-        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
-        if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
-        if (inc) atomicAdd(&acc[q - lo], inc);
-      }
-    }
-  }
-  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
-    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
-  }
-  __syncthreads();
-  if ((MODE == kMode_ALL || MODE == kMode_PAIR) && threadIdx.x == 0 && line >= lo && line < hi)
-    acc[line - lo] = diag_valid;
-  __syncthreads();
-
-  const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
-  for (int q = threadIdx.x; q < len_chunk; q += 256) {
-    const uint32_t v = acc[q];
-    if (MODE == kMode_PAIR) {
-      const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-      out[base + q] = accumulate ? out[base + q] + va : va;
-      out2[base + q] = accumulate ? out2[base + q] + vb : vb;
-    } else {
-      out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
-    }
-  }
+  gather_line_body<MODE, LAYOUT>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, blockIdx.x,
+                                 blockIdx.y, acc, diag_valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -536,8 +409,9 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   const int64_t slots = n * static_cast<int64_t>(ntree);
   const int64_t elems = LAYOUT == RFX_LAYOUT_UPPER ? n * (n + 1) / 2 : n * n;
   if (!accumulate) {
-    RFX_HIP_TRY(hipMemsetAsync(out, 0, 4 * elems, st));
-    if (out2) RFX_HIP_TRY(hipMemsetAsync(out2, 0, 4 * elems, st));
+    int rc = fill_u32(st, out, elems, 0u);
+    if (!rc && out2) rc = fill_u32(st, out2, elems, 0u);
+    if (rc) return rc;
   }
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
@@ -654,7 +528,8 @@ extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t 
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  RFX_HIP_TRY(hipMemsetAsync(w.stat, 0, 8, st));
+  rc = fill_u32(st, w.stat, 2, 0u);
+  if (rc) return rc;
   hipLaunchKernelGGL(sum_squares_kernel, dim3(grid_for(total_nodes, 256)), dim3(256), 0, st, w.bstart, w.bend,
                      total_nodes, w.stat);
   rc = rfx_launch_status();

@@ -1,0 +1,413 @@
+// A10: co-occurrence counts for a batch of independent forests in a fixed
+// number of launches, replayable as one hipGraph.
+//
+// The reference grows and accumulates one forest at a time, tree by tree,
+// from the host (fit_batch_gpu, src/rf_random_forest.cpp:1364-1570, with the
+// tree batch size of get_recommended_batch_size,
+// src/rf_cuda_config.cpp:225-295); many small problems then leave the GPU
+// mostly idle between launches.  Here the whole batch is one problem:
+//
+//  1. one leaf-assignment launch over every (problem, sample block, tree
+//     group), writing keys whose node ids are offset per problem, so that the
+//     batch's key arrays, laid end to end, form one key space;
+//  2. one segmented leaf-bucket build (radix sort by leaf) over all slots of
+//     all problems;
+//  3. one persistent gather launch: a grid sized to the machine pops work
+//     items (problem, run of output rows) from a queue in device memory,
+//     largest problems first, and counts each row in LDS with the same line
+//     body as the one-problem GATHER kernel.  The queue head is reset by a
+//     one-thread kernel, so the whole sequence is capturable.
+//
+// The tunables (rows per work item, workgroups per CU) are chosen by a small
+// autotuner that times the batch on the caller's data.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "bucket.hpp"
+#include "gather.hpp"
+#include "leaf.hpp"
+#include "rfx_common.hpp"
+
+namespace {
+
+struct LeafDesc {
+  const float* x;
+  const uint2* nodes;
+  const int64_t* node_off;
+  const uint32_t* inbag;
+  int32_t* keys;
+  int64_t n;
+  int64_t node_base;
+  int64_t bx_count;
+  int32_t ntree;
+};
+
+struct GatherDesc {
+  const uint2* pe;
+  int32_t* out;
+  int32_t* out2;
+  int64_t n;
+  int32_t ntree;
+};
+
+__global__ __launch_bounds__(kBlock) void batch_leaf_kernel(const LeafDesc* __restrict__ desc,
+                                                            const int64_t* __restrict__ block_start, int nprob) {
+  const int64_t blk = blockIdx.x;
+  int lo = 0, hi = nprob - 1;  // last problem whose first block <= blk
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (block_start[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const LeafDesc d = desc[lo];
+  const int64_t local = blk - block_start[lo];
+  leaf_numeric_body(d.x, d.n, d.nodes, d.node_off, d.ntree, d.inbag, nullptr, d.keys, d.node_base,
+                    local % d.bx_count, local / d.bx_count);
+}
+
+__global__ void batch_reset_kernel(uint32_t* counter) { *counter = 0u; }
+
+// Work item: (problem, first row, row count).  Every workgroup pops items
+// until the queue is empty; the queue is finite, so every wave exits.
+template <int MODE>
+__global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __restrict__ desc,
+                                                           const uint4* __restrict__ items, uint32_t nitems,
+                                                           uint32_t* __restrict__ counter,
+                                                           const uint32_t* __restrict__ bstart,
+                                                           const uint32_t* __restrict__ mbucket,
+                                                           const uint32_t* __restrict__ members) {
+  __shared__ uint32_t acc[kGatherChunk];
+  __shared__ uint32_t diag_valid;
+  __shared__ uint32_t item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t it = item_s;
+    if (it >= nitems) break;
+    const uint4 w = items[it];
+    const GatherDesc d = desc[w.x];
+    for (uint32_t l = 0; l < w.z; ++l) {
+      const int64_t line = static_cast<int64_t>(w.y) + l;
+      const int64_t nch = (d.n - line + kGatherChunk - 1) / kGatherChunk;
+      for (int64_t c = 0; c < nch; ++c) {
+        gather_line_body<MODE, RFX_LAYOUT_UPPER>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out, d.out2,
+                                                 line, c, acc, diag_valid);
+        __syncthreads();
+      }
+    }
+    __syncthreads();  // every thread has read item_s before it is rewritten
+  }
+}
+
+}  // namespace
+
+struct rfx_batch {
+  std::vector<rfx_batch_problem> prob;
+  int32_t variant = RFX_PROX_OOB_UPPER;
+  int64_t slots = 0, total_nodes = 0;
+  std::vector<int64_t> seg_start, seg_n, node_base, block_start;
+  std::vector<uint32_t> order;  // problems by decreasing work
+  int32_t lines_per_item = 4;
+  int32_t blocks_per_cu = 0;  // 0: as many as fit
+  // device layout inside the workspace
+  size_t off_leaf = 0, off_gather = 0, off_block = 0, off_segs = 0, off_segn = 0, off_items = 0, off_counter = 0,
+         off_keys = 0, off_bucket = 0, total_bytes = 0;
+  uint32_t nitems = 0;
+  std::vector<uint4> items;
+  void* uploaded_ws = nullptr;
+  int32_t uploaded_lpi = -1;
+  // graph
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* graph_ws = nullptr;
+  int32_t graph_lpi = -1, graph_bpc = -1;
+  int cu_count = 0, max_bpc = 0;
+};
+
+namespace {
+
+void build_items(rfx_batch* b) {
+  b->items.clear();
+  const uint32_t L = static_cast<uint32_t>(std::max(1, b->lines_per_item));
+  for (uint32_t pi : b->order) {
+    const int64_t n = b->prob[pi].n;
+    for (int64_t r = 0; r < n; r += L)
+      b->items.push_back(make_uint4(pi, static_cast<uint32_t>(r), static_cast<uint32_t>(std::min<int64_t>(L, n - r)), 0u));
+  }
+  b->nitems = static_cast<uint32_t>(b->items.size());
+}
+
+size_t layout_bytes(rfx_batch* b) {
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    const size_t at = o;
+    o += align256(bytes);
+    return at;
+  };
+  const size_t B = b->prob.size();
+  // items for the finest granularity (1 row per item) bound every tuning
+  int64_t max_items = 0;
+  for (const auto& p : b->prob) max_items += p.n;
+  b->off_leaf = take(sizeof(LeafDesc) * B);
+  b->off_gather = take(sizeof(GatherDesc) * B);
+  b->off_block = take(8 * (B + 1));
+  b->off_segs = take(8 * (B + 1));
+  b->off_segn = take(8 * B);
+  b->off_items = take(sizeof(uint4) * static_cast<size_t>(max_items));
+  b->off_counter = take(256);
+  b->off_keys = take(4 * static_cast<size_t>(b->slots));
+  b->off_bucket = take(bucket_ws_bytes(b->slots, 1, b->total_nodes));
+  return o;
+}
+
+int upload(hipStream_t st, rfx_batch* b, char* ws) {
+  const size_t B = b->prob.size();
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  std::vector<LeafDesc> ld(B);
+  std::vector<GatherDesc> gd(B);
+  int32_t* keys = reinterpret_cast<int32_t*>(ws + b->off_keys);
+  for (size_t i = 0; i < B; ++i) {
+    const auto& p = b->prob[i];
+    ld[i] = LeafDesc{p.x_col, reinterpret_cast<const uint2*>(p.nodes), p.node_off, p.inbag, keys + b->seg_start[i],
+                     p.n, b->node_base[i], rfx_ceil_div(p.n, kBlock), p.ntree};
+    gd[i] = GatherDesc{w.pe + b->seg_start[i], p.out, p.out2, p.n, p.ntree};
+  }
+  build_items(b);
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_leaf, ld.data(), sizeof(LeafDesc) * B, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_gather, gd.data(), sizeof(GatherDesc) * B, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_block, b->block_start.data(), 8 * (B + 1), hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_segs, b->seg_start.data(), 8 * (B + 1), hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_segn, b->seg_n.data(), 8 * B, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_items, b->items.data(), sizeof(uint4) * b->items.size(),
+                             hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));  // host vectors are local
+  b->uploaded_ws = ws;
+  b->uploaded_lpi = b->lines_per_item;
+  return RFX_OK;
+}
+
+template <int MODE>
+int launch_gather(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, unsigned grid) {
+  hipLaunchKernelGGL(batch_gather_kernel<MODE>, dim3(grid), dim3(256), 0, st,
+                     reinterpret_cast<const GatherDesc*>(ws + b->off_gather),
+                     reinterpret_cast<const uint4*>(ws + b->off_items), b->nitems,
+                     reinterpret_cast<uint32_t*>(ws + b->off_counter), w.bstart, w.mbucket, w.members);
+  return rfx_launch_status();
+}
+
+int max_blocks_per_cu(int32_t variant) {
+  int nb = 0;
+  hipError_t e;
+  if (variant == RFX_PROX_ALL)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_ALL>, 256, 0);
+  else if (variant == RFX_PROX_OOB_UPPER)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_OOB_UPPER>, 256, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_PAIR>, 256, 0);
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+// The launch sequence (eager, or recorded into a graph by the caller).
+int enqueue(hipStream_t st, rfx_batch* b, char* ws) {
+  const int B = static_cast<int>(b->prob.size());
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  const int64_t nblocks = b->block_start[B];
+  hipLaunchKernelGGL(batch_leaf_kernel, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, st,
+                     reinterpret_cast<const LeafDesc*>(ws + b->off_leaf),
+                     reinterpret_cast<const int64_t*>(ws + b->off_block), B);
+  int rc = rfx_launch_status();
+  if (rc) return rc;
+  rc = bucket_build_segmented(st, reinterpret_cast<const int32_t*>(ws + b->off_keys), b->slots, b->total_nodes,
+                              reinterpret_cast<const int64_t*>(ws + b->off_segs),
+                              reinterpret_cast<const int64_t*>(ws + b->off_segn), B, w);
+  if (rc) return rc;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, st, reinterpret_cast<uint32_t*>(ws + b->off_counter));
+  rc = rfx_launch_status();
+  if (rc) return rc;
+  const int bpc = b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, b->max_bpc) : b->max_bpc;
+  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(b->cu_count) * bpc, static_cast<int64_t>(b->nitems))));
+  switch (b->variant) {
+    case RFX_PROX_ALL:
+      return launch_gather<kMode_ALL>(st, b, ws, w, grid);
+    case RFX_PROX_OOB_UPPER:
+      return launch_gather<kMode_OOB_UPPER>(st, b, ws, w, grid);
+    default:
+      return launch_gather<kMode_PAIR>(st, b, ws, w, grid);
+  }
+}
+
+void drop_graph(rfx_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  b->graph_ws = nullptr;
+}
+
+}  // namespace
+
+extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count, int32_t variant,
+                                rfx_batch** out) {
+  if (!problems || !out || count <= 0) return RFX_ERR_INVALID;
+  if (variant != RFX_PROX_ALL && variant != RFX_PROX_OOB_UPPER && variant != RFX_PROX_PAIR) return RFX_ERR_INVALID;
+  *out = nullptr;
+  auto* b = new (std::nothrow) rfx_batch();
+  if (!b) return RFX_ERR_ALLOC;
+  b->prob.assign(problems, problems + count);
+  b->variant = variant;
+  b->seg_start.assign(count + 1, 0);
+  b->seg_n.assign(count, 0);
+  b->node_base.assign(count, 0);
+  b->block_start.assign(count + 1, 0);
+  for (int32_t i = 0; i < count; ++i) {
+    const auto& p = b->prob[i];
+    const bool bad = !p.x_col || !p.nodes || !p.node_off || !p.inbag || !p.out || p.n <= 0 ||
+                     p.n > RFX_MAX_SAMPLES || p.p <= 0 || p.p > RFX_MAX_FEATURES || p.ntree <= 0 ||
+                     p.ntree > 65535 || p.total_nodes <= 0 || (variant == RFX_PROX_PAIR && !p.out2);
+    if (bad) {
+      delete b;
+      return RFX_ERR_INVALID;
+    }
+    b->seg_start[i + 1] = b->seg_start[i] + p.n * static_cast<int64_t>(p.ntree);
+    b->seg_n[i] = p.n;
+    b->node_base[i] = b->total_nodes;
+    b->total_nodes += p.total_nodes;
+    b->block_start[i + 1] = b->block_start[i] + rfx_ceil_div(p.n, kBlock) * rfx_ceil_div(p.ntree, kFastTrees);
+  }
+  b->slots = b->seg_start[count];
+  if (b->slots >= (1LL << 31) || b->total_nodes > RFX_MAX_TOTAL_NODES || b->block_start[count] >= (1LL << 31)) {
+    delete b;
+    return RFX_ERR_UNSUPPORTED;
+  }
+  b->order.resize(count);
+  std::iota(b->order.begin(), b->order.end(), 0u);
+  std::stable_sort(b->order.begin(), b->order.end(), [b](uint32_t a, uint32_t c) {
+    const double wa = static_cast<double>(b->prob[a].n) * b->prob[a].n * b->prob[a].ntree;
+    const double wc = static_cast<double>(b->prob[c].n) * b->prob[c].n * b->prob[c].ntree;
+    return wa > wc;
+  });
+  b->total_bytes = layout_bytes(b);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&b->cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    delete b;
+    return RFX_ERR_HIP;
+  }
+  b->max_bpc = max_blocks_per_cu(variant);
+  *out = b;
+  return RFX_OK;
+}
+
+extern "C" size_t rfx_batch_workspace_bytes(const rfx_batch* b) { return b ? b->total_bytes : 0; }
+
+extern "C" int rfx_batch_set_tuning(rfx_batch* b, int32_t lines_per_item, int32_t blocks_per_cu) {
+  if (!b || lines_per_item < 1 || lines_per_item > 4096 || blocks_per_cu < 0) return RFX_ERR_INVALID;
+  b->lines_per_item = lines_per_item;
+  b->blocks_per_cu = blocks_per_cu;
+  return RFX_OK;
+}
+
+extern "C" int rfx_batch_get_tuning(const rfx_batch* b, int32_t* lines_per_item, int32_t* blocks_per_cu) {
+  if (!b) return RFX_ERR_INVALID;
+  if (lines_per_item) *lines_per_item = b->lines_per_item;
+  if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu > 0 ? std::min(b->blocks_per_cu, b->max_bpc) : b->max_bpc;
+  return RFX_OK;
+}
+
+extern "C" int rfx_batch_run(hipStream_t st, rfx_batch* b, void* workspace, size_t workspace_bytes,
+                             int32_t use_graph) {
+  if (!b || !workspace) return RFX_ERR_INVALID;
+  if (workspace_bytes < b->total_bytes) return RFX_ERR_WORKSPACE;
+  char* ws = static_cast<char*>(workspace);
+  if (b->uploaded_ws != ws || b->uploaded_lpi != b->lines_per_item) {
+    int rc = upload(st, b, ws);
+    if (rc) return rc;
+  }
+  if (!use_graph) return enqueue(st, b, ws);
+  if (!b->exec || b->graph_ws != ws || b->graph_lpi != b->lines_per_item || b->graph_bpc != b->blocks_per_cu) {
+    drop_graph(b);
+    if (!b->cap_stream) RFX_HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    RFX_HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(b->cap_stream, b, ws);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return RFX_ERR_HIP;
+    const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      b->exec = nullptr;
+      return RFX_ERR_HIP;
+    }
+    b->graph_ws = ws;
+    b->graph_lpi = b->lines_per_item;
+    b->graph_bpc = b->blocks_per_cu;
+  }
+  RFX_HIP_TRY(hipGraphLaunch(b->exec, st));
+  return RFX_OK;
+}
+
+extern "C" int rfx_batch_autotune(hipStream_t st, rfx_batch* b, void* workspace, size_t workspace_bytes,
+                                  int32_t reps, double* best_ms) {
+  if (!b || !workspace || reps < 1) return RFX_ERR_INVALID;
+  if (workspace_bytes < b->total_bytes) return RFX_ERR_WORKSPACE;
+  static const int32_t kLines[] = {1, 4, 16};
+  std::vector<int32_t> bpcs;
+  for (int v = 1; v <= b->max_bpc; v *= 2) bpcs.push_back(v);
+  if (bpcs.back() != b->max_bpc) bpcs.push_back(b->max_bpc);
+  hipEvent_t e0, e1;
+  RFX_HIP_TRY(hipEventCreate(&e0));
+  RFX_HIP_TRY(hipEventCreate(&e1));
+  double best = 1e300;
+  int32_t best_l = b->lines_per_item, best_b = b->blocks_per_cu;
+  int rc = RFX_OK;
+  for (int32_t L : kLines) {
+    for (int32_t bpc : bpcs) {
+      b->lines_per_item = L;
+      b->blocks_per_cu = bpc;
+      rc = rfx_batch_run(st, b, workspace, workspace_bytes, 0);  // upload + warm
+      if (rc) break;
+      if (hipEventRecord(e0, st) != hipSuccess) {
+        rc = RFX_ERR_HIP;
+        break;
+      }
+      for (int r = 0; r < reps && !rc; ++r) rc = rfx_batch_run(st, b, workspace, workspace_bytes, 0);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = RFX_ERR_HIP;
+        break;
+      }
+      if (ms / reps < best) {
+        best = ms / reps;
+        best_l = L;
+        best_b = bpc;
+      }
+    }
+    if (rc) break;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  b->lines_per_item = best_l;
+  b->blocks_per_cu = best_b;
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
+  return RFX_OK;
+}
+
+extern "C" void rfx_batch_free(rfx_batch* b) {
+  if (!b) return;
+  drop_graph(b);
+  if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+  delete b;
+}

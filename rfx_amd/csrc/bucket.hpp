@@ -219,6 +219,38 @@ __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t
   pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
 }
 
+// Segmented form for a batch of problems laid end to end: slots
+// [seg_start[b], seg_start[b+1]) belong to problem b with seg_n[b] samples,
+// so the sample index is (s - seg_start[b]) % seg_n[b].
+__global__ void rs_members_seg_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N,
+                                      const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_n,
+                                      int nseg, uint32_t total_nodes, const uint32_t* __restrict__ bend,
+                                      uint32_t* __restrict__ members, uint2* __restrict__ pe) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t v = V[p];
+  const uint32_t s = v >> 1, bag = v & 1u;
+  int lo = 0, hi = nseg - 1;  // last b with seg_start[b] <= s
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_start[mid] <= static_cast<int64_t>(s))
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint32_t i = static_cast<uint32_t>((static_cast<int64_t>(s) - seg_start[lo]) % seg_n[lo]);
+  members[p] = (i << 1) | bag;
+  const uint32_t k = K[p];
+  pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
+}
+
+// Graph-safe fill (no memset nodes in captured graphs).
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t len, uint32_t v) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < len;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[e] = v;
+}
+
 struct BucketWs {
   uint32_t* bstart;
   uint32_t* bend;
@@ -292,13 +324,19 @@ unsigned grid_for(int64_t work, int threads) {
   return static_cast<unsigned>(g);
 }
 
-// Full bucket build from the leaf keys (see the file comment).
-int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-                 const BucketWs& w) {
-  const int64_t N = n * static_cast<int64_t>(ntree);
+int fill_u32(hipStream_t st, void* p, int64_t len, uint32_t v) {
+  if (len <= 0) return RFX_OK;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(len, 256)), dim3(256), 0, st, static_cast<uint32_t*>(p), len, v);
+  return rfx_launch_status();
+}
+
+// Sort of N slots by leaf and the leaf ranges (everything but members/pe).
+int bucket_sort(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_nodes, const BucketWs& w) {
   const uint32_t tn = static_cast<uint32_t>(total_nodes);
-  RFX_HIP_TRY(hipMemsetAsync(w.bstart, 0, 4 * total_nodes, st));
-  RFX_HIP_TRY(hipMemsetAsync(w.bend, 0, 4 * total_nodes, st));
+  int rc = fill_u32(st, w.bstart, total_nodes, 0u);
+  if (rc) return rc;
+  rc = fill_u32(st, w.bend, total_nodes, 0u);
+  if (rc) return rc;
   hipLaunchKernelGGL(rs_prep_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, N, tn, w.K[0], w.V[0]);
   const int64_t hlen = 256 * w.ntiles;
   for (int pass = 0; pass < w.passes; ++pass) {
@@ -316,9 +354,31 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   const int fin = w.passes & 1;
   hipLaunchKernelGGL(rs_bounds_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], N, tn, w.bstart, w.bend,
                      w.nvalid);
+  return rfx_launch_status();
+}
+
+// Full bucket build from the leaf keys [T][n] (see the file comment).
+int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                 const BucketWs& w) {
+  const int64_t N = n * static_cast<int64_t>(ntree);
+  int rc = bucket_sort(st, keys, N, total_nodes, w);
+  if (rc) return rc;
+  const int fin = w.passes & 1;
   hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
-                     static_cast<uint32_t>(n), tn,
-                     w.bend, w.members, w.pe);
+                     static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.pe);
+  return rfx_launch_status();
+}
+
+// Bucket build over a batch of problems whose key arrays lie end to end
+// (problem b: slots [seg_start[b], seg_start[b+1]), keys [T_b][seg_n[b]],
+// node ids already offset so that all leaves are distinct).
+int bucket_build_segmented(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_nodes,
+                           const int64_t* seg_start_dev, const int64_t* seg_n_dev, int nseg, const BucketWs& w) {
+  int rc = bucket_sort(st, keys, N, total_nodes, w);
+  if (rc) return rc;
+  const int fin = w.passes & 1;
+  hipLaunchKernelGGL(rs_members_seg_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
+                     seg_start_dev, seg_n_dev, nseg, static_cast<uint32_t>(total_nodes), w.bend, w.members, w.pe);
   return rfx_launch_status();
 }
 

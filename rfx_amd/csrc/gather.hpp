@@ -1,0 +1,156 @@
+// Per-line gather of the GATHER co-occurrence algorithm, shared by the
+// one-problem kernels and the batched persistent kernel.
+//
+// A workgroup owns one output line (row i of UPPER, column i of FULL) chunk
+// [lo, hi) and counts it in LDS: every tree's member run of the line's leaf
+// (the samples that follow i in the leaf, or the whole leaf for FULL),
+// restricted to [lo, hi) by two binary searches, is flattened across the 64
+// lanes of a wavefront (prefix sum of run lengths, each lane finds its run by
+// a 6-step search over the prefix) so that every lane issues independent
+// member loads regardless of how ragged the runs are; matches are added with
+// LDS atomics and the chunk is written (or added) once.
+#pragma once
+
+#include "rfx_common.hpp"
+
+namespace {
+
+constexpr int kMode_ALL = RFX_PROX_ALL;
+constexpr int kMode_OOB_ASYM = RFX_PROX_OOB_ASYM;
+constexpr int kMode_OOB_UPPER = RFX_PROX_OOB_UPPER;
+constexpr int kMode_PAIR = 3;
+
+#ifndef RFX_GATHER_CHUNK
+#define RFX_GATHER_CHUNK 8192
+#endif
+#ifndef RFX_GATHER_UNROLL
+#define RFX_GATHER_UNROLL 8
+#endif
+#ifndef RFX_GATHER_LINE_MAX_N
+#define RFX_GATHER_LINE_MAX_N 32768
+#endif
+constexpr int kGatherChunk = RFX_GATHER_CHUNK;  // LDS words per workgroup
+constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
+// lines up to this length use the flattened line kernel, longer ones the walk
+constexpr int64_t kGatherLineMaxN = RFX_GATHER_LINE_MAX_N;
+
+// first position in members[lo, hi) whose sample index is >= key
+__device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
+                                              uint32_t key) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((members[mid] >> 1) < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int MODE, int LAYOUT>
+__device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
+                                                 const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
+                                                 const uint32_t* __restrict__ members, int32_t accumulate,
+                                                 int32_t* __restrict__ out, int32_t* __restrict__ out2, int64_t line,
+                                                 int64_t chunk, uint32_t* acc, uint32_t& diag_valid) {
+  // UPPER rows are chunked from the diagonal: chunk y of row i covers
+  // [i + y*CH, i + (y+1)*CH); FULL columns in aligned chunks.
+  const int64_t lo = (LAYOUT == RFX_LAYOUT_UPPER ? line : 0) + chunk * kGatherChunk;
+  if (lo >= n) return;
+  const int64_t hi = min(n, lo + kGatherChunk);
+  const int len_chunk = static_cast<int>(hi - lo);
+  for (int q = threadIdx.x; q < len_chunk; q += 256) acc[q] = 0u;
+  if (threadIdx.x == 0) diag_valid = 0u;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t valid = 0;
+  for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
+    const int32_t t = tb + lane;
+    uint32_t s = 0, len = 0, bag = 0;
+    if (t < ntree) {
+      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
+      const uint32_t end = q.y & 0x7fffffffu;
+      if (end > 0) {
+        ++valid;
+        bag = q.y >> 31;
+        uint32_t b0, b1;
+        if (LAYOUT == RFX_LAYOUT_UPPER) {
+          b0 = q.x + 1;
+          if (lo > line) b0 = member_lb(members, b0, end, static_cast<uint32_t>(lo));
+        } else {
+          const uint32_t bs = bstart[mbucket[q.x]];
+          b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
+        }
+        b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
+        s = b0;
+        len = b1 > b0 ? b1 - b0 : 0u;
+      }
+    }
+    if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
+    uint32_t pre = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(pre, d, 64);
+      if (lane >= d) pre += y;
+    }
+    const uint32_t excl = pre - len;
+    const uint32_t total = __shfl(pre, 63, 64);
+    for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
+      uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
+      bool ok[kGatherUnroll];
+#pragma unroll
+      for (int u = 0; u < kGatherUnroll; ++u) {
+        const uint32_t g = g0 + u * 64 + lane;
+        int o = 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          const uint32_t v = __shfl(pre, o + d - 1, 64);
+          if (v <= g) o += d;
+        }
+        o = min(o, 63);
+        const uint32_t so = __shfl(s, o, 64);
+        const uint32_t eo = __shfl(excl, o, 64);
+        bo[u] = __shfl(bag, o, 64);
+        ok[u] = g < total;
+        mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kGatherUnroll; ++u) {
+        if (!ok[u]) continue;
+        const int64_t q = mv[u] >> 1;
+        const uint32_t bq = mv[u] & 1u;
+        if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
+        uint32_t inc = 0;
+        if (MODE == kMode_ALL) inc = 1u;
+        if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
+        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+        if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
+        if (inc) atomicAdd(&acc[q - lo], inc);
+      }
+    }
+  }
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  __syncthreads();
+  if ((MODE == kMode_ALL || MODE == kMode_PAIR) && threadIdx.x == 0 && line >= lo && line < hi)
+    acc[line - lo] = diag_valid;
+  __syncthreads();
+
+  const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
+  for (int q = threadIdx.x; q < len_chunk; q += 256) {
+    const uint32_t v = acc[q];
+    if (MODE == kMode_PAIR) {
+      const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+      out[base + q] = accumulate ? out[base + q] + va : va;
+      out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+    } else {
+      out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+    }
+  }
+}
+
+}  // namespace

@@ -18,12 +18,11 @@
 // With all features numeric both rules reduce to x <= threshold -> left.
 
 #include "rfx_common.hpp"
+#include "leaf.hpp"
 
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kTreesPerBlock = 8;
-constexpr int kMaxOobDescents = 999;
 
 __global__ __launch_bounds__(kBlock) void leaf_assign_kernel(const float* __restrict__ x, int64_t n,
                                                              const uint2* __restrict__ nodes,
@@ -96,91 +95,15 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_kernel(const float* __rest
   }
 }
 
-// All-numeric fast path: the block's 256 samples have their features staged
-// in LDS (feature-major, so a lane reads its own sample conflict-free) and
-// each thread advances kIlp traversals of different trees in lockstep, so
-// every step issues kIlp independent node gathers instead of one dependent
-// node + feature chain.
-#ifndef RFX_LEAF_ILP
-#define RFX_LEAF_ILP 2
-#endif
-#ifndef RFX_LEAF_TREES
-#define RFX_LEAF_TREES 4
-#endif
-#ifndef RFX_LEAF_STAGE
-#define RFX_LEAF_STAGE 0
-#endif
-constexpr int kIlp = RFX_LEAF_ILP;
-constexpr int kFastTrees = RFX_LEAF_TREES;  // trees per block (grid.y = ceil(T / kFastTrees))
-constexpr bool kStage = RFX_LEAF_STAGE != 0;  // features staged in LDS
-constexpr int kMaxStagedFeatures = 64;      // 64 KiB of LDS at most
-
-__global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float* __restrict__ x, int64_t n, int p,
+// All-numeric fast path (body in leaf.hpp).
+__global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float* __restrict__ x, int64_t n,
                                                                      const uint2* __restrict__ nodes,
                                                                      const int64_t* __restrict__ node_off,
                                                                      int32_t ntree,
                                                                      const uint32_t* __restrict__ inbag,
                                                                      int32_t* __restrict__ nodextr,
                                                                      int32_t* __restrict__ keys) {
-  extern __shared__ float xs_flat[];  // [p][kBlock]
-  float(*xs)[kBlock] = reinterpret_cast<float(*)[kBlock]>(xs_flat);
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kBlock;
-  const int tid = threadIdx.x;
-  if (kStage) {
-    for (int f = 0; f < p; ++f) xs[f][tid] = i0 + tid < n ? x[static_cast<int64_t>(f) * n + i0 + tid] : 0.0f;
-    __syncthreads();
-  }
-  const int64_t i = i0 + tid;
-  if (i >= n) return;
-  const int64_t words = (n + 31) / 32;
-  const int32_t tb0 = blockIdx.y * kFastTrees;
-  const int32_t tb1 = min(tb0 + kFastTrees, ntree);
-  for (int32_t t0 = tb0; t0 < tb1; t0 += kIlp) {
-    int64_t base[kIlp];
-    uint32_t kt[kIlp];
-    uint2 nd[kIlp];
-    bool bag[kIlp], live[kIlp];
-    int steps[kIlp];
-#pragma unroll
-    for (int u = 0; u < kIlp; ++u) {
-      const int32_t t = t0 + u;
-      live[u] = t < tb1;
-      base[u] = live[u] ? node_off[t] : 0;
-      bag[u] = live[u] ? ((inbag[t * words + (i >> 5)] >> (i & 31)) & 1u) : true;
-      kt[u] = 0;
-      steps[u] = 0;
-      nd[u] = live[u] ? nodes[base[u]] : make_uint2(0u, 0xffffffffu);
-    }
-    bool any = true;
-    while (any) {
-      any = false;
-      bool go[kIlp];
-#pragma unroll
-      for (int u = 0; u < kIlp; ++u) {
-        go[u] = nd[u].y != 0xffffffffu && (bag[u] || steps[u] < kMaxOobDescents);
-        if (go[u]) {
-          const uint32_t f = nd[u].y & 1023u, l = nd[u].y >> 10;
-          const float v = kStage ? xs[f][tid] : x[static_cast<int64_t>(f) * n + i];
-          kt[u] = v <= __uint_as_float(nd[u].x) ? l : l + 1;
-          ++steps[u];
-        }
-        any = any || go[u];
-      }
-      // the kIlp node gathers of this step are independent
-#pragma unroll
-      for (int u = 0; u < kIlp; ++u)
-        if (go[u]) nd[u] = nodes[base[u] + kt[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < kIlp; ++u) {
-      if (!live[u]) continue;
-      if (nd[u].y != 0xffffffffu) kt[u] = 0;  // out-of-bag depth cap reached
-      const int64_t o = static_cast<int64_t>(t0 + u) * n + i;
-      if (nodextr) nodextr[o] = static_cast<int32_t>(kt[u]);
-      if (keys)
-        keys[o] = nd[u].y == 0xffffffffu ? static_cast<int32_t>(((base[u] + kt[u]) << 1) | (bag[u] ? 1 : 0)) : -1;
-    }
-  }
+  leaf_numeric_body(x, n, nodes, node_off, ntree, inbag, nodextr, keys, 0, blockIdx.x, blockIdx.y);
 }
 
 }  // namespace
@@ -192,10 +115,9 @@ extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n
       ntree <= 0)
     return RFX_ERR_INVALID;
   if (!nodextr && !keys) return RFX_OK;
-  if (cat == nullptr && p <= kMaxStagedFeatures) {
+  if (cat == nullptr) {
     dim3 g(rfx_ceil_div(n, kBlock), rfx_ceil_div(ntree, kFastTrees));
-    hipLaunchKernelGGL(leaf_assign_numeric_kernel, g, dim3(kBlock), kStage ? sizeof(float) * kBlock * p : 0, stream,
-                       x_col, n, p,
+    hipLaunchKernelGGL(leaf_assign_numeric_kernel, g, dim3(kBlock), 0, stream, x_col, n,
                        reinterpret_cast<const uint2*>(nodes), node_off, ntree, inbag, nodextr, keys);
     return rfx_launch_status();
   }

@@ -1,0 +1,92 @@
+// All-numeric leaf assignment body, shared by the one-forest kernel and the
+// batched kernel.
+//
+// One thread follows one sample down kFastTrees consecutive trees, kIlp
+// traversals of different trees in lockstep, so every step issues kIlp
+// independent node gathers instead of one dependent node + feature chain.
+// Node records are 8 bytes (threshold bits, left << 10 | feature;
+// 0xffffffff in the second word marks a terminal node).  With all features
+// numeric the reference's out-of-bag rule (src/rf_testreebag.cpp:51-122, at
+// most 999 descents, else node 0) and in-bag rule
+// (src/rf_random_forest.cpp:2194-2244) both reduce to x <= threshold -> left.
+#pragma once
+
+#include "rfx_common.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxOobDescents = 999;
+#ifndef RFX_LEAF_ILP
+#define RFX_LEAF_ILP 2
+#endif
+#ifndef RFX_LEAF_TREES
+#define RFX_LEAF_TREES 4
+#endif
+constexpr int kIlp = RFX_LEAF_ILP;
+constexpr int kFastTrees = RFX_LEAF_TREES;  // trees per workgroup
+
+// Sample block bx (kBlock samples), tree group by (kFastTrees trees).  Keys
+// are ((key_node_base + node_off[t] + node) << 1) | in-bag, -1 off-leaf.
+__device__ __forceinline__ void leaf_numeric_body(const float* __restrict__ x, int64_t n,
+                                                  const uint2* __restrict__ nodes,
+                                                  const int64_t* __restrict__ node_off, int32_t ntree,
+                                                  const uint32_t* __restrict__ inbag, int32_t* __restrict__ nodextr,
+                                                  int32_t* __restrict__ keys, int64_t key_node_base, int64_t bx,
+                                                  int64_t by) {
+  const int64_t i = bx * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t words = (n + 31) / 32;
+  const int32_t tb0 = static_cast<int32_t>(by) * kFastTrees;
+  const int32_t tb1 = min(tb0 + kFastTrees, ntree);
+  for (int32_t t0 = tb0; t0 < tb1; t0 += kIlp) {
+    int64_t base[kIlp];
+    uint32_t kt[kIlp];
+    uint2 nd[kIlp];
+    bool bag[kIlp], live[kIlp];
+    int steps[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const int32_t t = t0 + u;
+      live[u] = t < tb1;
+      base[u] = live[u] ? node_off[t] : 0;
+      bag[u] = live[u] ? ((inbag[t * words + (i >> 5)] >> (i & 31)) & 1u) : true;
+      kt[u] = 0;
+      steps[u] = 0;
+      nd[u] = live[u] ? nodes[base[u]] : make_uint2(0u, 0xffffffffu);
+    }
+    bool any = true;
+    while (any) {
+      any = false;
+      bool go[kIlp];
+#pragma unroll
+      for (int u = 0; u < kIlp; ++u) {
+        go[u] = nd[u].y != 0xffffffffu && (bag[u] || steps[u] < kMaxOobDescents);
+        if (go[u]) {
+          const uint32_t f = nd[u].y & 1023u, l = nd[u].y >> 10;
+          const float v = x[static_cast<int64_t>(f) * n + i];
+          kt[u] = v <= __uint_as_float(nd[u].x) ? l : l + 1;
+          ++steps[u];
+        }
+        any = any || go[u];
+      }
+      // the kIlp node gathers of this step are independent
+#pragma unroll
+      for (int u = 0; u < kIlp; ++u)
+        if (go[u]) nd[u] = nodes[base[u] + kt[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (!live[u]) continue;
+      if (nd[u].y != 0xffffffffu) kt[u] = 0;  // out-of-bag depth cap reached
+      const int64_t o = static_cast<int64_t>(t0 + u) * n + i;
+      if (nodextr) nodextr[o] = static_cast<int32_t>(kt[u]);
+      if (keys)
+        keys[o] = nd[u].y == 0xffffffffu
+                      ? static_cast<int32_t>(((key_node_base + base[u] + kt[u]) << 1) | (bag[u] ? 1 : 0))
+                      : -1;
+    }
+  }
+}
+
+}  // namespace

@@ -16,6 +16,10 @@
 //              1221-1252; src/rf_mds_cpu.cpp:154-186): OOB_UPPER counts ->
 //              finishprox S (FP64) -> float32 -> M_ij = 1/2 (S_ij/nout_i +
 //              S_ij/nout_j), M_ii = 1 (rows with nout_i = 0 are not divided)
+//   MATRIX     a given FP64 n x n column-major proximity S, as
+//              rf::compute_mds_3d_cpu receives it: with nout, M_ij = 1/2
+//              (S_ij/nout_i + S_ji/nout_j) and M_ii = 1
+//              (src/rf_mds_cpu.cpp:154-186); without, M = S as it is
 
 #include <cstring>
 #include <vector>
@@ -30,11 +34,13 @@ constexpr int kCols = 32;   // columns per LDS tile
 constexpr int kBlockK = 16; // subspace block size
 
 struct ProxForm {
-  int32_t mode;    // RFX_MDS_COUNTS / RFX_MDS_REFERENCE
+  int32_t mode;    // RFX_MDS_COUNTS / RFX_MDS_REFERENCE / RFX_MDS_MATRIX
   int32_t layout;  // RFX_LAYOUT_UPPER / RFX_LAYOUT_FULL
   double scale;
   double mx;       // max of M
   const int32_t* nout;
+  const double* mat;  // MATRIX: S, column-major
+  int64_t n;
 };
 
 __device__ __forceinline__ int64_t src_index(int32_t layout, int64_t i, int64_t j, int64_t n) {
@@ -45,9 +51,18 @@ __device__ __forceinline__ int64_t src_index(int32_t layout, int64_t i, int64_t 
 
 __device__ __forceinline__ double recip_or_one(int32_t c) { return c > 0 ? 1.0 / static_cast<double>(c) : 1.0; }
 
-// M_ij (i != j) from the stored count c_ij
+// M_ij (i != j) from the stored count c_ij (MATRIX: from S itself)
 __device__ __forceinline__ double prox_value(const ProxForm& f, int32_t c, int64_t i, int64_t j) {
   if (f.mode == RFX_MDS_COUNTS) return static_cast<double>(c) * f.scale;
+  if (f.mode == RFX_MDS_MATRIX) {
+    const double sij = f.mat[i + j * f.n];
+    if (!f.nout) return sij;
+    const double sji = f.mat[j + i * f.n];
+    const int32_t ni = f.nout[i], nj = f.nout[j];
+    const double ti = ni > 0 ? sij / static_cast<double>(ni) : sij;
+    const double tj = nj > 0 ? sji / static_cast<double>(nj) : sji;
+    return 0.5 * (ti + tj);
+  }
   const int32_t ni = f.nout[i], nj = f.nout[j];
   const double a = static_cast<double>(c) / 2.0;
   const double S = 0.5 * (a * recip_or_one(ni) + a * recip_or_one(nj));
@@ -57,8 +72,13 @@ __device__ __forceinline__ double prox_value(const ProxForm& f, int32_t c, int64
   return 0.5 * (ti + tj);
 }
 
-__device__ __forceinline__ double diag_value(const ProxForm& f, int32_t c) {
+__device__ __forceinline__ double diag_value(const ProxForm& f, int32_t c, int64_t i) {
+  if (f.mode == RFX_MDS_MATRIX) return f.nout ? 1.0 : f.mat[i + i * f.n];
   return f.mode == RFX_MDS_COUNTS ? static_cast<double>(c) * f.scale : 1.0;
+}
+
+__device__ __forceinline__ int32_t count_at(const ProxForm& f, const int32_t* counts, int64_t idx) {
+  return f.mode == RFX_MDS_MATRIX ? 0 : counts[idx];
 }
 
 // Max of M over all entries (diagonal included), FP64 bits via atomicMax.
@@ -69,8 +89,8 @@ __global__ void mds_max_kernel(const int32_t* __restrict__ counts, int64_t n, Pr
   const int64_t j0 = f.layout == RFX_LAYOUT_UPPER ? i : 0;
   for (int64_t j = j0 + threadIdx.x; j < n; j += blockDim.x) {
     // row i of the symmetric matrix, read along the contiguous direction
-    const int32_t c = counts[f.layout == RFX_LAYOUT_UPPER ? src_index(f.layout, i, j, n) : j + i * n];
-    const double v = (i == j) ? diag_value(f, c) : prox_value(f, c, i, j);
+    const int32_t c = count_at(f, counts, f.layout == RFX_LAYOUT_UPPER ? src_index(f.layout, i, j, n) : j + i * n);
+    const double v = (i == j) ? diag_value(f, c, i) : prox_value(f, c, i, j);
     m = v > m ? v : m;
   }
   for (int d = 32; d > 0; d >>= 1) {
@@ -113,7 +133,7 @@ __global__ __launch_bounds__(kRows) void mds_apply_kernel(const int32_t* __restr
       const int64_t ii = i0 + r, jj = j0 + q;
       Real v = Real(0);
       if (ii < n && jj < je && ii != jj) {
-        const int32_t c = counts[src_index(f.layout, ii, jj, n)];
+        const int32_t c = count_at(f, counts, src_index(f.layout, ii, jj, n));
         const double d = f.mx - prox_value(f, c, ii, jj);
         v = static_cast<Real>(d * d);
       }
@@ -154,6 +174,10 @@ __global__ void mds_sum_splits_kernel(const Real* __restrict__ Ypart, int splits
   Y[e] = static_cast<Out>(s);
 }
 
+__global__ void mds_zero_kernel(unsigned long long* p) {
+  if (threadIdx.x < 4) p[threadIdx.x] = 0ull;
+}
+
 __global__ void mds_scale_kernel(double* __restrict__ r, int64_t n, double inv) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) r[i] *= inv;
@@ -179,8 +203,8 @@ size_t dense_ws(int64_t n) {
 }
 
 template <typename Real>
-int run_dense(hipStream_t st, const int32_t* counts, int32_t layout, int64_t n, int32_t mode, double scale,
-              const int32_t* nout, int32_t max_iter, double tol, void* ws, double* coords, double* eigvals_host,
+int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t layout, int64_t n, int32_t mode,
+              double scale, const int32_t* nout, int32_t max_iter, double tol, void* ws, double* coords, double* eigvals_host,
               int32_t* iters_host) {
   char* p = static_cast<char*>(ws);
   double* r = reinterpret_cast<double*>(p);
@@ -192,9 +216,9 @@ int run_dense(hipStream_t st, const int32_t* counts, int32_t layout, int64_t n, 
   p += al(sizeof(Real) * sp * n * kBlockK);
   double* rpart = reinterpret_cast<double*>(p);
   p += al(8 * sp * n);
-  ProxForm f{mode, layout, scale, 0.0, nout};
+  ProxForm f{mode, layout, scale, 0.0, nout, mat, n};
 
-  RFX_HIP_TRY(hipMemsetAsync(mx_bits, 0, 8, st));
+  hipLaunchKernelGGL(mds_zero_kernel, dim3(1), dim3(64), 0, st, mx_bits);
   hipLaunchKernelGGL(mds_max_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, counts, n, f, mx_bits);
   unsigned long long bits = 0;
   RFX_HIP_TRY(hipMemcpyAsync(&bits, mx_bits, 8, hipMemcpyDeviceToHost, st));
@@ -262,8 +286,21 @@ extern "C" int rfx_mds_dense(hipStream_t st, const int32_t* counts, int32_t layo
   if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
   if (workspace_bytes < rfx_mds_dense_workspace_bytes(n, precision)) return RFX_ERR_WORKSPACE;
   if (precision == RFX_PREC_FP64)
-    return run_dense<double>(st, counts, layout, n, mode, scale, nout, max_iter, tol, workspace, coords, eigvals_host,
-                             iters_host);
-  return run_dense<float>(st, counts, layout, n, mode, scale, nout, max_iter, tol, workspace, coords, eigvals_host,
-                          iters_host);
+    return run_dense<double>(st, counts, nullptr, layout, n, mode, scale, nout, max_iter, tol, workspace, coords,
+                             eigvals_host, iters_host);
+  return run_dense<float>(st, counts, nullptr, layout, n, mode, scale, nout, max_iter, tol, workspace, coords,
+                          eigvals_host, iters_host);
+}
+
+extern "C" int rfx_mds_dense_matrix(hipStream_t st, const double* prox, int64_t n, const int32_t* nout,
+                                    int32_t precision, int32_t max_iter, double tol, void* workspace,
+                                    size_t workspace_bytes, double* coords, double* eigvals_host, int32_t* iters_host) {
+  if (!prox || !coords || !workspace || n < 4 || n > RFX_MAX_SAMPLES || max_iter < 1) return RFX_ERR_INVALID;
+  if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
+  if (workspace_bytes < rfx_mds_dense_workspace_bytes(n, precision)) return RFX_ERR_WORKSPACE;
+  if (precision == RFX_PREC_FP64)
+    return run_dense<double>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_MATRIX, 1.0, nout, max_iter, tol,
+                             workspace, coords, eigvals_host, iters_host);
+  return run_dense<float>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_MATRIX, 1.0, nout, max_iter, tol, workspace,
+                          coords, eigvals_host, iters_host);
 }

@@ -1,0 +1,129 @@
+"""GPU parity of the batched path (many forests in one leaf launch, one
+segmented leaf-bucket sort and one persistent gather launch): every
+problem's UPPER counts bit-exact against the oracle, eager and as replays of
+the captured hipGraph, and after autotuning."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rfx_amd import _lib
+from rfx_amd.batch import ForestBatch
+from rfx_amd.forest import Forest, make_dataset
+from rfx_amd.proximity import Accumulator, DeviceForest
+
+pytestmark = pytest.mark.gpu
+
+# (n, p, classes, T, seed): ragged sizes, n not a multiple of 256, T not a
+# multiple of the 4-tree leaf groups, a one-tree forest and a tiny problem
+SHAPES = [(1000, 8, 3, 50, 1), (37, 5, 2, 3, 2), (2500, 12, 4, 21, 3), (300, 20, 6, 1, 4), (4097, 10, 5, 13, 5)]
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+def _problems(shapes):
+    out = []
+    for n, p, ncl, T, seed in shapes:
+        x, y = make_dataset(_lib.DATA_GAUSS_MIXTURE, n, p, ncl, seed)
+        f = Forest.grow(x, y, T, iseed=seed)
+        out.append((f, x))
+    return out
+
+
+def _expected(problems, variant):
+    exp = []
+    for f, x in problems:
+        nodextr = O.leaf_assign(f, x)
+        exp.append(O.counts(f, nodextr, variant, _lib.LAYOUT_UPPER))
+    return exp
+
+
+@pytest.fixture(scope="module")
+def problems(gpu):
+    probs = _problems(SHAPES)
+    dfs = [DeviceForest(f, x) for f, x in probs]
+    return probs, dfs
+
+
+@pytest.mark.parametrize("variant", ["oob_upper", "all", "pair"])
+def test_batch_counts_match_oracle(gpu, problems, variant):
+    probs, dfs = problems
+    _say(f"batch eager variant {variant} shapes {SHAPES}")
+    b = ForestBatch(dfs, variant)
+    b.run(use_graph=False)
+    torch.cuda.synchronize()
+    if variant == "pair":
+        ea, eo = _expected(probs, _lib.PROX_ALL), _expected(probs, _lib.PROX_OOB_UPPER)
+        for k in range(len(dfs)):
+            assert np.array_equal(b.out[k].cpu().numpy(), ea[k]), f"problem {k} ALL"
+            assert np.array_equal(b.out2[k].cpu().numpy(), eo[k]), f"problem {k} OOB"
+    else:
+        v = _lib.PROX_OOB_UPPER if variant == "oob_upper" else _lib.PROX_ALL
+        for k, e in enumerate(_expected(probs, v)):
+            assert np.array_equal(b.out[k].cpu().numpy(), e), f"problem {k}"
+    b.close()
+
+
+def test_batch_graph_replays_and_tuning(gpu, problems):
+    probs, dfs = problems
+    exp = _expected(probs, _lib.PROX_OOB_UPPER)
+    b = ForestBatch(dfs, "oob_upper")
+    for tuning in [(1, 1), (16, 0)]:
+        b.set_tuning(*tuning)
+        for rep in range(3):
+            _say(f"batch graph replay {rep} tuning {tuning}")
+            for o in b.out:
+                o.fill_(-7)  # eager work between replays; the replay must overwrite all of it
+            torch.cuda.synchronize()
+            b.run(use_graph=True)
+            torch.cuda.synchronize()
+            for k, e in enumerate(exp):
+                assert np.array_equal(b.out[k].cpu().numpy(), e), f"replay {rep} problem {k}"
+    _say("batch autotune")
+    ms = b.autotune(reps=1)
+    assert ms > 0
+    lpi, bpc = b.tuning()
+    assert lpi in (1, 4, 16) and bpc >= 1
+    for o in b.out:
+        o.fill_(-7)
+    b.run(use_graph=True)
+    torch.cuda.synchronize()
+    for k, e in enumerate(exp):
+        assert np.array_equal(b.out[k].cpu().numpy(), e), f"after autotune problem {k}"
+    b.close()
+
+
+def test_batch_long_rows_match_single_problem_path(gpu):
+    # n > the 8192-column LDS chunk: rows span several chunks; checked against
+    # the one-problem GATHER path (itself pinned to the oracle)
+    shapes = [(20000, 6, 3, 4, 11), (9000, 6, 3, 6, 12)]
+    probs = _problems(shapes)
+    dfs = [DeviceForest(f, x) for f, x in probs]
+    _say(f"batch long rows {shapes}")
+    b = ForestBatch(dfs, "pair")
+    b.run(use_graph=True)
+    torch.cuda.synchronize()
+    for k, (df, (f, x)) in enumerate(zip(dfs, probs)):
+        keys, _ = df.leaf_assign()
+        acc = Accumulator(df.n, df.ntree, df.total_nodes)
+        ra, ro = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+        torch.cuda.synchronize()
+        assert torch.equal(b.out[k], ra), f"problem {k} ALL"
+        assert torch.equal(b.out2[k], ro), f"problem {k} OOB"
+    # and the oracle on the largest problem
+    f, x = probs[0]
+    exp = O.counts(f, O.leaf_assign(f, x), _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
+    assert np.array_equal(b.out2[0].cpu().numpy(), exp)
+    b.close()
+
+
+def test_batch_rejects_bad_input(gpu, problems):
+    _, dfs = problems
+    with pytest.raises(ValueError):
+        ForestBatch(dfs, "oob_asym")
+    with pytest.raises(RuntimeError):
+        ForestBatch([], "all")
