@@ -18,9 +18,10 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "librfx.so"
+PYMOD_SRC = CSRC / "ref_py.cpp"  # pybind11 module of the reference-signature entry points
 
 HIP_SOURCES = ["leaf.hip", "accum.hip", "finish.hip", "mds.hip", "factor.hip", "batch.hip"]
-CXX_SOURCES = ["forest.cpp"]
+CXX_SOURCES = ["forest.cpp", "shim.cpp"]
 # Per-file extra flags: finishprox must not contract a*b+c into an FMA.
 HIP_EXTRA = {"finish.hip": ["-ffp-contract=off"]}
 
@@ -72,7 +73,24 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", "-o", str(tmp), *objs, "-pthread"])
     os.replace(tmp, LIB)
+    build_pymod()
     return LIB
+
+
+def build_pymod() -> Path:
+    """rfx_amd/_rfxref*.so, linked to librfx.so beside it."""
+    import sysconfig
+
+    import pybind11
+
+    out = PKG / ("_rfxref" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          f"-I{ROOT / 'include'}", f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+          str(PYMOD_SRC), f"-L{PKG}", "-lrfx", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

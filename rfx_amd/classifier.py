@@ -1,13 +1,47 @@
 """RandomForestClassifier mirror of the reference pybind class, proximity path.
 
-Keeps the reference's constructor options and the four proximity/MDS methods
-of python/randomforest_py.cpp:606-1420 (get_proximity_matrix,
-get_lowrank_factors, compute_mds_3d_cpu, compute_mds_from_factors), with the
-reference's return dtypes, backed by the HIP path in librfx.
+Keeps the reference's constructor options (python/randomforest_py.cpp:617-647,
+1352-1385, same names, order and defaults) and its proximity/MDS methods
+(1114-1318: get_proximity_matrix, compute_proximity_matrix,
+get_lowrank_factors, compute_mds_3d_cpu, compute_mds_from_factors,
+compute_mds_3d_from_factors) with the reference's return dtypes and shapes,
+backed by the HIP path in librfx:
+
+  fit                  grows the forest on the host exactly as the reference
+                       CPU grower (bootstrap seed 12345 + t, grower seed
+                       iseed + t), categorical auto-detection as the
+                       reference's fit (771-803), then on the GPU: leaf
+                       assignment, OOB_UPPER counts (or, with use_qlora, the
+                       rank-`rank` factors of C/T) and nout = 2 * #OOB trees.
+  get_proximity_matrix float32 n x n of rf::cpu_finishprox's result
+                       (1138-1146); raises in low-rank mode as the reference.
+  compute_mds_3d_cpu   the reference chain's 3-D MDS (OOB counts ->
+                       finishprox -> float32 -> nout renormalisation ->
+                       classical MDS), computed on the GPU in FP64.
+  compute_mds_from_factors / compute_mds_3d_from_factors
+                       MDS from the stored factors (k <= 3).
+
+Options outside the proximity path (importance, prediction, RF-GAP,
+case-wise weights, regression losses) are accepted for signature
+compatibility; the ones that would change the proximity (use_rfgap,
+use_casewise) raise NotImplementedError, as do the methods outside the path.
+use_gpu is accepted: this implementation always runs on the MI355X.
 """
 from __future__ import annotations
 
 import numpy as np
+
+from . import _lib
+
+# reference quant_mode names -> stored factor format here
+QUANT_FORMATS = {
+    "fp32": _lib.PREC_FP32,
+    "bf16": _lib.PREC_BF16,
+    "fp16": _lib.PREC_BF16,  # 16-bit storage: bfloat16 here
+    "int8": _lib.PREC_INT8,
+    "fp8": _lib.PREC_FP8,
+    "nf4": _lib.PREC_FP8,  # the lowest-precision storage here: FP8 e4m3 with per-column scales
+}
 
 
 def detect_categorical(x: np.ndarray, maxcat: int) -> np.ndarray:
@@ -26,3 +60,145 @@ def detect_categorical(x: np.ndarray, maxcat: int) -> np.ndarray:
         if all_int and 1 < nu <= maxcat and col.min() >= 0.0 and col.max() < np.float32(maxcat):
             cat[m] = nu
     return cat
+
+
+class RandomForestClassifier:
+    def __init__(self, nsample: int = 1000, ntree: int = 100, mdim: int = 10, nclass: int = 2, maxcat: int = 10,
+                 mtry: int = 0, maxnode: int = 0, minndsize: int = 1, ncsplit: int = 25, ncmax: int = 25,
+                 iseed: int = 12345, compute_proximity: bool = False, compute_importance: bool = True,
+                 compute_local_importance: bool = False, use_gpu: bool = False, use_qlora: bool = False,
+                 quant_mode: str = "nf4", use_sparse: bool = False, sparsity_threshold: float = 1e-6,
+                 batch_size: int = 0, nodesize: int = 5, cutoff: float = 0.01, show_progress: bool = True,
+                 progress_desc: str = "Training Random Forest", gpu_loss_function: str = "gini", rank: int = 32,
+                 n_threads_cpu: int = 0, use_rfgap: bool = False, importance_method: str = "local_imp",
+                 clique_M: int = -1, use_casewise: bool = False):
+        if use_rfgap or use_casewise:
+            raise NotImplementedError("RF-GAP and case-wise proximities are not part of this proximity path")
+        if quant_mode not in QUANT_FORMATS:
+            raise ValueError(f"quant_mode must be one of {sorted(QUANT_FORMATS)}")
+        if gpu_loss_function != "gini":
+            raise NotImplementedError("only the classification (gini) grower is part of this path")
+        self.nsample, self.ntree, self.mdim, self.nclass = nsample, ntree, mdim, nclass
+        self.maxcat, self.mtry, self.maxnode, self.minndsize = maxcat, mtry, maxnode, minndsize
+        self.iseed = iseed
+        self.compute_proximity = compute_proximity
+        self.use_gpu, self.use_qlora, self.quant_mode = use_gpu, use_qlora, quant_mode
+        self.rank = rank
+        self.n_threads_cpu = n_threads_cpu
+        self.batch_size = batch_size
+        # accepted for signature compatibility, outside the proximity path
+        self._unused = dict(ncsplit=ncsplit, ncmax=ncmax, compute_importance=compute_importance,
+                            compute_local_importance=compute_local_importance, use_sparse=use_sparse,
+                            sparsity_threshold=sparsity_threshold, nodesize=nodesize, cutoff=cutoff,
+                            show_progress=show_progress, progress_desc=progress_desc,
+                            importance_method=importance_method, clique_M=clique_M)
+        self.forest = None
+        self.cat = None
+        self._counts = None  # device OOB_UPPER counts (UPPER int32)
+        self._nout = None
+        self._factors = None  # (stored factors, scales, format, U float32)
+
+    # ------------------------------------------------------------------ fit
+    def fit(self, X: np.ndarray, y: np.ndarray, sample_weight=None) -> "RandomForestClassifier":
+        import torch
+
+        from .forest import Forest
+        from .proximity import Accumulator, DeviceForest, lowrank_factors, quantize_factors
+
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.asarray(y)
+        if y.dtype != np.int32:
+            raise TypeError("y must be an int32 array (the reference reinterprets the buffer as int32)")
+        n, p = X.shape
+        self.nsample, self.mdim = n, p
+        self.nclass = max(self.nclass, int(y.max()) + 1)
+        self.cat = detect_categorical(X, self.maxcat) if self.maxcat > 1 else np.ones(p, np.int32)
+        self.forest = Forest.grow(X, y, self.ntree, mtry=self.mtry, minndsize=self.minndsize, maxnode=self.maxnode,
+                                  iseed=self.iseed, nthreads=self.n_threads_cpu)
+        self._counts = self._factors = None
+        if not self.compute_proximity:
+            return self
+        f = self.forest
+        df = DeviceForest(f, X, self.cat)
+        keys, _ = df.leaf_assign()
+        self._nout = torch.from_numpy(f.nout(2)).to(df.device)
+        if self.use_qlora:
+            rank = min(self.rank, 128, n)
+            U, _ = lowrank_factors(keys, f, rank=rank)
+            fmt = QUANT_FORMATS[self.quant_mode]
+            Uq, sc = quantize_factors(U, fmt)
+            self._factors = (Uq, sc, fmt, U)
+        else:
+            acc = Accumulator(n, f.ntree, f.total_nodes, device=str(df.device))
+            self._counts = acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
+        torch.cuda.synchronize()
+        return self
+
+    def _need_fit(self, what: str) -> None:
+        if self.forest is None:
+            raise RuntimeError(f"Model must be fitted before calling {what}()")
+
+    # ------------------------------------------------------------ proximity
+    def get_proximity_matrix(self) -> np.ndarray:
+        from .proximity import finishprox, unpack_upper
+
+        self._need_fit("get_proximity_matrix")
+        if self._factors is not None:
+            raise RuntimeError("the proximity is held as low-rank factors (use_qlora=True): use "
+                               "get_lowrank_factors() or compute_mds_from_factors()")
+        if self._counts is None:
+            raise RuntimeError("Proximity matrix not available. Set compute_proximity=True when fitting.")
+        n = self.nsample
+        S = finishprox(self._counts, _lib.LAYOUT_UPPER, self._nout, n, _lib.LAYOUT_UPPER)
+        return unpack_upper(S.cpu().numpy(), n).astype(np.float32)
+
+    compute_proximity_matrix = get_proximity_matrix
+
+    def get_lowrank_factors(self) -> tuple[np.ndarray, np.ndarray, int]:
+        self._need_fit("get_lowrank_factors")
+        if self._factors is None:
+            raise RuntimeError("low-rank factors need compute_proximity=True and use_qlora=True")
+        U = self._factors[3].cpu().numpy().astype(np.float64)
+        return U, U.copy(), U.shape[1]
+
+    # ------------------------------------------------------------------ MDS
+    def compute_mds_3d_cpu(self) -> np.ndarray:
+        from .proximity import mds_dense
+
+        self._need_fit("compute_mds_3d_cpu")
+        if self._counts is None:
+            raise RuntimeError("Proximity matrix not available. Set compute_proximity=True when fitting.")
+        X, _, _ = mds_dense(self._counts, _lib.LAYOUT_UPPER, self.nsample, _lib.MDS_REFERENCE, nout=self._nout,
+                            precision=_lib.PREC_FP64)
+        return X.cpu().numpy()
+
+    def compute_mds_from_factors(self, k: int = 3) -> np.ndarray:
+        from .proximity import mds_factors
+
+        self._need_fit("compute_mds_from_factors")
+        if self._factors is None:
+            raise RuntimeError("compute_mds_from_factors needs use_qlora=True and compute_proximity=True")
+        if not 1 <= k <= 3:
+            raise ValueError("k must be 1, 2 or 3")
+        Uq, sc, fmt, _ = self._factors
+        X, _, _ = mds_factors(Uq, sc, fmt)
+        return X.cpu().numpy()[:, :k].copy()
+
+    def compute_mds_3d_from_factors(self) -> np.ndarray:
+        return self.compute_mds_from_factors(3)
+
+    # --------------------------------------------------------------- shape
+    def get_n_samples(self) -> int:
+        return self.nsample
+
+    def get_n_features(self) -> int:
+        return self.mdim
+
+    def get_n_classes(self) -> int:
+        return self.nclass
+
+    def _outside(self, *a, **k):
+        raise NotImplementedError("outside the proximity path of this implementation")
+
+    predict = predict_proba = get_oob_error = feature_importances_ = get_local_importance = _outside
+    get_oob_predictions = _outside

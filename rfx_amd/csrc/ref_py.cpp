@@ -1,0 +1,135 @@
+// pybind11 module rfx_amd._rfxref: the reference-signature C++ entry points
+// of rfx_ref.hpp for Python callers and tests (numpy arrays in and out).
+
+#include <hip/hip_runtime_api.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <stdexcept>
+
+#include "rfx_ref.hpp"
+
+namespace py = pybind11;
+using rfx::ref::integer_t;
+
+namespace {
+
+using I32 = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+using F64F = py::array_t<double, py::array::f_style | py::array::forcecast>;
+
+void need(bool ok, const char* what) {
+  if (!ok) throw std::invalid_argument(what);
+}
+
+// prox: n x n float64 Fortran-ordered (column-major, as the reference), updated in place.
+void gpu_proximity(I32 nodestatus, I32 nodextr, I32 nin, py::array_t<double> prox) {
+  const integer_t n = static_cast<integer_t>(nodextr.size());
+  need(nin.size() == n && prox.ndim() == 2 && prox.shape(0) == n && prox.shape(1) == n, "shape mismatch");
+  need((prox.flags() & py::array::f_style) != 0 && prox.writeable(), "prox must be a writable Fortran-ordered array");
+  rfx::ref::gpu_proximity(nodestatus.data(), nodextr.data(), nin.data(), n,
+                          static_cast<integer_t>(nodestatus.size()), prox.mutable_data(), nullptr, nullptr, nullptr,
+                          nullptr, nullptr, nullptr);
+}
+
+py::array_t<uint16_t> upper_triangle_fp16(I32 nodestatus, I32 nodextr, I32 nin) {
+  const integer_t n = static_cast<integer_t>(nodextr.size());
+  need(nin.size() == n, "shape mismatch");
+  const size_t len = static_cast<size_t>(n) * (n + 1) / 2;
+  void* d = nullptr;
+  if (hipMalloc(&d, 2 * len) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+  py::array_t<uint16_t> out(len);
+  try {
+    rfx::ref::gpu_proximity_upper_triangle_fp16(nodestatus.data(), nodextr.data(), nin.data(), n,
+                                                static_cast<integer_t>(nodestatus.size()),
+                                                static_cast<uint16_t*>(d), nullptr, nullptr, nullptr, nullptr,
+                                                nullptr, nullptr);
+    if (hipMemcpy(out.mutable_data(), d, 2 * len, hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("hipMemcpy failed");
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  (void)hipFree(d);
+  return out;
+}
+
+F64F finishprox(I32 nout, F64F prox) {
+  const integer_t n = static_cast<integer_t>(nout.size());
+  need(prox.ndim() == 2 && prox.shape(0) == n && prox.shape(1) == n, "shape mismatch");
+  F64F out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(n)});
+  rfx::ref::cpu_finishprox(n, nout.data(), const_cast<double*>(prox.data()), out.mutable_data());
+  return out;
+}
+
+py::array_t<double> compute_mds_3d(F64F prox, py::object oob_counts, bool use_rfgap) {
+  need(prox.ndim() == 2 && prox.shape(0) == prox.shape(1), "prox must be square");
+  const integer_t n = static_cast<integer_t>(prox.shape(0));
+  I32 oc;
+  const integer_t* ocp = nullptr;
+  if (!oob_counts.is_none()) {
+    oc = oob_counts.cast<I32>();
+    need(oc.size() == n, "oob_counts length");
+    ocp = oc.data();
+  }
+  std::vector<double> c = rfx::ref::compute_mds_3d_cpu(prox.data(), n, true, ocp, use_rfgap);
+  py::array_t<double> out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(3)});
+  std::copy(c.begin(), c.end(), out.mutable_data());
+  return out;
+}
+
+class LowRank {
+ public:
+  LowRank(integer_t n, integer_t rank, int32_t format) : m_(n, rank, static_cast<rfx::ref::FactorFormat>(format)) {
+    if (!m_.initialize()) throw std::runtime_error("initialize failed");
+  }
+  void add_tree(I32 nodestatus, I32 nodextr, I32 nin) {
+    need(nodextr.size() == m_.get_n_samples() && nin.size() == m_.get_n_samples(), "shape mismatch");
+    m_.add_tree_contribution(nodestatus.data(), nodextr.data(), nin.data(),
+                             static_cast<integer_t>(nodestatus.size()));
+  }
+  void finalize() { m_.finalize_accumulation(0, true); }
+  py::tuple factors() {
+    double *A = nullptr, *B = nullptr;
+    integer_t r = 0;
+    if (!rfx::ref::get_lowrank_factors_host(&m_, &A, &B, &r, m_.get_n_samples()))
+      throw std::runtime_error("get_lowrank_factors_host failed");
+    const py::ssize_t n = m_.get_n_samples();
+    // column-major n x r -> numpy Fortran arrays
+    F64F a({n, static_cast<py::ssize_t>(r)}), b({n, static_cast<py::ssize_t>(r)});
+    std::copy(A, A + n * r, a.mutable_data());
+    std::copy(B, B + n * r, b.mutable_data());
+    delete[] A;
+    delete[] B;
+    return py::make_tuple(a, b, r);
+  }
+  py::array_t<double> mds(integer_t k) {
+    std::vector<double> c = rfx::ref::compute_mds_from_factors_host(&m_, k);
+    if (c.empty()) throw std::runtime_error("compute_mds_from_factors_host failed");
+    py::array_t<double> out({static_cast<py::ssize_t>(m_.get_n_samples()), static_cast<py::ssize_t>(k)});
+    std::copy(c.begin(), c.end(), out.mutable_data());
+    return out;
+  }
+  std::vector<double> eigenvalues() const { return m_.top_eigenvalues(); }
+
+ private:
+  rfx::ref::LowRankProximityMatrix m_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_rfxref, m) {
+  m.doc() = "Reference-signature C++ entry points of the MI355X proximity path";
+  m.def("gpu_proximity", &gpu_proximity, "per-tree OOB proximity accumulated into prox (n x n, column-major)");
+  m.def("upper_triangle_fp16", &upper_triangle_fp16, "one tree's OOB fold as IEEE half bits, packed upper");
+  m.def("finishprox", &finishprox, "rf::cpu_finishprox semantics");
+  m.def("compute_mds_3d", &compute_mds_3d, py::arg("prox"), py::arg("oob_counts") = py::none(),
+        py::arg("use_rfgap") = false);
+  py::class_<LowRank>(m, "LowRankProximityMatrix")
+      .def(py::init<integer_t, integer_t, int32_t>())
+      .def("add_tree", &LowRank::add_tree)
+      .def("finalize", &LowRank::finalize)
+      .def("factors", &LowRank::factors)
+      .def("mds", &LowRank::mds, py::arg("k") = 3)
+      .def("eigenvalues", &LowRank::eigenvalues);
+}

@@ -38,3 +38,17 @@ def keys_from_nodextr(forest: Forest, nodextr: np.ndarray) -> np.ndarray:
         term = st[nd] == -1
         keys[t, term] = ((b + nd[term]) << 1) | (forest.nin[t, term] > 0)
     return keys.astype(np.int32)
+
+
+def centred(M: np.ndarray) -> np.ndarray:
+    D = M.max() - M
+    np.fill_diagonal(D, 0.0)
+    D2 = D * D
+    r = D2.mean(axis=1)
+    return -0.5 * (D2 - r[:, None] - r[None, :] + r.mean())
+
+
+def eig_residuals(B: np.ndarray, X: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """||B v - lambda v|| / (lambda ||v||) per embedding column (v = x / sqrt(lambda))."""
+    V = X / np.sqrt(lam[None, :])
+    return np.linalg.norm(B @ V - V * lam[None, :], axis=0) / (np.abs(lam) * np.linalg.norm(V, axis=0))

@@ -7,6 +7,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 from rfx_amd import _lib
 
@@ -63,3 +64,32 @@ def test_workspace_size_formula():
     assert L.rfx_prox_workspace_bytes(1000, 10, 500, _lib.ALGO_DENSE) == 0
     b = L.rfx_prox_workspace_bytes(1000, 10, 500, _lib.ALGO_BUCKET)
     assert b >= 8 * 500 + 8 * 1000 * 10
+
+
+def test_reference_signature_module_loads():
+    import rfx_amd._rfxref as R
+
+    for name in ("gpu_proximity", "upper_triangle_fp16", "finishprox", "compute_mds_3d", "LowRankProximityMatrix"):
+        assert hasattr(R, name)
+
+
+def test_classifier_options_mirror_reference():
+    import inspect
+
+    from rfx_amd.classifier import QUANT_FORMATS, RandomForestClassifier
+
+    params = list(inspect.signature(RandomForestClassifier.__init__).parameters)[1:]
+    assert params[:12] == ["nsample", "ntree", "mdim", "nclass", "maxcat", "mtry", "maxnode", "minndsize", "ncsplit",
+                           "ncmax", "iseed", "compute_proximity"]
+    assert params[-6:] == ["rank", "n_threads_cpu", "use_rfgap", "importance_method", "clique_M", "use_casewise"]
+    clf = RandomForestClassifier()
+    assert clf.rank == 32 and clf.quant_mode == "nf4" and clf.iseed == 12345
+    assert set(QUANT_FORMATS) >= {"nf4", "int8", "fp16", "fp32"}
+    with pytest.raises(NotImplementedError):
+        RandomForestClassifier(use_rfgap=True)
+    with pytest.raises(NotImplementedError):
+        RandomForestClassifier(use_casewise=True)
+    with pytest.raises(RuntimeError):
+        clf.get_proximity_matrix()
+    with pytest.raises(TypeError):
+        clf.fit(np.zeros((10, 2), np.float32), np.zeros(10, np.int64))

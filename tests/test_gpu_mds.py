@@ -11,7 +11,7 @@ from oracle import oracle as O
 from rfx_amd import _lib
 from rfx_amd.proximity import Accumulator, DeviceForest, mds_dense, unpack_upper
 
-from helpers import small_forest
+from helpers import centred, eig_residuals, small_forest
 
 pytestmark = pytest.mark.gpu
 
@@ -38,20 +38,6 @@ def test_mds_counts_over_T(gpu, precision, tol_rel):
     print(it, eig[:4], w[:4], e, flush=True)
     assert e["rel_frobenius"] < tol_rel
     assert np.allclose(eig[:3], w[:3], rtol=tol_rel)
-
-
-def centred(M: np.ndarray) -> np.ndarray:
-    D = M.max() - M
-    np.fill_diagonal(D, 0.0)
-    D2 = D * D
-    r = D2.mean(axis=1)
-    return -0.5 * (D2 - r[:, None] - r[None, :] + r.mean())
-
-
-def eig_residuals(B: np.ndarray, X: np.ndarray, lam: np.ndarray) -> np.ndarray:
-    """||B v - lambda v|| / (lambda ||v||) per embedding column (v = x / sqrt(lambda))."""
-    V = X / np.sqrt(lam[None, :])
-    return np.linalg.norm(B @ V - V * lam[None, :], axis=0) / (np.abs(lam) * np.linalg.norm(V, axis=0))
 
 
 @pytest.mark.parametrize("kind,n,p,ncl,T,seed", [(_lib.DATA_COVTYPE_LIKE, 1500, 54, 7, 100, 12),

@@ -1,0 +1,112 @@
+"""The reference-signature C++ entry points (include/rfx_ref.hpp, through the
+pybind11 module rfx_amd._rfxref) on the GPU: per-tree proximity and
+finishprox bit-exact against the reference's golden vectors, the per-tree
+FP16 upper triangle against the oracle, the matrix MDS against the FP64
+classical-MDS oracle, and the low-rank factor object's host helpers."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rfx_amd import _lib
+
+from helpers import centred, eig_residuals, load_golden, small_forest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def R(gpu):
+    import rfx_amd._rfxref as mod
+    return mod
+
+
+def _tree(f, t):
+    b, e = f.node_offset[t], f.node_offset[t + 1]
+    return np.ascontiguousarray(f.nodestatus[b:e], np.int32), int(e - b)
+
+
+@pytest.mark.parametrize("name", ["gauss_n200.npz", "covtype_cat_n240.npz"])
+def test_per_tree_proximity_and_finishprox_match_golden(R, name):
+    _say(f"ref api per-tree proximity {name}")
+    g = load_golden(name)
+    f = g["forest"]
+    n = f.n
+    prox = np.zeros((n, n), np.float64, order="F")
+    for t in range(f.ntree):
+        st, _ = _tree(f, t)
+        R.gpu_proximity(st, np.ascontiguousarray(g["nodextr"][t], np.int32),
+                        np.ascontiguousarray(f.nin[t], np.int32), prox)
+    assert np.array_equal(prox.reshape(-1, order="F"), g["prox"])
+    S = R.finishprox(np.ascontiguousarray(g["nout"], np.int32), prox)
+    assert np.array_equal(S.reshape(-1, order="F").view(np.uint64), g["proxsym"].view(np.uint64))
+
+
+def test_per_tree_upper_fp16_matches_oracle(R):
+    x, y, f = small_forest(_lib.DATA_GAUSS_MIXTURE, 900, 6, 3, 3, seed=21)
+    nodextr = O.leaf_assign(f, x)
+    for t in range(f.ntree):
+        _say(f"ref api upper fp16 tree {t}")
+        st, _ = _tree(f, t)
+        got = R.upper_triangle_fp16(st, np.ascontiguousarray(nodextr[t], np.int32),
+                                    np.ascontiguousarray(f.nin[t], np.int32))
+        sub = f.subset(slice(t, t + 1))
+        exp = O.counts(sub, nodextr[t:t + 1], _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
+        assert exp.max() <= 1
+        assert np.array_equal(got, exp.astype(np.float16).view(np.uint16))
+
+
+def test_matrix_mds_matches_oracle(R):
+    _say("ref api compute_mds_3d")
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 800, 8, 3, 60, seed=22)
+    nodextr = O.leaf_assign(f, x)
+    n = f.n
+    P = O.counts(f, nodextr, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL).astype(np.float64)
+    nout = f.nout(2)
+    S = O.finishprox(n, nout, P).reshape(n, n, order="F")
+    Sf = np.asfortranarray(S.astype(np.float32).astype(np.float64))  # the pybind path's float32 round trip
+    X = R.compute_mds_3d(Sf, nout, False)
+    M = O.mds_input_reference(S, nout)
+    Xref, w = O.classical_mds(M, 3)
+    lam = np.sum(X * X, axis=0)  # x_c = v_c sqrt(lambda_c)
+    _say(f"eigs {w[:5]} got {lam}")
+    # eigenvalues and eigen-residuals pin the result whatever the eigengap
+    assert np.allclose(lam, w[:3], rtol=1e-9)
+    assert eig_residuals(centred(M), X, lam).max() < 1e-5
+    if w[2] / w[3] > 1.02:
+        assert O.embedding_errors(X, Xref)["rel_frobenius"] < 1e-6
+    # without OOB counts the matrix is embedded as it is
+    X2 = R.compute_mds_3d(Sf, None, False)
+    lam2 = np.sum(X2 * X2, axis=0)
+    _, w2 = O.classical_mds(Sf, 3)
+    assert np.allclose(lam2, w2[:3], rtol=1e-9)
+    assert eig_residuals(centred(Sf), X2, lam2).max() < 1e-5
+    with pytest.raises(RuntimeError):
+        R.compute_mds_3d(Sf, nout, True)  # RF-GAP is not part of this path
+
+
+def test_lowrank_object_and_host_helpers(R):
+    _say("ref api LowRankProximityMatrix")
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 1500, 8, 3, 40, seed=23)
+    nodextr = O.leaf_assign(f, x)
+    n, rank = f.n, 16
+    m = R.LowRankProximityMatrix(n, rank, _lib.PREC_FP32)
+    for t in range(f.ntree):
+        st, _ = _tree(f, t)
+        m.add_tree(st, np.ascontiguousarray(nodextr[t], np.int32), np.ascontiguousarray(f.nin[t], np.int32))
+    m.finalize()
+    A, B, r = m.factors()
+    assert r == rank and A.shape == (n, rank) and np.array_equal(A, B)
+    C = O.counts(f, nodextr, _lib.PROX_ALL, _lib.LAYOUT_FULL).reshape(n, n).astype(np.float64) / f.ntree
+    w = np.linalg.eigvalsh(C)[::-1]
+    best = np.sqrt(np.sum(w[rank:] ** 2))  # Eckart-Young optimum of a rank-r PSD approximation
+    err = np.linalg.norm(C - A @ A.T)
+    _say(f"factor error {err:.4f} vs optimum {best:.4f}")
+    assert err <= 1.01 * best
+    X = m.mds(3)
+    assert X.shape == (n, 3) and np.all(np.isfinite(X))
