@@ -89,6 +89,9 @@ def main() -> int:
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mds", action="store_true", help="c5: also time the 3-D MDS of every instance")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one device-to-device copy of 1 GiB (a known-size kernel that scales the HBM "
+                         "byte counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the proximity path runs on a single MI355X here "
@@ -102,6 +105,13 @@ def main() -> int:
     from rfx_amd.forest import Forest, make_dataset
     from rfx_amd.proximity import Accumulator, DeviceForest
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 1 GiB read + 1 GiB written")
     if args.workload == "c5":
         return bench_c5(args)
     NAMES = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket", _lib.ALGO_GATHER: "gather"}
