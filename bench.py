@@ -225,7 +225,7 @@ def factor_mds_section(df, forest, rank: int) -> dict:
     from rfx_amd import _lib
     from rfx_amd.forest import Forest, make_dataset
     from rfx_amd.proximity import (Accumulator, DeviceForest, lowrank_factors, mds_dense, mds_factors,
-                                   quantize_factors)
+                                   mds_factors_mixed, quantize_factors)
 
     def errs(X, Xref):
         X = X.cpu().numpy()
@@ -267,6 +267,13 @@ def factor_mds_section(df, forest, rank: int) -> dict:
         out[name] = {"mds_s": round(t, 4), "iters": it, "rel_err_vs_fp64": rel, "max_angle_rad": ang,
                      "bytes_per_factor_row": rank * {"fp32": 4, "bf16": 2, "int8": 1, "fp8": 1}[name]}
         log(f"factor mds {name}: {t:.4f} s rel {rel:.2e}")
+        # bf16 matrix-core operator on the stored factors + FP32 refinement sweeps on the FP32 factors
+        mds_factors_mixed(Uq, sc, fmt, U32)  # warm-up
+        (X, ev, itm), t = timed(lambda: mds_factors_mixed(Uq, sc, fmt, U32))
+        rel, ang = errs(X, Xref)
+        out[name]["mfma_refined"] = {"mds_s": round(t, 4), "mfma_iters": itm[0], "fp32_refine_sweeps": itm[1],
+                                     "rel_err_vs_fp64": rel, "max_angle_rad": ang}
+        log(f"factor mds {name} mfma+refine: {t:.4f} s iters {itm} rel {rel:.2e}")
     # truncation: the exact dense FP64 MDS of a seeded 20k problem vs its factor MDS
     xs, ys = make_dataset(_lib.DATA_COVTYPE_LIKE, 20000, 54, 7, 7)
     fs = Forest.grow(xs, ys, forest.ntree, mtry=7, iseed=0)

@@ -89,6 +89,9 @@ _SIGS = {
     "rfx_quantize_factors": (_i32, [_p, _p, _i64, _i32, _i32, _p, _p]),
     "rfx_mds_factors_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "rfx_mds_factors": (_i32, [_p, _p, _p, _i32, _i64, _i32, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_mds_factors_mixed_workspace_bytes": (_sz, [_i64, _i32]),
+    "rfx_mds_factors_mixed": (_i32, [_p, _p, _p, _i32, _p, _i64, _i32, _i32, C.c_double, _i32, C.c_double, _p, _sz,
+                                     _p, _p, _p]),
     "rfx_mds_dense_matrix": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
     "rfx_counts_convert": (_i32, [_p, _p, _i64, _i32, _p]),
     "rfx_batch_create": (_i32, [_p, _i32, _i32, C.POINTER(_p)]),

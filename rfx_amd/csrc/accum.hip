@@ -267,8 +267,10 @@ __global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restric
                                                           int32_t* __restrict__ out2) {
   __shared__ uint32_t acc[kGatherChunk];
   __shared__ uint32_t diag_valid;
-  gather_line_body<MODE, LAYOUT>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, blockIdx.x,
-                                 blockIdx.y, acc, diag_valid);
+  const int64_t line = xcd_row(blockIdx.x);
+  if (line >= n) return;
+  gather_line_body<MODE, LAYOUT>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line, blockIdx.y,
+                                 acc, diag_valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,7 +293,8 @@ __global__ __launch_bounds__(256) void gather_walk_kernel(const uint2* __restric
                                                           int32_t* __restrict__ out2) {
   __shared__ uint32_t acc[kWalkChunk];
   __shared__ uint32_t diag_valid;
-  const int64_t line = blockIdx.x;
+  const int64_t line = xcd_row(blockIdx.x);
+  if (line >= n) return;
   const int tid = threadIdx.x;
   uint32_t cur[TPT], end[TPT], nv[TPT];
   uint32_t bagm = 0, valid = 0;
@@ -429,12 +432,12 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   if (n <= kGatherLineMaxN) {
-    dim3 grid(static_cast<unsigned>(n), rfx_ceil_div(n, kGatherChunk));
+    dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), rfx_ceil_div(n, kGatherChunk));
     hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, w.pe, w.bstart, w.mbucket, n,
                        ntree, w.members, accumulate, out, out2);
     return rfx_launch_status();
   }
-  const dim3 grid(static_cast<unsigned>(n)), blk(256);
+  const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n))), blk(256);
   if (ntree <= 256)
     hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 1>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
                        w.members, accumulate, out, out2);

@@ -158,6 +158,12 @@ __global__ void quantize_kernel(const float* __restrict__ U, int64_t n, int r, i
   }
 }
 
+}  // namespace
+
+#include "factor_mfma.hpp"
+
+namespace {
+
 // ---------------------------------------------------------------------------
 // matrix-free C V through the leaf buckets
 // ---------------------------------------------------------------------------
@@ -373,96 +379,214 @@ size_t fac_mds_ws(int64_t n, int r) {
          al(8 * P) + al(8 * n) + rfx::SubspaceSolver<Real>::workspace_bytes(n, kMdsK);
 }
 
-template <int FMT, typename Real>
-int fac_mds_run(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, int max_iter, double tol,
-                char* p, double* coords, rfx::SubspaceResult& res, double* r_dev_keep, const Real* V0,
-                const PairTable& pt, int64_t nrb, Real* partial, double* partial64, Real* Z, double* Z64,
-                double* ones) {
-  using ST = typename Store<FMT>::T;
-  const ST* U = static_cast<const ST*>(Uq);
-  const int64_t rpb = fac_rows_per_blk(n);
-  dim3 gr(rfx_ceil_div(pt.count, 256), static_cast<unsigned>(nrb));
-  // FP64 row means of E: E 1 / n
-  hipLaunchKernelGGL((fac_reduce_kernel<FMT, double, 1>), gr, dim3(256), 0, st, U, scale, n, r, pt, ones, rpb,
-                     partial64);
-  hipLaunchKernelGGL((fac_reduce_partials_kernel<double, 1>), dim3(rfx_ceil_div(pt.count, 256)), dim3(256), 0, st,
-                     partial64, static_cast<int>(nrb), pt, Z64);
-  hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
-                     r, pt, Z64, ones, r_dev_keep);
-  std::vector<double> rh(n);
-  RFX_HIP_TRY(hipMemcpyAsync(rh.data(), r_dev_keep, 8 * n, hipMemcpyDeviceToHost, st));
-  RFX_HIP_TRY(hipStreamSynchronize(st));
-  double g = 0.0;
-  for (int64_t i = 0; i < n; ++i) {
-    rh[i] /= static_cast<double>(n);
-    g += rh[i];
-  }
-  g /= static_cast<double>(n);
-  RFX_HIP_TRY(hipMemcpyAsync(r_dev_keep, rh.data(), 8 * n, hipMemcpyHostToDevice, st));
-  RFX_HIP_TRY(hipStreamSynchronize(st));
-
-  rfx::SubspaceSolver<Real> solver(st, n, kMdsK);
-  solver.carve(p);
-  auto apply = [&](const Real* V, Real* W, int) -> int {
-    hipLaunchKernelGGL((fac_reduce_kernel<FMT, Real, kMdsK>), gr, dim3(256), 0, st, U, scale, n, r, pt, V, rpb,
-                       partial);
-    hipLaunchKernelGGL((fac_reduce_partials_kernel<Real, kMdsK>), dim3(rfx_ceil_div(pt.count * kMdsK, 256)),
-                       dim3(256), 0, st, partial, static_cast<int>(nrb), pt, Z);
-    hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale,
-                       n, r, pt, Z, V, W);
-    return rfx_launch_status();
-  };
-  (void)V0;
-  return solver.run(apply, r_dev_keep, g, 3, max_iter, tol, 0x5eedull, coords, res);
-}
+// Workspace of the VALU operator path (also the FP32 refinement of the mixed path).
+template <typename Real>
+struct FacWs {
+  double* rdev;
+  PairTable pt;
+  int64_t nrb;
+  Real* partial;
+  double* partial64;
+  Real* Z;
+  double* Z64;
+  double* ones;
+  char* solver;
+};
 
 template <typename Real>
-int fac_mds_dispatch(hipStream_t st, const void* Uq, const float* scale, int fmt, int64_t n, int r, int max_iter,
-                     double tol, void* ws, double* coords, rfx::SubspaceResult& res) {
+int fac_ws_setup(hipStream_t st, void* ws, int64_t n, int r, FacWs<Real>& f) {
   const int P = r * (r + 1) / 2 + r + 1;
   const int64_t rpb = fac_rows_per_blk(n);
-  const int64_t nrb = (n + rpb - 1) / rpb;
+  f.nrb = (n + rpb - 1) / rpb;
   char* p = static_cast<char*>(ws);
-  double* rdev = reinterpret_cast<double*>(p);
+  f.rdev = reinterpret_cast<double*>(p);
   p += al(8 * n);
   uint8_t* pa = reinterpret_cast<uint8_t*>(p);
   uint8_t* pb = pa + P;
   float* pw = reinterpret_cast<float*>(p + ((2 * P + 15) / 16) * 16);
   p += al(P * 6 + 16);
-  Real* partial = reinterpret_cast<Real*>(p);
-  p += al(sizeof(Real) * nrb * P * kMdsK);
-  double* partial64 = reinterpret_cast<double*>(p);
-  p += al(8 * nrb * P);
-  Real* Z = reinterpret_cast<Real*>(p);
+  f.partial = reinterpret_cast<Real*>(p);
+  p += al(sizeof(Real) * f.nrb * P * kMdsK);
+  f.partial64 = reinterpret_cast<double*>(p);
+  p += al(8 * f.nrb * P);
+  f.Z = reinterpret_cast<Real*>(p);
   p += al(sizeof(Real) * P * kMdsK);
-  double* Z64 = reinterpret_cast<double*>(p);
+  f.Z64 = reinterpret_cast<double*>(p);
   p += al(8 * P);
-  double* ones = reinterpret_cast<double*>(p);
+  f.ones = reinterpret_cast<double*>(p);
   p += al(8 * n);
+  f.solver = p;
   PairHost h = make_pairs(r);
   RFX_HIP_TRY(hipMemcpyAsync(pa, h.a.data(), P, hipMemcpyHostToDevice, st));
   RFX_HIP_TRY(hipMemcpyAsync(pb, h.b.data(), P, hipMemcpyHostToDevice, st));
   RFX_HIP_TRY(hipMemcpyAsync(pw, h.w.data(), 4 * P, hipMemcpyHostToDevice, st));
   std::vector<double> onesh(n, 1.0);
-  RFX_HIP_TRY(hipMemcpyAsync(ones, onesh.data(), 8 * n, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipMemcpyAsync(f.ones, onesh.data(), 8 * n, hipMemcpyHostToDevice, st));
   RFX_HIP_TRY(hipStreamSynchronize(st));
-  PairTable pt{pa, pb, pw, P};
+  f.pt = PairTable{pa, pb, pw, P};
+  return RFX_OK;
+}
+
+// FP64 row means of E (r_i = (E 1)_i / n) into f.rdev and their mean g.
+template <int FMT, typename Real>
+int fac_row_means(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const FacWs<Real>& f,
+                  double* g) {
+  using ST = typename Store<FMT>::T;
+  const ST* U = static_cast<const ST*>(Uq);
+  dim3 gr(rfx_ceil_div(f.pt.count, 256), static_cast<unsigned>(f.nrb));
+  hipLaunchKernelGGL((fac_reduce_kernel<FMT, double, 1>), gr, dim3(256), 0, st, U, scale, n, r, f.pt, f.ones,
+                     fac_rows_per_blk(n), f.partial64);
+  hipLaunchKernelGGL((fac_reduce_partials_kernel<double, 1>), dim3(rfx_ceil_div(f.pt.count, 256)), dim3(256), 0, st,
+                     f.partial64, static_cast<int>(f.nrb), f.pt, f.Z64);
+  hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
+                     r, f.pt, f.Z64, f.ones, f.rdev);
+  std::vector<double> rh(n);
+  RFX_HIP_TRY(hipMemcpyAsync(rh.data(), f.rdev, 8 * n, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    rh[i] /= static_cast<double>(n);
+    s += rh[i];
+  }
+  *g = s / static_cast<double>(n);
+  RFX_HIP_TRY(hipMemcpyAsync(f.rdev, rh.data(), 8 * n, hipMemcpyHostToDevice, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  return RFX_OK;
+}
+
+// E V on the VALU (FP32 or FP64 arithmetic) through the Khatri-Rao pairs.
+template <int FMT, typename Real>
+int fac_valu_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const FacWs<Real>& f,
+                   const Real* V, Real* W) {
+  using ST = typename Store<FMT>::T;
+  const ST* U = static_cast<const ST*>(Uq);
+  dim3 gr(rfx_ceil_div(f.pt.count, 256), static_cast<unsigned>(f.nrb));
+  hipLaunchKernelGGL((fac_reduce_kernel<FMT, Real, kMdsK>), gr, dim3(256), 0, st, U, scale, n, r, f.pt, V,
+                     fac_rows_per_blk(n), f.partial);
+  hipLaunchKernelGGL((fac_reduce_partials_kernel<Real, kMdsK>), dim3(rfx_ceil_div(f.pt.count * kMdsK, 256)),
+                     dim3(256), 0, st, f.partial, static_cast<int>(f.nrb), f.pt, f.Z);
+  hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
+                     r, f.pt, f.Z, V, W);
+  return rfx_launch_status();
+}
+
+template <int FMT, typename Real>
+int fac_mds_run(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, int max_iter, double tol,
+                const FacWs<Real>& f, double* coords, rfx::SubspaceResult& res) {
+  double g = 0.0;
+  int rc = fac_row_means<FMT, Real>(st, Uq, scale, n, r, f, &g);
+  if (rc) return rc;
+  rfx::SubspaceSolver<Real> solver(st, n, kMdsK);
+  solver.carve(f.solver);
+  auto apply = [&](const Real* V, Real* W, int) -> int {
+    return fac_valu_apply<FMT, Real>(st, Uq, scale, n, r, f, V, W);
+  };
+  return solver.run(apply, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res);
+}
+
+template <typename Real>
+int fac_mds_dispatch(hipStream_t st, const void* Uq, const float* scale, int fmt, int64_t n, int r, int max_iter,
+                     double tol, void* ws, double* coords, rfx::SubspaceResult& res) {
+  FacWs<Real> f;
+  int rc = fac_ws_setup<Real>(st, ws, n, r, f);
+  if (rc) return rc;
   switch (fmt) {
     case RFX_PREC_FP32:
-      return fac_mds_run<RFX_PREC_FP32, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
-                                              nrb, partial, partial64, Z, Z64, ones);
+      return fac_mds_run<RFX_PREC_FP32, Real>(st, Uq, scale, n, r, max_iter, tol, f, coords, res);
     case RFX_PREC_BF16:
-      return fac_mds_run<RFX_PREC_BF16, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
-                                              nrb, partial, partial64, Z, Z64, ones);
+      return fac_mds_run<RFX_PREC_BF16, Real>(st, Uq, scale, n, r, max_iter, tol, f, coords, res);
     case RFX_PREC_INT8:
-      return fac_mds_run<RFX_PREC_INT8, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
-                                              nrb, partial, partial64, Z, Z64, ones);
+      return fac_mds_run<RFX_PREC_INT8, Real>(st, Uq, scale, n, r, max_iter, tol, f, coords, res);
     case RFX_PREC_FP8:
-      return fac_mds_run<RFX_PREC_FP8, Real>(st, Uq, scale, n, r, max_iter, tol, p, coords, res, rdev, nullptr, pt,
-                                             nrb, partial, partial64, Z, Z64, ones);
+      return fac_mds_run<RFX_PREC_FP8, Real>(st, Uq, scale, n, r, max_iter, tol, f, coords, res);
     default:
       return RFX_ERR_INVALID;
   }
+}
+
+// ---------------------------------------------------------------------------
+// mixed path: bf16 MFMA operator on the stored factors, then FP32 refinement
+// sweeps of the VALU operator on the FP32 factors from the converged block
+// ---------------------------------------------------------------------------
+int mf_nt(int r) { return (r + 1 + 15) / 16; }
+
+int64_t mf_rows_per_chunk(int64_t n) {
+  int64_t rpc = (n + 255) / 256;  // about 256 chunks: one 1024-thread workgroup per CU
+  rpc = (rpc + 15) / 16 * 16;
+  return rpc < kMfStage ? kMfStage : rpc;
+}
+
+size_t mf_ws_bytes(int64_t n, int r) {
+  const int NT = mf_nt(r);
+  const int64_t nch = (n + mf_rows_per_chunk(n) - 1) / mf_rows_per_chunk(n);
+  return al(4 * static_cast<size_t>(nch) * kMfK * (NT * (NT + 1) / 2) * 256) +
+         al(2 * 2 * static_cast<size_t>(kMfK) * NT * NT * 256);
+}
+
+template <int FMT, int NT>
+int mf_apply_nt(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W,
+                float* P, uint16_t* Qb) {
+  using ST = typename Store<FMT>::T;
+  const ST* U = static_cast<const ST*>(Uq);
+  const int64_t rpc = mf_rows_per_chunk(n);
+  const int64_t nch = (n + rpc - 1) / rpc;
+  constexpr int ntiles = NT * (NT + 1) / 2;
+  hipLaunchKernelGGL((fac_mfma_reduce_kernel<FMT, NT>), dim3(static_cast<unsigned>(nch)), dim3(1024), 0, st, U,
+                     scale, n, r, V, rpc, P);
+  hipLaunchKernelGGL(fac_mfma_finalize_kernel, dim3(rfx_ceil_div(kMfK * ntiles * 256, 64)), dim3(256), 0, st, P,
+                     static_cast<int>(nch), NT, Qb);
+  hipLaunchKernelGGL((fac_mfma_apply_kernel<FMT, NT>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n, r,
+                     Qb, V, W);
+  return rfx_launch_status();
+}
+
+template <int FMT>
+int mf_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W, float* P,
+             uint16_t* Qb) {
+  switch (mf_nt(r)) {
+    case 1: return mf_apply_nt<FMT, 1>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 2: return mf_apply_nt<FMT, 2>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 3: return mf_apply_nt<FMT, 3>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 4: return mf_apply_nt<FMT, 4>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 5: return mf_apply_nt<FMT, 5>(st, Uq, scale, n, r, V, W, P, Qb);
+    default: return RFX_ERR_UNSUPPORTED;
+  }
+}
+
+template <int FMT>
+int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U32, int64_t n, int r, int max_iter,
+              double tol, int refine_max, double refine_tol, void* ws, double* coords, rfx::SubspaceResult& res,
+              int32_t* iters) {
+  FacWs<float> f;
+  int rc = fac_ws_setup<float>(st, ws, n, r, f);
+  if (rc) return rc;
+  char* p = static_cast<char*>(ws) + fac_mds_ws<float>(n, r);
+  const int NT = mf_nt(r);
+  const int64_t nch = (n + mf_rows_per_chunk(n) - 1) / mf_rows_per_chunk(n);
+  float* P = reinterpret_cast<float*>(p);
+  p += al(4 * static_cast<size_t>(nch) * kMfK * (NT * (NT + 1) / 2) * 256);
+  uint16_t* Qb = reinterpret_cast<uint16_t*>(p);
+
+  double g = 0.0;
+  rc = fac_row_means<FMT, float>(st, Uq, scale, n, r, f, &g);
+  if (rc) return rc;
+  rfx::SubspaceSolver<float> solver(st, n, kMdsK);
+  solver.carve(f.solver);
+  auto apply_mf = [&](const float* V, float* W, int) -> int { return mf_apply<FMT>(st, Uq, scale, n, r, V, W, P, Qb); };
+  rc = solver.run(apply_mf, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res);
+  if (rc) return rc;
+  iters[0] = res.iterations;
+  iters[1] = 0;
+  if (!U32 || refine_max <= 0) return RFX_OK;
+  rc = fac_row_means<RFX_PREC_FP32, float>(st, U32, nullptr, n, r, f, &g);
+  if (rc) return rc;
+  auto apply_32 = [&](const float* V, float* W, int) -> int {
+    return fac_valu_apply<RFX_PREC_FP32, float>(st, U32, nullptr, n, r, f, V, W);
+  };
+  rc = solver.run(apply_32, f.rdev, g, 3, refine_max, refine_tol, 0x5eedull, coords, res, 0.0, true);
+  if (rc) return rc;
+  iters[1] = res.iterations;
+  return RFX_OK;
 }
 
 }  // namespace
@@ -598,6 +722,51 @@ extern "C" int rfx_mds_factors(hipStream_t st, const void* Uq, const float* scal
   if (eigvals_host)
     for (int c = 0; c < kMdsK; ++c) eigvals_host[c] = res.eigvals[c];
   if (iters_host) *iters_host = res.iterations;
+  if (!(res.eigvals[2] > 1e-10)) return RFX_ERR_INVALID;
+  return RFX_OK;
+}
+
+extern "C" size_t rfx_mds_factors_mixed_workspace_bytes(int64_t n, int32_t rank) {
+  if (n <= 0 || rank <= 0 || rank > 64) return 0;
+  return fac_mds_ws<float>(n, rank) + mf_ws_bytes(n, rank);
+}
+
+extern "C" int rfx_mds_factors_mixed(hipStream_t st, const void* Uq, const float* scales, int32_t format,
+                                     const float* U32, int64_t n, int32_t rank, int32_t max_iter, double tol,
+                                     int32_t refine_max, double refine_tol, void* ws, size_t ws_bytes,
+                                     double* coords, double* eigvals_host, int32_t* iters_host) {
+  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > 64 || fmt_bytes(format) == 0 || max_iter < 1 ||
+      refine_max < 0)
+    return RFX_ERR_INVALID;
+  if (ws_bytes < rfx_mds_factors_mixed_workspace_bytes(n, rank)) return RFX_ERR_WORKSPACE;
+  rfx::SubspaceResult res;
+  int32_t iters[2] = {0, 0};
+  int rc;
+  switch (format) {
+    case RFX_PREC_FP32:
+      rc = mixed_run<RFX_PREC_FP32>(st, Uq, scales, U32, n, rank, max_iter, tol, refine_max, refine_tol, ws, coords,
+                                    res, iters);
+      break;
+    case RFX_PREC_BF16:
+      rc = mixed_run<RFX_PREC_BF16>(st, Uq, scales, U32, n, rank, max_iter, tol, refine_max, refine_tol, ws, coords,
+                                    res, iters);
+      break;
+    case RFX_PREC_INT8:
+      rc = mixed_run<RFX_PREC_INT8>(st, Uq, scales, U32, n, rank, max_iter, tol, refine_max, refine_tol, ws, coords,
+                                    res, iters);
+      break;
+    default:
+      rc = mixed_run<RFX_PREC_FP8>(st, Uq, scales, U32, n, rank, max_iter, tol, refine_max, refine_tol, ws, coords,
+                                   res, iters);
+      break;
+  }
+  if (rc) return rc;
+  if (eigvals_host)
+    for (int c = 0; c < kMdsK; ++c) eigvals_host[c] = res.eigvals[c];
+  if (iters_host) {
+    iters_host[0] = iters[0];
+    iters_host[1] = iters[1];
+  }
   if (!(res.eigvals[2] > 1e-10)) return RFX_ERR_INVALID;
   return RFX_OK;
 }

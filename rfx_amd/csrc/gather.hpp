@@ -34,6 +34,21 @@ constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
 // lines up to this length use the flattened line kernel, longer ones the walk
 constexpr int64_t kGatherLineMaxN = RFX_GATHER_LINE_MAX_N;
 
+// Rows are spread over the 8 XCDs (each with its own L2) in groups of
+// kXcdGroup consecutive rows: the dispatcher sends workgroup b to XCD b % 8,
+// so workgroups b = x + 8 * local on XCD x take the rows of groups
+// x, x + 8, x + 16, ...  Neighbouring rows, which share pe cache lines, then
+// run on one XCD, and the long rows of the upper triangle are still spread
+// evenly.  The grid is padded to a multiple of 8 * kXcdGroup rows.
+constexpr int64_t kXcdGroup = 64;
+
+inline int64_t xcd_rows_padded(int64_t n) { return (n + 8 * kXcdGroup - 1) / (8 * kXcdGroup) * (8 * kXcdGroup); }
+
+__device__ __forceinline__ int64_t xcd_row(int64_t b) {
+  const int64_t x = b & 7, local = b >> 3;
+  return (8 * (local / kXcdGroup) + x) * kXcdGroup + local % kXcdGroup;
+}
+
 // first position in members[lo, hi) whose sample index is >= key
 __device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
                                               uint32_t key) {

@@ -149,6 +149,72 @@ __global__ __launch_bounds__(256) void ss_fused_gram_kernel(const Real* __restri
   }
 }
 
+// The same sums for a block of 16 columns on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64): X = [1, r, V, W] (34 columns, padded to 48) and
+// Y = [V, W] (32 columns) as 3 x 2 output tiles of 16 x 16, K = 4 rows per
+// instruction; the four wavefronts of a block combine in LDS and write the
+// block's partial in the layout of ss_fused_gram_kernel.
+// Register layout (wave64): A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
+// D[i][j] at lane j + 16 * (i % 4), element i / 4 (the f64 form; checked with
+// a lane probe on gfx950).
+typedef double ss_f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename Real>
+__device__ __forceinline__ double ss_xcol(const Real* __restrict__ V, const Real* __restrict__ W,
+                                          const double* __restrict__ r, int64_t i, int c) {
+  if (c == 0) return 1.0;
+  if (c == 1) return r[i];
+  if (c < 18) return static_cast<double>(V[i * 16 + (c - 2)]);
+  if (c < 34) return static_cast<double>(W[i * 16 + (c - 18)]);
+  return 0.0;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void ss_fused_gram16_mfma_kernel(const Real* __restrict__ V,
+                                                                   const Real* __restrict__ W,
+                                                                   const double* __restrict__ r, int64_t n,
+                                                                   int64_t rows_per_block,
+                                                                   double* __restrict__ partial) {
+  __shared__ double red[4][6][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+  const int64_t b1 = min(n, b0 + rows_per_block);
+  const int64_t per = (b1 - b0 + 3) / 4;
+  const int64_t r0 = b0 + wave * per, r1 = min(b1, r0 + per);
+  ss_f64x4 acc[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) acc[t] = ss_f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t j0 = r0; j0 < r1; j0 += 4) {
+    const int64_t i = j0 + kq;
+    const bool ok = i < r1;
+    double xa[3], yb[2];
+#pragma unroll
+    for (int tx = 0; tx < 3; ++tx) xa[tx] = ok ? ss_xcol(V, W, r, i, tx * 16 + l16) : 0.0;
+    yb[0] = ok ? static_cast<double>(V[i * 16 + l16]) : 0.0;
+    yb[1] = ok ? static_cast<double>(W[i * 16 + l16]) : 0.0;
+#pragma unroll
+    for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+      for (int ty = 0; ty < 2; ++ty)
+        acc[tx * 2 + ty] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[tx], yb[ty], acc[tx * 2 + ty], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 6; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[wave][t][lane * 4 + e] = acc[t][e];
+  __syncthreads();
+  constexpr int nc = 32, pairs = 34 * 32;
+  for (int q = tid; q < 6 * 256; q += 256) {
+    const int t = q / 256, le = q % 256, ln = le / 4, e = le % 4;
+    const int a = (t / 2) * 16 + (ln >> 4) + 4 * e, c = (t % 2) * 16 + (ln & 15);
+    if (a < 34) {
+      const double v = red[0][t][le] + red[1][t][le] + red[2][t][le] + red[3][t][le];
+      partial[static_cast<int64_t>(blockIdx.x) * pairs + a * nc + c] = v;
+    }
+  }
+}
+
 // out[i, :] = V[i, :] * M   (M k x kout, FP64, row-major)
 template <typename Real>
 __global__ void ss_rotate_kernel(const Real* __restrict__ V, int k, const double* __restrict__ M, int kout,
@@ -325,8 +391,12 @@ class SubspaceSolver {
     for (int it = 0;; ++it) {
       rc = apply(V_, W_, k);
       if (rc) return rc;
-      hipLaunchKernelGGL((ss_fused_gram_kernel<Real>), dim3(static_cast<unsigned>(nb)), dim3(256),
-                         8 * kGramRows * wdt, st_, V_, W_, r_dev, k, n_, rpb, partial_);
+      if (k == 16)
+        hipLaunchKernelGGL((ss_fused_gram16_mfma_kernel<Real>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, st_,
+                           V_, W_, r_dev, n_, rpb, partial_);
+      else
+        hipLaunchKernelGGL((ss_fused_gram_kernel<Real>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                           8 * kGramRows * wdt, st_, V_, W_, r_dev, k, n_, rpb, partial_);
       hipLaunchKernelGGL(ss_reduce_partials_kernel, dim3(rfx_ceil_div(pairs, 4)), dim3(256), 0, st_, partial_,
                          static_cast<int>(nb), pairs, kk_);
       RFX_HIP_TRY(hipMemcpyAsync(S.data(), kk_, 8 * pairs, hipMemcpyDeviceToHost, st_));
@@ -400,7 +470,7 @@ class SubspaceSolver {
         worst = std::max(worst, std::sqrt(r2) / std::max(std::fabs(theta[c] - sh), 1e-300));
       }
       res.residual = worst;
-      if ((it >= 1 && worst < tol) || it + 1 >= max_iter) {
+      if ((it >= (start_from_v ? 0 : 1) && worst < tol) || it + 1 >= max_iter) {
         // Ritz vectors X = V Z (orthonormal: Z^T Gvv Z = I), scaled by sqrt(lambda)
         std::vector<double> Zw(static_cast<size_t>(k) * want), sc(want);
         for (int a = 0; a < k; ++a)

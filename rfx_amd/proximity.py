@@ -203,3 +203,24 @@ def mds_factors(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, precision: int
     check(lib().rfx_mds_factors(_stream(), _dptr(Uq), _dptr(scales), fmt, n, r, precision, max_iter, tol, _dptr(ws),
                                 nb, _dptr(coords), eig.ctypes.data, C.addressof(it)), "rfx_mds_factors")
     return coords, eig, it.value
+
+
+def mds_factors_mixed(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, U32: torch.Tensor | None = None,
+                      max_iter: int = 500, tol: float = 2e-3, refine_max: int = 8,
+                      refine_tol: float = 1e-5) -> tuple[torch.Tensor, np.ndarray, tuple[int, int]]:
+    """3-D MDS from stored factors with the bf16 matrix-core operator, then (with
+    the FP32 factors U32) FP32 refinement sweeps from the converged block.
+    Returns (coords n x 3 float64, the 16 Ritz values, (MFMA iterations,
+    refinement sweeps))."""
+    import ctypes as C
+
+    n, r = Uq.shape
+    nb = lib().rfx_mds_factors_mixed_workspace_bytes(n, r)
+    ws = torch.empty(nb, dtype=torch.uint8, device=Uq.device)
+    coords = torch.empty((n, 3), dtype=torch.float64, device=Uq.device)
+    eig = np.zeros(16, np.float64)
+    it = (C.c_int32 * 2)()
+    check(lib().rfx_mds_factors_mixed(_stream(), _dptr(Uq), _dptr(scales), fmt, _dptr(U32), n, r, max_iter, tol,
+                                      refine_max if U32 is not None else 0, refine_tol, _dptr(ws), nb, _dptr(coords),
+                                      eig.ctypes.data, C.addressof(it)), "rfx_mds_factors_mixed")
+    return coords, eig, (it[0], it[1])

@@ -7,7 +7,7 @@ import torch
 
 from oracle import oracle as O
 from rfx_amd import _lib
-from rfx_amd.proximity import DeviceForest, lowrank_factors, mds_factors, quantize_factors
+from rfx_amd.proximity import DeviceForest, lowrank_factors, mds_factors, mds_factors_mixed, quantize_factors
 
 from helpers import small_forest
 
@@ -116,3 +116,43 @@ def test_factor_mds_storage_precisions(gpu, fmt):
     print("fmt", fmt, "iters", it, "eig", eig[:4], "ref", w[:4], e, flush=True)
     assert np.allclose(eig[:3], w[:3], rtol={_lib.PREC_FP32: 1e-4, _lib.PREC_BF16: 2e-2, _lib.PREC_INT8: 2e-2,
                                              _lib.PREC_FP8: 1e-1}[fmt])
+
+
+def _fp64_factor_B(Uh: np.ndarray) -> np.ndarray:
+    Pt = Uh @ Uh.T
+    np.fill_diagonal(Pt, 1.0)
+    D = 1.0 - Pt
+    np.fill_diagonal(D, 0.0)
+    D2 = D * D
+    rr = D2.mean(axis=1)
+    return -0.5 * (D2 - rr[:, None] - rr[None, :] + rr.mean())
+
+
+@pytest.mark.parametrize("rank", [32, 64])
+@pytest.mark.parametrize("fmt", [_lib.PREC_BF16, _lib.PREC_INT8, _lib.PREC_FP8, _lib.PREC_FP32])
+def test_mixed_factor_mds_mfma_then_fp32_refinement(gpu, rank, fmt):
+    # The bf16 matrix-core operator (MFMA) on the stored factors gives
+    # eigenvalues at bf16/storage accuracy; FP32 refinement sweeps on the FP32
+    # factors then reach the FP64 closed form of those factors.
+    print(f"mixed factor mds rank {rank} fmt {fmt}", flush=True)
+    f, keys, P = _setup(n=1300, T=50, seed=24)
+    U, _ = lowrank_factors(keys, f, rank=rank, oversample=16, power_iters=3)
+    Uh = U.cpu().numpy().astype(np.float64)
+    B = _fp64_factor_B(Uh)
+    from scipy.linalg import eigh
+    n = B.shape[0]
+    w, Vr = eigh(B, subset_by_index=[n - 8, n - 1])
+    w = w[::-1]
+    Uq, sc = quantize_factors(U, fmt)
+    X0, e0, it0 = mds_factors_mixed(Uq, sc, fmt, None, max_iter=300, tol=2e-3)
+    storage_tol = {_lib.PREC_FP32: 1e-2, _lib.PREC_BF16: 2e-2, _lib.PREC_INT8: 5e-2, _lib.PREC_FP8: 1e-1}[fmt]
+    print("mfma only", it0, e0[:4], w[:4], flush=True)
+    assert np.allclose(e0[:3], w[:3], rtol=storage_tol)
+    X, e, it = mds_factors_mixed(Uq, sc, fmt, U, max_iter=300, tol=2e-3, refine_max=30, refine_tol=1e-6)
+    Xh = X.cpu().numpy()
+    lam = np.sum(Xh * Xh, axis=0)
+    V = Xh / np.sqrt(lam)
+    res = np.linalg.norm(B @ V - V * lam, axis=0) / lam
+    print("refined", it, e[:4], res, flush=True)
+    assert np.allclose(e[:3], w[:3], rtol=1e-6)
+    assert res.max() < 1e-5
