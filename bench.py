@@ -333,18 +333,21 @@ def bench_c5(args) -> int:
     batch = ForestBatch(dfs, "oob_upper")
     best_ms = batch.autotune(reps=1)
     lpi, bpc = batch.tuning()
-    log(f"autotune: {lpi} rows per item, {bpc} workgroups per CU, {best_ms:.2f} ms")
+    algos, sumsq = batch.stats()
+    n_dense = int(np.sum(algos == _lib.ALGO_DENSE))
+    log(f"autotune: {lpi} rows per item, {bpc} workgroups per CU, {n_dense} dense / {len(algos) - n_dense} gather "
+        f"forests, {best_ms:.2f} ms")
 
-    # 64 sequential launches of the one-problem path (GATHER), same outputs
+    # 64 sequential launches of the one-problem path with the same per-forest algorithm, same outputs
     seq = []
     for k, (df, f) in enumerate(zip(dfs, forests)):
         keys = torch.empty((f.ntree, f.n), dtype=torch.int32, device=batch.device)
-        seq.append((df, Accumulator(f.n, f.ntree, f.total_nodes), keys, batch.out[k]))
+        seq.append((df, Accumulator(f.n, f.ntree, f.total_nodes), keys, batch.out[k], int(algos[k])))
 
     def run_seq():
-        for df, acc, keys, out in seq:
+        for df, acc, keys, out, algo in seq:
             df.leaf_assign(keys=keys)
-            acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER, out)
+            acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, algo, out)
 
     run_seq()
     torch.cuda.synchronize()
@@ -384,13 +387,16 @@ def bench_c5(args) -> int:
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": WORKLOADS["c5"][6], "instances": len(forests), "tree_pair_updates": work,
                    "sum_n": int(sum(f.n for f in forests)), "layout": "upper", "graph": True,
-                   "rows_per_item": lpi, "workgroups_per_cu": bpc,
+                   "rows_per_item": lpi, "workgroups_per_cu": bpc, "dense_forests": n_dense,
+                   "gather_forests": len(algos) - n_dense,
+                   "pair_density": float(np.sum(sumsq) / sum(f.ntree * float(f.n) ** 2 for f in forests)),
                    "sequential_s": round(seq_s, 5), "batched_s": round(ms_step / 1e3, 5),
                    "speedup_vs_sequential": round(seq_s / (ms_step / 1e3), 3),
                    "sequential_value": work / seq_s},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole batched step (leaf + segmented sort + persistent gather), HIP events",
+                     "kernel": "whole batched step (leaf + segmented sort + persistent dense/gather queue), "
+                               "HIP events",
                      "algorithmic_bytes": step_bytes, "avg_ms": round(ev_ms, 4)},
     }
     if not args.no_cpu:

@@ -322,11 +322,20 @@ size_t rfx_batch_workspace_bytes(const rfx_batch* b);
  * the descriptors and synchronises; later calls do not.  Two runs of one
  * plan must not overlap (they share the workspace). */
 int rfx_batch_run(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t use_graph);
-/* Time every (rows per work item, workgroups per CU) candidate on the
- * batch's own data (reps eager runs each), keep the fastest; synchronises. */
+/* Measure every problem's leaf statistics on the batch's own data, choose
+ * GATHER or DENSE per problem (DENSE: T n^2 / 2 tile compares; GATHER:
+ * sum |leaf|^2 / 2 member visits at 8 compares each plus one write per
+ * output entry at 4), then time every (rows per work item, workgroups per
+ * CU) candidate (reps eager runs each) and keep the fastest; synchronises. */
 int rfx_batch_autotune(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t reps,
                        double* best_ms);
 int rfx_batch_set_tuning(rfx_batch* b, int32_t lines_per_item, int32_t blocks_per_cu);
+/* Per-problem algorithm (RFX_ALGO_GATHER or RFX_ALGO_DENSE, `count`
+ * entries); the autotuner sets them from measured leaf statistics. */
+int rfx_batch_set_algos(rfx_batch* b, const int32_t* algos);
+/* Per-problem algorithm and measured sum over trees and leaves of |leaf|^2
+ * (0 before the autotuner ran); either pointer may be NULL. */
+int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* sum_leaf_sq);
 int rfx_batch_get_tuning(const rfx_batch* b, int32_t* lines_per_item, int32_t* blocks_per_cu);
 void rfx_batch_free(rfx_batch* b);
 

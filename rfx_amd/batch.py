@@ -71,6 +71,24 @@ class ForestBatch:
         check(lib().rfx_batch_get_tuning(self._h, C.byref(a), C.byref(b)), "rfx_batch_get_tuning")
         return a.value, b.value
 
+    def set_algos(self, algos: list[int]) -> None:
+        """Per-forest algorithm: _lib.ALGO_GATHER or _lib.ALGO_DENSE."""
+        import numpy as np
+
+        a = np.ascontiguousarray(algos, np.int32)
+        if a.shape != (len(self.forests),):
+            raise ValueError("one algorithm per forest")
+        check(lib().rfx_batch_set_algos(self._h, a.ctypes.data), "rfx_batch_set_algos")
+
+    def stats(self):
+        """(algorithm per forest, measured sum |leaf|^2 per forest)."""
+        import numpy as np
+
+        a = np.zeros(len(self.forests), np.int32)
+        q = np.zeros(len(self.forests), np.float64)
+        check(lib().rfx_batch_get_stats(self._h, a.ctypes.data, q.ctypes.data), "rfx_batch_get_stats")
+        return a, q
+
     def work(self) -> int:
         """Tree-pair updates of one run: sum over forests of T * n(n+1)/2."""
         return sum(f.ntree * upper_size(f.n) for f in self.forests)

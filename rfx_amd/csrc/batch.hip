@@ -12,14 +12,17 @@
 //     batch's key arrays, laid end to end, form one key space;
 //  2. one segmented leaf-bucket build (radix sort by leaf) over all slots of
 //     all problems;
-//  3. one persistent gather launch: a grid sized to the machine pops work
-//     items (problem, run of output rows) from a queue in device memory,
-//     largest problems first, and counts each row in LDS with the same line
-//     body as the one-problem GATHER kernel.  The queue head is reset by a
-//     one-thread kernel, so the whole sequence is capturable.
+//  3. one persistent launch: a grid sized to the machine pops work items
+//     from a queue in device memory, most expensive problems first.  An item
+//     is either a run of output rows of a GATHER problem (counted in LDS by
+//     the one-problem GATHER line body) or a run of 128 x 128 upper tiles of
+//     a DENSE problem (the one-problem DENSE tile body).  The queue head is
+//     reset by a one-thread kernel, so the whole sequence is capturable.
 //
-// The tunables (rows per work item, workgroups per CU) are chosen by a small
-// autotuner that times the batch on the caller's data.
+// The autotuner measures every problem's leaf statistics (sum over trees and
+// leaves of |leaf|^2) on the caller's data, chooses GATHER or DENSE per
+// problem with the one-problem cost model, and times the tunables (rows per
+// work item, workgroups per CU).
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +31,7 @@
 #include <vector>
 
 #include "bucket.hpp"
+#include "dense.hpp"
 #include "gather.hpp"
 #include "leaf.hpp"
 #include "rfx_common.hpp"
@@ -48,11 +52,14 @@ struct LeafDesc {
 
 struct GatherDesc {
   const uint2* pe;
+  const int32_t* keys;  // DENSE items read the keys
   int32_t* out;
   int32_t* out2;
   int64_t n;
   int32_t ntree;
 };
+
+constexpr uint32_t kItemGather = 0, kItemDense = 1;
 
 __global__ __launch_bounds__(kBlock) void batch_leaf_kernel(const LeafDesc* __restrict__ desc,
                                                             const int64_t* __restrict__ block_start, int nprob) {
@@ -73,8 +80,10 @@ __global__ __launch_bounds__(kBlock) void batch_leaf_kernel(const LeafDesc* __re
 
 __global__ void batch_reset_kernel(uint32_t* counter) { *counter = 0u; }
 
-// Work item: (problem, first row, row count).  Every workgroup pops items
-// until the queue is empty; the queue is finite, so every wave exits.
+// Work item: (problem, first, count, kind) -- rows [first, first + count)
+// of a GATHER problem or upper tiles [first, first + count) of a DENSE
+// problem.  Every workgroup pops items until the queue is empty; the queue is
+// finite, so every wave exits.
 template <int MODE>
 __global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __restrict__ desc,
                                                            const uint4* __restrict__ items, uint32_t nitems,
@@ -82,9 +91,13 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __r
                                                            const uint32_t* __restrict__ bstart,
                                                            const uint32_t* __restrict__ mbucket,
                                                            const uint32_t* __restrict__ members) {
-  __shared__ uint32_t acc[kGatherChunk];
+  // 32 KiB: the GATHER line counters or the two DENSE key stages
+  __shared__ uint32_t smem[kGatherChunk];
   __shared__ uint32_t diag_valid;
   __shared__ uint32_t item_s;
+  static_assert(2 * kTreeChunk * kTile <= kGatherChunk, "DENSE stages must fit the shared buffer");
+  int32_t(*sR)[kTile] = reinterpret_cast<int32_t(*)[kTile]>(smem);
+  int32_t(*sS)[kTile] = reinterpret_cast<int32_t(*)[kTile]>(smem + kTreeChunk * kTile);
   for (;;) {
     if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
     __syncthreads();
@@ -92,17 +105,40 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __r
     if (it >= nitems) break;
     const uint4 w = items[it];
     const GatherDesc d = desc[w.x];
-    for (uint32_t l = 0; l < w.z; ++l) {
-      const int64_t line = static_cast<int64_t>(w.y) + l;
-      const int64_t nch = (d.n - line + kGatherChunk - 1) / kGatherChunk;
-      for (int64_t c = 0; c < nch; ++c) {
-        gather_line_body<MODE, RFX_LAYOUT_UPPER>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out, d.out2,
-                                                 line, c, acc, diag_valid);
+    if (w.w == kItemGather) {
+      for (uint32_t l = 0; l < w.z; ++l) {
+        const int64_t line = static_cast<int64_t>(w.y) + l;
+        const int64_t nch = (d.n - line + kGatherChunk - 1) / kGatherChunk;
+        for (int64_t c = 0; c < nch; ++c) {
+          gather_line_body<MODE, RFX_LAYOUT_UPPER>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out, d.out2,
+                                                   line, c, smem, diag_valid);
+          __syncthreads();
+        }
+      }
+    } else {
+      const int64_t nb = (d.n + kTile - 1) / kTile;
+      for (uint32_t q = 0; q < w.z; ++q) {
+        int64_t bi, bj;
+        upper_tile(static_cast<int64_t>(w.y) + q, nb, bi, bj);
+        dense_tile_body<MODE, RFX_LAYOUT_UPPER>(d.keys, d.n, d.ntree, bi, bj, 0, d.out, d.out2, sR, sS);
         __syncthreads();
       }
     }
     __syncthreads();  // every thread has read item_s before it is rewritten
   }
+}
+
+// sum over the leaves [lo, hi) of |leaf|^2
+__global__ void batch_sumsq_kernel(const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend, int64_t lo,
+                                   int64_t hi, unsigned long long* __restrict__ acc) {
+  unsigned long long s = 0;
+  for (int64_t e = lo + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < hi;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const unsigned long long c = bend[e] - bstart[e];
+    s += c * c;
+  }
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
 }
 
 }  // namespace
@@ -113,6 +149,8 @@ struct rfx_batch {
   int64_t slots = 0, total_nodes = 0;
   std::vector<int64_t> seg_start, seg_n, node_base, block_start;
   std::vector<uint32_t> order;  // problems by decreasing work
+  std::vector<int32_t> algo;    // RFX_ALGO_GATHER or RFX_ALGO_DENSE per problem
+  std::vector<double> sumsq;    // measured sum of |leaf|^2 per problem (0 until measured)
   int32_t lines_per_item = 4;
   int32_t blocks_per_cu = 0;  // 0: as many as fit
   // device layout inside the workspace
@@ -122,23 +160,59 @@ struct rfx_batch {
   std::vector<uint4> items;
   void* uploaded_ws = nullptr;
   int32_t uploaded_lpi = -1;
+  uint64_t uploaded_algo_sig = 0;
   // graph
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   void* graph_ws = nullptr;
   int32_t graph_lpi = -1, graph_bpc = -1;
+  uint64_t graph_algo_sig = 0;
   int cu_count = 0, max_bpc = 0;
 };
 
 namespace {
+
+// Relative costs of the one-problem AUTO rule (one tile compare = 1).
+constexpr double kBatchVisit = 8.0, kBatchWrite = 4.0;
+
+int64_t dense_tiles(int64_t n) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  return nb * (nb + 1) / 2;
+}
+
+double problem_cost(const rfx_batch* b, size_t i) {
+  const double n = static_cast<double>(b->prob[i].n), T = b->prob[i].ntree;
+  if (b->algo[i] == RFX_ALGO_DENSE) return 0.5 * T * n * n;
+  const double sq = b->sumsq[i] > 0 ? b->sumsq[i] : T * n * n;
+  return 0.5 * sq * kBatchVisit + 0.5 * n * n * kBatchWrite;
+}
+
+void order_problems(rfx_batch* b) {
+  b->order.resize(b->prob.size());
+  std::iota(b->order.begin(), b->order.end(), 0u);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [b](uint32_t a, uint32_t c) { return problem_cost(b, a) > problem_cost(b, c); });
+}
+
+uint64_t algo_signature(const rfx_batch* b) {
+  uint64_t h = 1469598103934665603ull;
+  for (int32_t a : b->algo) h = (h ^ static_cast<uint64_t>(a + 1)) * 1099511628211ull;
+  return h;
+}
 
 void build_items(rfx_batch* b) {
   b->items.clear();
   const uint32_t L = static_cast<uint32_t>(std::max(1, b->lines_per_item));
   for (uint32_t pi : b->order) {
     const int64_t n = b->prob[pi].n;
-    for (int64_t r = 0; r < n; r += L)
-      b->items.push_back(make_uint4(pi, static_cast<uint32_t>(r), static_cast<uint32_t>(std::min<int64_t>(L, n - r)), 0u));
+    if (b->algo[pi] == RFX_ALGO_DENSE) {
+      const int64_t tiles = dense_tiles(n);
+      for (int64_t t = 0; t < tiles; ++t) b->items.push_back(make_uint4(pi, static_cast<uint32_t>(t), 1u, kItemDense));
+    } else {
+      for (int64_t r = 0; r < n; r += L)
+        b->items.push_back(make_uint4(pi, static_cast<uint32_t>(r),
+                                      static_cast<uint32_t>(std::min<int64_t>(L, n - r)), kItemGather));
+    }
   }
   b->nitems = static_cast<uint32_t>(b->items.size());
 }
@@ -151,9 +225,9 @@ size_t layout_bytes(rfx_batch* b) {
     return at;
   };
   const size_t B = b->prob.size();
-  // items for the finest granularity (1 row per item) bound every tuning
+  // items at the finest granularity (1 row or 1 tile per item) bound every tuning
   int64_t max_items = 0;
-  for (const auto& p : b->prob) max_items += p.n;
+  for (const auto& p : b->prob) max_items += std::max(p.n, dense_tiles(p.n));
   b->off_leaf = take(sizeof(LeafDesc) * B);
   b->off_gather = take(sizeof(GatherDesc) * B);
   b->off_block = take(8 * (B + 1));
@@ -176,7 +250,7 @@ int upload(hipStream_t st, rfx_batch* b, char* ws) {
     const auto& p = b->prob[i];
     ld[i] = LeafDesc{p.x_col, reinterpret_cast<const uint2*>(p.nodes), p.node_off, p.inbag, keys + b->seg_start[i],
                      p.n, b->node_base[i], rfx_ceil_div(p.n, kBlock), p.ntree};
-    gd[i] = GatherDesc{w.pe + b->seg_start[i], p.out, p.out2, p.n, p.ntree};
+    gd[i] = GatherDesc{w.pe + b->seg_start[i], keys + b->seg_start[i], p.out, p.out2, p.n, p.ntree};
   }
   build_items(b);
   RFX_HIP_TRY(hipMemcpyAsync(ws + b->off_leaf, ld.data(), sizeof(LeafDesc) * B, hipMemcpyHostToDevice, st));
@@ -189,6 +263,7 @@ int upload(hipStream_t st, rfx_batch* b, char* ws) {
   RFX_HIP_TRY(hipStreamSynchronize(st));  // host vectors are local
   b->uploaded_ws = ws;
   b->uploaded_lpi = b->lines_per_item;
+  b->uploaded_algo_sig = algo_signature(b);
   return RFX_OK;
 }
 
@@ -285,13 +360,9 @@ extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count
     delete b;
     return RFX_ERR_UNSUPPORTED;
   }
-  b->order.resize(count);
-  std::iota(b->order.begin(), b->order.end(), 0u);
-  std::stable_sort(b->order.begin(), b->order.end(), [b](uint32_t a, uint32_t c) {
-    const double wa = static_cast<double>(b->prob[a].n) * b->prob[a].n * b->prob[a].ntree;
-    const double wc = static_cast<double>(b->prob[c].n) * b->prob[c].n * b->prob[c].ntree;
-    return wa > wc;
-  });
+  b->algo.assign(count, RFX_ALGO_GATHER);
+  b->sumsq.assign(count, 0.0);
+  order_problems(b);
   b->total_bytes = layout_bytes(b);
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -325,12 +396,13 @@ extern "C" int rfx_batch_run(hipStream_t st, rfx_batch* b, void* workspace, size
   if (!b || !workspace) return RFX_ERR_INVALID;
   if (workspace_bytes < b->total_bytes) return RFX_ERR_WORKSPACE;
   char* ws = static_cast<char*>(workspace);
-  if (b->uploaded_ws != ws || b->uploaded_lpi != b->lines_per_item) {
+  if (b->uploaded_ws != ws || b->uploaded_lpi != b->lines_per_item || b->uploaded_algo_sig != algo_signature(b)) {
     int rc = upload(st, b, ws);
     if (rc) return rc;
   }
   if (!use_graph) return enqueue(st, b, ws);
-  if (!b->exec || b->graph_ws != ws || b->graph_lpi != b->lines_per_item || b->graph_bpc != b->blocks_per_cu) {
+  if (!b->exec || b->graph_ws != ws || b->graph_lpi != b->lines_per_item || b->graph_bpc != b->blocks_per_cu ||
+      b->graph_algo_sig != algo_signature(b)) {
     drop_graph(b);
     if (!b->cap_stream) RFX_HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
     RFX_HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -351,8 +423,59 @@ extern "C" int rfx_batch_run(hipStream_t st, rfx_batch* b, void* workspace, size
     b->graph_ws = ws;
     b->graph_lpi = b->lines_per_item;
     b->graph_bpc = b->blocks_per_cu;
+    b->graph_algo_sig = algo_signature(b);
   }
   RFX_HIP_TRY(hipGraphLaunch(b->exec, st));
+  return RFX_OK;
+}
+
+// Leaf statistics of every problem from the buckets of the last run (the
+// workspace holds them after any run), then GATHER or DENSE per problem by
+// the one-problem cost model: DENSE costs T n^2 / 2 tile compares, GATHER
+// sum|L|^2 / 2 member visits plus one write per output entry.
+int measure_and_choose(hipStream_t st, rfx_batch* b, char* ws) {
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(w.stat);
+  const size_t B = b->prob.size();
+  std::vector<double> sq(B);
+  for (size_t i = 0; i < B; ++i) {
+    int rc = fill_u32(st, acc, 2, 0u);
+    if (rc) return rc;
+    const int64_t lo = b->node_base[i], hi = lo + b->prob[i].total_nodes;
+    hipLaunchKernelGGL(batch_sumsq_kernel, dim3(grid_for(hi - lo, 256)), dim3(256), 0, st, w.bstart, w.bend, lo, hi,
+                       acc);
+    rc = rfx_launch_status();
+    if (rc) return rc;
+    unsigned long long h = 0;
+    RFX_HIP_TRY(hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, st));
+    RFX_HIP_TRY(hipStreamSynchronize(st));
+    sq[i] = static_cast<double>(h);
+  }
+  for (size_t i = 0; i < B; ++i) {
+    const double n = static_cast<double>(b->prob[i].n), T = b->prob[i].ntree;
+    b->sumsq[i] = sq[i];
+    const double dense = 0.5 * T * n * n, gather = 0.5 * sq[i] * kBatchVisit + 0.5 * n * n * kBatchWrite;
+    b->algo[i] = dense < gather ? RFX_ALGO_DENSE : RFX_ALGO_GATHER;
+  }
+  order_problems(b);
+  return RFX_OK;
+}
+
+extern "C" int rfx_batch_set_algos(rfx_batch* b, const int32_t* algos) {
+  if (!b || !algos) return RFX_ERR_INVALID;
+  for (size_t i = 0; i < b->prob.size(); ++i)
+    if (algos[i] != RFX_ALGO_GATHER && algos[i] != RFX_ALGO_DENSE) return RFX_ERR_INVALID;
+  b->algo.assign(algos, algos + b->prob.size());
+  order_problems(b);
+  return RFX_OK;
+}
+
+extern "C" int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* sum_leaf_sq) {
+  if (!b) return RFX_ERR_INVALID;
+  for (size_t i = 0; i < b->prob.size(); ++i) {
+    if (algos) algos[i] = b->algo[i];
+    if (sum_leaf_sq) sum_leaf_sq[i] = b->sumsq[i];
+  }
   return RFX_OK;
 }
 
@@ -360,6 +483,12 @@ extern "C" int rfx_batch_autotune(hipStream_t st, rfx_batch* b, void* workspace,
                                   int32_t reps, double* best_ms) {
   if (!b || !workspace || reps < 1) return RFX_ERR_INVALID;
   if (workspace_bytes < b->total_bytes) return RFX_ERR_WORKSPACE;
+  {
+    int rc = rfx_batch_run(st, b, workspace, workspace_bytes, 0);  // buckets for the statistics
+    if (rc) return rc;
+    rc = measure_and_choose(st, b, static_cast<char*>(workspace));
+    if (rc) return rc;
+  }
   static const int32_t kLines[] = {1, 4, 16};
   std::vector<int32_t> bpcs;
   for (int v = 1; v <= b->max_bpc; v *= 2) bpcs.push_back(v);

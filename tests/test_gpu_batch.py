@@ -97,6 +97,43 @@ def test_batch_graph_replays_and_tuning(gpu, problems):
     b.close()
 
 
+@pytest.mark.parametrize("variant", ["oob_upper", "pair"])
+def test_batch_dense_and_gather_items_mixed(gpu, problems, variant):
+    # DENSE tile items and GATHER row items of different forests in one queue
+    probs, dfs = problems
+    _say(f"batch mixed dense/gather items variant {variant}")
+    b = ForestBatch(dfs, variant)
+    b.set_algos([_lib.ALGO_DENSE if k % 2 == 0 else _lib.ALGO_GATHER for k in range(len(dfs))])
+    for use_graph in (False, True, True):
+        for o in b.out:
+            o.fill_(-7)
+        b.run(use_graph=use_graph)
+        torch.cuda.synchronize()
+        if variant == "pair":
+            ea, eo = _expected(probs, _lib.PROX_ALL), _expected(probs, _lib.PROX_OOB_UPPER)
+            for k in range(len(dfs)):
+                assert np.array_equal(b.out[k].cpu().numpy(), ea[k]), f"problem {k} ALL"
+                assert np.array_equal(b.out2[k].cpu().numpy(), eo[k]), f"problem {k} OOB"
+        else:
+            for k, e in enumerate(_expected(probs, _lib.PROX_OOB_UPPER)):
+                assert np.array_equal(b.out[k].cpu().numpy(), e), f"problem {k}"
+    algos, sq = b.stats()
+    assert list(algos) == [_lib.ALGO_DENSE if k % 2 == 0 else _lib.ALGO_GATHER for k in range(len(dfs))]
+    b.autotune(reps=1)
+    algos, sq = b.stats()
+    # measured leaf statistics: sum over trees and leaves of |leaf|^2 from the oracle's leaves
+    for k, (f, x) in enumerate(probs):
+        nd = O.leaf_assign(f, x)
+        want = 0
+        for t in range(f.ntree):
+            b0 = f.node_offset[t]
+            st = f.nodestatus[b0:f.node_offset[t + 1]]
+            term = nd[t][st[nd[t]] == -1]
+            want += int(np.sum(np.bincount(term).astype(np.int64) ** 2))
+        assert sq[k] == want, f"problem {k}"
+    b.close()
+
+
 def test_batch_long_rows_match_single_problem_path(gpu):
     # n > the 8192-column LDS chunk: rows span several chunks; checked against
     # the one-problem GATHER path (itself pinned to the oracle)
