@@ -62,6 +62,80 @@ __device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ membe
   return lo;
 }
 
+// The wavefront's 64 member runs (lane l: run [s, s + len), its sample's
+// in-bag bit) flattened across the lanes: each step every lane takes one
+// member of the union (owner lane by a 6-step search over the prefix sums of
+// the run lengths), kGatherUnroll members in flight, and adds its count to
+// the LDS counters acc[column - lo].  All lanes of the wavefront must call.
+template <int MODE, int LAYOUT>
+__device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t bag, int lane,
+                                            const uint32_t* __restrict__ members, uint32_t* acc, int64_t lo,
+                                            int64_t line) {
+  uint32_t pre = len;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(pre, d, 64);
+    if (lane >= d) pre += y;
+  }
+  const uint32_t excl = pre - len;
+  const uint32_t total = __shfl(pre, 63, 64);
+  for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
+    uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
+    bool ok[kGatherUnroll];
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const uint32_t g = g0 + u * 64 + lane;
+      int o = 0;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t v = __shfl(pre, o + d - 1, 64);
+        if (v <= g) o += d;
+      }
+      o = min(o, 63);
+      const uint32_t so = __shfl(s, o, 64);
+      const uint32_t eo = __shfl(excl, o, 64);
+      bo[u] = __shfl(bag, o, 64);
+      ok[u] = g < total;
+      mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      if (!ok[u]) continue;
+      const int64_t q = mv[u] >> 1;
+      const uint32_t bq = mv[u] & 1u;
+      if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
+      uint32_t inc = 0;
+      if (MODE == kMode_ALL) inc = 1u;
+      if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
+      if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+      if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
+      if (inc) atomicAdd(&acc[q - lo], inc);
+    }
+  }
+}
+
+// First position p in [cur, end) whose member column is >= key: a galloping
+// search from cur (the runs are sorted by column), cost O(log(p - cur)).
+__device__ __forceinline__ uint32_t gallop_lb(const uint32_t* __restrict__ members, uint32_t cur, uint32_t end,
+                                              uint32_t key) {
+  if (cur >= end || (members[cur] >> 1) >= key) return cur;
+  uint32_t lo = cur, step = 1, hi;
+  for (;;) {  // invariant: column of members[lo] < key
+    const uint32_t probe = lo + step;
+    if (probe >= end) {
+      hi = end;
+      break;
+    }
+    if ((members[probe] >> 1) >= key) {
+      hi = probe;
+      break;
+    }
+    lo = probe;
+    step <<= 1;
+  }
+  return member_lb(members, lo + 1, hi, key);
+}
+
 template <int MODE, int LAYOUT>
 __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
                                                  const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
@@ -103,47 +177,7 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
       }
     }
     if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
-    uint32_t pre = len;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(pre, d, 64);
-      if (lane >= d) pre += y;
-    }
-    const uint32_t excl = pre - len;
-    const uint32_t total = __shfl(pre, 63, 64);
-    for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
-      uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
-      bool ok[kGatherUnroll];
-#pragma unroll
-      for (int u = 0; u < kGatherUnroll; ++u) {
-        const uint32_t g = g0 + u * 64 + lane;
-        int o = 0;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-          const uint32_t v = __shfl(pre, o + d - 1, 64);
-          if (v <= g) o += d;
-        }
-        o = min(o, 63);
-        const uint32_t so = __shfl(s, o, 64);
-        const uint32_t eo = __shfl(excl, o, 64);
-        bo[u] = __shfl(bag, o, 64);
-        ok[u] = g < total;
-        mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < kGatherUnroll; ++u) {
-        if (!ok[u]) continue;
-        const int64_t q = mv[u] >> 1;
-        const uint32_t bq = mv[u] & 1u;
-        if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
-        uint32_t inc = 0;
-        if (MODE == kMode_ALL) inc = 1u;
-        if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
-        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
-        if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
-        if (inc) atomicAdd(&acc[q - lo], inc);
-      }
-    }
+    flat_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
   }
   if (MODE == kMode_ALL || MODE == kMode_PAIR) {
 #pragma unroll

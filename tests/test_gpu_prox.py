@@ -221,3 +221,28 @@ def test_long_rows_walk_kernel(gpu):
     del b, o, want_o
     P = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
     assert np.array_equal(P.cpu().numpy(), expect(f, nd, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL))
+
+
+def test_long_rows_many_trees_match_bucket_form(gpu):
+    """n = 40,000 and T = 600: the long-row gather with several tree groups
+    per lane (cursors of 4 trees) against the independent global-atomic form,
+    for ALL + OOB (fused pair) and the FULL OOB_ASYM layout."""
+    _say("long rows n=40000 T=600")
+    x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, 40000, 54, 7, 600, 9, mtry=7)
+    df, keys, _ = run_keys(f, x)
+    acc = Accumulator(f.n, f.ntree, f.total_nodes)
+    a, o = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+    ba = acc.accumulate(keys, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET)
+    assert torch.equal(a, ba)
+    del ba
+    bo = acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET)
+    assert torch.equal(o, bo)
+    # size-independent properties: C_ii = number of trees with a terminal leaf, row sums
+    n = f.n
+    diag_idx = torch.arange(n, device=a.device, dtype=torch.int64)
+    diag_idx = diag_idx * n - diag_idx * (diag_idx - 1) // 2
+    assert int(a[diag_idx].max()) <= f.ntree
+    del a, o, bo
+    P = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
+    Q = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_BUCKET)
+    assert torch.equal(P, Q)
