@@ -270,10 +270,12 @@ int rfx_mds_factors(hipStream_t stream, const void* Uq, const float* scales, int
  * the E V products as bf16 MFMA GEMMs (u * v and Q carried as bf16 hi + lo
  * pairs, FP32 accumulation, FP64 partial sums) until the Ritz residual is
  * below tol, then, when U32 (device FP32 n x rank row-major, the factors the
- * stored copy was made from) is given, up to refine_max sweeps of the FP32
- * VALU operator on U32 from the converged block until refine_tol.  FP64 row
- * means, CholeskyQR2 and Rayleigh-Ritz throughout.  iters_host[0] = matrix-core
- * iterations, iters_host[1] = refinement sweeps.  rank <= 64.  Synchronises.
+ * stored copy was made from) is given, up to refine_max refinement sweeps
+ * from the converged block until refine_tol, with an FP32-accurate operator
+ * on U32 (the same GEMMs with u carried as a bf16 pair hi + lo).  FP64 row
+ * means, CholeskyQR2, Grams and Rayleigh-Ritz throughout.  iters_host[0] =
+ * matrix-core iterations, iters_host[1] = refinement sweeps.  rank <= 64.
+ * Synchronises.
  * Replaces LowRankProximityMatrix::compute_mds_from_factors
  * (cuda/rf_proximity_lowrank.cu:1935-2531). */
 size_t rfx_mds_factors_mixed_workspace_bytes(int64_t n, int32_t rank);

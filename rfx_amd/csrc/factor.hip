@@ -523,7 +523,7 @@ size_t mf_ws_bytes(int64_t n, int r) {
          al(2 * 2 * static_cast<size_t>(kMfK) * NT * NT * 256);
 }
 
-template <int FMT, int NT>
+template <int FMT, int NT, bool SPLIT>
 int mf_apply_nt(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W,
                 float* P, uint16_t* Qb) {
   using ST = typename Store<FMT>::T;
@@ -531,24 +531,24 @@ int mf_apply_nt(hipStream_t st, const void* Uq, const float* scale, int64_t n, i
   const int64_t rpc = mf_rows_per_chunk(n);
   const int64_t nch = (n + rpc - 1) / rpc;
   constexpr int ntiles = NT * (NT + 1) / 2;
-  hipLaunchKernelGGL((fac_mfma_reduce_kernel<FMT, NT>), dim3(static_cast<unsigned>(nch)), dim3(1024), 0, st, U,
+  hipLaunchKernelGGL((fac_mfma_reduce_kernel<FMT, NT, SPLIT>), dim3(static_cast<unsigned>(nch)), dim3(1024), 0, st, U,
                      scale, n, r, V, rpc, P);
   hipLaunchKernelGGL(fac_mfma_finalize_kernel, dim3(rfx_ceil_div(kMfK * ntiles * 256, 64)), dim3(256), 0, st, P,
                      static_cast<int>(nch), NT, Qb);
-  hipLaunchKernelGGL((fac_mfma_apply_kernel<FMT, NT>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n, r,
+  hipLaunchKernelGGL((fac_mfma_apply_kernel<FMT, NT, SPLIT>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n, r,
                      Qb, V, W);
   return rfx_launch_status();
 }
 
-template <int FMT>
+template <int FMT, bool SPLIT = false>
 int mf_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W, float* P,
              uint16_t* Qb) {
   switch (mf_nt(r)) {
-    case 1: return mf_apply_nt<FMT, 1>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 2: return mf_apply_nt<FMT, 2>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 3: return mf_apply_nt<FMT, 3>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 4: return mf_apply_nt<FMT, 4>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 5: return mf_apply_nt<FMT, 5>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 1: return mf_apply_nt<FMT, 1, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 2: return mf_apply_nt<FMT, 2, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 3: return mf_apply_nt<FMT, 3, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 4: return mf_apply_nt<FMT, 4, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
+    case 5: return mf_apply_nt<FMT, 5, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
     default: return RFX_ERR_UNSUPPORTED;
   }
 }
@@ -580,8 +580,9 @@ int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U
   if (!U32 || refine_max <= 0) return RFX_OK;
   rc = fac_row_means<RFX_PREC_FP32, float>(st, U32, nullptr, n, r, f, &g);
   if (rc) return rc;
+  // FP32-accurate matrix-core operator on the FP32 factors (u as bf16 hi + lo)
   auto apply_32 = [&](const float* V, float* W, int) -> int {
-    return fac_valu_apply<RFX_PREC_FP32, float>(st, U32, nullptr, n, r, f, V, W);
+    return mf_apply<RFX_PREC_FP32, true>(st, U32, nullptr, n, r, V, W, P, Qb);
   };
   rc = solver.run(apply_32, f.rdev, g, 3, refine_max, refine_tol, 0x5eedull, coords, res, 0.0, true);
   if (rc) return rc;

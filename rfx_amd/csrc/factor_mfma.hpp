@@ -14,7 +14,9 @@
 // dequantised (q * s_c) and rounded to bf16.  The products u * v and the
 // entries of Q are carried as unevaluated bf16 pairs hi + lo (two MFMAs), so
 // the operator error is set by the stored factors, not by the arithmetic.
-// Refinement sweeps with the FP32 operator on the FP32 factors follow.
+// SPLIT (FP32 factors) also carries u as a pair hi + lo (three MFMAs per
+// product: hi*hi, lo*hi, hi*lo), an FP32-accurate operator on the matrix
+// cores that the refinement sweeps use.
 //
 // MFMA 16x16x16 bf16 (v_mfma_f32_16x16x16bf16_1k), wave64 register layout:
 //   A[i][k]: lane i + 16*(k/4), element k%4
@@ -63,14 +65,15 @@ __device__ __forceinline__ void mf_tile(int t, int NT, int& ta, int& tb) {
 
 // Partials: P[chunk][c][tile][lane*4 + e] (FP32), one workgroup per row chunk,
 // 16 waves: wave c accumulates the upper tiles of Q_c.
-template <int FMT, int NT>
+template <int FMT, int NT, bool SPLIT>
 __global__ __launch_bounds__(1024) void fac_mfma_reduce_kernel(const typename Store<FMT>::T* __restrict__ U,
                                                                const float* __restrict__ scale, int64_t n, int r,
                                                                const float* __restrict__ V,
                                                                int64_t rows_per_chunk, float* __restrict__ P) {
   constexpr int W = NT * 16;
   constexpr int ntiles = NT * (NT + 1) / 2;
-  __shared__ uint16_t Us[W][kMfStride];
+  // factor planes: bf16(u), and with SPLIT bf16(u - bf16(u)) as well
+  __shared__ uint16_t Us[SPLIT ? 2 : 1][W][kMfStride];
   __shared__ float Vs[kMfK][kMfStride];
   const int tid = threadIdx.x, lane = tid & 63, c = tid >> 6;
   const int g = lane >> 4, l16 = lane & 15;
@@ -85,14 +88,16 @@ __global__ __launch_bounds__(1024) void fac_mfma_reduce_kernel(const typename St
     for (int e = tid; e < kMfStage * W; e += 1024) {
       const int row = e / W, col = e % W;  // consecutive threads read one row
       const int64_t j = j0 + row;
-      uint16_t h = 0;
+      float u = 0.f;
       if (j < r1) {
         if (col < r)
-          h = mf_bf16(Store<FMT>::load(U, j * r + col, scale, col));
+          u = Store<FMT>::load(U, j * r + col, scale, col);
         else if (col == r)
-          h = 0xbf80;  // -1
+          u = -1.f;
       }
-      Us[col][row] = h;
+      const uint16_t h = mf_bf16(u);
+      Us[0][col][row] = h;
+      if (SPLIT) Us[SPLIT ? 1 : 0][col][row] = mf_bf16(u - mf_f32(h));
     }
     for (int e = tid; e < kMfStage * kMfK; e += 1024) {
       const int row = e / kMfK, cc = e % kMfK;
@@ -106,13 +111,23 @@ __global__ __launch_bounds__(1024) void fac_mfma_reduce_kernel(const typename St
       const float4 vv = *reinterpret_cast<const float4*>(&Vs[c][jr]);
       // A = u^ o v_c as an unevaluated bf16 sum hi + lo (the FP32 product
       // u * v is exact to 2^-24; hi + lo carries it to about 2^-17)
-      mf_bf16x4 Bop[NT], Ahi[NT], Alo[NT];
+      mf_bf16x4 Bop[NT], Blo[NT], Ahi[NT], Alo[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const uint2 w = *reinterpret_cast<const uint2*>(&Us[t * 16 + l16][jr]);
+        const uint2 w = *reinterpret_cast<const uint2*>(&Us[0][t * 16 + l16][jr]);
         const uint16_t h0 = w.x & 0xffffu, h1 = w.x >> 16, h2 = w.y & 0xffffu, h3 = w.y >> 16;
         Bop[t] = mf_pack(h0, h1, h2, h3);
-        const float p0 = mf_f32(h0) * vv.x, p1 = mf_f32(h1) * vv.y, p2 = mf_f32(h2) * vv.z, p3 = mf_f32(h3) * vv.w;
+        float u0 = mf_f32(h0), u1 = mf_f32(h1), u2 = mf_f32(h2), u3 = mf_f32(h3);
+        if (SPLIT) {
+          const uint2 z = *reinterpret_cast<const uint2*>(&Us[SPLIT ? 1 : 0][t * 16 + l16][jr]);
+          const uint16_t z0 = z.x & 0xffffu, z1 = z.x >> 16, z2 = z.y & 0xffffu, z3 = z.y >> 16;
+          Blo[t] = mf_pack(z0, z1, z2, z3);
+          u0 += mf_f32(z0);
+          u1 += mf_f32(z1);
+          u2 += mf_f32(z2);
+          u3 += mf_f32(z3);
+        }
+        const float p0 = u0 * vv.x, p1 = u1 * vv.y, p2 = u2 * vv.z, p3 = u3 * vv.w;
         const uint16_t q0 = mf_bf16(p0), q1 = mf_bf16(p1), q2 = mf_bf16(p2), q3 = mf_bf16(p3);
         Ahi[t] = mf_pack(q0, q1, q2, q3);
         Alo[t] = mf_pack(mf_bf16(p0 - mf_f32(q0)), mf_bf16(p1 - mf_f32(q1)), mf_bf16(p2 - mf_f32(q2)),
@@ -125,6 +140,7 @@ __global__ __launch_bounds__(1024) void fac_mfma_reduce_kernel(const typename St
           const int t = ta * NT - ta * (ta - 1) / 2 + (tb - ta);
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ahi[ta], Bop[tb], acc[t], 0, 0, 0);
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Alo[ta], Bop[tb], acc[t], 0, 0, 0);
+          if (SPLIT) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ahi[ta], Blo[tb], acc[t], 0, 0, 0);
         }
     }
   }
@@ -180,14 +196,14 @@ __global__ __launch_bounds__(256) void fac_mfma_finalize_kernel(const float* __r
 }
 
 // Y[i][c] = u~_i^T Q_c u~_i - (1 - |u_i|^2)^2 V[i][c]; 4 waves x 64 rows.
-template <int FMT, int NT>
+template <int FMT, int NT, bool SPLIT>
 __global__ __launch_bounds__(256) void fac_mfma_apply_kernel(const typename Store<FMT>::T* __restrict__ U,
                                                               const float* __restrict__ scale, int64_t n, int r,
                                                               const uint16_t* __restrict__ Qb,
                                                               const float* __restrict__ V, float* __restrict__ Y) {
   constexpr int W = NT * 16;
   constexpr int kRowStride = W + 4;
-  __shared__ uint16_t Ut[256][kRowStride];
+  __shared__ uint16_t Ut[SPLIT ? 2 : 1][256][kRowStride];
   __shared__ uint16_t Qs[2][NT * NT * 256];
   __shared__ float nrm[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -196,32 +212,44 @@ __global__ __launch_bounds__(256) void fac_mfma_apply_kernel(const typename Stor
   for (int e = tid; e < 256 * W; e += 256) {
     const int row = e / W, col = e % W;
     const int64_t i = i0 + row;
-    uint16_t h = 0;
+    float u = 0.f;
     if (i < n) {
       if (col < r)
-        h = mf_bf16(Store<FMT>::load(U, i * r + col, scale, col));
+        u = Store<FMT>::load(U, i * r + col, scale, col);
       else if (col == r)
-        h = 0x3f80;  // +1
+        u = 1.f;
     }
-    Ut[row][col] = h;
+    const uint16_t h = mf_bf16(u);
+    Ut[0][row][col] = h;
+    if (SPLIT) Ut[SPLIT ? 1 : 0][row][col] = mf_bf16(u - mf_f32(h));
   }
   __syncthreads();
+  auto ut = [&](int row, int col) -> float {
+    float v = mf_f32(Ut[0][row][col]);
+    if (SPLIT) v += mf_f32(Ut[SPLIT ? 1 : 0][row][col]);
+    return v;
+  };
   {
     float s = 0.f;
     for (int col = 0; col < r; ++col) {
-      const float u = mf_f32(Ut[tid][col]);
+      const float u = ut(tid, col);
       s += u * u;
     }
     nrm[tid] = s;
   }
   // A operands of the wave's 4 row tiles: A[i][k] = u~[row][ka*16 + k]
-  mf_bf16x4 Aop[4][NT];
+  mf_bf16x4 Aop[4][NT], Alo[SPLIT ? 4 : 1][NT];
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int ka = 0; ka < NT; ++ka) {
-      const uint2 w = *reinterpret_cast<const uint2*>(&Ut[wave * 64 + rt * 16 + l16][ka * 16 + 4 * g]);
+      const uint2 w = *reinterpret_cast<const uint2*>(&Ut[0][wave * 64 + rt * 16 + l16][ka * 16 + 4 * g]);
       Aop[rt][ka] = mf_pack(w.x & 0xffffu, w.x >> 16, w.y & 0xffffu, w.y >> 16);
+      if (SPLIT) {
+        const uint2 z =
+            *reinterpret_cast<const uint2*>(&Ut[SPLIT ? 1 : 0][wave * 64 + rt * 16 + l16][ka * 16 + 4 * g]);
+        Alo[SPLIT ? rt : 0][ka] = mf_pack(z.x & 0xffffu, z.x >> 16, z.y & 0xffffu, z.y >> 16);
+      }
     }
   float yv[4][4];
 #pragma unroll
@@ -257,10 +285,11 @@ __global__ __launch_bounds__(256) void fac_mfma_apply_kernel(const typename Stor
         for (int ka = 0; ka < NT; ++ka) {
           d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop[rt][ka], Bhi[tn][ka], d, 0, 0, 0);
           d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop[rt][ka], Blo[tn][ka], d, 0, 0, 0);
+          if (SPLIT) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Alo[SPLIT ? rt : 0][ka], Bhi[tn][ka], d, 0, 0, 0);
         }
         // D[i][b]: row 4g + e of the tile, column tn*16 + l16
 #pragma unroll
-        for (int e = 0; e < 4; ++e) s[e] += d[e] * mf_f32(Ut[wave * 64 + rt * 16 + 4 * g + e][tn * 16 + l16]);
+        for (int e = 0; e < 4; ++e) s[e] += d[e] * ut(wave * 64 + rt * 16 + 4 * g + e, tn * 16 + l16);
       }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {

@@ -206,10 +206,11 @@ def mds_factors(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, precision: int
 
 
 def mds_factors_mixed(Uq: torch.Tensor, scales: torch.Tensor, fmt: int, U32: torch.Tensor | None = None,
-                      max_iter: int = 500, tol: float = 1e-4, refine_max: int = 8,
-                      refine_tol: float = 1e-6) -> tuple[torch.Tensor, np.ndarray, tuple[int, int]]:
+                      max_iter: int = 500, tol: float = 1e-4, refine_max: int = 60,
+                      refine_tol: float = 5e-5) -> tuple[torch.Tensor, np.ndarray, tuple[int, int]]:
     """3-D MDS from stored factors with the bf16 matrix-core operator, then (with
-    the FP32 factors U32) FP32 refinement sweeps from the converged block.
+    the FP32 factors U32) refinement sweeps with the FP32-accurate matrix-core
+    operator from the converged block.
     Returns (coords n x 3 float64, the 16 Ritz values, (MFMA iterations,
     refinement sweeps))."""
     import ctypes as C
