@@ -51,6 +51,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(key: str):
+    """HBM bytes per launch of the counting kernel for this kernel and shape,
+    as measured by the rocprofv3 counter runs summarised under profiles/
+    (None when that kernel and shape have not been measured)."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None, None
+    e = json.loads(path.read_text()).get(key)
+    if not e:
+        return None, None
+    b = (e["fetch_kib"] * e["fetch_scale"] + e["write_kib"] * e["write_scale"]) * 1024
+    return int(b), e["summary"]
+
+
 def cpu_baseline(forest, x, sample_trees: int) -> dict:
     """The oracle port of the reference CPU path (per tree: leaf assignment,
     then rf::cpu_proximity's accumulating OpenMP branch) on sample_trees trees."""
@@ -89,6 +103,7 @@ def main() -> int:
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mds", action="store_true", help="c5: also time the 3-D MDS of every instance")
+    ap.add_argument("--no-mds", action="store_true", help="c3: counts only, no factor / MDS section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one device-to-device copy of 1 GiB (a known-size kernel that scales the HBM "
                          "byte counters of a profiler run)")
@@ -152,18 +167,24 @@ def main() -> int:
     algo_name = NAMES[algo]
 
     stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(args.steps)]
+    sparse = algo in (_lib.ALGO_GATHER, _lib.ALGO_BUCKET)
 
+    # leaf assignment, then (sparse algorithms) the leaf buckets, then the
+    # counting kernel alone, each bracketed by events on the launch stream
     def step(e=None):
         if e:
             e[0].record(stream)
         df.leaf_assign(keys=keys)
         if e:
             e[1].record(stream)
-        acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, algo, out_all, out_oob)
+        if sparse:
+            acc.build_buckets(keys)
         if e:
             e[2].record(stream)
+        acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, algo | (_lib.ALGO_PREBUILT if sparse else 0), out_all, out_oob)
+        if e:
+            e[3].record(stream)
 
     for _ in range(args.warmup):
         step()
@@ -174,19 +195,26 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_step = wall * 1e3 / args.steps
-    leaf_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
-    acc_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    leaf_ms = float(np.mean([a.elapsed_time(b) for a, b, _, _ in ev]))
+    build_ms = float(np.mean([b.elapsed_time(c) for _, b, c, _ in ev]))
+    kern_ms = float(np.mean([c.elapsed_time(d) for _, _, c, d in ev]))
+    acc_ms = build_ms + kern_ms
     P = n * (n + 1) // 2
     value = T * P / (ms_step / 1e3)
 
-    # Algorithmic bytes of the accumulation launch (one pass over all trees):
+    # Algorithmic bytes of the counting launch (one pass over all trees):
     # two int32 upper triangles written + the int32 key array read.
+    KERNEL = {_lib.ALGO_GATHER: "gather_line_kernel<PAIR,UPPER>" if n <= 32768 else
+              "gather_gallop_kernel<PAIR,UPPER>", _lib.ALGO_BUCKET: "bucket_pair_kernel<PAIR,UPPER>",
+              _lib.ALGO_DENSE: "dense_tile_kernel<PAIR,UPPER>"}
     acc_bytes = 2 * 4 * P + 4 * n * T
-    achieved = acc_bytes / (acc_ms / 1e3) / 1e9
+    achieved = acc_bytes / (kern_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(f"{KERNEL[algo]} n={n} T={T}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"rfx_prox_accumulate_pair ({algo_name})", "algorithmic_bytes": acc_bytes,
-                "avg_ms": round(acc_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": KERNEL[algo], "algorithmic_bytes": acc_bytes, "avg_ms": round(kern_ms, 4)}
+    if traffic_src:
+        roofline["traffic_source"] = traffic_src
 
     line = {
         "metric": METRIC, "value": value, "unit": "tree·pair updates/s", "n_gpus": 1, "steps": args.steps,
@@ -194,6 +222,7 @@ def main() -> int:
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": desc, "n": n, "p": p, "ntree": T, "mtry": mtry, "algo": algo_name,
                    "layout": "upper", "leaf_assign_ms": round(leaf_ms, 4), "accumulate_ms": round(acc_ms, 4),
+                   "bucket_build_ms": round(build_ms, 4), "count_kernel_ms": round(kern_ms, 4),
                    "sum_leaf_sq": sq, "pair_density": sq / (T * float(n) * n)},
         "roofline": roofline,
     }
@@ -205,7 +234,7 @@ def main() -> int:
         cb = cpu_baseline(forest, x, trees)
         cb.pop("seconds")
         line["cpu_baseline"] = cb
-    if args.workload == "c3":
+    if args.workload == "c3" and not args.no_mds:
         del keys
         out_all = out_oob = None
         torch.cuda.empty_cache()

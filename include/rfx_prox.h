@@ -55,6 +55,9 @@ extern "C" {
 #define RFX_PROX_ALL 0
 #define RFX_PROX_OOB_ASYM 1
 #define RFX_PROX_OOB_UPPER 2
+/* OOB_ROW  P4[i,j] = #{t : i out-of-bag, same leaf as j}, any j including i
+ *          (cuda/rf_proximity.cu:32-67, non-case-wise); FULL layout only. */
+#define RFX_PROX_OOB_ROW 4
 
 /* layouts: FULL = n x n, element (i,j) at i + j*n (column-major, as the
  * reference's prox[n + kk*nsample]); UPPER = packed upper triangle with the
@@ -67,6 +70,10 @@ extern "C" {
 #define RFX_ALGO_DENSE 1  /* LDS-staged key tiles, register-tiled compare   */
 #define RFX_ALGO_BUCKET 2 /* leaf buckets, integer atomics per matching pair */
 #define RFX_ALGO_GATHER 3 /* leaf buckets gathered per output line in LDS  */
+/* Flag or-ed into a BUCKET or GATHER algo: the workspace already holds the
+ * leaf buckets of these keys (rfx_prox_build_buckets), so only the counting
+ * kernel runs. */
+#define RFX_ALGO_PREBUILT 0x100
 
 /* ========================================================================
  * Host side: forest input contract (not timed)
@@ -147,6 +154,12 @@ size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, i
 int rfx_prox_accumulate(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                         int32_t variant, int32_t layout, int32_t algo, int32_t accumulate, int32_t* out,
                         void* workspace, size_t workspace_bytes);
+
+/* A2 alone: the leaf buckets (the reference's per-tree nod/ncount/ndbegin/
+ * npcase CSR of src/rf_proximity.cpp:108-171, for all trees at once) of the
+ * keys into the workspace, for calls with RFX_ALGO_PREBUILT. */
+int rfx_prox_build_buckets(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                           void* workspace, size_t workspace_bytes);
 
 /* ALL and OOB_UPPER in one pass (same layout for both outputs). */
 int rfx_prox_accumulate_pair(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree,

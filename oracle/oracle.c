@@ -21,7 +21,8 @@
  *   orc_counts_bucket   the ALL / OOB_ASYM / OOB_UPPER integer counts by leaf
  *                       buckets (ALL has no reference implementation; it is
  *                       pinned by brute force and closed-form invariants)
- *   orc_counts_brute    the same by the O(T n^2) definition (tiny n only)
+ *   orc_counts_brute    the same by the O(T n^2) definition (tiny n only);
+ *                       OOB_ROW (variant 4) follows cuda/rf_proximity.cu:32-67
  *   orc_finishprox      src/rf_finishprox.cpp:12-74
  */
 #include <stdint.h>
@@ -37,6 +38,7 @@
 #define ORC_ALL 0
 #define ORC_OOB_ASYM 1
 #define ORC_OOB_UPPER 2
+#define ORC_OOB_ROW 4 /* cuda/rf_proximity.cu:32-67: i out-of-bag, any j of its leaf, i itself included */
 
 /* Terminal node reached by sample i in tree t (node id local to the tree). */
 static int32_t orc_traverse(const float* xi, int32_t inbag, const int32_t* st, const int32_t* bv, const float* xs,
@@ -187,7 +189,7 @@ static int valid_leaf(const int32_t* st, int32_t node, int32_t nnode) {
 /* Integer counts by leaf buckets.  layout 0 = FULL, 1 = UPPER. */
 int orc_counts_bucket(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr,
                       const int32_t* nin, int64_t n, int32_t ntree, int32_t variant, int32_t layout, int32_t* out) {
-  if (variant == ORC_OOB_ASYM && layout != 0) return -1;
+  if ((variant == ORC_OOB_ASYM || variant == ORC_OOB_ROW) && layout != 0) return -1;
   int32_t maxnn = 0;
   for (int32_t t = 0; t < ntree; ++t) {
     int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
@@ -219,6 +221,16 @@ int orc_counts_bucket(const int32_t* nodestatus, const int64_t* node_offset, con
     for (int32_t k = 0; k < nn; ++k) {
       for (int32_t a = head[k]; a < head[k + 1]; ++a) {
         const int64_t i = memb[a];
+        if (variant == ORC_OOB_ROW) {
+          const int bi = bag[i] > 0;
+          if (!bi) out[i + i * n] += 1;
+          for (int32_t b2 = a + 1; b2 < head[k + 1]; ++b2) {
+            const int64_t j = memb[b2];
+            if (!bi) out[i + j * n] += 1;
+            if (!(bag[j] > 0)) out[j + i * n] += 1;
+          }
+          continue;
+        }
         if (variant == ORC_ALL) {
           /* diagonal and pairs j > i; rows are owned by distinct i within a leaf,
              but different leaves never share (i, j), so no race */
@@ -276,6 +288,8 @@ int orc_counts_brute(const int32_t* nodestatus, const int64_t* node_offset, cons
           hit = 1;
         else if (variant == ORC_OOB_ASYM)
           hit = bag[i] == 0 && bag[j] > 0;
+        else if (variant == ORC_OOB_ROW)
+          hit = bag[i] == 0;
         else
           hit = (bag[i] == 0) != (bag[j] == 0);
         out[i + j * n] += hit;

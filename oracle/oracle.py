@@ -22,7 +22,7 @@ HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "librfxref.so"
 
-ALL, OOB_ASYM, OOB_UPPER = 0, 1, 2
+ALL, OOB_ASYM, OOB_UPPER, OOB_ROW = 0, 1, 2, 4
 
 _p = C.c_void_p
 _i32 = C.c_int32

@@ -31,6 +31,7 @@ PROX_ALL = 0
 PROX_OOB_ASYM = 1
 PROX_OOB_UPPER = 2
 PROX_PAIR = 3  # batch: ALL into out, OOB_UPPER into out2
+PROX_OOB_ROW = 4
 
 LAYOUT_FULL = 0
 LAYOUT_UPPER = 1
@@ -39,6 +40,7 @@ ALGO_AUTO = 0
 ALGO_DENSE = 1
 ALGO_BUCKET = 2
 ALGO_GATHER = 3
+ALGO_PREBUILT = 0x100  # or-ed into BUCKET/GATHER: buckets already in the workspace
 
 MAX_TOTAL_NODES = 1073741823
 
@@ -77,6 +79,7 @@ _SIGS = {
     # GPU side
     "rfx_leaf_assign": (_i32, [_p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _p]),
     "rfx_prox_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),
+    "rfx_prox_build_buckets": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz]),
     "rfx_prox_accumulate": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _i32, _p, _p, _sz]),
     "rfx_prox_accumulate_pair": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _p, _p, _p, _sz]),
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),

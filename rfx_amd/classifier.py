@@ -19,7 +19,8 @@ backed by the HIP path in librfx:
                        finishprox -> float32 -> nout renormalisation ->
                        classical MDS), computed on the GPU in FP64.
   compute_mds_from_factors / compute_mds_3d_from_factors
-                       MDS from the stored factors (k <= 3).
+                       MDS from the stored factors on the matrix cores,
+                       refined with the FP32 factors (k <= 3, rank <= 64).
 
 Options outside the proximity path (importance, prediction, RF-GAP,
 case-wise weights, regression losses) are accepted for signature
@@ -173,15 +174,18 @@ class RandomForestClassifier:
         return X.cpu().numpy()
 
     def compute_mds_from_factors(self, k: int = 3) -> np.ndarray:
-        from .proximity import mds_factors
+        """MDS from the stored factors on the matrix cores, refined with the FP32 factors."""
+        from .proximity import mds_factors_mixed
 
         self._need_fit("compute_mds_from_factors")
         if self._factors is None:
             raise RuntimeError("compute_mds_from_factors needs use_qlora=True and compute_proximity=True")
         if not 1 <= k <= 3:
             raise ValueError("k must be 1, 2 or 3")
-        Uq, sc, fmt, _ = self._factors
-        X, _, _ = mds_factors(Uq, sc, fmt)
+        Uq, sc, fmt, U = self._factors
+        if U.shape[1] > 64:
+            raise ValueError("the factor MDS supports rank <= 64")
+        X, _, _ = mds_factors_mixed(Uq, sc, fmt, U)
         return X.cpu().numpy()[:, :k].copy()
 
     def compute_mds_3d_from_factors(self) -> np.ndarray:

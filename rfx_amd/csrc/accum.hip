@@ -89,10 +89,16 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
     const uint32_t e = bend[mbucket[m]];
     const int64_t i = me >> 1;
     const uint32_t bi = me & 1u;
+    if (MODE == kMode_OOB_ROW && bi == 0) atomicAdd(&out[i + i * n], 1);  // i with itself
     for (uint32_t q = static_cast<uint32_t>(m) + 1; q < e; ++q) {
       const uint32_t o = members[q];
       const int64_t j = o >> 1;
       const uint32_t bj = o & 1u;
+      if (MODE == kMode_OOB_ROW) {
+        if (bi == 0) atomicAdd(&out[i + j * n], 1);
+        if (bj == 0) atomicAdd(&out[j + i * n], 1);
+        continue;
+      }
       if (MODE == kMode_ALL || MODE == kMode_PAIR) {
         if (LAYOUT == RFX_LAYOUT_UPPER) {
           atomicAdd(&out[rfx_upper_index(i, j, n)], 1);
@@ -226,11 +232,11 @@ __global__ __launch_bounds__(256) void gather_walk_kernel(const uint2* __restric
         const uint32_t bq = m & 1u, bu = (bagm >> u) & 1u;
         ++cur[u];
         nv[u] = cur[u] < end[u] ? members[cur[u]] : 0xffffffffu;
-        if (LAYOUT == RFX_LAYOUT_FULL && j == line) continue;
+        if (LAYOUT == RFX_LAYOUT_FULL && MODE != kMode_OOB_ROW && j == line) continue;
         uint32_t inc = 0;
         if (MODE == kMode_ALL) inc = 1u;
         if (MODE == kMode_OOB_UPPER) inc = bq != bu ? 1u : 0u;
-        if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+        if (MODE == kMode_OOB_ASYM || MODE == kMode_OOB_ROW) inc = bq == 0u ? 1u : 0u;
         if (MODE == kMode_PAIR) inc = 1u | (bq != bu ? 0x10000u : 0u);
         if (inc) atomicAdd(&acc[j - lo], inc);
       }
@@ -367,7 +373,7 @@ int run_dense(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int
 
 template <int MODE, int LAYOUT>
 int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-               int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+               int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
   const int64_t slots = n * static_cast<int64_t>(ntree);
   const int64_t elems = LAYOUT == RFX_LAYOUT_UPPER ? n * (n + 1) / 2 : n * n;
@@ -376,7 +382,7 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
     if (!rc && out2) rc = fill_u32(st, out2, elems, 0u);
     if (rc) return rc;
   }
-  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   hipLaunchKernelGGL((bucket_pair_kernel<MODE, LAYOUT>), dim3(grid_for(slots, 256)), dim3(256), 0, st, w.members,
                      w.mbucket, w.bend, w.nvalid, n, out, out2);
@@ -387,9 +393,9 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
 
 template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-               int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+               int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   if (n <= kGatherLineMaxN) {
     dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), rfx_ceil_div(n, kGatherChunk));
@@ -441,18 +447,20 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
 template <int MODE>
 int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout,
              int32_t algo, int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
+  const bool prebuilt = (algo & RFX_ALGO_PREBUILT) != 0;
+  algo &= ~RFX_ALGO_PREBUILT;
   if (algo == RFX_ALGO_DENSE) {
     return layout == RFX_LAYOUT_UPPER ? run_dense<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, accumulate, out, out2)
                                       : run_dense<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, accumulate, out, out2);
   }
   if (algo == RFX_ALGO_GATHER) {
     return layout == RFX_LAYOUT_UPPER
-               ? run_gather<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws)
-               : run_gather<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws);
+               ? run_gather<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws, prebuilt)
+               : run_gather<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws, prebuilt);
   }
   return layout == RFX_LAYOUT_UPPER
-             ? run_bucket<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws)
-             : run_bucket<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws);
+             ? run_bucket<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws, prebuilt)
+             : run_bucket<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, total_nodes, accumulate, out, out2, ws, prebuilt);
 }
 
 // AUTO: compare the compare-kernel cost T*n^2/2 (one tile compare per
@@ -487,6 +495,11 @@ int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, i
 int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout, int32_t algo) {
   if (!keys || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || ntree > 65535) return RFX_ERR_INVALID;
   if (layout != RFX_LAYOUT_FULL && layout != RFX_LAYOUT_UPPER) return RFX_ERR_INVALID;
+  if (algo & RFX_ALGO_PREBUILT) {
+    const int32_t base = algo & ~RFX_ALGO_PREBUILT;
+    if (base != RFX_ALGO_BUCKET && base != RFX_ALGO_GATHER) return RFX_ERR_INVALID;
+    algo = base;
+  }
   if (algo < RFX_ALGO_AUTO || algo > RFX_ALGO_GATHER) return RFX_ERR_INVALID;
   if (algo != RFX_ALGO_DENSE && (total_nodes <= 0 || total_nodes > RFX_MAX_TOTAL_NODES)) return RFX_ERR_INVALID;
   if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
@@ -524,19 +537,33 @@ extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t 
   return RFX_OK;
 }
 
+extern "C" int rfx_prox_build_buckets(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                      int64_t total_nodes, void* ws, size_t ws_bytes) {
+  if (!keys || !ws || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || total_nodes <= 0 ||
+      total_nodes > RFX_MAX_TOTAL_NODES)
+    return RFX_ERR_INVALID;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
+  return bucket_build(st, keys, n, ntree, total_nodes, bucket_ws_carve(ws, n, ntree, total_nodes));
+}
+
 extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
                                    int64_t total_nodes, int32_t variant, int32_t layout, int32_t algo,
                                    int32_t accumulate, int32_t* out, void* ws, size_t ws_bytes) {
   int rc = check_common(keys, n, ntree, total_nodes, layout, algo);
   if (rc) return rc;
   if (!out) return RFX_ERR_INVALID;
-  if (variant == RFX_PROX_OOB_ASYM && layout != RFX_LAYOUT_FULL) return RFX_ERR_UNSUPPORTED;
+  if ((variant == RFX_PROX_OOB_ASYM || variant == RFX_PROX_OOB_ROW) && layout != RFX_LAYOUT_FULL)
+    return RFX_ERR_UNSUPPORTED;
   if (algo == RFX_ALGO_AUTO) {
     rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
     if (rc) return rc;
   }
-  if ((algo == RFX_ALGO_BUCKET || algo == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
-    return RFX_ERR_WORKSPACE;
+  {
+    const int32_t base = algo & ~RFX_ALGO_PREBUILT;
+    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+      return RFX_ERR_WORKSPACE;
+  }
   switch (variant) {
     case RFX_PROX_ALL:
       return dispatch<kMode_ALL>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr, ws);
@@ -545,6 +572,8 @@ extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t 
     case RFX_PROX_OOB_UPPER:
       return dispatch<kMode_OOB_UPPER>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr,
                                        ws);
+    case RFX_PROX_OOB_ROW:
+      return dispatch<kMode_OOB_ROW>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out, nullptr, ws);
     default:
       return RFX_ERR_INVALID;
   }
@@ -560,8 +589,11 @@ extern "C" int rfx_prox_accumulate_pair(hipStream_t st, const int32_t* keys, int
     rc = choose_algo(st, keys, n, ntree, total_nodes, ws, ws_bytes, &algo);
     if (rc) return rc;
   }
-  if ((algo == RFX_ALGO_BUCKET || algo == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
-    return RFX_ERR_WORKSPACE;
+  {
+    const int32_t base = algo & ~RFX_ALGO_PREBUILT;
+    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+      return RFX_ERR_WORKSPACE;
+  }
   return dispatch<kMode_PAIR>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out_all, out_oob, ws);
 }
 
@@ -583,6 +615,10 @@ extern "C" int rfx_prox_rows(hipStream_t st, const int32_t* keys, int64_t n, int
     case RFX_PROX_OOB_UPPER:
       hipLaunchKernelGGL(dense_rows_kernel<kMode_OOB_UPPER>, grid, dim3(kThreads), 0, st, keys, n, ntree,
                          row_begin, row_end, out);
+      break;
+    case RFX_PROX_OOB_ROW:
+      hipLaunchKernelGGL(dense_rows_kernel<kMode_OOB_ROW>, grid, dim3(kThreads), 0, st, keys, n, ntree, row_begin,
+                         row_end, out);
       break;
     default:
       return RFX_ERR_INVALID;

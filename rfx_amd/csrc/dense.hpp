@@ -18,11 +18,12 @@ template <int MODE>
 __device__ __forceinline__ int32_t row_key(int32_t k) {
   if (MODE == kMode_ALL) return k >> 1;
   if (MODE == kMode_OOB_ASYM) return (k >= 0 && !(k & 1)) ? k : -1;
+  if (MODE == kMode_OOB_ROW) return (k >= 0 && !(k & 1)) ? (k >> 1) : -1;
   return k;  // OOB_UPPER, PAIR (invalid stays -1)
 }
 template <int MODE>
 __device__ __forceinline__ int32_t col_key(int32_t k) {
-  if (MODE == kMode_ALL) return k < 0 ? -2 : (k >> 1);
+  if (MODE == kMode_ALL || MODE == kMode_OOB_ROW) return k < 0 ? -2 : (k >> 1);
   if (MODE == kMode_OOB_ASYM) return (k >= 0 && (k & 1)) ? (k ^ 1) : -2;
   if (MODE == kMode_OOB_UPPER) return k < 0 ? -2 : (k ^ 1);
   return k < 0 ? 0x40000000 : k;  // PAIR

@@ -19,6 +19,7 @@ constexpr int kMode_ALL = RFX_PROX_ALL;
 constexpr int kMode_OOB_ASYM = RFX_PROX_OOB_ASYM;
 constexpr int kMode_OOB_UPPER = RFX_PROX_OOB_UPPER;
 constexpr int kMode_PAIR = 3;
+constexpr int kMode_OOB_ROW = RFX_PROX_OOB_ROW;
 
 #ifndef RFX_GATHER_CHUNK
 #define RFX_GATHER_CHUNK 8192
@@ -103,11 +104,11 @@ __device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t b
       if (!ok[u]) continue;
       const int64_t q = mv[u] >> 1;
       const uint32_t bq = mv[u] & 1u;
-      if (LAYOUT == RFX_LAYOUT_FULL && q == line) continue;
+      if (LAYOUT == RFX_LAYOUT_FULL && MODE != kMode_OOB_ROW && q == line) continue;
       uint32_t inc = 0;
       if (MODE == kMode_ALL) inc = 1u;
       if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
-      if (MODE == kMode_OOB_ASYM) inc = bq == 0u ? 1u : 0u;
+      if (MODE == kMode_OOB_ASYM || MODE == kMode_OOB_ROW) inc = bq == 0u ? 1u : 0u;
       if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
       if (inc) atomicAdd(&acc[q - lo], inc);
     }

@@ -252,13 +252,14 @@ std::vector<double> LowRankProximityMatrix::compute_mds_from_factors(integer_t k
   if (!ready_) throw std::runtime_error("LowRankProximityMatrix: factors not finalized");
   if (k < 1 || k > 3) throw std::runtime_error("compute_mds_from_factors supports k <= 3");
   if (rank_ > 64) throw std::runtime_error("factor MDS supports rank <= 64");
-  const size_t wsb = rfx_mds_factors_workspace_bytes(n_, rank_, RFX_PREC_FP32);
+  // matrix cores on the stored factors, refined with the FP32 factors
+  const size_t wsb = rfx_mds_factors_mixed_workspace_bytes(n_, rank_);
   DevBuf ws(wsb), cd(8 * 3 * static_cast<size_t>(n_));
   double eig[16];
-  int32_t iters = 0;
-  rfx_ok(rfx_mds_factors(nullptr, Uq_, scales_, static_cast<int32_t>(format_), n_, rank_, RFX_PREC_FP32, 500, 1e-5,
-                         ws.p, wsb, cd.as<double>(), eig, &iters),
-         "rfx_mds_factors");
+  int32_t iters[2] = {0, 0};
+  rfx_ok(rfx_mds_factors_mixed(nullptr, Uq_, scales_, static_cast<int32_t>(format_), U_, n_, rank_, 500, 1e-4, 60,
+                               5e-5, ws.p, wsb, cd.as<double>(), eig, iters),
+         "rfx_mds_factors_mixed");
   std::vector<double> c3(3 * static_cast<size_t>(n_));
   download(c3.data(), cd.p, c3.size());
   std::vector<double> out(static_cast<size_t>(k) * n_);

@@ -87,6 +87,11 @@ class Accumulator:
     def alloc(self, layout: int) -> torch.Tensor:
         return torch.empty(self.out_elems(layout), dtype=torch.int32, device=self.device)
 
+    def build_buckets(self, keys: torch.Tensor) -> None:
+        """Leaf buckets of keys into the workspace (for algo | ALGO_PREBUILT)."""
+        check(lib().rfx_prox_build_buckets(_stream(), _dptr(keys), self.n, self.ntree, self.total_nodes,
+                                           _dptr(self.ws), self.ws.numel()), "rfx_prox_build_buckets")
+
     def accumulate(self, keys: torch.Tensor, variant: int, layout: int, algo: int = _lib.ALGO_AUTO,
                    out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
         if out is None:

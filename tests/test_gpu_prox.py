@@ -93,7 +93,7 @@ def test_counts_all_variants_layouts_algos(gpu, case):
     acc = Accumulator(n, T, f.total_nodes)
     for variant, layout in [(_lib.PROX_ALL, _lib.LAYOUT_FULL), (_lib.PROX_ALL, _lib.LAYOUT_UPPER),
                             (_lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL), (_lib.PROX_OOB_UPPER, _lib.LAYOUT_FULL),
-                            (_lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)]:
+                            (_lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER), (_lib.PROX_OOB_ROW, _lib.LAYOUT_FULL)]:
         want = expect(f, nd_ref, variant, layout)
         for algo in ALGOS:
             got = acc.accumulate(keys, variant, layout, algo).cpu().numpy()
@@ -110,6 +110,14 @@ def test_counts_all_variants_layouts_algos(gpu, case):
             assert np.array_equal(a.cpu().numpy(), wa) and np.array_equal(o.cpu().numpy(), wo)
     auto = acc.accumulate(keys, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_AUTO).cpu().numpy()
     assert np.array_equal(auto, expect(f, nd_ref, _lib.PROX_ALL, _lib.LAYOUT_UPPER))
+    # buckets built once, then several counting calls on them
+    acc.build_buckets(keys)
+    for algo in (_lib.ALGO_GATHER, _lib.ALGO_BUCKET):
+        a, o = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, algo | _lib.ALGO_PREBUILT)
+        assert np.array_equal(a.cpu().numpy(), expect(f, nd_ref, _lib.PROX_ALL, _lib.LAYOUT_UPPER))
+        assert np.array_equal(o.cpu().numpy(), expect(f, nd_ref, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER))
+        P = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, algo | _lib.ALGO_PREBUILT)
+        assert np.array_equal(P.cpu().numpy(), expect(f, nd_ref, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL))
 
 
 def test_rows_match_full(gpu):
@@ -118,7 +126,7 @@ def test_rows_match_full(gpu):
     _, keys, _ = run_keys(f, x)
     nd = O.leaf_assign(f, x)
     acc = Accumulator(f.n, f.ntree, f.total_nodes)
-    for v in (_lib.PROX_ALL, _lib.PROX_OOB_ASYM, _lib.PROX_OOB_UPPER):
+    for v in (_lib.PROX_ALL, _lib.PROX_OOB_ASYM, _lib.PROX_OOB_UPPER, _lib.PROX_OOB_ROW):
         full = expect(f, nd, v, _lib.LAYOUT_FULL).reshape(f.n, f.n, order="F")
         r = acc.rows(keys, v, 700, 777).cpu().numpy()
         assert np.array_equal(r, full[700:777])
@@ -165,7 +173,7 @@ def test_oob_depth_cap_and_single_leaf_trees(gpu):
     assert nd_ref[0, 1] == 0 and nd_ref[0, 0] == 2 * depth  # OOB capped, in-bag reaches the end
     acc = Accumulator(n, 2, f.total_nodes)
     for v, lay in [(_lib.PROX_ALL, _lib.LAYOUT_FULL), (_lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL),
-                   (_lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)]:
+                   (_lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER), (_lib.PROX_OOB_ROW, _lib.LAYOUT_FULL)]:
         want = expect(f, nd_ref, v, lay)
         for algo in ALGOS:
             assert np.array_equal(acc.accumulate(keys, v, lay, algo).cpu().numpy(), want)
@@ -245,4 +253,8 @@ def test_long_rows_many_trees_match_bucket_form(gpu):
     del a, o, bo
     P = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
     Q = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL, _lib.ALGO_BUCKET)
+    assert torch.equal(P, Q)
+    del P, Q
+    P = acc.accumulate(keys, _lib.PROX_OOB_ROW, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
+    Q = acc.accumulate(keys, _lib.PROX_OOB_ROW, _lib.LAYOUT_FULL, _lib.ALGO_BUCKET)
     assert torch.equal(P, Q)

@@ -95,7 +95,7 @@ def test_finishprox_closed_form():
     assert np.allclose(S, g["proxsym"].reshape(n, n, order="F"), rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("variant", [O.ALL, O.OOB_ASYM, O.OOB_UPPER])
+@pytest.mark.parametrize("variant", [O.ALL, O.OOB_ASYM, O.OOB_UPPER, O.OOB_ROW])
 def test_bucket_counts_equal_brute_force(variant):
     x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 150, 5, 3, 7, seed=2)
     nd = O.leaf_assign(f, x)
@@ -103,7 +103,7 @@ def test_bucket_counts_equal_brute_force(variant):
     b = O.counts_brute(f, nd, variant)
     assert np.array_equal(a, b)
     n = f.n
-    if variant != O.OOB_ASYM:
+    if variant not in (O.OOB_ASYM, O.OOB_ROW):
         u = O.counts(f, nd, variant, 1)
         iu = np.triu_indices(n)
         assert np.array_equal(u, b.reshape(n, n, order="F")[iu])
@@ -173,3 +173,32 @@ def test_embedding_error_metrics():
     Y = X.copy()
     Y[:, 2] = rng.normal(size=50)
     assert O.embedding_errors(Y, X)["max_principal_angle"] > 0.1
+
+
+def test_oob_row_closed_forms():
+    """OOB_ROW (cuda/rf_proximity.cu:32-67) against identities that fix it
+    independently of the restatement: P4 = P1 + (OOB-OOB same-leaf counts),
+    P4_ii = #{t : i out-of-bag on a terminal leaf}, and row sums
+    sum_j P4_ij = sum over OOB trees of |leaf_t(i)| (i itself included)."""
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 160, 5, 3, 9, seed=6)
+    nd = O.leaf_assign(f, x)
+    n, T = f.n, f.ntree
+    P4 = O.counts(f, nd, O.OOB_ROW, 0).reshape(n, n, order="F").astype(np.int64)
+    P1 = O.counts(f, nd, O.OOB_ASYM, 0).reshape(n, n, order="F").astype(np.int64)
+    oo = np.zeros((n, n), np.int64)
+    diag = np.zeros(n, np.int64)
+    rows = np.zeros(n, np.int64)
+    for t in range(T):
+        b = f.node_offset[t]
+        st = f.nodestatus[b:f.node_offset[t + 1]]
+        leaf = nd[t]
+        term = st[leaf] == -1
+        oob = (f.nin[t] == 0) & term
+        same = (leaf[:, None] == leaf[None, :]) & term[:, None] & term[None, :]
+        oo += same & oob[:, None] & oob[None, :]
+        diag += oob
+        size = np.bincount(leaf[term], minlength=len(st))
+        rows += np.where(oob, size[leaf], 0)
+    assert np.array_equal(P4, P1 + oo)
+    assert np.array_equal(np.diag(P4), diag)
+    assert np.array_equal(P4.sum(axis=1), rows)
