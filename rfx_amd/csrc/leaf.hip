@@ -17,6 +17,8 @@
 //    is all zero) and anything else stops at node 0.
 // With all features numeric both rules reduce to x <= threshold -> left.
 
+#include <algorithm>
+
 #include "rfx_common.hpp"
 #include "leaf.hpp"
 
@@ -96,14 +98,50 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_kernel(const float* __rest
 }
 
 // All-numeric fast path (body in leaf.hpp).
+//
+// One-dimensional grid laid out for the 8 XCDs.  The dispatcher sends
+// workgroup b to XCD b % 8, and each XCD has its own L2, so XCD x owns the
+// contiguous sample blocks [x * nsbx, (x + 1) * nsbx) and walks them through
+// every tree: its share of the feature matrix stays in its L2 instead of the
+// whole matrix cycling through all eight.  Within an XCD the sample blocks
+// are taken in chunks of `sc` blocks (a feature footprint of about 1.5 MiB);
+// per chunk the tree groups run in order with the chunk's sample blocks
+// innermost, so a tree group's nodes are fetched once per chunk.  Workgroups
+// past the last sample block exit.
+struct LeafGrid {
+  int64_t nsb, ntg, nsbx, sc;
+};
+
+__device__ __forceinline__ bool leaf_grid_map(const LeafGrid& g, int64_t b, int64_t& sb, int64_t& tg) {
+  const int64_t xcd = b & 7, local = b >> 3;
+  const int64_t per_chunk = g.sc * g.ntg;
+  const int64_t c = local / per_chunk, rem = local - c * per_chunk;
+  const int64_t csize = min(g.sc, g.nsbx - c * g.sc);
+  tg = rem / csize;
+  sb = xcd * g.nsbx + c * g.sc + (rem - tg * csize);
+  return sb < g.nsb;
+}
+
+LeafGrid leaf_grid(int64_t n, int32_t p, int32_t ntree) {
+  LeafGrid g;
+  g.nsb = rfx_ceil_div(n, kBlock);
+  g.ntg = rfx_ceil_div(ntree, kFastTrees);
+  g.nsbx = (g.nsb + 7) / 8;
+  const int64_t block_bytes = static_cast<int64_t>(kBlock) * p * 4;
+  g.sc = std::max<int64_t>(1, std::min<int64_t>(g.nsbx, (int64_t(3) << 19) / std::max<int64_t>(block_bytes, 1)));
+  return g;
+}
+
 __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float* __restrict__ x, int64_t n,
                                                                      const uint2* __restrict__ nodes,
                                                                      const int64_t* __restrict__ node_off,
                                                                      int32_t ntree,
                                                                      const uint32_t* __restrict__ inbag,
                                                                      int32_t* __restrict__ nodextr,
-                                                                     int32_t* __restrict__ keys) {
-  leaf_numeric_body(x, n, nodes, node_off, ntree, inbag, nodextr, keys, 0, blockIdx.x, blockIdx.y);
+                                                                     int32_t* __restrict__ keys, LeafGrid g) {
+  int64_t sb, tg;
+  if (!leaf_grid_map(g, blockIdx.x, sb, tg)) return;
+  leaf_numeric_body(x, n, nodes, node_off, ntree, inbag, nodextr, keys, 0, sb, tg);
 }
 
 }  // namespace
@@ -116,9 +154,11 @@ extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n
     return RFX_ERR_INVALID;
   if (!nodextr && !keys) return RFX_OK;
   if (cat == nullptr) {
-    dim3 g(rfx_ceil_div(n, kBlock), rfx_ceil_div(ntree, kFastTrees));
-    hipLaunchKernelGGL(leaf_assign_numeric_kernel, g, dim3(kBlock), 0, stream, x_col, n,
-                       reinterpret_cast<const uint2*>(nodes), node_off, ntree, inbag, nodextr, keys);
+    const LeafGrid g = leaf_grid(n, p, ntree);
+    const int64_t blocks = 8 * g.nsbx * g.ntg;
+    if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(leaf_assign_numeric_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, x_col,
+                       n, reinterpret_cast<const uint2*>(nodes), node_off, ntree, inbag, nodextr, keys, g);
     return rfx_launch_status();
   }
   dim3 grid(rfx_ceil_div(n, kBlock), rfx_ceil_div(ntree, kTreesPerBlock));
