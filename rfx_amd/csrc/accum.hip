@@ -30,6 +30,8 @@
 // Reference semantics: src/rf_proximity.cpp:179-230 (OOB_ASYM, accumulated
 // per tree), cuda/rf_proximity_upper_triangle.cu:238-248 (OOB_UPPER fold).
 
+#include <algorithm>
+
 #include "rfx_common.hpp"
 #include "bucket.hpp"
 #include "gather.hpp"
@@ -142,19 +144,149 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
 // steps in flight, so a leaf of 1000 members and one of 2 cost the same per
 // member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
 // ---------------------------------------------------------------------------
+// Lines of at most kGatherLineMaxN <= kMembers16MaxN samples read the 16-bit
+// member copy.
+static_assert(kGatherLineMaxN <= kMembers16MaxN, "line kernel reads the 16-bit members");
+#ifndef RFX_GATHER_ROWS
+#define RFX_GATHER_ROWS 1
+#endif
+#ifndef RFX_LINE_MAX_LDS
+#define RFX_LINE_MAX_LDS 81920
+#endif
+#ifndef RFX_LINE_THREADS
+#define RFX_LINE_THREADS 512
+#endif
+#ifndef RFX_LINE_WAVES_PER_EU
+#define RFX_LINE_WAVES_PER_EU 1
+#endif
+constexpr int kLineThreads = RFX_LINE_THREADS;
+constexpr int64_t kLineMaxLds = RFX_LINE_MAX_LDS;  // bytes of LDS counters per workgroup
+
+// Counters of one line chunk in dynamic LDS: the whole line when it fits
+// kLineMaxLds (no chunk searches, every pe entry read once), else chunks of
+// kLineMaxLds / 4 columns.
+inline int64_t line_chunk_len(int64_t n) { return std::min<int64_t>(n, kLineMaxLds / 4 - 4); }
+
 template <int MODE, int LAYOUT>
-__global__ __launch_bounds__(256) void gather_line_kernel(const uint2* __restrict__ pe,
-                                                          const uint32_t* __restrict__ bstart,
-                                                          const uint32_t* __restrict__ mbucket, int64_t n,
-                                                          int32_t ntree, const uint32_t* __restrict__ members,
-                                                          int32_t accumulate, int32_t* __restrict__ out,
-                                                          int32_t* __restrict__ out2) {
-  __shared__ uint32_t acc[kGatherChunk];
-  __shared__ uint32_t diag_valid;
+__global__ __launch_bounds__(kLineThreads, RFX_LINE_WAVES_PER_EU) void gather_line_kernel(
+    const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ mbucket, int64_t n,
+    int32_t ntree, const uint16_t* __restrict__ members, int32_t accumulate, int32_t* __restrict__ out,
+    int32_t* __restrict__ out2, int64_t chunk_len) {
+  extern __shared__ uint32_t line_lds[];
   const int64_t line = xcd_row(blockIdx.x);
   if (line >= n) return;
-  gather_line_body<MODE, LAYOUT>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line, blockIdx.y,
-                                 acc, diag_valid);
+  gather_line_body<MODE, LAYOUT, uint16_t, kLineThreads>(pe, bstart, mbucket, n, ntree, members, accumulate, out,
+                                                         out2, line, blockIdx.y, line_lds + 4, line_lds[0],
+                                                         chunk_len);
+}
+
+// Whole lines, persistent: when a whole line's counters fit the LDS budget,
+// each workgroup loops over the lines b, b + G, b + 2G, ... (G, the grid, a
+// multiple of the XCD row group, so a workgroup stays on its XCD and its
+// lines spread over the whole triangle).  The pe entries of the next line
+// are loaded before the current line is visited, so their latency overlaps
+// the member visits instead of opening every line.  No chunk searches: a
+// line's runs are [position + 1, end) (UPPER) or the whole leaf (FULL).
+template <int MODE, int LAYOUT, int KT>
+__device__ __forceinline__ void line_whole_body(const uint2 (&q)[KT], const uint32_t* __restrict__ bstart,
+                                                const uint32_t* __restrict__ mbucket, int64_t n,
+                                                const uint16_t* __restrict__ members, int32_t accumulate,
+                                                int32_t* __restrict__ out, int32_t* __restrict__ out2, int64_t line,
+                                                uint32_t* acc, uint32_t& diag_valid) {
+  const int64_t lo = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
+  const int len_line = static_cast<int>(n - lo);
+  for (int c = threadIdx.x; c < len_line; c += kLineThreads) acc[c] = 0u;
+  if (threadIdx.x == 0) diag_valid = 0u;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  uint32_t valid = 0;
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    uint32_t s = 0, len = 0, bag = 0;
+    const uint32_t end = q[k].y & 0x7fffffffu;
+    if (end > 0) {
+      ++valid;
+      bag = q[k].y >> 31;
+      s = LAYOUT == RFX_LAYOUT_UPPER ? q[k].x + 1 : bstart[mbucket[q[k].x]];
+      len = end - s;
+    }
+    if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
+    line_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
+  }
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  __syncthreads();
+  if ((MODE == kMode_ALL || MODE == kMode_PAIR) && threadIdx.x == 0) acc[line - lo] = diag_valid;
+  __syncthreads();
+  const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, line, n) : line * n;
+  for (int c = threadIdx.x; c < len_line; c += kLineThreads) {
+    const uint32_t v = acc[c];
+    if (MODE == kMode_PAIR) {
+      const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+      out[base + c] = accumulate ? out[base + c] + va : va;
+      out2[base + c] = accumulate ? out2[base + c] + vb : vb;
+    } else {
+      out[base + c] = accumulate ? out[base + c] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+    }
+  }
+}
+
+template <int KT>
+__device__ __forceinline__ void line_pe_load(const uint2* __restrict__ pe, int64_t n, int32_t ntree, int64_t line,
+                                             uint2 (&q)[KT]) {
+  const int32_t t0 = static_cast<int32_t>(threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int32_t t = t0 + k * kLineThreads;
+    q[k] = t < ntree ? pe[static_cast<int64_t>(t) * n + line] : make_uint2(0u, 0u);
+  }
+}
+
+template <int MODE, int LAYOUT, int KT>
+__global__ __launch_bounds__(kLineThreads) void gather_rows_kernel(const uint2* __restrict__ pe,
+                                                                   const uint32_t* __restrict__ bstart,
+                                                                   const uint32_t* __restrict__ mbucket, int64_t n,
+                                                                   int32_t ntree,
+                                                                   const uint16_t* __restrict__ members,
+                                                                   int32_t accumulate, int32_t* __restrict__ out,
+                                                                   int32_t* __restrict__ out2, int64_t rows_padded) {
+  extern __shared__ uint32_t line_lds[];
+  const int64_t G = gridDim.x;
+  int64_t b = blockIdx.x;
+  while (b < rows_padded && xcd_row(b) >= n) b += G;
+  uint2 q[KT];
+  if (b < rows_padded) line_pe_load<KT>(pe, n, ntree, xcd_row(b), q);
+  while (b < rows_padded) {
+    const int64_t line = xcd_row(b);
+    uint2 cur[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) cur[k] = q[k];
+    int64_t bn = b + G;
+    while (bn < rows_padded && xcd_row(bn) >= n) bn += G;
+    if (bn < rows_padded) line_pe_load<KT>(pe, n, ntree, xcd_row(bn), q);
+    line_whole_body<MODE, LAYOUT, KT>(cur, bstart, mbucket, n, members, accumulate, out, out2, line, line_lds + 4,
+                                      line_lds[0]);
+    __syncthreads();  // the line's counters are read out before the next line zeroes them
+    b = bn;
+  }
+}
+
+#ifndef RFX_ROWS_WG_PER_CU
+#define RFX_ROWS_WG_PER_CU 16
+#endif
+
+template <int MODE, int LAYOUT, int KT>
+int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
+                int32_t* out2) {
+  const int64_t rows_padded = xcd_rows_padded(n);
+  const int64_t g = std::min<int64_t>(rows_padded, 256 * RFX_ROWS_WG_PER_CU / (8 * kXcdGroup) * (8 * kXcdGroup));
+  hipLaunchKernelGGL((gather_rows_kernel<MODE, LAYOUT, KT>), dim3(static_cast<unsigned>(g)), dim3(kLineThreads),
+                     static_cast<unsigned>(4 * (n + 4)), st, w.pe, w.bstart, w.mbucket, n, ntree, w.members16,
+                     accumulate, out, out2, rows_padded);
+  return rfx_launch_status();
 }
 
 // ---------------------------------------------------------------------------
@@ -318,7 +450,7 @@ __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restr
       const uint32_t e2 = hi < n ? gallop_lb(members, cur[g], end[g], static_cast<uint32_t>(hi)) : end[g];
       const uint32_t s0 = cur[g];
       cur[g] = e2;
-      flat_visits<MODE, LAYOUT>(s0, e2 - s0, (bagm >> g) & 1u, lane, members, acc, lo, line);
+      line_visits<MODE, LAYOUT>(s0, e2 - s0, (bagm >> g) & 1u, lane, members, acc, lo, line);
     }
     __syncthreads();
     if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
@@ -397,10 +529,18 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
   int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
+  if (RFX_GATHER_ROWS && 4 * (n + 4) <= kLineMaxLds) {
+    if (ntree <= kLineThreads) return launch_rows<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
+    if (ntree <= 2 * kLineThreads) return launch_rows<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
+    if (ntree <= 4 * kLineThreads) return launch_rows<MODE, LAYOUT, 4>(st, w, n, ntree, accumulate, out, out2);
+    if (ntree <= 8 * kLineThreads) return launch_rows<MODE, LAYOUT, 8>(st, w, n, ntree, accumulate, out, out2);
+  }
   if (n <= kGatherLineMaxN) {
-    dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), rfx_ceil_div(n, kGatherChunk));
-    hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(256), 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    const int64_t chunk_len = line_chunk_len(n);
+    dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), rfx_ceil_div(n, chunk_len));
+    hipLaunchKernelGGL((gather_line_kernel<MODE, LAYOUT>), grid, dim3(kLineThreads),
+                       static_cast<unsigned>(4 * (chunk_len + 4)), st, w.pe, w.bstart, w.mbucket, n, ntree,
+                       w.members16, accumulate, out, out2, chunk_len);
     return rfx_launch_status();
   }
   const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n))), blk(256);

@@ -10,6 +10,8 @@
 // From the sorted arrays:
 //   bstart[b], bend[b]  the member range of leaf b (0, 0 when empty)
 //   members[p]          (i << 1) | in-bag of the p-th sorted slot
+//   members16[p]        the same as 16 bits, kept when n <= kMembers16MaxN
+//                       (half the bytes for the gather kernels' member runs)
 //   mbucket[p]          its leaf (the sorted keys)
 //   pe[s]               (position of slot s in members, end of its leaf |
 //                       in-bag << 31); (0, 0) off-leaf
@@ -20,6 +22,7 @@
 
 namespace {
 
+constexpr int64_t kMembers16MaxN = 32768;
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanChunk = kScanThreads * kScanItems;
@@ -208,13 +211,15 @@ __global__ void rs_bounds_kernel(const uint32_t* __restrict__ K, int64_t N, uint
 // (0, 0) off-leaf.
 __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N, uint32_t n,
                                   uint32_t total_nodes, const uint32_t* __restrict__ bend,
-                                  uint32_t* __restrict__ members, uint2* __restrict__ pe) {
+                                  uint32_t* __restrict__ members, uint16_t* __restrict__ members16,
+                                  uint2* __restrict__ pe) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t v = V[p];
   const uint32_t s = v >> 1, bag = v & 1u;
   const uint32_t i = s % n;
   members[p] = (i << 1) | bag;
+  if (members16) members16[p] = static_cast<uint16_t>((i << 1) | bag);
   const uint32_t k = K[p];
   pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
 }
@@ -255,6 +260,7 @@ struct BucketWs {
   uint32_t* bstart;
   uint32_t* bend;
   uint32_t* members;
+  uint16_t* members16;  // nullptr unless n <= kMembers16MaxN
   uint32_t* mbucket;  // sorted keys (one of the two key buffers)
   uint2* pe;
   uint32_t* nvalid;
@@ -282,7 +288,7 @@ size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
   const int64_t hlen = 256 * ntiles;
   const int64_t nparts = (hlen + kScanChunk - 1) / kScanChunk;
   return align256(4 * total_nodes) * 2 + align256(4 * slots) * 5 + align256(8 * slots) + align256(4 * hlen) +
-         align256(4 * (nparts + 1)) + 512;
+         align256(4 * (nparts + 1)) + 512 + (n <= kMembers16MaxN ? align256(2 * slots) : 0);
 }
 
 BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes) {
@@ -313,6 +319,8 @@ BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes
   p += align256(4 * (w.nparts + 1));
   w.nvalid = reinterpret_cast<uint32_t*>(p);
   w.stat = reinterpret_cast<unsigned long long*>(p + 256);
+  p += 512;
+  w.members16 = n <= kMembers16MaxN ? reinterpret_cast<uint16_t*>(p) : nullptr;
   w.mbucket = w.K[w.passes & 1];
   return w;
 }
@@ -365,7 +373,8 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   if (rc) return rc;
   const int fin = w.passes & 1;
   hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
-                     static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.pe);
+                     static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.members16,
+                     w.pe);
   return rfx_launch_status();
 }
 

@@ -50,12 +50,13 @@ __device__ __forceinline__ int64_t xcd_row(int64_t b) {
   return (8 * (local / kXcdGroup) + x) * kXcdGroup + local % kXcdGroup;
 }
 
-// first position in members[lo, hi) whose sample index is >= key
-__device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ members, uint32_t lo, uint32_t hi,
-                                              uint32_t key) {
+// first position in members[lo, hi) whose sample index is >= key; M is
+// uint32_t, or uint16_t for the 16-bit copy kept when n <= kMembers16MaxN
+template <class M>
+__device__ __forceinline__ uint32_t member_lb(const M* __restrict__ members, uint32_t lo, uint32_t hi, uint32_t key) {
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if ((members[mid] >> 1) < key)
+    if ((static_cast<uint32_t>(members[mid]) >> 1) < key)
       lo = mid + 1;
     else
       hi = mid;
@@ -68,10 +69,9 @@ __device__ __forceinline__ uint32_t member_lb(const uint32_t* __restrict__ membe
 // member of the union (owner lane by a 6-step search over the prefix sums of
 // the run lengths), kGatherUnroll members in flight, and adds its count to
 // the LDS counters acc[column - lo].  All lanes of the wavefront must call.
-template <int MODE, int LAYOUT>
+template <int MODE, int LAYOUT, class M>
 __device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t bag, int lane,
-                                            const uint32_t* __restrict__ members, uint32_t* acc, int64_t lo,
-                                            int64_t line) {
+                                            const M* __restrict__ members, uint32_t* acc, int64_t lo, int64_t line) {
   uint32_t pre = len;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t b
       const uint32_t eo = __shfl(excl, o, 64);
       bo[u] = __shfl(bag, o, 64);
       ok[u] = g < total;
-      mv[u] = ok[u] ? members[so + (g - eo)] : 0u;
+      mv[u] = ok[u] ? static_cast<uint32_t>(members[so + (g - eo)]) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kGatherUnroll; ++u) {
@@ -113,6 +113,91 @@ __device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t b
       if (inc) atomicAdd(&acc[q - lo], inc);
     }
   }
+}
+
+// The same visits, flattened over aligned 16-byte blocks of the member
+// array instead of single members: a lane's run [s, s + len) touches the
+// blocks floor(s / P) .. floor((s + len - 1) / P) (P = 16 / sizeof(M)
+// members per block); the union of all lanes' blocks is spread over the
+// lanes by the same prefix-sum owner search, so the search and its
+// shuffles are paid once per block, each block is one aligned 16-byte load,
+// and the members of a block outside the run are masked off.  Blocks may
+// reach up to P - 1 members past the end of the member array: the array's
+// allocation is padded to 256 bytes.
+#ifndef RFX_GATHER_PIECE_UNROLL
+#define RFX_GATHER_PIECE_UNROLL 2
+#endif
+template <int MODE, int LAYOUT, class M>
+__device__ __forceinline__ void flat_visit_blocks(uint32_t s, uint32_t len, uint32_t bag, int lane,
+                                                  const M* __restrict__ members, uint32_t* acc, int64_t lo,
+                                                  int64_t line) {
+  constexpr uint32_t P = 16 / sizeof(M);
+  constexpr int U = RFX_GATHER_PIECE_UNROLL;
+  const uint32_t e = s + len;
+  const uint32_t nb = len ? (e - 1) / P - s / P + 1 : 0u;
+  uint32_t pre = nb;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(pre, d, 64);
+    if (lane >= d) pre += y;
+  }
+  const uint32_t excl = pre - nb;
+  const uint32_t total = __shfl(pre, 63, 64);
+  for (uint32_t g0 = 0; g0 < total; g0 += 64 * U) {
+    uint4 w[U];
+    uint32_t blo[U], bhi[U], bb[U], bo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = g0 + u * 64 + lane;
+      int o = 0;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t v = __shfl(pre, o + d - 1, 64);
+        if (v <= g) o += d;
+      }
+      o = min(o, 63);
+      const uint32_t so = __shfl(s, o, 64);
+      const uint32_t eo = __shfl(e, o, 64);
+      const uint32_t xo = __shfl(excl, o, 64);
+      bo[u] = __shfl(bag, o, 64);
+      const bool ok = g < total;
+      bb[u] = (so / P + (g - xo)) * P;  // first member index of the block
+      blo[u] = ok ? (so > bb[u] ? so - bb[u] : 0u) : P;
+      bhi[u] = ok ? min(P, eo - bb[u]) : 0u;
+      w[u] = ok ? *reinterpret_cast<const uint4*>(members + bb[u]) : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t wv[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
+#pragma unroll
+      for (uint32_t k = 0; k < P; ++k) {
+        if (k < blo[u] || k >= bhi[u]) continue;
+        const uint32_t m = sizeof(M) == 2 ? ((wv[k / 2] >> (16 * (k % 2))) & 0xffffu) : wv[k];
+        const int64_t q = m >> 1;
+        const uint32_t bq = m & 1u;
+        if (LAYOUT == RFX_LAYOUT_FULL && MODE != kMode_OOB_ROW && q == line) continue;
+        uint32_t inc = 0;
+        if (MODE == kMode_ALL) inc = 1u;
+        if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
+        if (MODE == kMode_OOB_ASYM || MODE == kMode_OOB_ROW) inc = bq == 0u ? 1u : 0u;
+        if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
+        if (inc) atomicAdd(&acc[q - lo], inc);
+      }
+    }
+  }
+}
+
+#ifndef RFX_GATHER_BLOCKS
+#define RFX_GATHER_BLOCKS 1
+#endif
+// The visit form the kernels use.
+template <int MODE, int LAYOUT, class M>
+__device__ __forceinline__ void line_visits(uint32_t s, uint32_t len, uint32_t bag, int lane,
+                                            const M* __restrict__ members, uint32_t* acc, int64_t lo, int64_t line) {
+  if (RFX_GATHER_BLOCKS)
+    flat_visit_blocks<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
+  else
+    flat_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
 }
 
 // First position p in [cur, end) whose member column is >= key: a galloping
@@ -137,25 +222,28 @@ __device__ __forceinline__ uint32_t gallop_lb(const uint32_t* __restrict__ membe
   return member_lb(members, lo + 1, hi, key);
 }
 
-template <int MODE, int LAYOUT>
+// `chunk` LDS counters per workgroup of THREADS threads (THREADS / 64
+// wavefronts, each taking 64 trees at a time).
+template <int MODE, int LAYOUT, class M, int THREADS = 256>
 __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
                                                  const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
-                                                 const uint32_t* __restrict__ members, int32_t accumulate,
+                                                 const M* __restrict__ members, int32_t accumulate,
                                                  int32_t* __restrict__ out, int32_t* __restrict__ out2, int64_t line,
-                                                 int64_t chunk, uint32_t* acc, uint32_t& diag_valid) {
+                                                 int64_t chunk, uint32_t* acc, uint32_t& diag_valid,
+                                                 int64_t chunk_len = kGatherChunk) {
   // UPPER rows are chunked from the diagonal: chunk y of row i covers
   // [i + y*CH, i + (y+1)*CH); FULL columns in aligned chunks.
-  const int64_t lo = (LAYOUT == RFX_LAYOUT_UPPER ? line : 0) + chunk * kGatherChunk;
+  const int64_t lo = (LAYOUT == RFX_LAYOUT_UPPER ? line : 0) + chunk * chunk_len;
   if (lo >= n) return;
-  const int64_t hi = min(n, lo + kGatherChunk);
+  const int64_t hi = min(n, lo + chunk_len);
   const int len_chunk = static_cast<int>(hi - lo);
-  for (int q = threadIdx.x; q < len_chunk; q += 256) acc[q] = 0u;
+  for (int q = threadIdx.x; q < len_chunk; q += THREADS) acc[q] = 0u;
   if (threadIdx.x == 0) diag_valid = 0u;
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t valid = 0;
-  for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
+  for (int32_t tb = wave * 64; tb < ntree; tb += THREADS) {
     const int32_t t = tb + lane;
     uint32_t s = 0, len = 0, bag = 0;
     if (t < ntree) {
@@ -178,7 +266,7 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
       }
     }
     if (MODE == kMode_OOB_ASYM && !bag) len = 0;  // column j of P needs j in-bag
-    flat_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
+    line_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
   }
   if (MODE == kMode_ALL || MODE == kMode_PAIR) {
 #pragma unroll
@@ -191,7 +279,7 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
   __syncthreads();
 
   const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
-  for (int q = threadIdx.x; q < len_chunk; q += 256) {
+  for (int q = threadIdx.x; q < len_chunk; q += THREADS) {
     const uint32_t v = acc[q];
     if (MODE == kMode_PAIR) {
       const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);

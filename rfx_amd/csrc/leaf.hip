@@ -122,12 +122,12 @@ __device__ __forceinline__ bool leaf_grid_map(const LeafGrid& g, int64_t b, int6
   return sb < g.nsb;
 }
 
-LeafGrid leaf_grid(int64_t n, int32_t p, int32_t ntree) {
+LeafGrid leaf_grid(int64_t n, int32_t p, int32_t ntree, int block, int trees) {
   LeafGrid g;
-  g.nsb = rfx_ceil_div(n, kBlock);
-  g.ntg = rfx_ceil_div(ntree, kFastTrees);
+  g.nsb = rfx_ceil_div(n, block);
+  g.ntg = rfx_ceil_div(ntree, trees);
   g.nsbx = (g.nsb + 7) / 8;
-  const int64_t block_bytes = static_cast<int64_t>(kBlock) * p * 4;
+  const int64_t block_bytes = static_cast<int64_t>(block) * p * 4;
   g.sc = std::max<int64_t>(1, std::min<int64_t>(g.nsbx, (int64_t(3) << 19) / std::max<int64_t>(block_bytes, 1)));
   return g;
 }
@@ -144,6 +144,52 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
   leaf_numeric_body(x, n, nodes, node_off, ntree, inbag, nodextr, keys, 0, sb, tg);
 }
 
+// LDS-staged form: the workgroup first copies its samples' feature rows
+// into LDS ([p][LB] floats, coalesced column reads), then every traversal
+// step reads the feature from LDS, so the divergent global gathers of a step
+// are the node records alone.  Used while p * LB * 4 bytes fit kStageMaxLds;
+// at n = 200,000 (p = 54, 100 trees) it takes 1.93 ms against 2.49 ms for
+// the global-feature kernel.
+#ifndef RFX_LEAF_STAGE
+#define RFX_LEAF_STAGE 1
+#endif
+#ifndef RFX_LEAF_STAGE_LB
+#define RFX_LEAF_STAGE_LB 64
+#endif
+#ifndef RFX_LEAF_STAGE_TREES
+#define RFX_LEAF_STAGE_TREES 16
+#endif
+#ifndef RFX_LEAF_STAGE_ILP
+#define RFX_LEAF_STAGE_ILP 4
+#endif
+constexpr int kStageLb = RFX_LEAF_STAGE_LB;
+constexpr int kStageTrees = RFX_LEAF_STAGE_TREES;
+constexpr int kStageIlp = RFX_LEAF_STAGE_ILP;
+constexpr int64_t kStageMaxLds = 64 * 1024;
+
+__global__ __launch_bounds__(kStageLb) void leaf_assign_staged_kernel(const float* __restrict__ x, int64_t n,
+                                                                      int32_t p, const uint2* __restrict__ nodes,
+                                                                      const int64_t* __restrict__ node_off,
+                                                                      int32_t ntree,
+                                                                      const uint32_t* __restrict__ inbag,
+                                                                      int32_t* __restrict__ nodextr,
+                                                                      int32_t* __restrict__ keys, LeafGrid g) {
+  extern __shared__ float xs[];
+  int64_t sb, tg;
+  if (!leaf_grid_map(g, blockIdx.x, sb, tg)) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
+  const int64_t i = sb * kStageLb + tid;
+  const bool in = i < n;
+  for (int32_t f = 0; f < p; ++f) xs[f * kStageLb + tid] = in ? x[static_cast<int64_t>(f) * n + i] : 0.0f;
+  __syncthreads();
+  if (!in) return;
+  const int32_t tb0 = static_cast<int32_t>(tg) * kStageTrees;
+  const int32_t tb1 = min(tb0 + kStageTrees, ntree);
+  const float* xr = xs + tid;
+  leaf_walk_trees<kStageIlp>([&](uint32_t f) { return xr[f * kStageLb]; }, i, n, nodes, node_off, tb0, tb1, inbag,
+                             nodextr, keys, 0);
+}
+
 }  // namespace
 
 extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n, int32_t p, const uint32_t* nodes,
@@ -154,7 +200,17 @@ extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n
     return RFX_ERR_INVALID;
   if (!nodextr && !keys) return RFX_OK;
   if (cat == nullptr) {
-    const LeafGrid g = leaf_grid(n, p, ntree);
+    const int64_t lds = static_cast<int64_t>(p) * kStageLb * 4;
+    if (RFX_LEAF_STAGE && lds <= kStageMaxLds) {
+      const LeafGrid g = leaf_grid(n, p, ntree, kStageLb, kStageTrees);
+      const int64_t blocks = 8 * g.nsbx * g.ntg;
+      if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+      hipLaunchKernelGGL(leaf_assign_staged_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kStageLb),
+                         static_cast<unsigned>(lds), stream, x_col, n, p, reinterpret_cast<const uint2*>(nodes),
+                         node_off, ntree, inbag, nodextr, keys, g);
+      return rfx_launch_status();
+    }
+    const LeafGrid g = leaf_grid(n, p, ntree, kBlock, kFastTrees);
     const int64_t blocks = 8 * g.nsbx * g.ntg;
     if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(leaf_assign_numeric_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, x_col,

@@ -26,27 +26,25 @@ constexpr int kMaxOobDescents = 999;
 constexpr int kIlp = RFX_LEAF_ILP;
 constexpr int kFastTrees = RFX_LEAF_TREES;  // trees per workgroup
 
-// Sample block bx (kBlock samples), tree group by (kFastTrees trees).  Keys
-// are ((key_node_base + node_off[t] + node) << 1) | in-bag, -1 off-leaf.
-__device__ __forceinline__ void leaf_numeric_body(const float* __restrict__ x, int64_t n,
-                                                  const uint2* __restrict__ nodes,
-                                                  const int64_t* __restrict__ node_off, int32_t ntree,
-                                                  const uint32_t* __restrict__ inbag, int32_t* __restrict__ nodextr,
-                                                  int32_t* __restrict__ keys, int64_t key_node_base, int64_t bx,
-                                                  int64_t by) {
-  const int64_t i = bx * kBlock + threadIdx.x;
-  if (i >= n) return;
+// Walk of sample i through trees [tb0, tb1), ILP traversals in lockstep;
+// feat(f) returns the sample's value of feature f (from global memory or an
+// LDS tile).  Keys are ((key_node_base + node_off[t] + node) << 1) | in-bag,
+// -1 off-leaf.
+template <int ILP, class Feat>
+__device__ __forceinline__ void leaf_walk_trees(const Feat& feat, int64_t i, int64_t n,
+                                                const uint2* __restrict__ nodes,
+                                                const int64_t* __restrict__ node_off, int32_t tb0, int32_t tb1,
+                                                const uint32_t* __restrict__ inbag, int32_t* __restrict__ nodextr,
+                                                int32_t* __restrict__ keys, int64_t key_node_base) {
   const int64_t words = (n + 31) / 32;
-  const int32_t tb0 = static_cast<int32_t>(by) * kFastTrees;
-  const int32_t tb1 = min(tb0 + kFastTrees, ntree);
-  for (int32_t t0 = tb0; t0 < tb1; t0 += kIlp) {
-    int64_t base[kIlp];
-    uint32_t kt[kIlp];
-    uint2 nd[kIlp];
-    bool bag[kIlp], live[kIlp];
-    int steps[kIlp];
+  for (int32_t t0 = tb0; t0 < tb1; t0 += ILP) {
+    int64_t base[ILP];
+    uint32_t kt[ILP];
+    uint2 nd[ILP];
+    bool bag[ILP], live[ILP];
+    int steps[ILP];
 #pragma unroll
-    for (int u = 0; u < kIlp; ++u) {
+    for (int u = 0; u < ILP; ++u) {
       const int32_t t = t0 + u;
       live[u] = t < tb1;
       base[u] = live[u] ? node_off[t] : 0;
@@ -58,25 +56,25 @@ __device__ __forceinline__ void leaf_numeric_body(const float* __restrict__ x, i
     bool any = true;
     while (any) {
       any = false;
-      bool go[kIlp];
+      bool go[ILP];
 #pragma unroll
-      for (int u = 0; u < kIlp; ++u) {
+      for (int u = 0; u < ILP; ++u) {
         go[u] = nd[u].y != 0xffffffffu && (bag[u] || steps[u] < kMaxOobDescents);
         if (go[u]) {
           const uint32_t f = nd[u].y & 1023u, l = nd[u].y >> 10;
-          const float v = x[static_cast<int64_t>(f) * n + i];
+          const float v = feat(f);
           kt[u] = v <= __uint_as_float(nd[u].x) ? l : l + 1;
           ++steps[u];
         }
         any = any || go[u];
       }
-      // the kIlp node gathers of this step are independent
+      // the ILP node gathers of this step are independent
 #pragma unroll
-      for (int u = 0; u < kIlp; ++u)
+      for (int u = 0; u < ILP; ++u)
         if (go[u]) nd[u] = nodes[base[u] + kt[u]];
     }
 #pragma unroll
-    for (int u = 0; u < kIlp; ++u) {
+    for (int u = 0; u < ILP; ++u) {
       if (!live[u]) continue;
       if (nd[u].y != 0xffffffffu) kt[u] = 0;  // out-of-bag depth cap reached
       const int64_t o = static_cast<int64_t>(t0 + u) * n + i;
@@ -87,6 +85,23 @@ __device__ __forceinline__ void leaf_numeric_body(const float* __restrict__ x, i
                       : -1;
     }
   }
+}
+
+// Sample block bx (kBlock samples), tree group by (kFastTrees trees), the
+// features read from the column-major matrix in global memory.
+__device__ __forceinline__ void leaf_numeric_body(const float* __restrict__ x, int64_t n,
+                                                  const uint2* __restrict__ nodes,
+                                                  const int64_t* __restrict__ node_off, int32_t ntree,
+                                                  const uint32_t* __restrict__ inbag, int32_t* __restrict__ nodextr,
+                                                  int32_t* __restrict__ keys, int64_t key_node_base, int64_t bx,
+                                                  int64_t by) {
+  const int64_t i = bx * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t tb0 = static_cast<int32_t>(by) * kFastTrees;
+  const int32_t tb1 = min(tb0 + kFastTrees, ntree);
+  const float* xi = x + i;
+  leaf_walk_trees<kIlp>([&](uint32_t f) { return xi[static_cast<int64_t>(f) * n]; }, i, n, nodes, node_off, tb0, tb1,
+                        inbag, nodextr, keys, key_node_base);
 }
 
 }  // namespace
