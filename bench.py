@@ -204,8 +204,11 @@ def main() -> int:
 
     # Algorithmic bytes of the counting launch (one pass over all trees):
     # two int32 upper triangles written + the int32 key array read.
-    KERNEL = {_lib.ALGO_GATHER: "gather_line_kernel<PAIR,UPPER>" if n <= 32768 else
-              "gather_gallop_kernel<PAIR,UPPER>", _lib.ALGO_BUCKET: "bucket_pair_kernel<PAIR,UPPER>",
+    # the GATHER kernel run_gather (accum.hip) picks: whole lines in LDS up to
+    # n = 20476 and T = 4096, line chunks up to n = 32768, else the gallop walk
+    gather_kernel = ("gather_rows_kernel<PAIR,UPPER>" if n <= 20476 and T <= 4096 else
+                     "gather_line_kernel<PAIR,UPPER>" if n <= 32768 else "gather_gallop_kernel<PAIR,UPPER>")
+    KERNEL = {_lib.ALGO_GATHER: gather_kernel, _lib.ALGO_BUCKET: "bucket_pair_kernel<PAIR,UPPER>",
               _lib.ALGO_DENSE: "dense_tile_kernel<PAIR,UPPER>"}
     acc_bytes = 2 * 4 * P + 4 * n * T
     achieved = acc_bytes / (kern_ms / 1e3) / 1e9
