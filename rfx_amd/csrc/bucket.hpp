@@ -138,35 +138,40 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __r
   hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Stable scatter of one tile.  Keys are visited round by round (256
-// consecutive keys per round, in index order); within a wavefront the lanes
-// holding the same digit are found with 8 ballots, across the 4 wavefronts
-// through per-wave digit counts in LDS.
+// Stable scatter of one tile of kRsTile keys.  Wave w ranks the keys
+// [w * kRsTile / 4, (w + 1) * kRsTile / 4) of the tile in kRsRounds rounds of
+// 64 consecutive keys: the lanes holding the same digit in a round are found
+// with 8 ballots, and a wave-private LDS counter per digit carries the count
+// over the rounds (no barrier inside the loop).  One block scan over the
+// digits then gives every key its position in the tile sorted by digit; the
+// tile is sorted in LDS and written out in position order, so the stores of
+// consecutive threads fall into the same digit run (coalesced) instead of
+// one scattered 4-byte store per key.
 __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* __restrict__ Kin,
                                                                 const uint32_t* __restrict__ Vin, int64_t N, int shift,
                                                                 int64_t ntiles, const uint32_t* __restrict__ offs,
                                                                 uint32_t* __restrict__ Kout,
                                                                 uint32_t* __restrict__ Vout) {
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wcnt[4][256];
+  constexpr int kWaves = kRsThreads / 64;
+  constexpr int kPerWave = kRsTile / kWaves;
+  constexpr int kKeysPerLane = kPerWave / 64;
+  __shared__ uint32_t wcnt[kWaves][256];  // per-wave digit counts, then per-wave digit bases
+  __shared__ int64_t gofs[256];           // global position of the tile's digit run minus its tile start
+  __shared__ uint32_t wsum[kWaves];
+  __shared__ uint32_t sk[kRsTile], sv[kRsTile];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  running[tid] = offs[static_cast<int64_t>(tid) * ntiles + blockIdx.x];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  const int tile_len = static_cast<int>(min<int64_t>(kRsTile, N - tile0));
+  for (int d = lane; d < 256; d += 64) wcnt[wave][d] = 0u;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int q = 0; q < kRsRounds; ++q) {
-    wcnt[0][tid] = 0;
-    wcnt[1][tid] = 0;
-    wcnt[2][tid] = 0;
-    wcnt[3][tid] = 0;
-    __syncthreads();
-    const int64_t g = base + q * kRsThreads + tid;
-    const bool valid = g < N;
-    uint32_t k = 0, v = 0;
-    if (valid) {
-      k = Kin[g];
-      v = Vin[g];
-    }
-    const uint32_t d = (k >> shift) & 255u;
+  uint32_t k[kKeysPerLane], v[kKeysPerLane], rk[kKeysPerLane];
+#pragma unroll
+  for (int r = 0; r < kKeysPerLane; ++r) {
+    const int idx = wave * kPerWave + r * 64 + lane;  // index in the tile
+    const bool valid = idx < tile_len;
+    k[r] = valid ? Kin[tile0 + idx] : 0u;
+    v[r] = valid ? Vin[tile0 + idx] : 0u;
+    const uint32_t d = (k[r] >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
@@ -174,18 +179,58 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* 
       const uint64_t m = __ballot(on);
       peers &= on ? m : ~m;
     }
-    const uint32_t rank_w = __popcll(peers & lt_mask);
-    if (valid && rank_w == 0) wcnt[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = running[d] + rank_w;
-      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-      Kout[pos] = k;
-      Vout[pos] = v;
+    // rank among this wave's earlier keys of digit d; the wave's LDS ops run
+    // in program order, so every lane reads the count before the leader adds
+    const uint32_t before = valid ? wcnt[wave][d] : 0u;
+    rk[r] = before + static_cast<uint32_t>(__popcll(peers & lt_mask));
+    if (valid && (peers & lt_mask) == 0ull) wcnt[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+  // thread d: the tile's count of digit d, per-wave prefixes, block scan over d
+  const int d = tid;
+  uint32_t c[kWaves], tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    c[w] = wcnt[w][d];
+    tot += c[w];
+  }
+  uint32_t incl = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t start = incl - tot;
+  for (int w = 0; w < wave; ++w) start += wsum[w];
+  uint32_t run = start;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    wcnt[w][d] = run;
+    run += c[w];
+  }
+  gofs[d] = static_cast<int64_t>(offs[static_cast<int64_t>(d) * ntiles + blockIdx.x]) - start;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kKeysPerLane; ++r) {
+    const int idx = wave * kPerWave + r * 64 + lane;
+    if (idx < tile_len) {
+      const uint32_t pos = wcnt[wave][(k[r] >> shift) & 255u] + rk[r];
+      if (pos < static_cast<uint32_t>(tile_len)) {
+        sk[pos] = k[r];
+        sv[pos] = v[r];
+      }
     }
-    __syncthreads();
-    running[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int j = tid; j < tile_len; j += kRsThreads) {
+    const uint32_t key = sk[j];
+    const int64_t pos = gofs[(key >> shift) & 255u] + j;
+    if (pos >= 0 && pos < N) {
+      Kout[pos] = key;
+      Vout[pos] = sv[j];
+    }
   }
 }
 
@@ -249,11 +294,18 @@ __global__ void rs_members_seg_kernel(const uint32_t* __restrict__ K, const uint
   pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
 }
 
-// Graph-safe fill (no memset nodes in captured graphs).
+// Graph-safe fill (no memset nodes in captured graphs): 16-byte stores from
+// the first 16-byte boundary, single words for the unaligned head and tail.
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t len, uint32_t v) {
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < len;
-       e += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    p[e] = v;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t head = min<int64_t>(len, ((16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15) / 4);
+  const int64_t n4 = (len - head) / 4;
+  uint4* p4 = reinterpret_cast<uint4*>(p + head);
+  const uint4 v4 = make_uint4(v, v, v, v);
+  for (int64_t e = gid; e < n4; e += stride) p4[e] = v4;
+  if (gid < head) p[gid] = v;
+  for (int64_t e = head + 4 * n4 + gid; e < len; e += stride) p[e] = v;
 }
 
 struct BucketWs {
@@ -334,7 +386,8 @@ unsigned grid_for(int64_t work, int threads) {
 
 int fill_u32(hipStream_t st, void* p, int64_t len, uint32_t v) {
   if (len <= 0) return RFX_OK;
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(len, 256)), dim3(256), 0, st, static_cast<uint32_t*>(p), len, v);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for((len + 3) / 4, 256)), dim3(256), 0, st, static_cast<uint32_t*>(p),
+                     len, v);
   return rfx_launch_status();
 }
 
