@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 // into LDS ([p][LB] floats, coalesced column reads), then every traversal
 // step reads the feature from LDS, so the divergent global gathers of a step
 // are the node records alone.  Used while p * LB * 4 bytes fit kStageMaxLds;
-// at n = 200,000 (p = 54, 100 trees) it takes 1.93 ms against 2.49 ms for
+// at n = 200,000 (p = 54, 100 trees) it takes 1.70 ms against 2.49 ms for
 // the global-feature kernel.
 #ifndef RFX_LEAF_STAGE
 #define RFX_LEAF_STAGE 1
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 #define RFX_LEAF_STAGE_TREES 16
 #endif
 #ifndef RFX_LEAF_STAGE_ILP
-#define RFX_LEAF_STAGE_ILP 4
+#define RFX_LEAF_STAGE_ILP 8
 #endif
 constexpr int kStageLb = RFX_LEAF_STAGE_LB;
 constexpr int kStageTrees = RFX_LEAF_STAGE_TREES;
