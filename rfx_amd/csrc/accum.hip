@@ -83,12 +83,15 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ mbucket,
                                                           const uint32_t* __restrict__ bend,
                                                           const uint32_t* __restrict__ nvalid_p, int64_t n,
-                                                          int32_t* __restrict__ out, int32_t* __restrict__ out2) {
+                                                          uint32_t total_nodes, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ out2) {
   const uint32_t nvalid = *nvalid_p;
   for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < nvalid;
        m += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t leaf = mbucket[m];
+    if (leaf >= total_nodes) continue;  // off-leaf slot (per-tree layout: at the end of its tree's segment)
     const uint32_t me = members[m];
-    const uint32_t e = bend[mbucket[m]];
+    const uint32_t e = bend[leaf];
     const int64_t i = me >> 1;
     const uint32_t bi = me & 1u;
     if (MODE == kMode_OOB_ROW && bi == 0) atomicAdd(&out[i + i * n], 1);  // i with itself
@@ -517,7 +520,7 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   hipLaunchKernelGGL((bucket_pair_kernel<MODE, LAYOUT>), dim3(grid_for(slots, 256)), dim3(256), 0, st, w.members,
-                     w.mbucket, w.bend, w.nvalid, n, out, out2);
+                     w.mbucket, w.bend, w.nvalid, n, static_cast<uint32_t>(total_nodes), out, out2);
   if (MODE == kMode_ALL || MODE == kMode_PAIR)
     hipLaunchKernelGGL(all_diag_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, keys, n, ntree, LAYOUT, out);
   return rfx_launch_status();

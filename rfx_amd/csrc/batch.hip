@@ -242,7 +242,7 @@ size_t layout_bytes(rfx_batch* b) {
 
 int upload(hipStream_t st, rfx_batch* b, char* ws) {
   const size_t B = b->prob.size();
-  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes, false);
   std::vector<LeafDesc> ld(B);
   std::vector<GatherDesc> gd(B);
   int32_t* keys = reinterpret_cast<int32_t*>(ws + b->off_keys);
@@ -292,7 +292,7 @@ int max_blocks_per_cu(int32_t variant) {
 // The launch sequence (eager, or recorded into a graph by the caller).
 int enqueue(hipStream_t st, rfx_batch* b, char* ws) {
   const int B = static_cast<int>(b->prob.size());
-  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes, false);
   const int64_t nblocks = b->block_start[B];
   hipLaunchKernelGGL(batch_leaf_kernel, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, st,
                      reinterpret_cast<const LeafDesc*>(ws + b->off_leaf),
@@ -434,7 +434,7 @@ extern "C" int rfx_batch_run(hipStream_t st, rfx_batch* b, void* workspace, size
 // the one-problem cost model: DENSE costs T n^2 / 2 tile compares, GATHER
 // sum|L|^2 / 2 member visits plus one write per output entry.
 int measure_and_choose(hipStream_t st, rfx_batch* b, char* ws) {
-  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes);
+  const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes, false);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(w.stat);
   const size_t B = b->prob.size();
   std::vector<double> sq(B);

@@ -7,6 +7,14 @@
 // tile histograms, one exclusive scan over the digit-major histogram, and a
 // scatter whose in-tile ranks come from wavefront ballots on the digit bits)
 // groups the slots by leaf with the samples of a leaf in ascending order.
+// The one-problem build sorts every tree's segment of slots [t*n, (t+1)*n)
+// on its own, by the node id relative to the tree's smallest leaf id: a tree
+// has fewer than 2n + 1 nodes, so 2 passes instead of 3 at n = 20,000 (3
+// instead of 4 at n = 200,000).  The passes needed follow from the largest
+// leaf-id range of a tree, found on the device; the host launches the
+// maximum and the surplus passes return at once.  Off-leaf slots (sentinel
+// key) sort to the end of their tree's segment.  The batched build sorts the
+// concatenated key space in one segment, off-leaf slots at the very end.
 // From the sorted arrays:
 //   bstart[b], bend[b]  the member range of leaf b (0, 0 when empty)
 //   members[p]          (i << 1) | in-bag of the p-th sorted slot
@@ -69,9 +77,15 @@ __device__ uint32_t block_exclusive_scan(uint32_t (&v)[kScanItems], uint32_t* ld
   return total;
 }
 
+// The scans of a radix pass return at once when `passes` (device pass count
+// of the per-tree sort) says the pass is surplus.
+__device__ __forceinline__ bool rs_surplus(const int32_t* passes, int pass) { return passes && pass >= *passes; }
+
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t len,
-                                                                   uint32_t* __restrict__ part) {
+                                                                   uint32_t* __restrict__ part,
+                                                                   const int32_t* passes = nullptr, int pass = 0) {
   __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  if (rs_surplus(passes, pass)) return;
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
 #pragma unroll
@@ -80,8 +94,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ part, int64_t nparts) {
+__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ part, int64_t nparts,
+                                                                     const int32_t* passes = nullptr, int pass = 0) {
   __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  if (rs_surplus(passes, pass)) return;
   uint32_t carry = 0;
   for (int64_t b0 = 0; b0 < nparts; b0 += kScanChunk) {
     const int64_t base = b0 + threadIdx.x * kScanItems;
@@ -100,8 +116,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* _
 // Exclusive scan of in[] into out[] (in-place allowed: each element is read
 // and written by the same thread).
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* in, int64_t len,
-                                                                  const uint32_t* __restrict__ part, uint32_t* out) {
+                                                                  const uint32_t* __restrict__ part, uint32_t* out,
+                                                                  const int32_t* passes = nullptr, int pass = 0) {
   __shared__ uint32_t lw[kScanThreads / 64 + 1];
+  if (rs_surplus(passes, pass)) return;
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
 #pragma unroll
@@ -113,7 +131,46 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
     if (base + q < len) out[base + q] = v[q] + off;
 }
 
-// Sort input: key = leaf (global node id) or total_nodes, value = slot<<1|bag.
+// Tiles never straddle a segment: segment g holds seg_len slots starting at
+// g * seg_len, tile j of it [j * kRsTile, min((j + 1) * kRsTile, seg_len)),
+// and histograms are laid out [segment][digit][tile], so one exclusive scan
+// over them keeps every segment's keys inside the segment.  The global
+// layout is one segment of all N slots.
+struct RsTiles {
+  int64_t seg_len;  // slots per segment
+  int64_t tps;      // tiles per segment
+  int64_t ntiles;   // total tiles
+};
+
+__device__ __forceinline__ void rs_tile(const RsTiles& g, int64_t tile, int64_t& start, int& len, int64_t& seg,
+                                        int64_t& j) {
+  seg = tile / g.tps;
+  j = tile - seg * g.tps;
+  start = seg * g.seg_len + j * kRsTile;
+  len = static_cast<int>(min<int64_t>(kRsTile, g.seg_len - j * kRsTile));
+}
+
+__device__ __forceinline__ int64_t rs_hist_index(const RsTiles& g, int64_t seg, int64_t j, uint32_t d) {
+  return (seg * 256 + d) * g.tps + j;
+}
+
+// Pass p reads buffer (p + first) & 1 and writes the other, first being the
+// parity of the pass count (device value `passes`, per-tree sort) or 0 (host
+// pass count, global sort), so the sorted keys always end in buffer 0 for
+// the per-tree sort.  With `passes` given, a pass at or beyond it returns.
+__device__ __forceinline__ bool rs_pass_buffers(const int32_t* passes, int pass, int& src) {
+  if (passes) {
+    const int np = *passes;
+    if (pass >= np) return false;
+    src = (pass + (np & 1)) & 1;
+  } else {
+    src = pass & 1;
+  }
+  return true;
+}
+
+// Sort input, global layout: key = leaf (global node id) or total_nodes,
+// value = slot << 1 | bag.
 __global__ void rs_prep_kernel(const int32_t* __restrict__ keys, int64_t slots, uint32_t total_nodes,
                                uint32_t* __restrict__ K, uint32_t* __restrict__ V) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -123,19 +180,85 @@ __global__ void rs_prep_kernel(const int32_t* __restrict__ keys, int64_t slots, 
   V[s] = (static_cast<uint32_t>(s) << 1) | (k >= 0 ? (static_cast<uint32_t>(k) & 1u) : 0u);
 }
 
-// Digit histogram of one tile, written digit-major: hist[d * ntiles + tile].
-__global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __restrict__ K, int64_t N, int shift,
-                                                             int64_t ntiles, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t cnt[256];
-  cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
-  for (int q = 0; q < kRsRounds; ++q) {
-    const int64_t g = base + q * kRsThreads + threadIdx.x;
-    if (g < N) atomicAdd(&cnt[(K[g] >> shift) & 255u], 1u);
+// Per-tree layout, step 1: smallest and largest leaf id of each tree
+// (workgroup t), and the largest range over the trees (atomic).
+__global__ __launch_bounds__(256) void rs_tree_range_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                            uint32_t* __restrict__ tmin, uint32_t* __restrict__ info) {
+  __shared__ uint32_t smin[4], smax[4];
+  const int64_t t = blockIdx.x;
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const int32_t k = keys[t * n + i];
+    if (k >= 0) {
+      const uint32_t v = static_cast<uint32_t>(k) >> 1;
+      lo = min(lo, v);
+      hi = max(hi, v);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = min(lo, static_cast<uint32_t>(__shfl_xor(lo, d, 64)));
+    hi = max(hi, static_cast<uint32_t>(__shfl_xor(hi, d, 64)));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    smin[wave] = lo;
+    smax[wave] = hi;
   }
   __syncthreads();
-  hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+  if (threadIdx.x == 0) {
+    lo = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
+    hi = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+    tmin[t] = lo;
+    if (lo <= hi) atomicMax(&info[0], hi - lo);
+  }
+}
+
+// Step 2: the sentinel (all ones over the digits needed) and the pass count:
+// info[0] = largest range, info[1] = sentinel, info[2] = passes.
+__global__ void rs_tree_passes_kernel(uint32_t* __restrict__ info) {
+  const uint32_t range = info[0];
+  int bits = 0;
+  while (bits < 32 && (1ull << bits) <= static_cast<uint64_t>(range) + 1) ++bits;  // range + 1 < 2^bits
+  const int passes = max(1, (bits + 7) / 8);
+  info[1] = passes >= 4 ? 0xffffffffu : static_cast<uint32_t>((1ull << (8 * passes)) - 1);
+  info[2] = static_cast<uint32_t>(passes);
+}
+
+// Step 3: key = leaf id - tmin[t] or the sentinel, into the buffer the first
+// pass reads.
+__global__ void rs_prep_tree_kernel(const int32_t* __restrict__ keys, int64_t n, int64_t slots,
+                                    const uint32_t* __restrict__ tmin, const uint32_t* __restrict__ info,
+                                    uint32_t* __restrict__ K0, uint32_t* __restrict__ K1, uint32_t* __restrict__ V0,
+                                    uint32_t* __restrict__ V1) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= slots) return;
+  const bool odd = info[2] & 1u;
+  uint32_t* K = odd ? K1 : K0;
+  uint32_t* V = odd ? V1 : V0;
+  const int32_t k = keys[s];
+  K[s] = k >= 0 ? (static_cast<uint32_t>(k) >> 1) - tmin[s / n] : info[1];
+  V[s] = (static_cast<uint32_t>(s) << 1) | (k >= 0 ? (static_cast<uint32_t>(k) & 1u) : 0u);
+}
+
+// Digit histogram of one tile.
+__global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __restrict__ K0,
+                                                             const uint32_t* __restrict__ K1,
+                                                             const int32_t* __restrict__ passes, int pass, RsTiles g,
+                                                             uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  int src;
+  if (!rs_pass_buffers(passes, pass, src)) return;
+  const uint32_t* K = src ? K1 : K0;
+  const int shift = 8 * pass;
+  int64_t start, seg, j;
+  int len;
+  rs_tile(g, blockIdx.x, start, len, seg, j);
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < len; q += kRsThreads) atomicAdd(&cnt[(K[start + q] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[rs_hist_index(g, seg, j, threadIdx.x)] = cnt[threadIdx.x];
 }
 
 // Stable scatter of one tile of kRsTile keys.  Wave w ranks the keys
@@ -147,11 +270,10 @@ __global__ __launch_bounds__(kRsThreads) void rs_hist_kernel(const uint32_t* __r
 // tile is sorted in LDS and written out in position order, so the stores of
 // consecutive threads fall into the same digit run (coalesced) instead of
 // one scattered 4-byte store per key.
-__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* __restrict__ Kin,
-                                                                const uint32_t* __restrict__ Vin, int64_t N, int shift,
-                                                                int64_t ntiles, const uint32_t* __restrict__ offs,
-                                                                uint32_t* __restrict__ Kout,
-                                                                uint32_t* __restrict__ Vout) {
+__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(uint32_t* __restrict__ K0, uint32_t* __restrict__ K1,
+                                                                uint32_t* __restrict__ V0, uint32_t* __restrict__ V1,
+                                                                const int32_t* __restrict__ passes, int pass,
+                                                                RsTiles g, const uint32_t* __restrict__ offs) {
   constexpr int kWaves = kRsThreads / 64;
   constexpr int kPerWave = kRsTile / kWaves;
   constexpr int kKeysPerLane = kPerWave / 64;
@@ -159,9 +281,18 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* 
   __shared__ int64_t gofs[256];           // global position of the tile's digit run minus its tile start
   __shared__ uint32_t wsum[kWaves];
   __shared__ uint32_t sk[kRsTile], sv[kRsTile];
+  int src;
+  if (!rs_pass_buffers(passes, pass, src)) return;
+  const uint32_t* Kin = src ? K1 : K0;
+  const uint32_t* Vin = src ? V1 : V0;
+  uint32_t* Kout = src ? K0 : K1;
+  uint32_t* Vout = src ? V0 : V1;
+  const int shift = 8 * pass;
+  int64_t tile0, seg, jt;
+  int tile_len;
+  rs_tile(g, blockIdx.x, tile0, tile_len, seg, jt);
+  const int64_t seg_end = seg * g.seg_len + g.seg_len;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kRsTile;
-  const int tile_len = static_cast<int>(min<int64_t>(kRsTile, N - tile0));
   for (int d = lane; d < 256; d += 64) wcnt[wave][d] = 0u;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint32_t k[kKeysPerLane], v[kKeysPerLane], rk[kKeysPerLane];
@@ -210,7 +341,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* 
     wcnt[w][d] = run;
     run += c[w];
   }
-  gofs[d] = static_cast<int64_t>(offs[static_cast<int64_t>(d) * ntiles + blockIdx.x]) - start;
+  gofs[d] = static_cast<int64_t>(offs[rs_hist_index(g, seg, jt, d)]) - start;
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kKeysPerLane; ++r) {
@@ -224,10 +355,11 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint32_t* 
     }
   }
   __syncthreads();
+  const int64_t seg_begin = seg * g.seg_len;
   for (int j = tid; j < tile_len; j += kRsThreads) {
     const uint32_t key = sk[j];
     const int64_t pos = gofs[(key >> shift) & 255u] + j;
-    if (pos >= 0 && pos < N) {
+    if (pos >= seg_begin && pos < seg_end) {
       Kout[pos] = key;
       Vout[pos] = sv[j];
     }
@@ -250,6 +382,28 @@ __global__ void rs_bounds_kernel(const uint32_t* __restrict__ K, int64_t N, uint
   const uint32_t next = p + 1 < N ? K[p + 1] : 0xffffffffu;
   if (next != k) bend[k] = static_cast<uint32_t>(p + 1);
   if (p + 1 == N) *nvalid = static_cast<uint32_t>(N);
+}
+
+// Leaf ranges for the per-tree layout: sorted local keys in K (buffer 0)
+// back to global leaf ids (mbucket, total_nodes for an off-leaf slot), the
+// ranges of every leaf, nvalid = N (off-leaf slots end each tree's segment).
+__global__ void rs_bounds_tree_kernel(const uint32_t* __restrict__ K, int64_t N, int64_t n, uint32_t total_nodes,
+                                      const uint32_t* __restrict__ tmin, const uint32_t* __restrict__ info,
+                                      uint32_t* __restrict__ mbucket, uint32_t* __restrict__ bstart,
+                                      uint32_t* __restrict__ bend, uint32_t* __restrict__ nvalid) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t sentinel = info[1];
+  const auto global = [&](int64_t q) -> uint32_t {
+    const uint32_t k = K[q];
+    return k == sentinel ? total_nodes : k + tmin[q / n];
+  };
+  const uint32_t g = global(p);
+  mbucket[p] = g;
+  if (p == 0) *nvalid = static_cast<uint32_t>(N);
+  if (g == total_nodes) return;
+  if (p == 0 || global(p - 1) != g) bstart[g] = static_cast<uint32_t>(p);
+  if (p + 1 == N || global(p + 1) != g) bend[g] = static_cast<uint32_t>(p + 1);
 }
 
 // members[p] = (i << 1) | bag; pe[s] = (p, end of its leaf | bag << 31),
@@ -320,6 +474,9 @@ struct BucketWs {
   uint32_t* V[2];
   uint32_t* hist;
   uint32_t* part;
+  uint32_t* tmin;  // per-tree smallest leaf id (per-tree sort)
+  uint32_t* info;  // largest range, sentinel, pass count (per-tree sort)
+  bool per_tree;
   unsigned long long* stat;
   int64_t ntiles;
   int64_t nparts;
@@ -334,20 +491,28 @@ int radix_passes(int64_t total_nodes) {
   return (bits + 7) / 8;
 }
 
+// Radix tiles: ntree segments of n slots each (per-tree sort; never fewer
+// tiles than the global layout of the same slots needs).
+int64_t rs_tiles_per_tree(int64_t n) { return (n + kRsTile - 1) / kRsTile; }
+
 size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
   const int64_t slots = n * static_cast<int64_t>(ntree);
-  const int64_t ntiles = (slots + kRsTile - 1) / kRsTile;
+  const int64_t ntiles = ntree * rs_tiles_per_tree(n);
   const int64_t hlen = 256 * ntiles;
   const int64_t nparts = (hlen + kScanChunk - 1) / kScanChunk;
   return align256(4 * total_nodes) * 2 + align256(4 * slots) * 5 + align256(8 * slots) + align256(4 * hlen) +
-         align256(4 * (nparts + 1)) + 512 + (n <= kMembers16MaxN ? align256(2 * slots) : 0);
+         align256(4 * (nparts + 1)) + 512 + align256(4 * static_cast<int64_t>(ntree)) + 256 +
+         (n <= kMembers16MaxN ? align256(2 * slots) : 0);
 }
 
-BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes) {
+// per_tree: the one-problem layout (segments of n slots per tree); false for
+// the batched build (one segment of all slots).
+BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes, bool per_tree = true) {
   BucketWs w;
   char* p = static_cast<char*>(ws);
   const int64_t slots = n * static_cast<int64_t>(ntree);
-  w.ntiles = (slots + kRsTile - 1) / kRsTile;
+  w.per_tree = per_tree;
+  w.ntiles = per_tree ? ntree * rs_tiles_per_tree(n) : (slots + kRsTile - 1) / kRsTile;
   const int64_t hlen = 256 * w.ntiles;
   w.nparts = (hlen + kScanChunk - 1) / kScanChunk;
   w.passes = radix_passes(total_nodes);
@@ -366,14 +531,20 @@ BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes
   w.pe = reinterpret_cast<uint2*>(p);
   p += align256(8 * slots);
   w.hist = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * hlen);
+  p += align256(4 * 256 * ntree * rs_tiles_per_tree(n));
+  const int64_t nparts_max = (256 * ntree * rs_tiles_per_tree(n) + kScanChunk - 1) / kScanChunk;
   w.part = reinterpret_cast<uint32_t*>(p);
-  p += align256(4 * (w.nparts + 1));
+  p += align256(4 * (nparts_max + 1));
   w.nvalid = reinterpret_cast<uint32_t*>(p);
   w.stat = reinterpret_cast<unsigned long long*>(p + 256);
   p += 512;
+  w.tmin = reinterpret_cast<uint32_t*>(p);
+  p += align256(4 * static_cast<int64_t>(ntree));
+  w.info = reinterpret_cast<uint32_t*>(p);
+  p += 256;
   w.members16 = n <= kMembers16MaxN ? reinterpret_cast<uint16_t*>(p) : nullptr;
-  w.mbucket = w.K[w.passes & 1];
+  // the per-tree sort ends in buffer 0 and writes the global leaf ids into K[1]
+  w.mbucket = per_tree ? w.K[1] : w.K[w.passes & 1];
   return w;
 }
 
@@ -391,7 +562,26 @@ int fill_u32(hipStream_t st, void* p, int64_t len, uint32_t v) {
   return rfx_launch_status();
 }
 
-// Sort of N slots by leaf and the leaf ranges (everything but members/pe).
+// The radix passes over the prepared keys: `passes` is the device pass count
+// of the per-tree sort (surplus passes return at once) or nullptr.
+int rs_passes(hipStream_t st, const BucketWs& w, const RsTiles& g, const int32_t* passes) {
+  const int64_t hlen = 256 * g.ntiles;
+  for (int pass = 0; pass < w.passes; ++pass) {
+    hipLaunchKernelGGL(rs_hist_kernel, dim3(static_cast<unsigned>(g.ntiles)), dim3(kRsThreads), 0, st, w.K[0], w.K[1],
+                       passes, pass, g, w.hist);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
+                       hlen, w.part, passes, pass);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts, passes, pass);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
+                       hlen, w.part, w.hist, passes, pass);
+    hipLaunchKernelGGL(rs_scatter_kernel, dim3(static_cast<unsigned>(g.ntiles)), dim3(kRsThreads), 0, st, w.K[0],
+                       w.K[1], w.V[0], w.V[1], passes, pass, g, w.hist);
+  }
+  return rfx_launch_status();
+}
+
+// Sort of N slots by leaf and the leaf ranges (everything but members/pe),
+// one segment of all slots; sorted keys end in K[passes & 1] = mbucket.
 int bucket_sort(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_nodes, const BucketWs& w) {
   const uint32_t tn = static_cast<uint32_t>(total_nodes);
   int rc = fill_u32(st, w.bstart, total_nodes, 0u);
@@ -399,22 +589,35 @@ int bucket_sort(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_no
   rc = fill_u32(st, w.bend, total_nodes, 0u);
   if (rc) return rc;
   hipLaunchKernelGGL(rs_prep_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, N, tn, w.K[0], w.V[0]);
-  const int64_t hlen = 256 * w.ntiles;
-  for (int pass = 0; pass < w.passes; ++pass) {
-    const int src = pass & 1, dst = src ^ 1;
-    hipLaunchKernelGGL(rs_hist_kernel, dim3(static_cast<unsigned>(w.ntiles)), dim3(kRsThreads), 0, st, w.K[src], N,
-                       8 * pass, w.ntiles, w.hist);
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
-                       hlen, w.part);
-    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, w.part, w.nparts);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(w.nparts)), dim3(kScanThreads), 0, st, w.hist,
-                       hlen, w.part, w.hist);
-    hipLaunchKernelGGL(rs_scatter_kernel, dim3(static_cast<unsigned>(w.ntiles)), dim3(kRsThreads), 0, st, w.K[src],
-                       w.V[src], N, 8 * pass, w.ntiles, w.hist, w.K[dst], w.V[dst]);
-  }
+  const RsTiles g{N, (N + kRsTile - 1) / kRsTile, (N + kRsTile - 1) / kRsTile};
+  rc = rs_passes(st, w, g, nullptr);
+  if (rc) return rc;
   const int fin = w.passes & 1;
   hipLaunchKernelGGL(rs_bounds_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], N, tn, w.bstart, w.bend,
                      w.nvalid);
+  return rfx_launch_status();
+}
+
+// Per-tree sort of the keys [T][n]: sorted local keys end in K[0], V[0]; the
+// global leaf ids go to mbucket (K[1]).
+int bucket_sort_trees(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
+                      const BucketWs& w) {
+  const int64_t N = n * static_cast<int64_t>(ntree);
+  const uint32_t tn = static_cast<uint32_t>(total_nodes);
+  int rc = fill_u32(st, w.bstart, total_nodes, 0u);
+  if (!rc) rc = fill_u32(st, w.bend, total_nodes, 0u);
+  if (!rc) rc = fill_u32(st, w.info, 4, 0u);
+  if (rc) return rc;
+  hipLaunchKernelGGL(rs_tree_range_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.tmin,
+                     w.info);
+  hipLaunchKernelGGL(rs_tree_passes_kernel, dim3(1), dim3(1), 0, st, w.info);
+  hipLaunchKernelGGL(rs_prep_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, n, N, w.tmin, w.info,
+                     w.K[0], w.K[1], w.V[0], w.V[1]);
+  const RsTiles g{n, rs_tiles_per_tree(n), w.ntiles};
+  rc = rs_passes(st, w, g, reinterpret_cast<const int32_t*>(w.info + 2));
+  if (rc) return rc;
+  hipLaunchKernelGGL(rs_bounds_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[0], N, n, tn, w.tmin,
+                     w.info, w.mbucket, w.bstart, w.bend, w.nvalid);
   return rfx_launch_status();
 }
 
@@ -422,10 +625,10 @@ int bucket_sort(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_no
 int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                  const BucketWs& w) {
   const int64_t N = n * static_cast<int64_t>(ntree);
-  int rc = bucket_sort(st, keys, N, total_nodes, w);
+  if (!w.per_tree) return RFX_ERR_INVALID;
+  int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
-  const int fin = w.passes & 1;
-  hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
+  hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.V[0], N,
                      static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.members16,
                      w.pe);
   return rfx_launch_status();
@@ -436,6 +639,7 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
 // node ids already offset so that all leaves are distinct).
 int bucket_build_segmented(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_nodes,
                            const int64_t* seg_start_dev, const int64_t* seg_n_dev, int nseg, const BucketWs& w) {
+  if (w.per_tree) return RFX_ERR_INVALID;
   int rc = bucket_sort(st, keys, N, total_nodes, w);
   if (rc) return rc;
   const int fin = w.passes & 1;
