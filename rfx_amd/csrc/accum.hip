@@ -12,7 +12,7 @@
 //        matrix is written exactly once, with no atomics at all.  Cost is
 //        T*n^2 compares: VALU-bound, independent of the leaf structure.
 //
-// BUCKET (leaf buckets).  A counting sort groups the (tree, sample) slots by
+// BUCKET (leaf buckets).  A radix sort groups the (tree, sample) slots by
 //        terminal node (histogram, scan, scatter), then every thread walks
 //        the bucket of one slot and adds 1 with an integer atomic for each
 //        partner of larger sample index.  Cost is sum over trees and leaves
@@ -141,11 +141,12 @@ __global__ __launch_bounds__(256) void bucket_pair_kernel(const uint32_t* __rest
 // atomics and no separate zero fill.  Each wavefront takes 64 trees at a
 // time (lane = tree): one load of pe gives the line's position in its leaf
 // and the leaf's end, so the partners of larger index are a contiguous run
-// (members are sorted); binary searches trim the run to the column chunk.
-// The wavefront then walks the union of its 64 runs 64 members per step
-// (owner lane by a binary search over the prefix sum of run lengths), four
-// steps in flight, so a leaf of 1000 members and one of 2 cost the same per
-// member.  PAIR packs ALL (low 16 bits) and OOB (high 16) per LDS word.
+// (members are sorted); when the line is split into column chunks, binary or
+// galloping searches trim the run to the chunk.  The wavefront then visits
+// the union of its 64 runs in aligned 16-byte blocks of the member array
+// (owner lane of a block by a search over the prefix sum of block counts),
+// so a leaf of 1000 members and one of 2 cost the same per member.  PAIR
+// packs ALL (low 16 bits) and OOB (high 16) per LDS word.
 // ---------------------------------------------------------------------------
 // Lines of at most kGatherLineMaxN <= kMembers16MaxN samples read the 16-bit
 // member copy.
@@ -290,107 +291,6 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
                      static_cast<unsigned>(4 * (n + 4)), st, w.pe, w.bstart, w.mbucket, n, ntree, w.members16,
                      accumulate, out, out2, rows_padded);
   return rfx_launch_status();
-}
-
-// ---------------------------------------------------------------------------
-// GATHER, long rows: one workgroup walks a whole output line chunk by chunk
-// (LDS counters of kWalkChunk words).  Each thread owns TPT trees of the line
-// and keeps, per tree, a cursor into the tree's member run with the next
-// member in a register; members are sorted, so for every chunk a thread
-// advances each cursor while the next member's column is below the chunk
-// end.  No search and no flattening: a run is streamed once across all the
-// chunks of the line, and an empty (tree, chunk) costs one register compare.
-// ---------------------------------------------------------------------------
-constexpr int kWalkChunk = 8192;
-#ifndef RFX_GATHER_LONG_WALK
-#define RFX_GATHER_LONG_WALK 0  // 1: the per-thread cursor walk below for long lines
-#endif
-
-template <int MODE, int LAYOUT, int TPT>
-__global__ __launch_bounds__(256) void gather_walk_kernel(const uint2* __restrict__ pe,
-                                                          const uint32_t* __restrict__ bstart,
-                                                          const uint32_t* __restrict__ mbucket, int64_t n,
-                                                          int32_t ntree, const uint32_t* __restrict__ members,
-                                                          int32_t accumulate, int32_t* __restrict__ out,
-                                                          int32_t* __restrict__ out2) {
-  __shared__ uint32_t acc[kWalkChunk];
-  __shared__ uint32_t diag_valid;
-  const int64_t line = xcd_row(blockIdx.x);
-  if (line >= n) return;
-  const int tid = threadIdx.x;
-  uint32_t cur[TPT], end[TPT], nv[TPT];
-  uint32_t bagm = 0, valid = 0;
-#pragma unroll
-  for (int u = 0; u < TPT; ++u) {
-    const int32_t t = tid + u * 256;
-    cur[u] = 0;
-    end[u] = 0;
-    nv[u] = 0xffffffffu;
-    if (t < ntree) {
-      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
-      const uint32_t e = q.y & 0x7fffffffu;
-      if (e > 0) {
-        ++valid;
-        const uint32_t bag = q.y >> 31;
-        bagm |= bag << u;
-        cur[u] = LAYOUT == RFX_LAYOUT_UPPER ? q.x + 1 : bstart[mbucket[q.x]];
-        end[u] = (MODE == kMode_OOB_ASYM && !bag) ? cur[u] : e;
-        if (cur[u] < end[u]) nv[u] = members[cur[u]];
-      }
-    }
-  }
-  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
-    if (tid == 0) diag_valid = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
-    if ((tid & 63) == 0 && valid) atomicAdd(&diag_valid, valid);
-  }
-  const int64_t first = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
-  for (int64_t lo = first; lo < n; lo += kWalkChunk) {
-    const int64_t hi = min(n, lo + kWalkChunk);
-    const int len_chunk = static_cast<int>(hi - lo);
-    __syncthreads();
-    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
-    __syncthreads();
-    // the thread's TPT runs advance one member per step, interleaved, so
-    // TPT independent loads are in flight
-    bool more = true;
-    while (more) {
-      more = false;
-#pragma unroll
-      for (int u = 0; u < TPT; ++u) {
-        if ((nv[u] >> 1) >= static_cast<uint32_t>(hi)) continue;
-        more = true;
-        const uint32_t m = nv[u];
-        const int64_t j = m >> 1;
-        const uint32_t bq = m & 1u, bu = (bagm >> u) & 1u;
-        ++cur[u];
-        nv[u] = cur[u] < end[u] ? members[cur[u]] : 0xffffffffu;
-        if (LAYOUT == RFX_LAYOUT_FULL && MODE != kMode_OOB_ROW && j == line) continue;
-        uint32_t inc = 0;
-        if (MODE == kMode_ALL) inc = 1u;
-        if (MODE == kMode_OOB_UPPER) inc = bq != bu ? 1u : 0u;
-        if (MODE == kMode_OOB_ASYM || MODE == kMode_OOB_ROW) inc = bq == 0u ? 1u : 0u;
-        if (MODE == kMode_PAIR) inc = 1u | (bq != bu ? 0x10000u : 0u);
-        if (inc) atomicAdd(&acc[j - lo], inc);
-      }
-    }
-    __syncthreads();
-    if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
-    __syncthreads();
-    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
-    for (int q = tid; q < len_chunk; q += 256) {
-      const uint32_t v = acc[q];
-      if (MODE == kMode_PAIR) {
-        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-        out[base + q] = accumulate ? out[base + q] + va : va;
-        out2[base + q] = accumulate ? out2[base + q] + vb : vb;
-      } else {
-        out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +447,6 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
     return rfx_launch_status();
   }
   const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n))), blk(256);
-#if !RFX_GATHER_LONG_WALK
   if (ntree <= 256)
     hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 1>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
                        ntree, w.members, accumulate, out, out2);
@@ -563,25 +462,6 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
   else if (ntree <= 4096)
     hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 16>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
                        ntree, w.members, accumulate, out, out2);
-  else
-    return RFX_ERR_UNSUPPORTED;
-  return rfx_launch_status();
-#endif
-  if (ntree <= 256)
-    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 1>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                       w.members, accumulate, out, out2);
-  else if (ntree <= 512)
-    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 2>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                       w.members, accumulate, out, out2);
-  else if (ntree <= 1024)
-    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 4>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                       w.members, accumulate, out, out2);
-  else if (ntree <= 2048)
-    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 8>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                       w.members, accumulate, out, out2);
-  else if (ntree <= 4096)
-    hipLaunchKernelGGL((gather_walk_kernel<MODE, LAYOUT, 16>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n, ntree,
-                       w.members, accumulate, out, out2);
   else
     return RFX_ERR_UNSUPPORTED;
   return rfx_launch_status();
@@ -609,7 +489,7 @@ int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int6
 // AUTO: compare the compare-kernel cost T*n^2/2 (one tile compare per
 // tree-pair) with the leaf-bucket cost: sum over trees and leaves of |leaf|^2
 // partner visits plus one pass over the output.  GATHER (LDS row segments)
-// serves T <= 4096 (the walk kernel keeps per-tree cursors in registers);
+// serves T <= 4096 (the long-line kernel keeps per-tree cursors in registers);
 // beyond that the global-atomic BUCKET form is used.  The per-item weights
 // are MI355X
 // measurements on the covtype-like workload (n = 20,000, T = 500).

@@ -5,10 +5,11 @@
 // [lo, hi) and counts it in LDS: every tree's member run of the line's leaf
 // (the samples that follow i in the leaf, or the whole leaf for FULL),
 // restricted to [lo, hi) by two binary searches, is flattened across the 64
-// lanes of a wavefront (prefix sum of run lengths, each lane finds its run by
-// a 6-step search over the prefix) so that every lane issues independent
-// member loads regardless of how ragged the runs are; matches are added with
-// LDS atomics and the chunk is written (or added) once.
+// lanes of a wavefront in aligned 16-byte blocks of the member array (prefix
+// sum of block counts, each lane finds the run of its block by a 6-step
+// search over the prefix) so that every lane issues independent loads
+// regardless of how ragged the runs are; matches are added with LDS atomics
+// and the chunk is written (or added) once.
 #pragma once
 
 #include "rfx_common.hpp"
@@ -24,15 +25,11 @@ constexpr int kMode_OOB_ROW = RFX_PROX_OOB_ROW;
 #ifndef RFX_GATHER_CHUNK
 #define RFX_GATHER_CHUNK 8192
 #endif
-#ifndef RFX_GATHER_UNROLL
-#define RFX_GATHER_UNROLL 8
-#endif
 #ifndef RFX_GATHER_LINE_MAX_N
 #define RFX_GATHER_LINE_MAX_N 32768
 #endif
 constexpr int kGatherChunk = RFX_GATHER_CHUNK;  // LDS words per workgroup
-constexpr int kGatherUnroll = RFX_GATHER_UNROLL;
-// lines up to this length use the flattened line kernel, longer ones the walk
+// lines up to this length use the line kernels, longer ones the gallop kernel
 constexpr int64_t kGatherLineMaxN = RFX_GATHER_LINE_MAX_N;
 
 // Rows are spread over the 8 XCDs (each with its own L2) in groups of
@@ -65,58 +62,10 @@ __device__ __forceinline__ uint32_t member_lb(const M* __restrict__ members, uin
 }
 
 // The wavefront's 64 member runs (lane l: run [s, s + len), its sample's
-// in-bag bit) flattened across the lanes: each step every lane takes one
-// member of the union (owner lane by a 6-step search over the prefix sums of
-// the run lengths), kGatherUnroll members in flight, and adds its count to
-// the LDS counters acc[column - lo].  All lanes of the wavefront must call.
-template <int MODE, int LAYOUT, class M>
-__device__ __forceinline__ void flat_visits(uint32_t s, uint32_t len, uint32_t bag, int lane,
-                                            const M* __restrict__ members, uint32_t* acc, int64_t lo, int64_t line) {
-  uint32_t pre = len;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(pre, d, 64);
-    if (lane >= d) pre += y;
-  }
-  const uint32_t excl = pre - len;
-  const uint32_t total = __shfl(pre, 63, 64);
-  for (uint32_t g0 = 0; g0 < total; g0 += 64 * kGatherUnroll) {
-    uint32_t mv[kGatherUnroll], bo[kGatherUnroll];
-    bool ok[kGatherUnroll];
-#pragma unroll
-    for (int u = 0; u < kGatherUnroll; ++u) {
-      const uint32_t g = g0 + u * 64 + lane;
-      int o = 0;
-#pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        const uint32_t v = __shfl(pre, o + d - 1, 64);
-        if (v <= g) o += d;
-      }
-      o = min(o, 63);
-      const uint32_t so = __shfl(s, o, 64);
-      const uint32_t eo = __shfl(excl, o, 64);
-      bo[u] = __shfl(bag, o, 64);
-      ok[u] = g < total;
-      mv[u] = ok[u] ? static_cast<uint32_t>(members[so + (g - eo)]) : 0u;
-    }
-#pragma unroll
-    for (int u = 0; u < kGatherUnroll; ++u) {
-      if (!ok[u]) continue;
-      const int64_t q = mv[u] >> 1;
-      const uint32_t bq = mv[u] & 1u;
-      if (LAYOUT == RFX_LAYOUT_FULL && MODE != kMode_OOB_ROW && q == line) continue;
-      uint32_t inc = 0;
-      if (MODE == kMode_ALL) inc = 1u;
-      if (MODE == kMode_OOB_UPPER) inc = bq != bo[u] ? 1u : 0u;
-      if (MODE == kMode_OOB_ASYM || MODE == kMode_OOB_ROW) inc = bq == 0u ? 1u : 0u;
-      if (MODE == kMode_PAIR) inc = 1u | (bq != bo[u] ? 0x10000u : 0u);
-      if (inc) atomicAdd(&acc[q - lo], inc);
-    }
-  }
-}
-
-// The same visits, flattened over aligned 16-byte blocks of the member
-// array instead of single members: a lane's run [s, s + len) touches the
+// in-bag bit) flattened across the lanes in aligned 16-byte blocks of the
+// member array, and each member's count added to the LDS counters
+// acc[column - lo].  All lanes of the wavefront must call.  A lane's run
+// [s, s + len) touches the
 // blocks floor(s / P) .. floor((s + len - 1) / P) (P = 16 / sizeof(M)
 // members per block); the union of all lanes' blocks is spread over the
 // lanes by the same prefix-sum owner search, so the search and its
@@ -187,17 +136,11 @@ __device__ __forceinline__ void flat_visit_blocks(uint32_t s, uint32_t len, uint
   }
 }
 
-#ifndef RFX_GATHER_BLOCKS
-#define RFX_GATHER_BLOCKS 1
-#endif
 // The visit form the kernels use.
 template <int MODE, int LAYOUT, class M>
 __device__ __forceinline__ void line_visits(uint32_t s, uint32_t len, uint32_t bag, int lane,
                                             const M* __restrict__ members, uint32_t* acc, int64_t lo, int64_t line) {
-  if (RFX_GATHER_BLOCKS)
-    flat_visit_blocks<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
-  else
-    flat_visits<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
+  flat_visit_blocks<MODE, LAYOUT>(s, len, bag, lane, members, acc, lo, line);
 }
 
 // First position p in [cur, end) whose member column is >= key: a galloping
