@@ -322,8 +322,9 @@ typedef struct rfx_batch rfx_batch;
 
 /* Plan a batch: leaf assignment and co-occurrence counts (variant ALL,
  * OOB_UPPER or PAIR; UPPER layout) of `count` forests in one leaf launch,
- * one leaf-bucket sort over all of them and one persistent gather launch
- * fed by a device work queue (largest problems first).  The reference has
+ * one leaf-bucket sort over all of them, the large problems' own count
+ * kernels (after rfx_batch_autotune) and one persistent gather launch fed
+ * by a device work queue (largest problems first) for the rest.  The reference has
  * no multi-forest batching; it runs each forest tree by tree from the host
  * (fit_batch_gpu, src/rf_random_forest.cpp:1364-1570; batch sizing
  * get_recommended_batch_size, src/rf_cuda_config.cpp:225-295).  The plan
@@ -337,11 +338,15 @@ size_t rfx_batch_workspace_bytes(const rfx_batch* b);
  * the descriptors and synchronises; later calls do not.  Two runs of one
  * plan must not overlap (they share the workspace). */
 int rfx_batch_run(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t use_graph);
-/* Measure every problem's leaf statistics on the batch's own data, choose
- * GATHER or DENSE per problem (DENSE: T n^2 / 2 tile compares; GATHER:
- * sum |leaf|^2 / 2 member visits at 8 compares each plus one write per
- * output entry at 4), then time every (rows per work item, workgroups per
- * CU) candidate (reps eager runs each) and keep the fastest; synchronises. */
+/* Measure every problem on the batch's own data: its sum of |leaf|^2 and
+ * the time of its one-problem DENSE and GATHER kernels on its slice of the
+ * batched leaf buckets.  The faster algorithm is kept; a problem whose
+ * kernel takes 0.05 ms or more is from then on launched on its own (inside
+ * the same launch sequence or graph), the others stay in the persistent
+ * queue.  Then every (rows per work item, workgroups per CU) candidate of
+ * the queue is timed (reps eager runs each) and the fastest kept;
+ * synchronises.  Before autotune every problem goes to the queue, GATHER
+ * unless rfx_batch_set_algos says otherwise. */
 int rfx_batch_autotune(hipStream_t stream, rfx_batch* b, void* workspace, size_t workspace_bytes, int32_t reps,
                        double* best_ms);
 int rfx_batch_set_tuning(rfx_batch* b, int32_t lines_per_item, int32_t blocks_per_cu);

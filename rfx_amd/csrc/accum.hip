@@ -294,14 +294,8 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 }
 
 // ---------------------------------------------------------------------------
-// GATHER, long rows, balanced: one workgroup per output line, column chunks
-// of kGatherChunk LDS counters in order.  Lane l of wave w holds the cursors
-// of trees g*256 + w*64 + l (g < G) into their member runs.  For every chunk
-// a lane finds where each of its runs crosses the chunk end by a galloping
-// search from its cursor, and the wavefront then visits the 64 run pieces
-// flattened across its lanes -- a leaf of 10,000 members costs its wavefront
-// no more per member than a leaf of 2.  No binary search from the run start
-// is ever repeated: cursors only move forward.
+// GATHER, long rows (n > 32768): one workgroup per output line, its column
+// chunks walked with per-tree cursors (gather_gallop_body in gather.hpp).
 // ---------------------------------------------------------------------------
 template <int MODE, int LAYOUT, int G>
 __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restrict__ pe,
@@ -314,62 +308,8 @@ __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restr
   __shared__ uint32_t diag_valid;
   const int64_t line = xcd_row(blockIdx.x);
   if (line >= n) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t cur[G], end[G];
-  uint32_t bagm = 0, valid = 0;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const int32_t t = g * 256 + wave * 64 + lane;
-    cur[g] = 0;
-    end[g] = 0;
-    if (t < ntree) {
-      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
-      const uint32_t e = q.y & 0x7fffffffu;
-      if (e > 0) {
-        ++valid;
-        const uint32_t bag = q.y >> 31;
-        bagm |= bag << g;
-        cur[g] = LAYOUT == RFX_LAYOUT_UPPER ? q.x + 1 : bstart[mbucket[q.x]];
-        end[g] = (MODE == kMode_OOB_ASYM && !bag) ? cur[g] : e;
-      }
-    }
-  }
-  if (tid == 0) diag_valid = 0u;
-  __syncthreads();
-  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
-    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
-  }
-  const int64_t first = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
-  for (int64_t lo = first; lo < n; lo += kGatherChunk) {
-    const int64_t hi = min(n, lo + kGatherChunk);
-    const int len_chunk = static_cast<int>(hi - lo);
-    __syncthreads();
-    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const uint32_t e2 = hi < n ? gallop_lb(members, cur[g], end[g], static_cast<uint32_t>(hi)) : end[g];
-      const uint32_t s0 = cur[g];
-      cur[g] = e2;
-      line_visits<MODE, LAYOUT>(s0, e2 - s0, (bagm >> g) & 1u, lane, members, acc, lo, line);
-    }
-    __syncthreads();
-    if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
-    __syncthreads();
-    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
-    for (int q = tid; q < len_chunk; q += 256) {
-      const uint32_t v = acc[q];
-      if (MODE == kMode_PAIR) {
-        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-        out[base + q] = accumulate ? out[base + q] + va : va;
-        out2[base + q] = accumulate ? out2[base + q] + vb : vb;
-      } else {
-        out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
-      }
-    }
-  }
+  gather_gallop_body<MODE, LAYOUT, G>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line, acc,
+                                      diag_valid);
 }
 
 // Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
@@ -427,11 +367,23 @@ int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
 }
 
 template <int MODE, int LAYOUT>
+int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate,
+                          int32_t* out, int32_t* out2);
+
+template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
   int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
+  return launch_gather_kernels<MODE, LAYOUT>(st, w, n, ntree, accumulate, out, out2);
+}
+
+// The GATHER kernel for the shape: whole lines, line chunks, or the gallop
+// walk (reads w.pe, w.bstart, w.mbucket, w.members, w.members16 only).
+template <int MODE, int LAYOUT>
+int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate,
+                          int32_t* out, int32_t* out2) {
   if (RFX_GATHER_ROWS && 4 * (n + 4) <= kLineMaxLds) {
     if (ntree <= kLineThreads) return launch_rows<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
     if (ntree <= 2 * kLineThreads) return launch_rows<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
@@ -530,6 +482,43 @@ int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_no
 }
 
 }  // namespace
+
+// One problem's UPPER counts (not accumulated) on bucket structures that may
+// be the problem's slice of a batched build: pe of the problem's slots,
+// positions global to members / members16, keys of the problem.  Used by the
+// batch (batch.hip) for the forests it launches on their own.
+int count_problem_slice(hipStream_t st, int32_t variant, int32_t algo, const int32_t* keys, int64_t n,
+                        int32_t ntree, const uint2* pe, const uint32_t* bstart, const uint32_t* mbucket,
+                        const uint32_t* members, const uint16_t* members16, int32_t* out, int32_t* out2) {
+  if (algo == RFX_ALGO_DENSE) {
+    switch (variant) {
+      case RFX_PROX_ALL:
+        return run_dense<kMode_ALL, RFX_LAYOUT_UPPER>(st, keys, n, ntree, 0, out, out2);
+      case RFX_PROX_OOB_UPPER:
+        return run_dense<kMode_OOB_UPPER, RFX_LAYOUT_UPPER>(st, keys, n, ntree, 0, out, out2);
+      case RFX_PROX_PAIR:
+        return run_dense<kMode_PAIR, RFX_LAYOUT_UPPER>(st, keys, n, ntree, 0, out, out2);
+      default:
+        return RFX_ERR_INVALID;
+    }
+  }
+  BucketWs w{};
+  w.pe = const_cast<uint2*>(pe);
+  w.bstart = const_cast<uint32_t*>(bstart);
+  w.mbucket = const_cast<uint32_t*>(mbucket);
+  w.members = const_cast<uint32_t*>(members);
+  w.members16 = const_cast<uint16_t*>(members16);
+  switch (variant) {
+    case RFX_PROX_ALL:
+      return launch_gather_kernels<kMode_ALL, RFX_LAYOUT_UPPER>(st, w, n, ntree, 0, out, out2);
+    case RFX_PROX_OOB_UPPER:
+      return launch_gather_kernels<kMode_OOB_UPPER, RFX_LAYOUT_UPPER>(st, w, n, ntree, 0, out, out2);
+    case RFX_PROX_PAIR:
+      return launch_gather_kernels<kMode_PAIR, RFX_LAYOUT_UPPER>(st, w, n, ntree, 0, out, out2);
+    default:
+      return RFX_ERR_INVALID;
+  }
+}
 
 extern "C" size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo) {
   if (algo == RFX_ALGO_DENSE) return 0;

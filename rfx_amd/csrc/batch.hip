@@ -36,6 +36,11 @@
 #include "leaf.hpp"
 #include "rfx_common.hpp"
 
+// One problem's UPPER counts on its slice of a batched bucket build (accum.hip).
+int count_problem_slice(hipStream_t st, int32_t variant, int32_t algo, const int32_t* keys, int64_t n,
+                        int32_t ntree, const uint2* pe, const uint32_t* bstart, const uint32_t* mbucket,
+                        const uint32_t* members, const uint16_t* members16, int32_t* out, int32_t* out2);
+
 namespace {
 
 struct LeafDesc {
@@ -84,7 +89,10 @@ __global__ void batch_reset_kernel(uint32_t* counter) { *counter = 0u; }
 // of a GATHER problem or upper tiles [first, first + count) of a DENSE
 // problem.  Every workgroup pops items until the queue is empty; the queue is
 // finite, so every wave exits.
-template <int MODE>
+// G > 0: GATHER lines walk their column chunks with per-tree cursors
+// (gather_gallop_body, T <= 256 G); G = 0: chunk by chunk with binary
+// searches (gather_line_body, any T).
+template <int MODE, int G>
 __global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __restrict__ desc,
                                                            const uint4* __restrict__ items, uint32_t nitems,
                                                            uint32_t* __restrict__ counter,
@@ -108,11 +116,17 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const GatherDesc* __r
     if (w.w == kItemGather) {
       for (uint32_t l = 0; l < w.z; ++l) {
         const int64_t line = static_cast<int64_t>(w.y) + l;
-        const int64_t nch = (d.n - line + kGatherChunk - 1) / kGatherChunk;
-        for (int64_t c = 0; c < nch; ++c) {
-          gather_line_body<MODE, RFX_LAYOUT_UPPER>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out, d.out2,
-                                                   line, c, smem, diag_valid);
+        if constexpr (G > 0) {
+          gather_gallop_body<MODE, RFX_LAYOUT_UPPER, G>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out,
+                                                        d.out2, line, smem, diag_valid);
           __syncthreads();
+        } else {
+          const int64_t nch = (d.n - line + kGatherChunk - 1) / kGatherChunk;
+          for (int64_t c = 0; c < nch; ++c) {
+            gather_line_body<MODE, RFX_LAYOUT_UPPER>(d.pe, bstart, mbucket, d.n, d.ntree, members, 0, d.out, d.out2,
+                                                     line, c, smem, diag_valid);
+            __syncthreads();
+          }
         }
       }
     } else {
@@ -147,20 +161,26 @@ struct rfx_batch {
   std::vector<rfx_batch_problem> prob;
   int32_t variant = RFX_PROX_OOB_UPPER;
   int64_t slots = 0, total_nodes = 0;
+  int32_t max_ntree = 0;  // largest T: cursor groups of the GATHER lines
   std::vector<int64_t> seg_start, seg_n, node_base, block_start;
   std::vector<uint32_t> order;  // problems by decreasing work
   std::vector<int32_t> algo;    // RFX_ALGO_GATHER or RFX_ALGO_DENSE per problem
   std::vector<double> sumsq;    // measured sum of |leaf|^2 per problem (0 until measured)
+  std::vector<uint8_t> direct;  // 1: counted by its own one-problem kernel launch, not the queue
   int32_t lines_per_item = 4;
   int32_t blocks_per_cu = 0;  // 0: as many as fit
   // device layout inside the workspace
   size_t off_leaf = 0, off_gather = 0, off_block = 0, off_segs = 0, off_segn = 0, off_items = 0, off_counter = 0,
-         off_keys = 0, off_bucket = 0, total_bytes = 0;
+         off_keys = 0, off_bucket = 0, off_m16 = 0, total_bytes = 0;
   uint32_t nitems = 0;
   std::vector<uint4> items;
   void* uploaded_ws = nullptr;
   int32_t uploaded_lpi = -1;
   uint64_t uploaded_algo_sig = 0;
+  // second stream of the launch sequence (own-launch DENSE problems, which
+  // need only the leaf keys, overlap the bucket build and the GATHER work)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // graph
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -174,6 +194,8 @@ namespace {
 
 // Relative costs of the one-problem AUTO rule (one tile compare = 1).
 constexpr double kBatchVisit = 8.0, kBatchWrite = 4.0;
+// a problem whose one-problem kernel takes at least this is launched on its own
+constexpr float kDirectMinMs = 0.05f;
 
 int64_t dense_tiles(int64_t n) {
   const int64_t nb = (n + kTile - 1) / kTile;
@@ -196,7 +218,8 @@ void order_problems(rfx_batch* b) {
 
 uint64_t algo_signature(const rfx_batch* b) {
   uint64_t h = 1469598103934665603ull;
-  for (int32_t a : b->algo) h = (h ^ static_cast<uint64_t>(a + 1)) * 1099511628211ull;
+  for (size_t i = 0; i < b->algo.size(); ++i)
+    h = (h ^ static_cast<uint64_t>(b->algo[i] + 1 + 16 * b->direct[i])) * 1099511628211ull;
   return h;
 }
 
@@ -205,6 +228,7 @@ void build_items(rfx_batch* b) {
   const uint32_t L = static_cast<uint32_t>(std::max(1, b->lines_per_item));
   for (uint32_t pi : b->order) {
     const int64_t n = b->prob[pi].n;
+    if (b->direct[pi]) continue;
     if (b->algo[pi] == RFX_ALGO_DENSE) {
       const int64_t tiles = dense_tiles(n);
       for (int64_t t = 0; t < tiles; ++t) b->items.push_back(make_uint4(pi, static_cast<uint32_t>(t), 1u, kItemDense));
@@ -237,6 +261,7 @@ size_t layout_bytes(rfx_batch* b) {
   b->off_counter = take(256);
   b->off_keys = take(4 * static_cast<size_t>(b->slots));
   b->off_bucket = take(bucket_ws_bytes(b->slots, 1, b->total_nodes));
+  b->off_m16 = take(2 * static_cast<size_t>(b->slots));
   return o;
 }
 
@@ -267,29 +292,94 @@ int upload(hipStream_t st, rfx_batch* b, char* ws) {
   return RFX_OK;
 }
 
-template <int MODE>
-int launch_gather(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, unsigned grid) {
-  hipLaunchKernelGGL(batch_gather_kernel<MODE>, dim3(grid), dim3(256), 0, st,
+template <int MODE, int G>
+int launch_gather_g(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, unsigned grid) {
+  hipLaunchKernelGGL((batch_gather_kernel<MODE, G>), dim3(grid), dim3(256), 0, st,
                      reinterpret_cast<const GatherDesc*>(ws + b->off_gather),
                      reinterpret_cast<const uint4*>(ws + b->off_items), b->nitems,
                      reinterpret_cast<uint32_t*>(ws + b->off_counter), w.bstart, w.mbucket, w.members);
   return rfx_launch_status();
 }
 
-int max_blocks_per_cu(int32_t variant) {
-  int nb = 0;
-  hipError_t e;
-  if (variant == RFX_PROX_ALL)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_ALL>, 256, 0);
-  else if (variant == RFX_PROX_OOB_UPPER)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_OOB_UPPER>, 256, 0);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<kMode_PAIR>, 256, 0);
-  if (e != hipSuccess || nb < 1) nb = 1;
-  return nb;
+// cursor groups of the GATHER lines: T <= 256 G, or 0 (chunked form) past 4096
+int gallop_groups(int32_t max_ntree) {
+  for (int g = 1; g <= 16; g *= 2)
+    if (max_ntree <= 256 * g) return g;
+  return 0;
 }
 
-// The launch sequence (eager, or recorded into a graph by the caller).
+template <int MODE>
+int launch_gather(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, unsigned grid) {
+  switch (gallop_groups(b->max_ntree)) {
+    case 1:
+      return launch_gather_g<MODE, 1>(st, b, ws, w, grid);
+    case 2:
+      return launch_gather_g<MODE, 2>(st, b, ws, w, grid);
+    case 4:
+      return launch_gather_g<MODE, 4>(st, b, ws, w, grid);
+    case 8:
+      return launch_gather_g<MODE, 8>(st, b, ws, w, grid);
+    case 16:
+      return launch_gather_g<MODE, 16>(st, b, ws, w, grid);
+    default:
+      return launch_gather_g<MODE, 0>(st, b, ws, w, grid);
+  }
+}
+
+template <int MODE>
+int occupancy(int g) {
+  int nb = 0;
+  hipError_t e;
+  switch (g) {
+    case 1:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 1>, 256, 0);
+      break;
+    case 2:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 2>, 256, 0);
+      break;
+    case 4:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 4>, 256, 0);
+      break;
+    case 8:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 8>, 256, 0);
+      break;
+    case 16:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 16>, 256, 0);
+      break;
+    default:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_gather_kernel<MODE, 0>, 256, 0);
+  }
+  return e == hipSuccess ? nb : 0;
+}
+
+int max_blocks_per_cu(int32_t variant, int32_t max_ntree) {
+  const int g = gallop_groups(max_ntree);
+  int nb = variant == RFX_PROX_ALL ? occupancy<kMode_ALL>(g)
+           : variant == RFX_PROX_OOB_UPPER ? occupancy<kMode_OOB_UPPER>(g)
+                                           : occupancy<kMode_PAIR>(g);
+  return nb < 1 ? 1 : nb;
+}
+
+// Problem i counted by its own one-problem kernel (algo DENSE or GATHER) on
+// its slice of the batched buckets.
+int launch_direct(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, size_t i, int32_t algo) {
+  const auto& p = b->prob[i];
+  const int32_t* keys = reinterpret_cast<const int32_t*>(ws + b->off_keys) + b->seg_start[i];
+  return count_problem_slice(st, b->variant, algo, keys, p.n, p.ntree, w.pe + b->seg_start[i], w.bstart, w.mbucket,
+                             w.members, reinterpret_cast<const uint16_t*>(ws + b->off_m16), p.out, p.out2);
+}
+
+// join: `st` waits for the side stream's work
+int join_side(hipStream_t st, rfx_batch* b, bool side_used) {
+  if (!side_used) return RFX_OK;
+  RFX_HIP_TRY(hipEventRecord(b->ev_join, b->side));
+  RFX_HIP_TRY(hipStreamWaitEvent(st, b->ev_join, 0));
+  return RFX_OK;
+}
+
+// The launch sequence (eager, or recorded into a graph by the caller): the
+// leaf keys of all problems, one segmented bucket build, the problems
+// counted on their own, then the persistent queue for the rest.
 int enqueue(hipStream_t st, rfx_batch* b, char* ws) {
   const int B = static_cast<int>(b->prob.size());
   const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes, false);
@@ -299,10 +389,29 @@ int enqueue(hipStream_t st, rfx_batch* b, char* ws) {
                      reinterpret_cast<const int64_t*>(ws + b->off_block), B);
   int rc = rfx_launch_status();
   if (rc) return rc;
+  // fork: the own-launch DENSE problems on the side stream, right after the leaves
+  bool side_used = false;
+  for (uint32_t pi : b->order) {
+    if (!b->direct[pi] || b->algo[pi] != RFX_ALGO_DENSE) continue;
+    if (!side_used) {
+      RFX_HIP_TRY(hipEventRecord(b->ev_fork, st));
+      RFX_HIP_TRY(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+      side_used = true;
+    }
+    rc = launch_direct(b->side, b, ws, w, pi, RFX_ALGO_DENSE);
+    if (rc) return rc;
+  }
   rc = bucket_build_segmented(st, reinterpret_cast<const int32_t*>(ws + b->off_keys), b->slots, b->total_nodes,
                               reinterpret_cast<const int64_t*>(ws + b->off_segs),
-                              reinterpret_cast<const int64_t*>(ws + b->off_segn), B, w);
+                              reinterpret_cast<const int64_t*>(ws + b->off_segn), B, w,
+                              reinterpret_cast<uint16_t*>(ws + b->off_m16));
   if (rc) return rc;
+  for (uint32_t pi : b->order)
+    if (b->direct[pi] && b->algo[pi] == RFX_ALGO_GATHER) {
+      rc = launch_direct(st, b, ws, w, pi, RFX_ALGO_GATHER);
+      if (rc) return rc;
+    }
+  if (b->nitems == 0) return join_side(st, b, side_used);
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, st, reinterpret_cast<uint32_t*>(ws + b->off_counter));
   rc = rfx_launch_status();
   if (rc) return rc;
@@ -311,12 +420,16 @@ int enqueue(hipStream_t st, rfx_batch* b, char* ws) {
       1, std::min<int64_t>(static_cast<int64_t>(b->cu_count) * bpc, static_cast<int64_t>(b->nitems))));
   switch (b->variant) {
     case RFX_PROX_ALL:
-      return launch_gather<kMode_ALL>(st, b, ws, w, grid);
+      rc = launch_gather<kMode_ALL>(st, b, ws, w, grid);
+      break;
     case RFX_PROX_OOB_UPPER:
-      return launch_gather<kMode_OOB_UPPER>(st, b, ws, w, grid);
+      rc = launch_gather<kMode_OOB_UPPER>(st, b, ws, w, grid);
+      break;
     default:
-      return launch_gather<kMode_PAIR>(st, b, ws, w, grid);
+      rc = launch_gather<kMode_PAIR>(st, b, ws, w, grid);
   }
+  if (rc) return rc;
+  return join_side(st, b, side_used);
 }
 
 void drop_graph(rfx_batch* b) {
@@ -353,6 +466,7 @@ extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count
     b->seg_n[i] = p.n;
     b->node_base[i] = b->total_nodes;
     b->total_nodes += p.total_nodes;
+    b->max_ntree = std::max(b->max_ntree, p.ntree);
     b->block_start[i + 1] = b->block_start[i] + rfx_ceil_div(p.n, kBlock) * rfx_ceil_div(p.ntree, kFastTrees);
   }
   b->slots = b->seg_start[count];
@@ -362,6 +476,7 @@ extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count
   }
   b->algo.assign(count, RFX_ALGO_GATHER);
   b->sumsq.assign(count, 0.0);
+  b->direct.assign(count, 0);
   order_problems(b);
   b->total_bytes = layout_bytes(b);
   int dev = 0;
@@ -370,7 +485,13 @@ extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count
     delete b;
     return RFX_ERR_HIP;
   }
-  b->max_bpc = max_blocks_per_cu(variant);
+  b->max_bpc = max_blocks_per_cu(variant, b->max_ntree);
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+    rfx_batch_free(b);
+    return RFX_ERR_HIP;
+  }
   *out = b;
   return RFX_OK;
 }
@@ -430,9 +551,8 @@ extern "C" int rfx_batch_run(hipStream_t st, rfx_batch* b, void* workspace, size
 }
 
 // Leaf statistics of every problem from the buckets of the last run (the
-// workspace holds them after any run), then GATHER or DENSE per problem by
-// the one-problem cost model: DENSE costs T n^2 / 2 tile compares, GATHER
-// sum|L|^2 / 2 member visits plus one write per output entry.
+// workspace holds them after any run), then GATHER or DENSE per problem and
+// queue or own launch by timing (below).
 int measure_and_choose(hipStream_t st, rfx_batch* b, char* ws) {
   const BucketWs w = bucket_ws_carve(ws + b->off_bucket, b->slots, 1, b->total_nodes, false);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(w.stat);
@@ -451,12 +571,33 @@ int measure_and_choose(hipStream_t st, rfx_batch* b, char* ws) {
     RFX_HIP_TRY(hipStreamSynchronize(st));
     sq[i] = static_cast<double>(h);
   }
-  for (size_t i = 0; i < B; ++i) {
-    const double n = static_cast<double>(b->prob[i].n), T = b->prob[i].ntree;
-    b->sumsq[i] = sq[i];
-    const double dense = 0.5 * T * n * n, gather = 0.5 * sq[i] * kBatchVisit + 0.5 * n * n * kBatchWrite;
-    b->algo[i] = dense < gather ? RFX_ALGO_DENSE : RFX_ALGO_GATHER;
+  for (size_t i = 0; i < B; ++i) b->sumsq[i] = sq[i];
+  // Each problem's one-problem DENSE and GATHER kernels timed on its slice of
+  // the buckets: the faster decides the algorithm; problems that take at
+  // least kDirectMinMs are launched on their own, the rest go to the queue.
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return RFX_ERR_HIP;
   }
+  int rc = RFX_OK;
+  for (size_t i = 0; i < B && !rc; ++i) {
+    float t[2] = {0.f, 0.f};
+    const int32_t algos[2] = {RFX_ALGO_DENSE, RFX_ALGO_GATHER};
+    for (int a = 0; a < 2 && !rc; ++a) {
+      rc = launch_direct(st, b, ws, w, i, algos[a]);  // warm
+      if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = RFX_ERR_HIP;
+      if (!rc) rc = launch_direct(st, b, ws, w, i, algos[a]);
+      if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&t[a], e0, e1) != hipSuccess))
+        rc = RFX_ERR_HIP;
+    }
+    b->algo[i] = t[0] <= t[1] ? RFX_ALGO_DENSE : RFX_ALGO_GATHER;
+    b->direct[i] = std::min(t[0], t[1]) >= kDirectMinMs ? 1 : 0;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
   order_problems(b);
   return RFX_OK;
 }
@@ -538,5 +679,8 @@ extern "C" void rfx_batch_free(rfx_batch* b) {
   if (!b) return;
   drop_graph(b);
   if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+  if (b->side) hipStreamDestroy(b->side);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_join) hipEventDestroy(b->ev_join);
   delete b;
 }

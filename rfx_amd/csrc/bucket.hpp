@@ -429,7 +429,8 @@ __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t
 __global__ void rs_members_seg_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N,
                                       const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_n,
                                       int nseg, uint32_t total_nodes, const uint32_t* __restrict__ bend,
-                                      uint32_t* __restrict__ members, uint2* __restrict__ pe) {
+                                      uint32_t* __restrict__ members, uint16_t* __restrict__ members16,
+                                      uint2* __restrict__ pe) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t v = V[p];
@@ -444,6 +445,7 @@ __global__ void rs_members_seg_kernel(const uint32_t* __restrict__ K, const uint
   }
   const uint32_t i = static_cast<uint32_t>((static_cast<int64_t>(s) - seg_start[lo]) % seg_n[lo]);
   members[p] = (i << 1) | bag;
+  if (members16) members16[p] = static_cast<uint16_t>((i << 1) | bag);  // meaningful where seg_n <= 32768
   const uint32_t k = K[p];
   pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
 }
@@ -638,13 +640,15 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
 // (problem b: slots [seg_start[b], seg_start[b+1]), keys [T_b][seg_n[b]],
 // node ids already offset so that all leaves are distinct).
 int bucket_build_segmented(hipStream_t st, const int32_t* keys, int64_t N, int64_t total_nodes,
-                           const int64_t* seg_start_dev, const int64_t* seg_n_dev, int nseg, const BucketWs& w) {
+                           const int64_t* seg_start_dev, const int64_t* seg_n_dev, int nseg, const BucketWs& w,
+                           uint16_t* members16 = nullptr) {
   if (w.per_tree) return RFX_ERR_INVALID;
   int rc = bucket_sort(st, keys, N, total_nodes, w);
   if (rc) return rc;
   const int fin = w.passes & 1;
   hipLaunchKernelGGL(rs_members_seg_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[fin], w.V[fin], N,
-                     seg_start_dev, seg_n_dev, nseg, static_cast<uint32_t>(total_nodes), w.bend, w.members, w.pe);
+                     seg_start_dev, seg_n_dev, nseg, static_cast<uint32_t>(total_nodes), w.bend, w.members, members16,
+                     w.pe);
   return rfx_launch_status();
 }
 

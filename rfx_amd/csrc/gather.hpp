@@ -234,4 +234,78 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
   }
 }
 
+
+// Long line in chunks of kGatherChunk LDS counters, in order, for a
+// workgroup of 256 threads: lane l of wave w holds the cursors of trees
+// g*256 + w*64 + l (g < G, so T <= 256 G) into their member runs.  For every
+// chunk a lane finds where each of its runs crosses the chunk end by a
+// galloping search from its cursor, and the wavefront then visits the 64 run
+// pieces flattened across its lanes -- a leaf of 10,000 members costs its
+// wavefront no more per member than a leaf of 2.  No binary search from the
+// run start is ever repeated: cursors only move forward, and every pe entry
+// of the line is read once.
+template <int MODE, int LAYOUT, int G, class M>
+__device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
+                                                   const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
+                                                   const M* __restrict__ members, int32_t accumulate,
+                                                   int32_t* __restrict__ out, int32_t* __restrict__ out2,
+                                                   int64_t line, uint32_t* acc, uint32_t& diag_valid) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t cur[G], end[G];
+  uint32_t bagm = 0, valid = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int32_t t = g * 256 + wave * 64 + lane;
+    cur[g] = 0;
+    end[g] = 0;
+    if (t < ntree) {
+      const uint2 q = pe[static_cast<int64_t>(t) * n + line];
+      const uint32_t e = q.y & 0x7fffffffu;
+      if (e > 0) {
+        ++valid;
+        const uint32_t bag = q.y >> 31;
+        bagm |= bag << g;
+        cur[g] = LAYOUT == RFX_LAYOUT_UPPER ? q.x + 1 : bstart[mbucket[q.x]];
+        end[g] = (MODE == kMode_OOB_ASYM && !bag) ? cur[g] : e;
+      }
+    }
+  }
+  if (tid == 0) diag_valid = 0u;
+  __syncthreads();
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  const int64_t first = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
+  for (int64_t lo = first; lo < n; lo += kGatherChunk) {
+    const int64_t hi = min(n, lo + kGatherChunk);
+    const int len_chunk = static_cast<int>(hi - lo);
+    __syncthreads();
+    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint32_t e2 = hi < n ? gallop_lb(members, cur[g], end[g], static_cast<uint32_t>(hi)) : end[g];
+      const uint32_t s0 = cur[g];
+      cur[g] = e2;
+      line_visits<MODE, LAYOUT>(s0, e2 - s0, (bagm >> g) & 1u, lane, members, acc, lo, line);
+    }
+    __syncthreads();
+    if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
+    __syncthreads();
+    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
+    for (int q = tid; q < len_chunk; q += 256) {
+      const uint32_t v = acc[q];
+      if (MODE == kMode_PAIR) {
+        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+        out[base + q] = accumulate ? out[base + q] + va : va;
+        out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+      } else {
+        out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+      }
+    }
+  }
+}
+
 }  // namespace
