@@ -356,6 +356,10 @@ int rfx_batch_set_algos(rfx_batch* b, const int32_t* algos);
 /* Per-problem algorithm and measured sum over trees and leaves of |leaf|^2
  * (0 before the autotuner ran); either pointer may be NULL. */
 int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* sum_leaf_sq);
+/* flags[i] = 1 when problem i is counted by its own kernel launch (set by
+ * rfx_batch_autotune), 0 when it goes through the persistent queue.  No
+ * counterpart in the reference, which has no multi-forest batching. */
+int rfx_batch_get_own_launch(const rfx_batch* b, int32_t* flags);
 int rfx_batch_get_tuning(const rfx_batch* b, int32_t* lines_per_item, int32_t* blocks_per_cu);
 void rfx_batch_free(rfx_batch* b);
 

@@ -80,6 +80,15 @@ class ForestBatch:
             raise ValueError("one algorithm per forest")
         check(lib().rfx_batch_set_algos(self._h, a.ctypes.data), "rfx_batch_set_algos")
 
+    def own_launch(self):
+        """Per forest: True when autotune gave it its own kernel launch, False
+        when it goes through the persistent queue."""
+        import numpy as np
+
+        f = np.zeros(len(self.forests), np.int32)
+        check(lib().rfx_batch_get_own_launch(self._h, f.ctypes.data), "rfx_batch_get_own_launch")
+        return f.astype(bool)
+
     def stats(self):
         """(algorithm per forest, measured sum |leaf|^2 per forest)."""
         import numpy as np

@@ -620,6 +620,12 @@ extern "C" int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* s
   return RFX_OK;
 }
 
+extern "C" int rfx_batch_get_own_launch(const rfx_batch* b, int32_t* flags) {
+  if (!b || !flags) return RFX_ERR_INVALID;
+  for (size_t i = 0; i < b->prob.size(); ++i) flags[i] = b->direct[i];
+  return RFX_OK;
+}
+
 extern "C" int rfx_batch_autotune(hipStream_t st, rfx_batch* b, void* workspace, size_t workspace_bytes,
                                   int32_t reps, double* best_ms) {
   if (!b || !workspace || reps < 1) return RFX_ERR_INVALID;

@@ -158,6 +158,52 @@ def test_batch_long_rows_match_single_problem_path(gpu):
     b.close()
 
 
+@pytest.mark.parametrize("variant", ["oob_upper", "pair"])
+def test_batch_own_launches_and_queue_mixed(gpu, variant):
+    # after autotune the large forests get their own kernel launches (the
+    # DENSE ones on the second stream), the tiny ones stay in the queue;
+    # eager runs and graph replays match the one-problem path, which the
+    # other tests pin to the oracle, and the oracle itself on one problem
+    shapes = [(20000, 6, 3, 20, 21), (6000, 10, 4, 60, 22), (300, 20, 6, 1, 23), (37, 5, 2, 3, 24),
+              (1000, 8, 3, 50, 25)]
+    probs = _problems(shapes)
+    dfs = [DeviceForest(f, x) for f, x in probs]
+    _say(f"batch own launches + queue, variant {variant}, shapes {shapes}")
+    b = ForestBatch(dfs, variant)
+    b.autotune(reps=1)
+    own = b.own_launch()
+    algos, _ = b.stats()
+    _say(f"own launch {list(own)} algos {list(algos)}")
+    assert own[0] and not own[3], "the largest forest launches on its own, the 37-sample one is queued"
+    ref = []
+    for df in dfs:
+        keys, _ = df.leaf_assign()
+        acc = Accumulator(df.n, df.ntree, df.total_nodes)
+        if variant == "pair":
+            ref.append(acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER))
+        else:
+            ref.append((acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER), None))
+    torch.cuda.synchronize()
+    for use_graph in (False, True, True):
+        for o in b.out:
+            o.fill_(-7)
+        if variant == "pair":
+            for o in b.out2:
+                o.fill_(-7)
+        torch.cuda.synchronize()
+        b.run(use_graph=use_graph)
+        torch.cuda.synchronize()
+        for k, (ra, ro) in enumerate(ref):
+            assert torch.equal(b.out[k], ra), f"graph {use_graph} problem {k}"
+            if variant == "pair":
+                assert torch.equal(b.out2[k], ro), f"graph {use_graph} problem {k} OOB"
+    f, x = probs[1]
+    exp = O.counts(f, O.leaf_assign(f, x), _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
+    got = b.out2[1] if variant == "pair" else b.out[1]
+    assert np.array_equal(got.cpu().numpy(), exp)
+    b.close()
+
+
 def test_batch_rejects_bad_input(gpu, problems):
     _, dfs = problems
     with pytest.raises(ValueError):
