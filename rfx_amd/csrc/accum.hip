@@ -384,7 +384,10 @@ int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, in
 template <int MODE, int LAYOUT>
 int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate,
                           int32_t* out, int32_t* out2) {
-  if (RFX_GATHER_ROWS && 4 * (n + 4) <= kLineMaxLds) {
+  // the whole-line and line-chunk kernels read the 16-bit member copy, which
+  // exists only for n <= kGatherLineMaxN (<= kMembers16MaxN)
+  if (n <= kGatherLineMaxN && !w.members16) return RFX_ERR_INVALID;
+  if (RFX_GATHER_ROWS && n <= kGatherLineMaxN && 4 * (n + 4) <= kLineMaxLds) {
     if (ntree <= kLineThreads) return launch_rows<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
     if (ntree <= 2 * kLineThreads) return launch_rows<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
     if (ntree <= 4 * kLineThreads) return launch_rows<MODE, LAYOUT, 4>(st, w, n, ntree, accumulate, out, out2);
