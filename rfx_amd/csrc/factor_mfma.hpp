@@ -196,124 +196,123 @@ __global__ __launch_bounds__(256) void fac_mfma_finalize_kernel(const float* __r
 }
 
 // Y[i][c] = u~_i^T Q_c u~_i - (1 - |u_i|^2)^2 V[i][c]; 4 waves x 64 rows.
+//
+// Computed as T^T = Q_c U~^T per 16-row tile (Q_c is stored exactly
+// symmetric, so the finalize kernel's B-operand layout of a Q tile is also
+// its A-operand layout): with A = Q tile (tn, ka) and B = U~^T tile (ka, rt),
+// D[j][i] = T[i][tn*16 + j] lands in lane i + 16*(j/4), element j%4 -- the
+// same lane and element that hold u~[i][tn*16 + j] in the B operand of k-block
+// tn, so the quadratic form's epilogue multiplies registers by registers and
+// sums over the 4 lanes of a row with two shuffles.  Q tiles are read straight
+// from global memory (a few hundred KB shared by every workgroup, L2-resident)
+// in operand layout; no LDS, so occupancy is set by registers alone.
 template <int FMT, int NT, bool SPLIT>
 __global__ __launch_bounds__(256) void fac_mfma_apply_kernel(const typename Store<FMT>::T* __restrict__ U,
                                                               const float* __restrict__ scale, int64_t n, int r,
                                                               const uint16_t* __restrict__ Qb,
                                                               const float* __restrict__ V, float* __restrict__ Y) {
-  constexpr int W = NT * 16;
-  constexpr int kRowStride = W + 4;
-  __shared__ uint16_t Ut[SPLIT ? 2 : 1][256][kRowStride];
-  __shared__ uint16_t Qs[2][NT * NT * 256];
-  __shared__ float nrm[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, l16 = lane & 15;
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * 256;
-  for (int e = tid; e < 256 * W; e += 256) {
-    const int row = e / W, col = e % W;
-    const int64_t i = i0 + row;
-    float u = 0.f;
-    if (i < n) {
-      if (col < r)
-        u = Store<FMT>::load(U, i * r + col, scale, col);
-      else if (col == r)
-        u = 1.f;
-    }
-    const uint16_t h = mf_bf16(u);
-    Ut[0][row][col] = h;
-    if (SPLIT) Ut[SPLIT ? 1 : 0][row][col] = mf_bf16(u - mf_f32(h));
-  }
-  __syncthreads();
-  auto ut = [&](int row, int col) -> float {
-    float v = mf_f32(Ut[0][row][col]);
-    if (SPLIT) v += mf_f32(Ut[SPLIT ? 1 : 0][row][col]);
-    return v;
-  };
-  {
-    float s = 0.f;
-    for (int col = 0; col < r; ++col) {
-      const float u = ut(tid, col);
-      s += u * u;
-    }
-    nrm[tid] = s;
-  }
-  // A operands of the wave's 4 row tiles: A[i][k] = u~[row][ka*16 + k]
-  mf_bf16x4 Aop[4][NT], Alo[SPLIT ? 4 : 1][NT];
+  const int64_t w0 = static_cast<int64_t>(blockIdx.x) * 256 + wave * 64;
+  // B operands: lane (l16, g) holds u~[row][ka*16 + 4g + e] of row w0 + rt*16 + l16
+  mf_bf16x4 Bu[4][NT], Bl[SPLIT ? 4 : 1][NT];
+  float nrm[4];
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+  for (int rt = 0; rt < 4; ++rt) {
+    const int64_t i = w0 + rt * 16 + l16;
+    float s2 = 0.f;
 #pragma unroll
     for (int ka = 0; ka < NT; ++ka) {
-      const uint2 w = *reinterpret_cast<const uint2*>(&Ut[0][wave * 64 + rt * 16 + l16][ka * 16 + 4 * g]);
-      Aop[rt][ka] = mf_pack(w.x & 0xffffu, w.x >> 16, w.y & 0xffffu, w.y >> 16);
-      if (SPLIT) {
-        const uint2 z =
-            *reinterpret_cast<const uint2*>(&Ut[SPLIT ? 1 : 0][wave * 64 + rt * 16 + l16][ka * 16 + 4 * g]);
-        Alo[SPLIT ? rt : 0][ka] = mf_pack(z.x & 0xffffu, z.x >> 16, z.y & 0xffffu, z.y >> 16);
+      uint16_t h[4], lo[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = ka * 16 + 4 * g + e;
+        float u = 0.f;
+        if (i < n) {
+          if (col < r)
+            u = Store<FMT>::load(U, i * r + col, scale, col);
+          else if (col == r)
+            u = 1.f;
+        }
+        h[e] = mf_bf16(u);
+        lo[e] = SPLIT ? mf_bf16(u - mf_f32(h[e])) : static_cast<uint16_t>(0);
+        const float uv = SPLIT ? mf_f32(h[e]) + mf_f32(lo[e]) : mf_f32(h[e]);
+        if (col < r) s2 += uv * uv;
       }
+      Bu[rt][ka] = mf_pack(h[0], h[1], h[2], h[3]);
+      if (SPLIT) Bl[SPLIT ? rt : 0][ka] = mf_pack(lo[0], lo[1], lo[2], lo[3]);
     }
+    s2 += __shfl_xor(s2, 16, 64);
+    s2 += __shfl_xor(s2, 32, 64);
+    nrm[rt] = s2;
+  }
   float yv[4][4];
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) yv[rt][e] = 0.f;
+    for (int q = 0; q < 4; ++q) yv[rt][q] = 0.f;
   constexpr int qn = NT * NT * 256;
+  const int64_t plane = static_cast<int64_t>(kMfK) * qn;
   for (int c = 0; c < kMfK; ++c) {
-    __syncthreads();
-    for (int e = tid; e < qn / 4; e += 256) {
-      reinterpret_cast<uint2*>(Qs[0])[e] = reinterpret_cast<const uint2*>(Qb + static_cast<int64_t>(c) * qn)[e];
-      reinterpret_cast<uint2*>(Qs[1])[e] =
-          reinterpret_cast<const uint2*>(Qb + static_cast<int64_t>(kMfK) * qn + static_cast<int64_t>(c) * qn)[e];
-    }
-    __syncthreads();
-    mf_bf16x4 Bhi[NT][NT], Blo[NT][NT];
+    const uint16_t* Qh = Qb + static_cast<int64_t>(c) * qn;
+    float ys[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int tn = 0; tn < NT; ++tn)
+    for (int tn = 0; tn < NT; ++tn) {
+      mf_bf16x4 Ah[NT], Al[NT];
 #pragma unroll
       for (int ka = 0; ka < NT; ++ka) {
-        const uint2 w = *reinterpret_cast<const uint2*>(&Qs[0][((tn * NT + ka) * 64 + lane) * 4]);
-        Bhi[tn][ka] = mf_pack(w.x & 0xffffu, w.x >> 16, w.y & 0xffffu, w.y >> 16);
-        const uint2 z = *reinterpret_cast<const uint2*>(&Qs[1][((tn * NT + ka) * 64 + lane) * 4]);
-        Blo[tn][ka] = mf_pack(z.x & 0xffffu, z.x >> 16, z.y & 0xffffu, z.y >> 16);
+        const int64_t o = static_cast<int64_t>((tn * NT + ka) * 64 + lane) * 4;
+        const uint2 w = *reinterpret_cast<const uint2*>(Qh + o);
+        const uint2 z = *reinterpret_cast<const uint2*>(Qh + plane + o);
+        Ah[ka] = mf_pack(w.x & 0xffffu, w.x >> 16, w.y & 0xffffu, w.y >> 16);
+        Al[ka] = mf_pack(z.x & 0xffffu, z.x >> 16, z.y & 0xffffu, z.y >> 16);
       }
+      mf_f32x4 d[4];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) d[rt] = mf_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ka = 0; ka < NT; ++ka)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          d[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah[ka], Bu[rt][ka], d[rt], 0, 0, 0);
+          d[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al[ka], Bu[rt][ka], d[rt], 0, 0, 0);
+          if (SPLIT)
+            d[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah[ka], Bl[SPLIT ? rt : 0][ka], d[rt], 0, 0, 0);
+        }
+      // D[j][i]: lane l16 + 16g holds T[i][tn*16 + 4g + e], i = l16
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const mf_bf16x4 bu = Bu[rt][tn];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float uv = mf_f32(static_cast<uint16_t>(bu[e]));
+          if (SPLIT) uv += mf_f32(static_cast<uint16_t>(Bl[SPLIT ? rt : 0][tn][e]));
+          ys[rt] += d[rt][e] * uv;
+        }
+      }
+    }
+    // the row's 4 lanes l16 + 16g hold its partial sums; lane g = c % 4 keeps column c
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
-      float s[4] = {0.f, 0.f, 0.f, 0.f};
+      float v = ys[rt];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
 #pragma unroll
-      for (int tn = 0; tn < NT; ++tn) {
-        mf_f32x4 d = mf_f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ka = 0; ka < NT; ++ka) {
-          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop[rt][ka], Bhi[tn][ka], d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop[rt][ka], Blo[tn][ka], d, 0, 0, 0);
-          if (SPLIT) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Alo[SPLIT ? rt : 0][ka], Bhi[tn][ka], d, 0, 0, 0);
-        }
-        // D[i][b]: row 4g + e of the tile, column tn*16 + l16
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s[e] += d[e] * ut(wave * 64 + rt * 16 + 4 * g + e, tn * 16 + l16);
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float v = s[e];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
-        if (l16 == c) yv[rt][e] = v;
-      }
+      for (int q = 0; q < 4; ++q)
+        if (c == 4 * q + g) yv[rt][q] = v;
     }
   }
-  // lane 16g + c holds rows 4g + e of every row tile for column c = l16
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+  for (int rt = 0; rt < 4; ++rt) {
+    const int64_t i = w0 + rt * 16 + l16;
+    if (i >= n) continue;
+    const float dd = 1.f - nrm[rt];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int row = wave * 64 + rt * 16 + 4 * g + e;
-      const int64_t i = i0 + row;
-      if (i < n) {
-        const float d = 1.f - nrm[row];
-        Y[i * kMfK + l16] = yv[rt][e] - d * d * V[i * kMfK + l16];
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * q + g;
+      Y[i * kMfK + c] = yv[rt][q] - dd * dd * V[i * kMfK + c];
     }
+  }
 }
 
 }  // namespace
