@@ -340,6 +340,7 @@ __global__ __launch_bounds__(256) void fac_apply_kernel(const typename Store<FMT
 size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 
 constexpr int kMdsK = 16;
+constexpr int kStartDoubles = 64 + 64 * kMdsK;  // column means + r x 16 projection, r <= 64
 
 int64_t fac_rows_per_blk(int64_t n) {
   int64_t rb = (n + 127) / 128;  // about 128 row blocks
@@ -375,8 +376,9 @@ template <typename Real>
 size_t fac_mds_ws(int64_t n, int r) {
   const int P = r * (r + 1) / 2 + r + 1;
   const int64_t nrb = (n + fac_rows_per_blk(n) - 1) / fac_rows_per_blk(n);
-  return al(8 * n) + al(P * 6) + al(sizeof(Real) * nrb * P * kMdsK) + al(8 * nrb * P) + al(sizeof(Real) * P * kMdsK) +
-         al(8 * P) + al(8 * n) + rfx::SubspaceSolver<Real>::workspace_bytes(n, kMdsK);
+  return al(8 * n) + al(P * 6 + 16) + al(sizeof(Real) * nrb * P * kMdsK) + al(8 * nrb * P) +
+         al(sizeof(Real) * P * kMdsK) + al(8 * P) + al(8 * n) + al(8 * kStartDoubles) +
+         rfx::SubspaceSolver<Real>::workspace_bytes(n, kMdsK);
 }
 
 // Workspace of the VALU operator path (also the FP32 refinement of the mixed path).
@@ -390,6 +392,7 @@ struct FacWs {
   Real* Z;
   double* Z64;
   double* ones;
+  double* start;  // column means and projection of the start block (kStartDoubles)
   char* solver;
 };
 
@@ -415,6 +418,8 @@ int fac_ws_setup(hipStream_t st, void* ws, int64_t n, int r, FacWs<Real>& f) {
   p += al(8 * P);
   f.ones = reinterpret_cast<double*>(p);
   p += al(8 * n);
+  f.start = reinterpret_cast<double*>(p);
+  p += al(8 * kStartDoubles);
   f.solver = p;
   PairHost h = make_pairs(r);
   RFX_HIP_TRY(hipMemcpyAsync(pa, h.a.data(), P, hipMemcpyHostToDevice, st));
@@ -454,6 +459,175 @@ int fac_row_means(hipStream_t st, const void* Uq, const float* scale, int64_t n,
   return RFX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Start block of the factor MDS.  E_ij = (1 - u_i.u_j)^2 = 1 - 2 u_i.u_j +
+// (u_i.u_j)^2, so B = -1/2 J E J = J U U^T J - 1/2 J (U o U)(U o U)^T J:
+// where the proximities u_i.u_j are small the first term leads, and the
+// leading eigenvectors of B lie close to the leading left singular vectors
+// of the column-centred factors J U.  The iteration starts from
+// V0 = J U Y, Y the leading 16-dimensional subspace of the r x r Gram
+// (J U)^T (J U) (a few block power steps on the host) instead of a random
+// block.  Needs r >= 16.
+// ---------------------------------------------------------------------------
+constexpr int kGramTileRows = 32;
+
+// part[blk][m]: sum over the block's rows of u_a u_b (a <= b, packed upper
+// triangle in row order) and of u_a (last r entries), FP64.
+template <int FMT>
+__global__ __launch_bounds__(256) void fac_gram_kernel(const typename Store<FMT>::T* __restrict__ U,
+                                                       const float* __restrict__ scale, int64_t n, int r,
+                                                       int64_t rows_per_blk, double* __restrict__ part) {
+  __shared__ float us[kGramTileRows][64];
+  const int m = r * (r + 1) / 2 + r;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  constexpr int kPer = (64 * 65 / 2 + 64 + 255) / 256;
+  double acc[kPer];
+  int pa[kPer], pb[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    acc[q] = 0.0;
+    const int e = threadIdx.x + q * 256;
+    pa[q] = -1;
+    pb[q] = -1;
+    if (e < r * (r + 1) / 2) {  // (a, b), a <= b, row order
+      int a = 0, rem = e;
+      while (rem >= r - a) {
+        rem -= r - a;
+        ++a;
+      }
+      pa[q] = a;
+      pb[q] = a + rem;
+    } else if (e < m) {
+      pa[q] = e - r * (r + 1) / 2;
+    }
+  }
+  for (int64_t t0 = i0; t0 < i1; t0 += kGramTileRows) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kGramTileRows * r; e += 256) {
+      const int row = e / r, col = e % r;
+      const int64_t i = t0 + row;
+      us[row][col] = i < i1 ? Store<FMT>::load(U, i * r + col, scale, col) : 0.f;
+    }
+    __syncthreads();
+    const int rows = static_cast<int>(min<int64_t>(kGramTileRows, i1 - t0));
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (pa[q] < 0) continue;
+      double s = 0.0;
+      if (pb[q] >= 0)
+        for (int row = 0; row < rows; ++row)
+          s += static_cast<double>(us[row][pa[q]]) * static_cast<double>(us[row][pb[q]]);
+      else
+        for (int row = 0; row < rows; ++row) s += static_cast<double>(us[row][pa[q]]);
+      acc[q] += s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = threadIdx.x + q * 256;
+    if (e < m) part[static_cast<int64_t>(blockIdx.x) * m + e] = acc[q];
+  }
+}
+
+__global__ void fac_gram_reduce_kernel(const double* __restrict__ part, int nblk, int m, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[static_cast<int64_t>(b) * m + e];
+  out[e] = s;
+}
+
+// V[i][c] = sum_a (u_ia - mu_a) Y[a][c]; st = [mu (64) | Y (r x 16)]
+template <int FMT, typename Real>
+__global__ void fac_start_kernel(const typename Store<FMT>::T* __restrict__ U, const float* __restrict__ scale,
+                                 int64_t n, int r, const double* __restrict__ st, Real* __restrict__ V) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * kMdsK) return;
+  const int64_t i = e / kMdsK;
+  const int c = static_cast<int>(e % kMdsK);
+  double s = 0.0;
+  for (int a = 0; a < r; ++a)
+    s += (static_cast<double>(Store<FMT>::load(U, i * r + a, scale, a)) - st[a]) * st[64 + a * kMdsK + c];
+  V[e] = static_cast<Real>(s);
+}
+
+// Orthonormal columns (modified Gram-Schmidt) of the r x k row-major Y.
+void host_orthonormalize(std::vector<double>& Y, int r, int k) {
+  for (int c = 0; c < k; ++c) {
+    for (int d = 0; d < c; ++d) {
+      double dot = 0.0;
+      for (int a = 0; a < r; ++a) dot += Y[a * k + c] * Y[a * k + d];
+      for (int a = 0; a < r; ++a) Y[a * k + c] -= dot * Y[a * k + d];
+    }
+    double nn = 0.0;
+    for (int a = 0; a < r; ++a) nn += Y[a * k + c] * Y[a * k + c];
+    nn = std::sqrt(nn);
+    for (int a = 0; a < r; ++a) Y[a * k + c] = nn > 0 ? Y[a * k + c] / nn : 0.0;
+  }
+}
+
+// used = false (and V untouched) when the Gram's 16-dimensional leading
+// subspace is numerically rank-deficient: the caller then starts at random.
+template <int FMT, typename Real>
+int fac_start_block(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const FacWs<Real>& f,
+                    Real* V, bool& used) {
+  used = false;
+  using ST = typename Store<FMT>::T;
+  const ST* U = static_cast<const ST*>(Uq);
+  const int m = r * (r + 1) / 2 + r;
+  const int64_t rpb = fac_rows_per_blk(n);
+  hipLaunchKernelGGL((fac_gram_kernel<FMT>), dim3(static_cast<unsigned>(f.nrb)), dim3(256), 0, st, U, scale, n, r, rpb,
+                     f.partial64);
+  hipLaunchKernelGGL(fac_gram_reduce_kernel, dim3(rfx_ceil_div(m, 256)), dim3(256), 0, st, f.partial64,
+                     static_cast<int>(f.nrb), m, f.Z64);
+  std::vector<double> G(m);
+  RFX_HIP_TRY(hipMemcpyAsync(G.data(), f.Z64, 8 * m, hipMemcpyDeviceToHost, st));
+  RFX_HIP_TRY(hipStreamSynchronize(st));
+  // centred Gram C = U^T U - n mu mu^T
+  const int tri = r * (r + 1) / 2;
+  std::vector<double> mu(r), C(static_cast<size_t>(r) * r);
+  for (int a = 0; a < r; ++a) mu[a] = G[tri + a] / static_cast<double>(n);
+  for (int a = 0, e = 0; a < r; ++a)
+    for (int b = a; b < r; ++b, ++e) C[a * r + b] = C[b * r + a] = G[e] - static_cast<double>(n) * mu[a] * mu[b];
+  // leading 16-dimensional subspace of C: block power steps from a fixed start
+  std::vector<double> Y(static_cast<size_t>(r) * kMdsK), T(Y.size());
+  uint64_t z = 0x243f6a8885a308d3ull;
+  for (auto& y : Y) {
+    z = z * 6364136223846793005ull + 1442695040888963407ull;
+    y = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+  host_orthonormalize(Y, r, kMdsK);
+  for (int it = 0; it < 8; ++it) {
+    for (int a = 0; a < r; ++a)
+      for (int c = 0; c < kMdsK; ++c) {
+        double s = 0.0;
+        for (int b = 0; b < r; ++b) s += C[a * r + b] * Y[b * kMdsK + c];
+        T[a * kMdsK + c] = s;
+      }
+    Y = T;
+    host_orthonormalize(Y, r, kMdsK);
+  }
+  double qmax = 0.0, qmin = 1e300;
+  for (int c = 0; c < kMdsK; ++c) {
+    double q = 0.0;
+    for (int a = 0; a < r; ++a)
+      for (int b = 0; b < r; ++b) q += Y[a * kMdsK + c] * C[a * r + b] * Y[b * kMdsK + c];
+    qmax = std::max(qmax, q);
+    qmin = std::min(qmin, q);
+  }
+  if (!(qmax > 0.0) || qmin <= 1e-10 * qmax) return RFX_OK;
+  std::vector<double> up(kStartDoubles, 0.0);
+  for (int a = 0; a < r; ++a) up[a] = mu[a];
+  for (size_t e = 0; e < Y.size(); ++e) up[64 + e] = Y[e];
+  RFX_HIP_TRY(hipMemcpyAsync(f.start, up.data(), 8 * kStartDoubles, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL((fac_start_kernel<FMT, Real>), dim3(rfx_ceil_div(n * kMdsK, 256)), dim3(256), 0, st, U, scale, n,
+                     r, f.start, V);
+  RFX_HIP_TRY(hipStreamSynchronize(st));  // the host buffer `up` is local
+  used = true;
+  return rfx_launch_status();
+}
+
 // E V on the VALU (FP32 or FP64 arithmetic) through the Khatri-Rao pairs.
 template <int FMT, typename Real>
 int fac_valu_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const FacWs<Real>& f,
@@ -481,7 +655,12 @@ int fac_mds_run(hipStream_t st, const void* Uq, const float* scale, int64_t n, i
   auto apply = [&](const Real* V, Real* W, int) -> int {
     return fac_valu_apply<FMT, Real>(st, Uq, scale, n, r, f, V, W);
   };
-  return solver.run(apply, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res);
+  bool informed = false;
+  if (r >= kMdsK && n >= 2 * kMdsK) {
+    rc = fac_start_block<FMT, Real>(st, Uq, scale, n, r, f, solver.block_v(), informed);
+    if (rc) return rc;
+  }
+  return solver.run(apply, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res, 0.0, informed);
 }
 
 template <typename Real>
@@ -573,7 +752,12 @@ int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U
   rfx::SubspaceSolver<float> solver(st, n, kMdsK);
   solver.carve(f.solver);
   auto apply_mf = [&](const float* V, float* W, int) -> int { return mf_apply<FMT>(st, Uq, scale, n, r, V, W, P, Qb); };
-  rc = solver.run(apply_mf, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res);
+  bool informed = false;
+  if (r >= kMdsK && n >= 2 * kMdsK) {
+    rc = fac_start_block<FMT, float>(st, Uq, scale, n, r, f, solver.block_v(), informed);
+    if (rc) return rc;
+  }
+  rc = solver.run(apply_mf, f.rdev, g, 3, max_iter, tol, 0x5eedull, coords, res, 0.0, informed);
   if (rc) return rc;
   iters[0] = res.iterations;
   iters[1] = 0;
