@@ -161,6 +161,20 @@ int rfx_prox_accumulate(hipStream_t stream, const int32_t* keys, int64_t n, int3
 int rfx_prox_build_buckets(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                            void* workspace, size_t workspace_bytes);
 
+/* A3, case-wise weights: the use_casewise branch of the reference's
+ * accumulating proximity (rf::cpu_proximity, src/rf_proximity.cpp:186-223,
+ * weight nin(kk) / nodesize at 214-216).  For every tree, each out-of-bag
+ * case i and in-bag case j of the same leaf add nin_t(j) / (sum of nin_t over
+ * the leaf's in-bag cases) to out[i + j n] (FP64 n x n column-major, row =
+ * the out-of-bag case, the reference's prox layout); accumulate = 0
+ * overwrites, 1 adds.  nin: device int32 [T][n] bootstrap multiplicities (the
+ * in-bag bits of keys must agree with nin > 0).  The weights of different
+ * trees are summed in no fixed order: entries may differ from the
+ * reference's tree-ordered FP64 sums in the last bits. */
+size_t rfx_prox_casewise_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes);
+int rfx_prox_casewise(hipStream_t stream, const int32_t* keys, const int32_t* nin, int64_t n, int32_t ntree,
+                      int64_t total_nodes, int32_t accumulate, double* out, void* workspace, size_t workspace_bytes);
+
 /* ALL and OOB_UPPER in one pass (same layout for both outputs). */
 int rfx_prox_accumulate_pair(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree,
                              int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,

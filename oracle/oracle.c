@@ -18,6 +18,9 @@
  *   orc_prox_v1         src/rf_proximity.cpp:99-231, called once per tree as at
  *                       src/rf_random_forest.cpp:1305-1308 (OpenMP parallel-for
  *                       over out-of-bag rows with an atomic FP64 +=)
+ *   orc_prox_v1_casewise the same with use_casewise (src/rf_proximity.cpp:
+ *                       186-223): each in-bag partner kk of an out-of-bag i
+ *                       adds nin(kk) / sum over the leaf's in-bag cases of nin
  *   orc_counts_bucket   the ALL / OOB_ASYM / OOB_UPPER integer counts by leaf
  *                       buckets (ALL has no reference implementation; it is
  *                       pinned by brute force and closed-form invariants)
@@ -109,7 +112,7 @@ int orc_leaf_assign(const float* x, int64_t n, int32_t p, int32_t ntree, const i
 /* One tree of the reference's accumulating dense branch. */
 static void v1_one_tree(const int32_t* st, const int32_t* ndx, const int32_t* nin, int64_t n, int32_t nnode,
                         double* prox, int32_t* nod, int32_t* ncount, int32_t* ncn, int32_t* nodexb, int32_t* ndbegin,
-                        int32_t* npcase) {
+                        int32_t* npcase, int casewise) {
   int32_t nterm = 0;
   for (int32_t k = 0; k < nnode; ++k) nod[k] = -1;
   for (int32_t k = 0; k < nnode; ++k)
@@ -142,19 +145,26 @@ static void v1_one_tree(const int32_t* st, const int32_t* ndx, const int32_t* ni
     if (nin[i] != 0) continue;
     int32_t k = nodexb[i];
     if (k < 0 || k >= nterm) continue;
+    int64_t nodesize = 0; /* bootstrap draws that landed in the leaf */
+    for (int32_t j = ndbegin[k]; j < ndbegin[k + 1]; ++j) {
+      int32_t kk = npcase[j];
+      if (kk >= 0 && nin[kk] > 0) nodesize += nin[kk];
+    }
+    if (casewise && nodesize == 0) continue;
     for (int32_t j = ndbegin[k]; j < ndbegin[k + 1]; ++j) {
       int32_t kk = npcase[j];
       if (kk >= 0 && nin[kk] > 0) {
+        const double w = casewise ? (double)nin[kk] / (double)nodesize : 1.0;
 #pragma omp atomic
-        prox[i + (int64_t)kk * n] += 1.0;
+        prox[i + (int64_t)kk * n] += w;
       }
     }
   }
 }
 
 /* Accumulate all trees into prox (column-major n x n FP64, caller-zeroed). */
-int orc_prox_v1(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr, const int32_t* nin,
-                int64_t n, int32_t ntree, double* prox) {
+static int prox_v1_all(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr, const int32_t* nin,
+                int64_t n, int32_t ntree, double* prox, int casewise) {
   int32_t maxnn = 0;
   for (int32_t t = 0; t < ntree; ++t) {
     int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
@@ -171,12 +181,22 @@ int orc_prox_v1(const int32_t* nodestatus, const int64_t* node_offset, const int
   }
   for (int32_t t = 0; t < ntree; ++t) {
     v1_one_tree(nodestatus + node_offset[t], nodextr + (int64_t)t * n, nin + (int64_t)t * n, n,
-                (int32_t)(node_offset[t + 1] - node_offset[t]), prox, nod, ws, ws + n, ws + 2 * n, ndbegin, ws + 3 * n);
+                (int32_t)(node_offset[t + 1] - node_offset[t]), prox, nod, ws, ws + n, ws + 2 * n, ndbegin, ws + 3 * n, casewise);
   }
   free(nod);
   free(ws);
   free(ndbegin);
   return 0;
+}
+
+int orc_prox_v1(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr, const int32_t* nin,
+                int64_t n, int32_t ntree, double* prox) {
+  return prox_v1_all(nodestatus, node_offset, nodextr, nin, n, ntree, prox, 0);
+}
+
+int orc_prox_v1_casewise(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr,
+                         const int32_t* nin, int64_t n, int32_t ntree, double* prox) {
+  return prox_v1_all(nodestatus, node_offset, nodextr, nin, n, ntree, prox, 1);
 }
 
 /* Output index helpers: FULL column-major, UPPER packed row-major. */

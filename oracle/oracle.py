@@ -48,10 +48,12 @@ def orc() -> C.CDLL:
         L = C.CDLL(str(ORACLE_SO))
         L.orc_leaf_assign.argtypes = [_p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p]
         L.orc_prox_v1.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
+        L.orc_prox_v1_casewise.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
         L.orc_counts_bucket.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]
         L.orc_counts_brute.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _p]
         L.orc_finishprox.argtypes = [_i64, _p, _p, _p]
-        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_counts_bucket, L.orc_counts_brute, L.orc_finishprox):
+        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_counts_bucket, L.orc_counts_brute,
+                  L.orc_finishprox):
             f.restype = _i32
         L.orc_num_threads.restype = _i32
         _orc = L
@@ -67,7 +69,7 @@ def ref() -> C.CDLL:
     if _ref is None:
         L = C.CDLL(str(REF_SO))
         L.ref_run_forest.argtypes = [_p, _p, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _p, _i32, _i32,
-                                     _p, _p, _p, _p, _p, _p, _p, _p, _p]
+                                     _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32]
         L.ref_run_forest.restype = _i32
         L.ref_finishprox.argtypes = [_i32, _p, _p, _p]
         L.ref_finishprox.restype = _i32
@@ -98,12 +100,14 @@ def leaf_assign(forest, x: np.ndarray, cat: np.ndarray | None = None) -> np.ndar
     return out
 
 
-def prox_v1(forest, nodextr: np.ndarray) -> np.ndarray:
-    """Accumulated FP64 n x n (column-major as a flat array, shape (n, n) F-order)."""
+def prox_v1(forest, nodextr: np.ndarray, casewise: bool = False) -> np.ndarray:
+    """Accumulated FP64 n x n (column-major as a flat array, shape (n, n) F-order);
+    casewise: the reference's use_casewise weights nin(kk) / leaf draws."""
     n = forest.n
     prox = np.zeros(n * n, np.float64)
-    _ok(orc().orc_prox_v1(_a(forest.nodestatus), _a(forest.node_offset), _a(np.ascontiguousarray(nodextr)),
-                          _a(forest.nin), n, forest.ntree, _a(prox)), "orc_prox_v1")
+    fn = orc().orc_prox_v1_casewise if casewise else orc().orc_prox_v1
+    _ok(fn(_a(forest.nodestatus), _a(forest.node_offset), _a(np.ascontiguousarray(nodextr)), _a(forest.nin), n,
+           forest.ntree, _a(prox)), "orc_prox_v1")
     return prox
 
 
@@ -139,7 +143,8 @@ def num_threads() -> int:
 # the reference itself (compiled from its sources into _ref/)
 # ---------------------------------------------------------------------------
 def ref_run_forest(x: np.ndarray, y: np.ndarray, ntree: int, mtry: int, minndsize: int = 1, iseed: int = 12345,
-                   cat: np.ndarray | None = None, maxcat: int = 1, do_prox: bool = True) -> dict:
+                   cat: np.ndarray | None = None, maxcat: int = 1, do_prox: bool = True,
+                   casewise: bool = False) -> dict:
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.int32)
     n, p = x.shape
@@ -155,7 +160,8 @@ def ref_run_forest(x: np.ndarray, y: np.ndarray, ntree: int, mtry: int, minndsiz
     cat = None if cat is None else np.ascontiguousarray(cat, np.int32)
     _ok(ref().ref_run_forest(_a(x), _a(y), n, p, nclass, ntree, mtry, minndsize, iseed, _a(cat), maxcat, int(do_prox),
                              _a(r["nnode"]), _a(r["nodestatus"]), _a(r["bestvar"]), _a(r["xbestsplit"]),
-                             _a(r["treemap"]), _a(r["nin"]), _a(r["nodextr"]), _a(r["nout"]), _a(r["prox"])),
+                             _a(r["treemap"]), _a(r["nin"]), _a(r["nodextr"]), _a(r["nout"]), _a(r["prox"]),
+                             int(casewise)),
         "ref_run_forest")
     r["maxnode"] = maxnode
     return r

@@ -55,10 +55,10 @@ extern "C" int ref_run_forest(const float* x, const int32_t* y, int32_t n, int32
                               int32_t mtry, int32_t minndsize, int32_t iseed, const int32_t* cat_in, int32_t maxcat,
                               int32_t do_prox, int32_t* nnode_out, int32_t* nodestatus, int32_t* bestvar,
                               float* xbestsplit, int32_t* treemap, int32_t* nin_out, int32_t* nodextr_out,
-                              int32_t* nout, double* prox) {
+                              int32_t* nout, double* prox, int32_t casewise) {
   const integer_t maxnode = 2 * n + 1;
   if (maxcat < 1) maxcat = 1;
-  rf::g_config.use_casewise = false;
+  rf::g_config.use_casewise = casewise != 0;
   rf::g_config.use_sparse = false;
   std::vector<integer_t> cat(p, 1);
   if (cat_in)

@@ -80,6 +80,8 @@ _SIGS = {
     "rfx_leaf_assign": (_i32, [_p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _p]),
     "rfx_prox_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),
     "rfx_prox_build_buckets": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz]),
+    "rfx_prox_casewise_workspace_bytes": (_sz, [_i64, _i32, _i64]),
+    "rfx_prox_casewise": (_i32, [_p, _p, _p, _i64, _i32, _i64, _i32, _p, _p, _sz]),
     "rfx_prox_accumulate": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _i32, _p, _p, _sz]),
     "rfx_prox_accumulate_pair": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _p, _p, _p, _sz]),
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),

@@ -1,0 +1,136 @@
+// Case-wise (bootstrap-weighted) proximity, the reference's use_casewise
+// branch of the accumulating V1 proximity (src/rf_proximity.cpp:186-223):
+// for every tree t, an out-of-bag case i and an in-bag case j of the same
+// leaf add
+//     P_w[i, j] += nin_t(j) / sum over the leaf's in-bag cases k of nin_t(k)
+// into the FP64 n x n column-major matrix (row = the out-of-bag case), the
+// layout of the reference's prox.  The weights of one out-of-bag case over
+// its leaf sum to 1.
+//
+// Built on the leaf buckets of the integer counts: the leaf's bootstrap draws
+// sum_k nin_t(k) come from one integer-atomic pass over the slots, then one
+// workgroup per output column j (in-bag j only contributes) gathers, chunk by
+// chunk of kCwChunk rows, the out-of-bag members of j's leaf in every tree
+// into FP64 LDS counters, with j's weight nin_t(j) / draws, and writes the
+// column once.  The member runs of a wavefront's 64 trees are flattened
+// across its lanes as in the integer gather.  FP64 LDS atomics add the
+// weights of different trees in no fixed order, so an entry may differ from
+// the reference's tree-ordered sum in its last bits (about 1e-16 relative).
+#include <algorithm>
+
+#include "rfx_common.hpp"
+#include "bucket.hpp"
+#include "gather.hpp"
+
+namespace {
+
+constexpr int kCwChunk = 4096;  // FP64 LDS counters per workgroup (32 KiB)
+constexpr int kCwUnroll = 4;
+
+// draws[leaf] = sum over the leaf's in-bag slots of nin
+__global__ void cw_leaf_draws_kernel(const int32_t* __restrict__ keys, const int32_t* __restrict__ nin, int64_t slots,
+                                     uint32_t* __restrict__ draws) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= slots) return;
+  const int32_t k = keys[s];
+  if (k >= 0 && (k & 1)) atomicAdd(&draws[k >> 1], static_cast<uint32_t>(nin[s]));
+}
+
+__global__ __launch_bounds__(256) void cw_column_kernel(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
+                                                        const uint32_t* __restrict__ mbucket,
+                                                        const uint32_t* __restrict__ members,
+                                                        const uint32_t* __restrict__ draws,
+                                                        const int32_t* __restrict__ nin, int64_t n, int32_t ntree,
+                                                        int32_t accumulate, double* __restrict__ out) {
+  __shared__ double acc[kCwChunk];
+  const int64_t j = xcd_row(blockIdx.x);
+  if (j >= n) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int64_t lo = 0; lo < n; lo += kCwChunk) {
+    const int64_t hi = min(n, lo + kCwChunk);
+    const int len_chunk = static_cast<int>(hi - lo);
+    __syncthreads();
+    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0.0;
+    __syncthreads();
+    for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
+      const int32_t t = tb + lane;
+      uint32_t s = 0, len = 0;
+      double w = 0.0;
+      if (t < ntree) {
+        const uint2 q = pe[static_cast<int64_t>(t) * n + j];
+        const uint32_t end = q.y & 0x7fffffffu;
+        if (end > 0 && (q.y >> 31)) {  // j in bag in tree t
+          const uint32_t leaf = mbucket[q.x];
+          w = static_cast<double>(nin[static_cast<int64_t>(t) * n + j]) / static_cast<double>(draws[leaf]);
+          const uint32_t bs = bstart[leaf];
+          const uint32_t b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
+          const uint32_t b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
+          s = b0;
+          len = b1 > b0 ? b1 - b0 : 0u;
+        }
+      }
+      // the 64 runs flattened across the lanes: owner lane of element g by a
+      // 6-step search over the prefix sums of the run lengths
+      uint32_t pre = len;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(pre, d, 64);
+        if (lane >= d) pre += y;
+      }
+      const uint32_t excl = pre - len;
+      const uint32_t total = __shfl(pre, 63, 64);
+      for (uint32_t g0 = 0; g0 < total; g0 += 64 * kCwUnroll) {
+        uint32_t mv[kCwUnroll];
+        double wv[kCwUnroll];
+        bool ok[kCwUnroll];
+#pragma unroll
+        for (int u = 0; u < kCwUnroll; ++u) {
+          const uint32_t g = g0 + u * 64 + lane;
+          int o = 0;
+#pragma unroll
+          for (int d = 32; d >= 1; d >>= 1) {
+            const uint32_t v = __shfl(pre, o + d - 1, 64);
+            if (v <= g) o += d;
+          }
+          o = min(o, 63);
+          const uint32_t so = __shfl(s, o, 64), eo = __shfl(excl, o, 64);
+          wv[u] = __shfl(w, o, 64);
+          ok[u] = g < total;
+          mv[u] = ok[u] ? members[so + (g - eo)] : 1u;
+        }
+#pragma unroll
+        for (int u = 0; u < kCwUnroll; ++u)
+          if (ok[u] && !(mv[u] & 1u)) atomicAdd(&acc[static_cast<int64_t>(mv[u] >> 1) - lo], wv[u]);
+      }
+    }
+    __syncthreads();
+    double* col = out + j * n + lo;
+    for (int q = tid; q < len_chunk; q += 256) col[q] = accumulate ? col[q] + acc[q] : acc[q];
+  }
+}
+
+}  // namespace
+
+extern "C" size_t rfx_prox_casewise_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
+  if (n <= 0 || ntree <= 0 || total_nodes <= 0) return 0;
+  return bucket_ws_bytes(n, ntree, total_nodes) + align256(4 * static_cast<size_t>(total_nodes));
+}
+
+extern "C" int rfx_prox_casewise(hipStream_t st, const int32_t* keys, const int32_t* nin, int64_t n, int32_t ntree,
+                                 int64_t total_nodes, int32_t accumulate, double* out, void* ws, size_t ws_bytes) {
+  if (!keys || !nin || !out || !ws || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || total_nodes <= 0 ||
+      total_nodes > RFX_MAX_TOTAL_NODES)
+    return RFX_ERR_INVALID;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+  if (ws_bytes < rfx_prox_casewise_workspace_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  uint32_t* draws = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + bucket_ws_bytes(n, ntree, total_nodes));
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (!rc) rc = fill_u32(st, draws, total_nodes, 0u);
+  if (rc) return rc;
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, keys, nin, slots, draws);
+  hipLaunchKernelGGL(cw_column_kernel, dim3(static_cast<unsigned>(xcd_rows_padded(n))), dim3(256), 0, st, w.pe,
+                     w.bstart, w.mbucket, w.members, draws, nin, n, ntree, accumulate, out);
+  return rfx_launch_status();
+}

@@ -92,6 +92,22 @@ class Accumulator:
         check(lib().rfx_prox_build_buckets(_stream(), _dptr(keys), self.n, self.ntree, self.total_nodes,
                                            _dptr(self.ws), self.ws.numel()), "rfx_prox_build_buckets")
 
+    def casewise(self, keys: torch.Tensor, nin: torch.Tensor, out: torch.Tensor | None = None,
+                 accumulate: bool = False) -> torch.Tensor:
+        """Case-wise weighted proximity (the reference's use_casewise): FP64
+        n x n column-major as a flat tensor, row = the out-of-bag case.  nin:
+        device int32 [T, n] bootstrap multiplicities."""
+        if nin.dtype != torch.int32 or nin.numel() != self.n * self.ntree:
+            raise ValueError("nin must be an int32 [T, n] tensor")
+        if out is None:
+            out = torch.empty(self.n * self.n, dtype=torch.float64, device=self.device)
+            accumulate = False
+        need = lib().rfx_prox_casewise_workspace_bytes(self.n, self.ntree, self.total_nodes)
+        ws = self.ws if self.ws.numel() >= need else torch.empty(need, dtype=torch.uint8, device=self.device)
+        check(lib().rfx_prox_casewise(_stream(), _dptr(keys), _dptr(nin), self.n, self.ntree, self.total_nodes,
+                                      int(accumulate), _dptr(out), _dptr(ws), ws.numel()), "rfx_prox_casewise")
+        return out
+
     def accumulate(self, keys: torch.Tensor, variant: int, layout: int, algo: int = _lib.ALGO_AUTO,
                    out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
         if out is None:

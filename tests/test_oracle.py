@@ -150,6 +150,35 @@ def test_oracle_leaf_assign_against_reference_fresh():
     assert np.array_equal(g.nodestatus, f.nodestatus) and np.array_equal(g.treemap, f.treemap)
 
 
+def test_oracle_casewise_against_reference_fresh():
+    # use_casewise: the reference's own cpu_proximity (dense branch, n <= 5000)
+    # compiled from its sources, against the restatement, bit for bit
+    if not O.ref_available():
+        pytest.skip("reference build oracle/_ref not present")
+    x, y = make_dataset(_lib.DATA_OVERLAP_MIXTURE, 300, 7, 3, 9)
+    r = O.ref_run_forest(x, y, 6, 2, iseed=3, do_prox=True, casewise=True)
+    f = Forest.from_arrays(300, 7, r["nnode"], r["nodestatus"], r["bestvar"], r["xbestsplit"], r["treemap"], r["nin"],
+                           maxnode=r["maxnode"])
+    nd = O.leaf_assign(f, x)
+    pw = O.prox_v1(f, nd, casewise=True)
+    assert np.array_equal(pw, r["prox"])
+    assert not np.array_equal(pw, O.prox_v1(f, nd))  # the weights do change the result
+
+
+def test_casewise_weights_sum_to_one_per_tree():
+    # every leaf of a grown tree holds in-bag cases, and an out-of-bag case's
+    # weights over its leaf's in-bag cases sum to 1: row i of the case-wise
+    # matrix sums to the number of trees in which i is out of bag
+    x, _, f = small_forest(_lib.DATA_GAUSS_MIXTURE, 150, 5, 3, 9, seed=6)
+    nd = O.leaf_assign(f, x)
+    pw = O.prox_v1(f, nd, casewise=True).reshape(f.n, f.n, order="F")
+    nin = f.nin.reshape(f.ntree, f.n)
+    oob_trees = np.sum(nin == 0, axis=0)
+    assert np.allclose(pw.sum(axis=1), oob_trees, rtol=0, atol=1e-12)
+    # rows of cases that are never out of bag stay zero
+    assert np.all(pw[oob_trees == 0, :] == 0)
+
+
 def test_mds_oracle_recovers_euclidean_configuration():
     # a proximity built from a Euclidean 3-D configuration: P = max - D  ->
     # classical MDS recovers the centred points up to an orthogonal map
