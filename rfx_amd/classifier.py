@@ -22,10 +22,15 @@ backed by the HIP path in librfx:
                        MDS from the stored factors on the matrix cores,
                        refined with the FP32 factors (k <= 3, rank <= 64).
 
+With use_casewise the proximity is the reference's case-wise weighted one
+(src/rf_proximity.cpp:186-223: an out-of-bag case and an in-bag case of a
+leaf add nin(j) / the leaf's bootstrap draws), accumulated in FP64 and fed
+through the same finishprox and MDS chain.
+
 Options outside the proximity path (importance, prediction, RF-GAP,
-case-wise weights, regression losses) are accepted for signature
-compatibility; the ones that would change the proximity (use_rfgap,
-use_casewise) raise NotImplementedError, as do the methods outside the path.
+regression losses) are accepted for signature compatibility; use_rfgap,
+which would change the proximity, raises NotImplementedError, as do the
+methods outside the path.
 use_gpu is accepted: this implementation always runs on the MI355X.
 """
 from __future__ import annotations
@@ -73,8 +78,10 @@ class RandomForestClassifier:
                  progress_desc: str = "Training Random Forest", gpu_loss_function: str = "gini", rank: int = 32,
                  n_threads_cpu: int = 0, use_rfgap: bool = False, importance_method: str = "local_imp",
                  clique_M: int = -1, use_casewise: bool = False):
-        if use_rfgap or use_casewise:
-            raise NotImplementedError("RF-GAP and case-wise proximities are not part of this proximity path")
+        if use_rfgap:
+            raise NotImplementedError("RF-GAP proximities are not part of this proximity path")
+        if use_casewise and use_qlora:
+            raise NotImplementedError("case-wise weights apply to the full proximity, not the low-rank factors")
         if quant_mode not in QUANT_FORMATS:
             raise ValueError(f"quant_mode must be one of {sorted(QUANT_FORMATS)}")
         if gpu_loss_function != "gini":
@@ -85,6 +92,7 @@ class RandomForestClassifier:
         self.compute_proximity = compute_proximity
         self.use_gpu, self.use_qlora, self.quant_mode = use_gpu, use_qlora, quant_mode
         self.rank = rank
+        self.use_casewise = use_casewise
         self.n_threads_cpu = n_threads_cpu
         self.batch_size = batch_size
         # accepted for signature compatibility, outside the proximity path
@@ -96,6 +104,7 @@ class RandomForestClassifier:
         self.forest = None
         self.cat = None
         self._counts = None  # device OOB_UPPER counts (UPPER int32)
+        self._prox_w = None  # device case-wise FP64 n x n (FULL), use_casewise
         self._nout = None
         self._factors = None  # (stored factors, scales, format, U float32)
 
@@ -116,7 +125,7 @@ class RandomForestClassifier:
         self.cat = detect_categorical(X, self.maxcat) if self.maxcat > 1 else np.ones(p, np.int32)
         self.forest = Forest.grow(X, y, self.ntree, mtry=self.mtry, minndsize=self.minndsize, maxnode=self.maxnode,
                                   iseed=self.iseed, nthreads=self.n_threads_cpu)
-        self._counts = self._factors = None
+        self._counts = self._factors = self._prox_w = None
         if not self.compute_proximity:
             return self
         f = self.forest
@@ -129,6 +138,10 @@ class RandomForestClassifier:
             fmt = QUANT_FORMATS[self.quant_mode]
             Uq, sc = quantize_factors(U, fmt)
             self._factors = (Uq, sc, fmt, U)
+        elif self.use_casewise:
+            acc = Accumulator(n, f.ntree, f.total_nodes, device=str(df.device))
+            nin = torch.from_numpy(np.ascontiguousarray(f.nin.reshape(f.ntree, n), np.int32)).to(df.device)
+            self._prox_w = acc.casewise(keys, nin)
         else:
             acc = Accumulator(n, f.ntree, f.total_nodes, device=str(df.device))
             self._counts = acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
@@ -141,12 +154,16 @@ class RandomForestClassifier:
 
     # ------------------------------------------------------------ proximity
     def get_proximity_matrix(self) -> np.ndarray:
-        from .proximity import finishprox, unpack_upper
+        from .proximity import finishprox, finishprox_f64, unpack_upper
 
         self._need_fit("get_proximity_matrix")
         if self._factors is not None:
             raise RuntimeError("the proximity is held as low-rank factors (use_qlora=True): use "
                                "get_lowrank_factors() or compute_mds_from_factors()")
+        if self._prox_w is not None:
+            n = self.nsample
+            S = finishprox_f64(self._prox_w, self._nout, n, _lib.LAYOUT_FULL)
+            return S.cpu().numpy().reshape(n, n, order="F").astype(np.float32)
         if self._counts is None:
             raise RuntimeError("Proximity matrix not available. Set compute_proximity=True when fitting.")
         n = self.nsample
@@ -164,9 +181,14 @@ class RandomForestClassifier:
 
     # ------------------------------------------------------------------ MDS
     def compute_mds_3d_cpu(self) -> np.ndarray:
-        from .proximity import mds_dense
+        from .proximity import finishprox_f64, mds_dense, mds_dense_matrix
 
         self._need_fit("compute_mds_3d_cpu")
+        if self._prox_w is not None:
+            # the reference chain: finishprox, float32 at the boundary, nout renormalisation, MDS
+            S = finishprox_f64(self._prox_w, self._nout, self.nsample, _lib.LAYOUT_FULL)
+            X, _, _ = mds_dense_matrix(S.float().double(), self.nsample, self._nout)
+            return X.cpu().numpy()
         if self._counts is None:
             raise RuntimeError("Proximity matrix not available. Set compute_proximity=True when fitting.")
         X, _, _ = mds_dense(self._counts, _lib.LAYOUT_UPPER, self.nsample, _lib.MDS_REFERENCE, nout=self._nout,

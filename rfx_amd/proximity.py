@@ -153,6 +153,34 @@ def finishprox(counts: torch.Tensor, in_layout: int, nout: torch.Tensor, n: int,
     return out
 
 
+def finishprox_f64(prox: torch.Tensor, nout: torch.Tensor, n: int, out_layout: int = _lib.LAYOUT_FULL) -> torch.Tensor:
+    """rf::cpu_finishprox of an FP64 accumulated matrix (FULL column-major,
+    e.g. case-wise weights) on the GPU, bit-identical."""
+    elems = upper_size(n) if out_layout == _lib.LAYOUT_UPPER else n * n
+    out = torch.empty(elems, dtype=torch.float64, device=prox.device)
+    check(lib().rfx_finishprox_f64(_stream(), _dptr(prox), _dptr(nout), n, _dptr(out), out_layout),
+          "rfx_finishprox_f64")
+    return out
+
+
+def mds_dense_matrix(prox: torch.Tensor, n: int, nout: torch.Tensor | None = None,
+                     precision: int = _lib.PREC_FP64, max_iter: int = 1000,
+                     tol: float = 1e-10) -> tuple[torch.Tensor, np.ndarray, int]:
+    """3-D classical MDS of a given FP64 n x n proximity (FULL column-major),
+    with the reference's nout renormalisation when nout is given."""
+    import ctypes as C
+
+    nb = lib().rfx_mds_dense_workspace_bytes(n, precision)
+    ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=prox.device)
+    X = torch.empty((n, 3), dtype=torch.float64, device=prox.device)
+    eig = np.zeros(16, np.float64)
+    it = C.c_int32()
+    check(lib().rfx_mds_dense_matrix(_stream(), _dptr(prox), n, _dptr(nout) if nout is not None else None, precision,
+                                     max_iter, tol, _dptr(ws), ws.numel(), _dptr(X), eig.ctypes.data, C.byref(it)),
+          "rfx_mds_dense_matrix")
+    return X, eig, it.value
+
+
 def unpack_upper(u: np.ndarray, n: int) -> np.ndarray:
     """Packed row-major upper triangle -> symmetric dense (host helper)."""
     m = np.zeros((n, n), dtype=u.dtype)

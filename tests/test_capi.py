@@ -88,7 +88,8 @@ def test_classifier_options_mirror_reference():
     with pytest.raises(NotImplementedError):
         RandomForestClassifier(use_rfgap=True)
     with pytest.raises(NotImplementedError):
-        RandomForestClassifier(use_casewise=True)
+        RandomForestClassifier(use_casewise=True, use_qlora=True)
+    assert RandomForestClassifier(use_casewise=True).use_casewise
     with pytest.raises(RuntimeError):
         clf.get_proximity_matrix()
     with pytest.raises(TypeError):

@@ -53,3 +53,29 @@ def test_lowrank_mode(gpu):
     X = clf.compute_mds_from_factors(k=2)
     assert X.shape == (200, 2) and np.all(np.isfinite(X))
     assert clf.compute_mds_3d_from_factors().shape == (200, 3)
+
+
+def test_casewise_proximity_and_mds(gpu):
+    # use_casewise: the case-wise FP64 accumulation, finishprox and the MDS
+    # chain, against the oracle chain on the same forest (the oracle's
+    # case-wise V1 is pinned bit for bit to the reference's cpu_proximity)
+    print("classifier case-wise", flush=True)
+    g = load_golden("gauss_n200.npz")
+    x, y = g["x"], g["y"].astype(np.int32)
+    clf = RandomForestClassifier(ntree=30, mtry=4, iseed=2, compute_proximity=True, use_gpu=True, use_casewise=True)
+    clf.fit(x, y)
+    f = clf.forest
+    n = len(y)
+    pw = O.prox_v1(f, O.leaf_assign(f, x), casewise=True)
+    nout = f.nout(2)
+    S_ref = O.finishprox(n, nout, pw).reshape(n, n, order="F")
+    S = clf.get_proximity_matrix()
+    assert S.dtype == np.float32 and S.shape == (n, n)
+    assert np.allclose(S, S_ref.astype(np.float32), rtol=1e-6, atol=1e-7)
+    X = clf.compute_mds_3d_cpu()
+    M = O.mds_input_reference(S_ref, nout)
+    Xref, w = O.classical_mds(M, 3)
+    if w[2] / w[3] > 1.02:
+        assert O.embedding_errors(X, Xref)["rel_frobenius"] < 1e-5
+    else:
+        assert np.allclose(np.sort(np.linalg.norm(X, axis=0) ** 2), np.sort(w[:3]), rtol=1e-6)
