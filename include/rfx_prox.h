@@ -175,6 +175,29 @@ size_t rfx_prox_casewise_workspace_bytes(int64_t n, int32_t ntree, int64_t total
 int rfx_prox_casewise(hipStream_t stream, const int32_t* keys, const int32_t* nin, int64_t n, int32_t ntree,
                       int64_t total_nodes, int32_t accumulate, double* out, void* workspace, size_t workspace_bytes);
 
+/* RF-GAP proximity.  Replaces rf::gpu_proximity_rfgap (include/
+ * rf_proximity.hpp:42, cuda/rf_proximity.cu:541-786) and
+ * rf::cpu_proximity_rfgap (include/rf_proximity.hpp:34, src/rf_proximity.cpp:
+ * 238-391), called from the forest's finalisation (src/rf_random_forest.cpp:
+ * 1702, 1707, 1798) with the per-tree nin and reached nodes, flattened
+ * [tree * n + sample].  For every case i, p_ij = (1 / |S_i|) * sum over the
+ * trees t in which i is out of bag (nin_t(i) = 0) of nin_t(j) * (1 / M_t)
+ * for each in-bag j that reached the same node (M_t = that node's sum of
+ * nin_t over in-bag cases); then out[i + j n] = out[j + i n] =
+ * (p_ij + p_ji) / 2, and out[i + i n] = 1 for a case that is never out of
+ * bag.  Cases are matched by equal node index, terminal or not, as in the
+ * reference.
+ *   nin       device int32 [T][n] bootstrap multiplicities
+ *   nodextr   device int32 [T][n] node reached (rfx_leaf_assign's nodextr;
+ *             negative entries take part in no pair)
+ *   node_off  device int64 [T+1] node offsets (total_nodes = node_off[T])
+ *   out       device FP64 n x n, overwritten, symmetric
+ * Sums run in no fixed tree order: entries agree with the reference to about
+ * 1e-16 relative. */
+size_t rfx_prox_rfgap_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes);
+int rfx_prox_rfgap(hipStream_t stream, const int32_t* nin, const int32_t* nodextr, const int64_t* node_off, int64_t n,
+                   int32_t ntree, int64_t total_nodes, double* out, void* workspace, size_t workspace_bytes);
+
 /* ALL and OOB_UPPER in one pass (same layout for both outputs). */
 int rfx_prox_accumulate_pair(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree,
                              int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,
@@ -213,6 +236,7 @@ int rfx_finishprox_f64(hipStream_t stream, const double* prox, const int32_t* no
 #define RFX_MDS_COUNTS 0    /* M_ij = c_ij * scale (e.g. ALL counts / T)       */
 #define RFX_MDS_REFERENCE 1 /* OOB_UPPER counts -> finishprox -> float32 -> nout renormalisation */
 #define RFX_MDS_MATRIX 2    /* a given FP64 proximity matrix (rfx_mds_dense_matrix) */
+#define RFX_MDS_RFGAP 3     /* a given FP64 RF-GAP matrix (rfx_mds_dense_rfgap) */
 
 #define RFX_PREC_FP32 0
 #define RFX_PREC_FP64 1
@@ -254,6 +278,17 @@ int rfx_mds_dense(hipStream_t stream, const int32_t* counts, int32_t layout, int
 int rfx_mds_dense_matrix(hipStream_t stream, const double* prox, int64_t n, const int32_t* nout, int32_t precision,
                          int32_t max_iter, double tol, void* workspace, size_t workspace_bytes, double* coords,
                          double* eigvals_host, int32_t* iters_host);
+
+/* A6+A7 on a given FP64 RF-GAP proximity: rf::compute_mds_3d_cpu with
+ * use_rfgap and OOB counts (src/rf_mds_cpu.cpp:125-153), as the classifier
+ * calls it (python/randomforest_py.cpp:1236-1244, nout = the reference's
+ * doubled OOB counts): M_ij = 1/2 (P_ij/nout_i + P_ji/nout_j) off the
+ * diagonal and M_ii = P_ii/nout_i, rows with nout_i = 0 undivided (no unit
+ * diagonal is imposed).  nout is required.  Then as rfx_mds_dense (same
+ * workspace size). */
+int rfx_mds_dense_rfgap(hipStream_t stream, const double* prox, int64_t n, const int32_t* nout, int32_t precision,
+                        int32_t max_iter, double tol, void* workspace, size_t workspace_bytes, double* coords,
+                        double* eigvals_host, int32_t* iters_host);
 
 /* Element-wise conversion of int32 counts to FP64, FP32 or FP16 (IEEE
  * round-to-nearest-even; exact up to 2048 for FP16).  The reference keeps

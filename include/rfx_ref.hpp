@@ -58,14 +58,26 @@ void gpu_finishprox(integer_t nsample, const integer_t* nout, dp_t* prox, dp_t* 
 
 // Replaces rf::compute_mds_3d_cpu (include/rf_mds_cpu.hpp:27-33,
 // src/rf_mds_cpu.cpp:88-341): n x 3 row-major coordinates of the classical
-// MDS of the host float64 column-major proximity.  With oob_counts and
-// use_rfgap == false the reference's nout renormalisation is applied
-// (src/rf_mds_cpu.cpp:154-186).  Throws std::runtime_error when fewer than
-// three eigenvalues exceed 1e-10 (as the reference) and for use_rfgap ==
-// true (RF-GAP is not part of this path).  memory_check is accepted; the
-// GPU form needs O(n) memory besides the input.
+// MDS of the host float64 column-major proximity.  With oob_counts the
+// reference's renormalisation is applied: by nout with a unit diagonal
+// (src/rf_mds_cpu.cpp:154-186), or with use_rfgap the RF-GAP form that keeps
+// the diagonal (125-153).  Throws std::runtime_error when fewer than three
+// eigenvalues exceed 1e-10 (as the reference).  memory_check is accepted;
+// the GPU form needs O(n) memory besides the input.
 std::vector<double> compute_mds_3d_cpu(const dp_t* proximity_matrix, integer_t n_samples, bool memory_check = true,
                                        const integer_t* oob_counts_rfgap = nullptr, bool use_rfgap = false);
+
+// RF-GAP proximity over all trees into the host float64 n x n prox
+// (overwritten, symmetric).  Replaces rf::cpu_proximity_rfgap and
+// rf::gpu_proximity_rfgap (include/rf_proximity.hpp:34-46,
+// src/rf_proximity.cpp:238-391, cuda/rf_proximity.cu:541-786): per-tree
+// bootstrap multiplicities and reached nodes, nested or flattened
+// [tree * nsample + sample].  Runs rfx_prox_rfgap; node offsets are formed
+// from each tree's largest node index.
+void cpu_proximity_rfgap(integer_t ntree, integer_t nsample, const std::vector<std::vector<integer_t>>& tree_nin,
+                         const std::vector<std::vector<integer_t>>& tree_nodextr, dp_t* prox);
+void gpu_proximity_rfgap(integer_t ntree, integer_t nsample, const integer_t* tree_nin_flat,
+                         const integer_t* tree_nodextr_flat, dp_t* prox);
 
 // Storage levels of the factors (the reference's QuantizationLevel names
 // FP32/FP16/INT8/NF4; here the stored formats of rfx_quantize_factors).

@@ -21,6 +21,10 @@
  *   orc_prox_v1_casewise the same with use_casewise (src/rf_proximity.cpp:
  *                       186-223): each in-bag partner kk of an out-of-bag i
  *                       adds nin(kk) / sum over the leaf's in-bag cases of nin
+ *   orc_prox_rfgap      src/rf_proximity.cpp:238-391 (cpu_proximity_rfgap):
+ *                       nin(j) * (1 / node draws) summed per (i, j) in tree
+ *                       order, rows scaled by 1 / |S_i|, then the symmetric
+ *                       average; cases matched by equal reached node
  *   orc_counts_bucket   the ALL / OOB_ASYM / OOB_UPPER integer counts by leaf
  *                       buckets (ALL has no reference implementation; it is
  *                       pinned by brute force and closed-form invariants)
@@ -197,6 +201,86 @@ int orc_prox_v1(const int32_t* nodestatus, const int64_t* node_offset, const int
 int orc_prox_v1_casewise(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr,
                          const int32_t* nin, int64_t n, int32_t ntree, double* prox) {
   return prox_v1_all(nodestatus, node_offset, nodextr, nin, n, ntree, prox, 1);
+}
+
+/* RF-GAP: the reference visits, for each case i, the trees in which i is out
+ * of bag in order and adds nin_t(j) * (1 / M) to prox[i + j n] for every
+ * in-bag j with nodextr_t(j) == nodextr_t(i) >= 0.  Here the tree loop is
+ * outermost and the in-bag cases of a node are grouped once per tree; every
+ * (i, j) entry still receives its terms in tree order, so the sums are the
+ * reference's bit for bit.  prox: n x n FP64 column-major, overwritten. */
+int orc_prox_rfgap(const int32_t* nin, const int32_t* nodextr, int64_t n, int32_t ntree, double* prox) {
+  memset(prox, 0, sizeof(double) * (size_t)(n * n));
+  int32_t* oobn = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+  int32_t* start = NULL;
+  int32_t* memb = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int64_t* draws = NULL;
+  if (!oobn || !memb) {
+    free(oobn);
+    free(memb);
+    return -2;
+  }
+  for (int32_t t = 0; t < ntree; ++t)
+    for (int64_t i = 0; i < n; ++i) oobn[i] += nin[(int64_t)t * n + i] == 0;
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int32_t* nt = nin + (int64_t)t * n;
+    const int32_t* xt = nodextr + (int64_t)t * n;
+    int32_t range = 0;
+    for (int64_t i = 0; i < n; ++i)
+      if (xt[i] + 1 > range) range = xt[i] + 1;
+    free(start);
+    free(draws);
+    start = (int32_t*)calloc((size_t)range + 2, sizeof(int32_t));
+    draws = (int64_t*)calloc((size_t)range + 1, sizeof(int64_t));
+    if (!start || !draws) {
+      free(oobn);
+      free(memb);
+      free(start);
+      free(draws);
+      return -2;
+    }
+    /* in-bag cases grouped by node (counting sort), draws per node */
+    for (int64_t j = 0; j < n; ++j)
+      if (nt[j] > 0 && xt[j] >= 0) {
+        start[xt[j] + 1]++;
+        draws[xt[j]] += nt[j];
+      }
+    for (int32_t k = 0; k < range; ++k) start[k + 1] += start[k];
+    for (int64_t j = 0; j < n; ++j)
+      if (nt[j] > 0 && xt[j] >= 0) memb[start[xt[j]]++] = (int32_t)j;
+    for (int32_t k = range; k > 0; --k) start[k] = start[k - 1];
+    start[0] = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+      if (nt[i] != 0) continue;
+      const int32_t k = xt[i];
+      if (k < 0 || draws[k] == 0) continue;
+      const double inv = 1.0 / (double)draws[k];
+      for (int32_t q = start[k]; q < start[k + 1]; ++q) {
+        const int32_t j = memb[q];
+        prox[i + (int64_t)j * n] += (double)nt[j] * inv;
+      }
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    if (oobn[i] == 0) {
+      prox[i + i * n] = 1.0;
+      continue;
+    }
+    const double f = 1.0 / (double)oobn[i];
+    for (int64_t j = 0; j < n; ++j) prox[i + j * n] *= f;
+  }
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = i + 1; j < n; ++j) {
+      const double avg = (prox[i + j * n] + prox[j + i * n]) / 2.0;
+      prox[i + j * n] = avg;
+      prox[j + i * n] = avg;
+    }
+  free(oobn);
+  free(memb);
+  free(start);
+  free(draws);
+  return 0;
 }
 
 /* Output index helpers: FULL column-major, UPPER packed row-major. */

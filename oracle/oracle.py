@@ -49,11 +49,12 @@ def orc() -> C.CDLL:
         L.orc_leaf_assign.argtypes = [_p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p]
         L.orc_prox_v1.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
         L.orc_prox_v1_casewise.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
+        L.orc_prox_rfgap.argtypes = [_p, _p, _i64, _i32, _p]
         L.orc_counts_bucket.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]
         L.orc_counts_brute.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _p]
         L.orc_finishprox.argtypes = [_i64, _p, _p, _p]
-        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_counts_bucket, L.orc_counts_brute,
-                  L.orc_finishprox):
+        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_prox_rfgap, L.orc_counts_bucket,
+                  L.orc_counts_brute, L.orc_finishprox):
             f.restype = _i32
         L.orc_num_threads.restype = _i32
         _orc = L
@@ -73,6 +74,8 @@ def ref() -> C.CDLL:
         L.ref_run_forest.restype = _i32
         L.ref_finishprox.argtypes = [_i32, _p, _p, _p]
         L.ref_finishprox.restype = _i32
+        L.ref_proximity_rfgap.argtypes = [_i32, _i32, _p, _p, _p]
+        L.ref_proximity_rfgap.restype = _i32
         _ref = L
     return _ref
 
@@ -108,6 +111,17 @@ def prox_v1(forest, nodextr: np.ndarray, casewise: bool = False) -> np.ndarray:
     fn = orc().orc_prox_v1_casewise if casewise else orc().orc_prox_v1
     _ok(fn(_a(forest.nodestatus), _a(forest.node_offset), _a(np.ascontiguousarray(nodextr)), _a(forest.nin), n,
            forest.ntree, _a(prox)), "orc_prox_v1")
+    return prox
+
+
+def prox_rfgap(nin: np.ndarray, nodextr: np.ndarray) -> np.ndarray:
+    """RF-GAP proximity, FP64 n x n (symmetric) as a flat array; nin and
+    nodextr are [T, n] int32."""
+    nin = np.ascontiguousarray(nin, np.int32)
+    nodextr = np.ascontiguousarray(nodextr, np.int32)
+    T, n = nin.shape
+    prox = np.empty(n * n, np.float64)
+    _ok(orc().orc_prox_rfgap(_a(nin), _a(nodextr), n, T, _a(prox)), "orc_prox_rfgap")
     return prox
 
 
@@ -191,6 +205,21 @@ def mds_input_reference(S: np.ndarray, nout: np.ndarray) -> np.ndarray:
     return M
 
 
+def mds_input_rfgap(P: np.ndarray, nout: np.ndarray) -> np.ndarray:
+    """The matrix compute_mds_3d_cpu embeds for an RF-GAP proximity
+    (src/rf_mds_cpu.cpp:125-153 with use_rfgap, as called at
+    python/randomforest_py.cpp:1236-1244 on the float32 matrix of
+    get_proximity_matrix): rows divided by nout_i where nout_i > 0, then
+    M_ij = 0.5 * (R_ij + R_ji) off the diagonal, M_ii = R_ii."""
+    Pf = P.astype(np.float32).astype(np.float64)
+    d = np.where(nout > 0, nout.astype(np.float64), 1.0)
+    R = Pf / d[:, None]
+    R = np.where((nout > 0)[:, None], R, Pf)
+    M = 0.5 * (R + R.T)
+    np.fill_diagonal(M, np.diag(R))
+    return M
+
+
 def classical_mds(prox: np.ndarray, k: int = 3, eig_floor: float = 1e-10) -> tuple[np.ndarray, np.ndarray]:
     """src/rf_mds_cpu.cpp:196-338 in float64: D = max(prox) - prox with zero
     diagonal, B = -1/2 J D^2 J, top-k eigenpairs with lambda > floor, sorted
@@ -229,3 +258,13 @@ def embedding_errors(X: np.ndarray, Xref: np.ndarray) -> dict:
     sv = np.linalg.svd(qa.T @ qb, compute_uv=False)
     ang = float(np.arccos(np.clip(sv.min(), -1.0, 1.0)))
     return {"rel_frobenius": rel, "max_principal_angle": ang}
+
+
+def ref_proximity_rfgap(nin: np.ndarray, nodextr: np.ndarray) -> np.ndarray:
+    """The reference's own rf::cpu_proximity_rfgap on [T, n] arrays."""
+    nin = np.ascontiguousarray(nin, np.int32)
+    nodextr = np.ascontiguousarray(nodextr, np.int32)
+    T, n = nin.shape
+    prox = np.empty(n * n, np.float64)
+    _ok(ref().ref_proximity_rfgap(T, n, _a(nin), _a(nodextr), _a(prox)), "ref_proximity_rfgap")
+    return prox

@@ -10,7 +10,7 @@
 // RandomForest::find_terminal_node (src/rf_random_forest.cpp:2194-2244, a
 // member function whose class cannot be compiled standalone), the second
 // nout increment (src/rf_random_forest.cpp:1108-1110), and rf::cpu_proximity;
-// rf::cpu_finishprox is exported separately.  rf_mds_cpu.cpp is not built:
+// rf::cpu_finishprox and rf::cpu_proximity_rfgap are exported separately.  rf_mds_cpu.cpp is not built:
 // it needs a LAPACK dsyevd_ that this image does not provide.
 #include <cstdint>
 #include <vector>
@@ -113,5 +113,17 @@ extern "C" int ref_run_forest(const float* x, const int32_t* y, int32_t n, int32
 
 extern "C" int ref_finishprox(int32_t n, const int32_t* nout, double* prox, double* proxsym) {
   rf::cpu_finishprox(n, nout, prox, proxsym);
+  return 0;
+}
+
+// rf::cpu_proximity_rfgap on flattened [tree * n + sample] arrays (its
+// per-tree vectors are built here, as RandomForest keeps them).
+extern "C" int ref_proximity_rfgap(int32_t ntree, int32_t n, const int32_t* nin, const int32_t* nodextr, double* prox) {
+  std::vector<std::vector<integer_t>> tn(ntree), tx(ntree);
+  for (int32_t t = 0; t < ntree; ++t) {
+    tn[t].assign(nin + static_cast<size_t>(t) * n, nin + static_cast<size_t>(t + 1) * n);
+    tx[t].assign(nodextr + static_cast<size_t>(t) * n, nodextr + static_cast<size_t>(t + 1) * n);
+  }
+  rf::cpu_proximity_rfgap(ntree, n, tn, tx, prox);
   return 0;
 }

@@ -47,6 +47,7 @@ MAX_TOTAL_NODES = 1073741823
 MDS_COUNTS = 0
 MDS_REFERENCE = 1
 MDS_MATRIX = 2
+MDS_RFGAP = 3
 
 PREC_FP32 = 0
 PREC_FP64 = 1
@@ -82,6 +83,8 @@ _SIGS = {
     "rfx_prox_build_buckets": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz]),
     "rfx_prox_casewise_workspace_bytes": (_sz, [_i64, _i32, _i64]),
     "rfx_prox_casewise": (_i32, [_p, _p, _p, _i64, _i32, _i64, _i32, _p, _p, _sz]),
+    "rfx_prox_rfgap_workspace_bytes": (_sz, [_i64, _i32, _i64]),
+    "rfx_prox_rfgap": (_i32, [_p, _p, _p, _p, _i64, _i32, _i64, _p, _p, _sz]),
     "rfx_prox_accumulate": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _i32, _p, _p, _sz]),
     "rfx_prox_accumulate_pair": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _p, _p, _p, _sz]),
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),
@@ -98,6 +101,7 @@ _SIGS = {
     "rfx_mds_factors_mixed": (_i32, [_p, _p, _p, _i32, _p, _i64, _i32, _i32, C.c_double, _i32, C.c_double, _p, _sz,
                                      _p, _p, _p]),
     "rfx_mds_dense_matrix": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_mds_dense_rfgap": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
     "rfx_counts_convert": (_i32, [_p, _p, _i64, _i32, _p]),
     "rfx_batch_create": (_i32, [_p, _i32, _i32, C.POINTER(_p)]),
     "rfx_batch_workspace_bytes": (_sz, [_p]),

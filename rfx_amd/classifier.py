@@ -27,10 +27,16 @@ With use_casewise the proximity is the reference's case-wise weighted one
 leaf add nin(j) / the leaf's bootstrap draws), accumulated in FP64 and fed
 through the same finishprox and MDS chain.
 
-Options outside the proximity path (importance, prediction, RF-GAP,
-regression losses) are accepted for signature compatibility; use_rfgap,
-which would change the proximity, raises NotImplementedError, as do the
-methods outside the path.
+With use_rfgap the proximity is the reference's RF-GAP one
+(src/rf_random_forest.cpp:1661-1712 -> rf::gpu_proximity_rfgap): FP64,
+symmetric, already normalised; get_proximity_matrix returns it as float32 and
+compute_mds_3d_cpu applies the reference's use_rfgap renormalisation by nout
+(src/rf_mds_cpu.cpp:125-153).  RF-GAP takes precedence over use_casewise, as
+in the reference's finalisation.
+
+Options outside the proximity path (importance, prediction, regression
+losses) are accepted for signature compatibility; the methods outside the
+path raise NotImplementedError.
 use_gpu is accepted: this implementation always runs on the MI355X.
 """
 from __future__ import annotations
@@ -78,8 +84,8 @@ class RandomForestClassifier:
                  progress_desc: str = "Training Random Forest", gpu_loss_function: str = "gini", rank: int = 32,
                  n_threads_cpu: int = 0, use_rfgap: bool = False, importance_method: str = "local_imp",
                  clique_M: int = -1, use_casewise: bool = False):
-        if use_rfgap:
-            raise NotImplementedError("RF-GAP proximities are not part of this proximity path")
+        if use_rfgap and use_qlora:
+            raise NotImplementedError("RF-GAP applies to the full proximity, not the low-rank factors")
         if use_casewise and use_qlora:
             raise NotImplementedError("case-wise weights apply to the full proximity, not the low-rank factors")
         if quant_mode not in QUANT_FORMATS:
@@ -93,6 +99,7 @@ class RandomForestClassifier:
         self.use_gpu, self.use_qlora, self.quant_mode = use_gpu, use_qlora, quant_mode
         self.rank = rank
         self.use_casewise = use_casewise
+        self.use_rfgap = use_rfgap
         self.n_threads_cpu = n_threads_cpu
         self.batch_size = batch_size
         # accepted for signature compatibility, outside the proximity path
@@ -105,6 +112,7 @@ class RandomForestClassifier:
         self.cat = None
         self._counts = None  # device OOB_UPPER counts (UPPER int32)
         self._prox_w = None  # device case-wise FP64 n x n (FULL), use_casewise
+        self._prox_g = None  # device RF-GAP FP64 n x n (FULL, symmetric), use_rfgap
         self._nout = None
         self._factors = None  # (stored factors, scales, format, U float32)
 
@@ -125,12 +133,12 @@ class RandomForestClassifier:
         self.cat = detect_categorical(X, self.maxcat) if self.maxcat > 1 else np.ones(p, np.int32)
         self.forest = Forest.grow(X, y, self.ntree, mtry=self.mtry, minndsize=self.minndsize, maxnode=self.maxnode,
                                   iseed=self.iseed, nthreads=self.n_threads_cpu)
-        self._counts = self._factors = self._prox_w = None
+        self._counts = self._factors = self._prox_w = self._prox_g = None
         if not self.compute_proximity:
             return self
         f = self.forest
         df = DeviceForest(f, X, self.cat)
-        keys, _ = df.leaf_assign()
+        keys, nodextr = df.leaf_assign(want_nodextr=self.use_rfgap)
         self._nout = torch.from_numpy(f.nout(2)).to(df.device)
         if self.use_qlora:
             rank = min(self.rank, 128, n)
@@ -138,6 +146,10 @@ class RandomForestClassifier:
             fmt = QUANT_FORMATS[self.quant_mode]
             Uq, sc = quantize_factors(U, fmt)
             self._factors = (Uq, sc, fmt, U)
+        elif self.use_rfgap:
+            acc = Accumulator(n, f.ntree, f.total_nodes, device=str(df.device))
+            nin = torch.from_numpy(np.ascontiguousarray(f.nin.reshape(f.ntree, n), np.int32)).to(df.device)
+            self._prox_g = acc.rfgap(nin, nodextr, df.node_off)
         elif self.use_casewise:
             acc = Accumulator(n, f.ntree, f.total_nodes, device=str(df.device))
             nin = torch.from_numpy(np.ascontiguousarray(f.nin.reshape(f.ntree, n), np.int32)).to(df.device)
@@ -160,6 +172,9 @@ class RandomForestClassifier:
         if self._factors is not None:
             raise RuntimeError("the proximity is held as low-rank factors (use_qlora=True): use "
                                "get_lowrank_factors() or compute_mds_from_factors()")
+        if self._prox_g is not None:
+            n = self.nsample
+            return self._prox_g.float().cpu().numpy().reshape(n, n, order="F")
         if self._prox_w is not None:
             n = self.nsample
             S = finishprox_f64(self._prox_w, self._nout, n, _lib.LAYOUT_FULL)
@@ -181,9 +196,13 @@ class RandomForestClassifier:
 
     # ------------------------------------------------------------------ MDS
     def compute_mds_3d_cpu(self) -> np.ndarray:
-        from .proximity import finishprox_f64, mds_dense, mds_dense_matrix
+        from .proximity import finishprox_f64, mds_dense, mds_dense_matrix, mds_dense_rfgap
 
         self._need_fit("compute_mds_3d_cpu")
+        if self._prox_g is not None:
+            # float32 at the boundary, then the use_rfgap renormalisation and MDS
+            X, _, _ = mds_dense_rfgap(self._prox_g.float().double(), self.nsample, self._nout)
+            return X.cpu().numpy()
         if self._prox_w is not None:
             # the reference chain: finishprox, float32 at the boundary, nout renormalisation, MDS
             S = finishprox_f64(self._prox_w, self._nout, self.nsample, _lib.LAYOUT_FULL)

@@ -16,6 +16,13 @@
 // across its lanes as in the integer gather.  FP64 LDS atomics add the
 // weights of different trees in no fixed order, so an entry may differ from
 // the reference's tree-ordered sum in its last bits (about 1e-16 relative).
+//
+// RF-GAP (src/rf_proximity.cpp:238-391, cuda/rf_proximity.cu:541-786) is the
+// same leaf sum with the weight formed as nin_t(j) * (1 / draws), followed by
+// a row scaling by 1 / |S_i| (S_i = the trees in which i is out of bag), the
+// symmetric average (p_ij + p_ji) / 2 and a unit diagonal for cases that are
+// never out of bag.  The scaling and the average run in one in-place pass
+// over pairs of mirrored 64 x 64 tiles.
 #include <algorithm>
 
 #include "rfx_common.hpp"
@@ -36,6 +43,8 @@ __global__ void cw_leaf_draws_kernel(const int32_t* __restrict__ keys, const int
   if (k >= 0 && (k & 1)) atomicAdd(&draws[k >> 1], static_cast<uint32_t>(nin[s]));
 }
 
+// kRecip: the weight as nin * (1 / draws) (RF-GAP) rather than nin / draws
+template <bool kRecip>
 __global__ __launch_bounds__(256) void cw_column_kernel(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
                                                         const uint32_t* __restrict__ mbucket,
                                                         const uint32_t* __restrict__ members,
@@ -61,7 +70,9 @@ __global__ __launch_bounds__(256) void cw_column_kernel(const uint2* __restrict_
         const uint32_t end = q.y & 0x7fffffffu;
         if (end > 0 && (q.y >> 31)) {  // j in bag in tree t
           const uint32_t leaf = mbucket[q.x];
-          w = static_cast<double>(nin[static_cast<int64_t>(t) * n + j]) / static_cast<double>(draws[leaf]);
+          const double c = static_cast<double>(nin[static_cast<int64_t>(t) * n + j]);
+          const double m = static_cast<double>(draws[leaf]);
+          w = kRecip ? c * (1.0 / m) : c / m;
           const uint32_t bs = bstart[leaf];
           const uint32_t b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
           const uint32_t b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
@@ -109,6 +120,103 @@ __global__ __launch_bounds__(256) void cw_column_kernel(const uint2* __restrict_
   }
 }
 
+// RF-GAP keys from the node each case reached, terminal or not: the
+// reference matches cases by equal node index (src/rf_proximity.cpp:
+// 320-355), so a case whose traversal stopped early still meets the cases
+// that stopped at the same node.  key = (node_off[t] + node) << 1 | in-bag,
+// -1 for a negative node index, which the reference skips.
+__global__ void rg_keys_kernel(const int32_t* __restrict__ nin, const int32_t* __restrict__ nodextr,
+                               const int64_t* __restrict__ node_off, int64_t n, int32_t* __restrict__ keys) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int32_t t = blockIdx.y;
+  if (i >= n) return;
+  const int64_t s = static_cast<int64_t>(t) * n + i;
+  const int32_t k = nodextr[s];
+  const int64_t nn = node_off[t + 1] - node_off[t];
+  keys[s] = (k >= 0 && k < nn) ? static_cast<int32_t>(((node_off[t] + k) << 1) | (nin[s] > 0 ? 1 : 0)) : -1;
+}
+
+// oob[i] = number of trees in which case i is out of bag (nin == 0)
+__global__ void rg_oob_trees_kernel(const int32_t* __restrict__ nin, int64_t n, int32_t ntree,
+                                    uint32_t* __restrict__ oob) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = 0;
+  for (int32_t t = 0; t < ntree; ++t) c += nin[static_cast<int64_t>(t) * n + i] == 0 ? 1u : 0u;
+  oob[i] = c;
+}
+
+constexpr int kRgTile = 64;
+
+// In place on the column-major n x n matrix W (W[i + j n], row = the
+// out-of-bag case): p_ij = W_ij * (1 / S_i) (rows with S_i = 0 are all zero
+// and stay so), then P_ij = P_ji = (p_ij + p_ji) / 2 off the diagonal, and
+// P_ii = 1 where S_i = 0, else p_ii.  Workgroup (I, J), I <= J, owns the
+// mirrored tiles (I, J) and (J, I): the (J, I) tile is read into LDS
+// transposed, the (I, J) tile is read, averaged and written directly, and
+// the averages are written back to (J, I) from LDS.  Both global passes run
+// along the contiguous (row) direction.
+__global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ w, const uint32_t* __restrict__ oob,
+                                                           int64_t n) {
+  // p_ij is rounded before the sum, as in the reference: a fused
+  // multiply-add of w_ij * (1 / S_i) + p_ji would differ between the two
+  // mirrored entries of the diagonal tile, which are computed separately
+#pragma clang fp contract(off)
+  const int64_t I = blockIdx.y, J = blockIdx.x;
+  if (I > J) return;
+  __shared__ double L[kRgTile][kRgTile + 1];  // L[b][a] = element (J0 + b, I0 + a)
+  const int64_t i0 = I * kRgTile, j0 = J * kRgTile;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const double inv_b = (j0 + tx < n && oob[j0 + tx] > 0) ? 1.0 / static_cast<double>(oob[j0 + tx]) : 1.0;
+  const double inv_a = (i0 + tx < n && oob[i0 + tx] > 0) ? 1.0 / static_cast<double>(oob[i0 + tx]) : 1.0;
+  // (J, I) tile: row J0 + b (b = tx, contiguous), column I0 + a; scaled by its row's 1 / S
+  for (int a = ty; a < kRgTile; a += 4) {
+    const int64_t r = j0 + tx, c = i0 + a;
+    L[tx][a] = (r < n && c < n) ? w[r + c * n] * inv_b : 0.0;
+  }
+  __syncthreads();
+  // (I, J) tile: row I0 + a (a = tx, contiguous), column J0 + b
+  for (int b = ty; b < kRgTile; b += 4) {
+    const int64_t r = i0 + tx, c = j0 + b;
+    if (r >= n || c >= n) continue;
+    double v;
+    if (r == c)
+      v = oob[r] > 0 ? w[r + c * n] * inv_a : 1.0;
+    else
+      v = (w[r + c * n] * inv_a + L[b][tx]) / 2.0;
+    w[r + c * n] = v;
+    L[b][tx] = v;
+  }
+  __syncthreads();
+  if (I == J) return;  // the diagonal tile is its own mirror: written above
+  for (int a = ty; a < kRgTile; a += 4) {
+    const int64_t r = j0 + tx, c = i0 + a;
+    if (r < n && c < n) w[r + c * n] = L[tx][a];
+  }
+}
+
+// The leaf sums of W: bucket build, leaf draws, then the column gather.
+template <bool kRecip>
+int weighted_leaf_sums(hipStream_t st, const int32_t* keys, const int32_t* nin, int64_t n, int32_t ntree,
+                       int64_t total_nodes, int32_t accumulate, double* out, void* ws) {
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  uint32_t* draws = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + bucket_ws_bytes(n, ntree, total_nodes));
+  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
+  if (!rc) rc = fill_u32(st, draws, total_nodes, 0u);
+  if (rc) return rc;
+  const int64_t slots = n * static_cast<int64_t>(ntree);
+  hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, keys, nin, slots, draws);
+  hipLaunchKernelGGL(cw_column_kernel<kRecip>, dim3(static_cast<unsigned>(xcd_rows_padded(n))), dim3(256), 0, st,
+                     w.pe, w.bstart, w.mbucket, w.members, draws, nin, n, ntree, accumulate, out);
+  return rfx_launch_status();
+}
+
+bool weighted_args_ok(const int32_t* keys, const int32_t* nin, const double* out, const void* ws, int64_t n,
+                      int32_t ntree, int64_t total_nodes) {
+  return keys && nin && out && ws && n > 0 && n <= RFX_MAX_SAMPLES && ntree > 0 && total_nodes > 0 &&
+         total_nodes <= RFX_MAX_TOTAL_NODES;
+}
+
 }  // namespace
 
 extern "C" size_t rfx_prox_casewise_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
@@ -118,19 +226,33 @@ extern "C" size_t rfx_prox_casewise_workspace_bytes(int64_t n, int32_t ntree, in
 
 extern "C" int rfx_prox_casewise(hipStream_t st, const int32_t* keys, const int32_t* nin, int64_t n, int32_t ntree,
                                  int64_t total_nodes, int32_t accumulate, double* out, void* ws, size_t ws_bytes) {
-  if (!keys || !nin || !out || !ws || n <= 0 || n > RFX_MAX_SAMPLES || ntree <= 0 || total_nodes <= 0 ||
-      total_nodes > RFX_MAX_TOTAL_NODES)
-    return RFX_ERR_INVALID;
+  if (!weighted_args_ok(keys, nin, out, ws, n, ntree, total_nodes)) return RFX_ERR_INVALID;
   if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
   if (ws_bytes < rfx_prox_casewise_workspace_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
-  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  uint32_t* draws = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + bucket_ws_bytes(n, ntree, total_nodes));
-  int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
-  if (!rc) rc = fill_u32(st, draws, total_nodes, 0u);
+  return weighted_leaf_sums<false>(st, keys, nin, n, ntree, total_nodes, accumulate, out, ws);
+}
+
+extern "C" size_t rfx_prox_rfgap_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
+  if (n <= 0 || ntree <= 0 || total_nodes <= 0) return 0;
+  return rfx_prox_casewise_workspace_bytes(n, ntree, total_nodes) + align256(4 * static_cast<size_t>(n)) +
+         align256(4 * static_cast<size_t>(n) * static_cast<size_t>(ntree));
+}
+
+extern "C" int rfx_prox_rfgap(hipStream_t st, const int32_t* nin, const int32_t* nodextr, const int64_t* node_off,
+                              int64_t n, int32_t ntree, int64_t total_nodes, double* out, void* ws, size_t ws_bytes) {
+  if (!nodextr || !node_off || !weighted_args_ok(nin, nin, out, ws, n, ntree, total_nodes)) return RFX_ERR_INVALID;
+  // grid y of the key kernel is the tree index
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31) || ntree > 65535) return RFX_ERR_UNSUPPORTED;
+  if (ws_bytes < rfx_prox_rfgap_workspace_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
+  char* tail = static_cast<char*>(ws) + rfx_prox_casewise_workspace_bytes(n, ntree, total_nodes);
+  uint32_t* oob = reinterpret_cast<uint32_t*>(tail);
+  int32_t* keys = reinterpret_cast<int32_t*>(tail + align256(4 * static_cast<size_t>(n)));
+  hipLaunchKernelGGL(rg_keys_kernel, dim3(rfx_ceil_div(n, 256), static_cast<unsigned>(ntree)), dim3(256), 0, st, nin,
+                     nodextr, node_off, n, keys);
+  int rc = weighted_leaf_sums<true>(st, keys, nin, n, ntree, total_nodes, 0, out, ws);
   if (rc) return rc;
-  const int64_t slots = n * static_cast<int64_t>(ntree);
-  hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, keys, nin, slots, draws);
-  hipLaunchKernelGGL(cw_column_kernel, dim3(static_cast<unsigned>(xcd_rows_padded(n))), dim3(256), 0, st, w.pe,
-                     w.bstart, w.mbucket, w.members, draws, nin, n, ntree, accumulate, out);
+  hipLaunchKernelGGL(rg_oob_trees_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, nin, n, ntree, oob);
+  const unsigned nt = static_cast<unsigned>(rfx_ceil_div(n, kRgTile));
+  hipLaunchKernelGGL(rg_symmetric_kernel, dim3(nt, nt), dim3(256), 0, st, out, oob, n);
   return rfx_launch_status();
 }

@@ -20,6 +20,10 @@
 //              rf::compute_mds_3d_cpu receives it: with nout, M_ij = 1/2
 //              (S_ij/nout_i + S_ji/nout_j) and M_ii = 1
 //              (src/rf_mds_cpu.cpp:154-186); without, M = S as it is
+//   RFGAP      a given FP64 RF-GAP matrix P with the reference's use_rfgap
+//              renormalisation (src/rf_mds_cpu.cpp:125-153): off the
+//              diagonal as MATRIX with nout, M_ii = P_ii/nout_i (rows with
+//              nout_i = 0 undivided)
 
 #include <cstring>
 #include <vector>
@@ -34,12 +38,12 @@ constexpr int kCols = 32;   // columns per LDS tile
 constexpr int kBlockK = 16; // subspace block size
 
 struct ProxForm {
-  int32_t mode;    // RFX_MDS_COUNTS / RFX_MDS_REFERENCE / RFX_MDS_MATRIX
+  int32_t mode;    // RFX_MDS_COUNTS / _REFERENCE / _MATRIX / _RFGAP
   int32_t layout;  // RFX_LAYOUT_UPPER / RFX_LAYOUT_FULL
   double scale;
   double mx;       // max of M
   const int32_t* nout;
-  const double* mat;  // MATRIX: S, column-major
+  const double* mat;  // MATRIX, RFGAP: the given matrix, column-major
   int64_t n;
 };
 
@@ -54,7 +58,7 @@ __device__ __forceinline__ double recip_or_one(int32_t c) { return c > 0 ? 1.0 /
 // M_ij (i != j) from the stored count c_ij (MATRIX: from S itself)
 __device__ __forceinline__ double prox_value(const ProxForm& f, int32_t c, int64_t i, int64_t j) {
   if (f.mode == RFX_MDS_COUNTS) return static_cast<double>(c) * f.scale;
-  if (f.mode == RFX_MDS_MATRIX) {
+  if (f.mode == RFX_MDS_MATRIX || f.mode == RFX_MDS_RFGAP) {
     const double sij = f.mat[i + j * f.n];
     if (!f.nout) return sij;
     const double sji = f.mat[j + i * f.n];
@@ -74,11 +78,16 @@ __device__ __forceinline__ double prox_value(const ProxForm& f, int32_t c, int64
 
 __device__ __forceinline__ double diag_value(const ProxForm& f, int32_t c, int64_t i) {
   if (f.mode == RFX_MDS_MATRIX) return f.nout ? 1.0 : f.mat[i + i * f.n];
+  if (f.mode == RFX_MDS_RFGAP) {
+    const double sii = f.mat[i + i * f.n];
+    const int32_t ni = f.nout[i];
+    return ni > 0 ? sii / static_cast<double>(ni) : sii;
+  }
   return f.mode == RFX_MDS_COUNTS ? static_cast<double>(c) * f.scale : 1.0;
 }
 
 __device__ __forceinline__ int32_t count_at(const ProxForm& f, const int32_t* counts, int64_t idx) {
-  return f.mode == RFX_MDS_MATRIX ? 0 : counts[idx];
+  return (f.mode == RFX_MDS_MATRIX || f.mode == RFX_MDS_RFGAP) ? 0 : counts[idx];
 }
 
 // Max of M over all entries (diagonal included), FP64 bits via atomicMax.
@@ -302,5 +311,19 @@ extern "C" int rfx_mds_dense_matrix(hipStream_t st, const double* prox, int64_t 
     return run_dense<double>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_MATRIX, 1.0, nout, max_iter, tol,
                              workspace, coords, eigvals_host, iters_host);
   return run_dense<float>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_MATRIX, 1.0, nout, max_iter, tol, workspace,
+                          coords, eigvals_host, iters_host);
+}
+
+extern "C" int rfx_mds_dense_rfgap(hipStream_t st, const double* prox, int64_t n, const int32_t* nout,
+                                   int32_t precision, int32_t max_iter, double tol, void* workspace,
+                                   size_t workspace_bytes, double* coords, double* eigvals_host, int32_t* iters_host) {
+  if (!prox || !nout || !coords || !workspace || n < 4 || n > RFX_MAX_SAMPLES || max_iter < 1)
+    return RFX_ERR_INVALID;
+  if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
+  if (workspace_bytes < rfx_mds_dense_workspace_bytes(n, precision)) return RFX_ERR_WORKSPACE;
+  if (precision == RFX_PREC_FP64)
+    return run_dense<double>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_RFGAP, 1.0, nout, max_iter, tol,
+                             workspace, coords, eigvals_host, iters_host);
+  return run_dense<float>(st, nullptr, prox, RFX_LAYOUT_FULL, n, RFX_MDS_RFGAP, 1.0, nout, max_iter, tol, workspace,
                           coords, eigvals_host, iters_host);
 }

@@ -62,6 +62,17 @@ F64F finishprox(I32 nout, F64F prox) {
   return out;
 }
 
+// nin, nodextr: [T, n] int32; returns the n x n float64 RF-GAP matrix.
+F64F proximity_rfgap(I32 nin, I32 nodextr) {
+  need(nin.ndim() == 2 && nodextr.ndim() == 2 && nin.shape(0) == nodextr.shape(0) &&
+           nin.shape(1) == nodextr.shape(1),
+       "nin and nodextr must be [T, n] arrays of one shape");
+  const integer_t T = static_cast<integer_t>(nin.shape(0)), n = static_cast<integer_t>(nin.shape(1));
+  F64F out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(n)});
+  rfx::ref::gpu_proximity_rfgap(T, n, nin.data(), nodextr.data(), out.mutable_data());
+  return out;
+}
+
 py::array_t<double> compute_mds_3d(F64F prox, py::object oob_counts, bool use_rfgap) {
   need(prox.ndim() == 2 && prox.shape(0) == prox.shape(1), "prox must be square");
   const integer_t n = static_cast<integer_t>(prox.shape(0));
@@ -123,6 +134,7 @@ PYBIND11_MODULE(_rfxref, m) {
   m.def("gpu_proximity", &gpu_proximity, "per-tree OOB proximity accumulated into prox (n x n, column-major)");
   m.def("upper_triangle_fp16", &upper_triangle_fp16, "one tree's OOB fold as IEEE half bits, packed upper");
   m.def("finishprox", &finishprox, "rf::cpu_finishprox semantics");
+  m.def("proximity_rfgap", &proximity_rfgap, "rf::gpu_proximity_rfgap semantics over all trees");
   m.def("compute_mds_3d", &compute_mds_3d, py::arg("prox"), py::arg("oob_counts") = py::none(),
         py::arg("use_rfgap") = false);
   py::class_<LowRank>(m, "LowRankProximityMatrix")

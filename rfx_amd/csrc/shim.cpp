@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -141,7 +142,6 @@ void cpu_finishprox(integer_t nsample, const integer_t* nout, dp_t* prox, dp_t* 
 std::vector<double> compute_mds_3d_cpu(const dp_t* proximity_matrix, integer_t n_samples, bool,
                                        const integer_t* oob_counts_rfgap, bool use_rfgap) {
   if (!proximity_matrix || n_samples < 4) throw std::runtime_error("MDS needs a proximity matrix with n >= 4");
-  if (use_rfgap) throw std::runtime_error("RF-GAP normalisation is not supported by this MDS");
   const int64_t n = n_samples;
   const size_t nn = static_cast<size_t>(n) * n;
   DevBuf pd(8 * nn), od(oob_counts_rfgap ? 4 * static_cast<size_t>(n) : 0), cd(8 * 3 * static_cast<size_t>(n));
@@ -151,14 +151,57 @@ std::vector<double> compute_mds_3d_cpu(const dp_t* proximity_matrix, integer_t n
   DevBuf ws(wsb);
   double eig[16];
   int32_t iters = 0;
-  const int rc = rfx_mds_dense_matrix(nullptr, pd.as<double>(), n, od.as<int32_t>(), RFX_PREC_FP64, 1000, 1e-10, ws.p,
-                                      wsb, cd.as<double>(), eig, &iters);
+  const int rc = (use_rfgap && oob_counts_rfgap)
+                     ? rfx_mds_dense_rfgap(nullptr, pd.as<double>(), n, od.as<int32_t>(), RFX_PREC_FP64, 1000, 1e-10,
+                                           ws.p, wsb, cd.as<double>(), eig, &iters)
+                     : rfx_mds_dense_matrix(nullptr, pd.as<double>(), n, od.as<int32_t>(), RFX_PREC_FP64, 1000, 1e-10,
+                                            ws.p, wsb, cd.as<double>(), eig, &iters);
   if (rc == RFX_ERR_INVALID)
     throw std::runtime_error("MDS failed: fewer than 3 positive eigenvalues");
-  rfx_ok(rc, "rfx_mds_dense_matrix");
+  rfx_ok(rc, "rfx_mds_dense");
   std::vector<double> coords(3 * static_cast<size_t>(n));
   download(coords.data(), cd.p, coords.size());
   return coords;
+}
+
+void gpu_proximity_rfgap(integer_t ntree, integer_t nsample, const integer_t* tree_nin_flat,
+                         const integer_t* tree_nodextr_flat, dp_t* prox) {
+  if (ntree <= 0 || nsample <= 0 || !tree_nin_flat || !tree_nodextr_flat || !prox)
+    throw std::runtime_error("invalid RF-GAP arguments");
+  const size_t slots = static_cast<size_t>(ntree) * nsample, nn = static_cast<size_t>(nsample) * nsample;
+  // node offsets from each tree's largest reached node
+  std::vector<int64_t> off(static_cast<size_t>(ntree) + 1, 0);
+  for (integer_t t = 0; t < ntree; ++t) {
+    integer_t mx = -1;
+    for (integer_t i = 0; i < nsample; ++i) mx = std::max(mx, tree_nodextr_flat[static_cast<size_t>(t) * nsample + i]);
+    off[t + 1] = off[t] + std::max<int64_t>(mx + 1, 1);
+  }
+  DevBuf nd(4 * slots), xd(4 * slots), od(8 * off.size()), pd(8 * nn);
+  upload(nd.p, tree_nin_flat, slots);
+  upload(xd.p, tree_nodextr_flat, slots);
+  upload(od.p, off.data(), off.size());
+  const size_t wsb = rfx_prox_rfgap_workspace_bytes(nsample, ntree, off.back());
+  DevBuf ws(wsb);
+  rfx_ok(rfx_prox_rfgap(nullptr, nd.as<int32_t>(), xd.as<int32_t>(), od.as<int64_t>(), nsample, ntree, off.back(),
+                        pd.as<double>(), ws.p, wsb),
+         "rfx_prox_rfgap");
+  download(prox, pd.p, nn);
+}
+
+void cpu_proximity_rfgap(integer_t ntree, integer_t nsample, const std::vector<std::vector<integer_t>>& tree_nin,
+                         const std::vector<std::vector<integer_t>>& tree_nodextr, dp_t* prox) {
+  if (ntree <= 0 || nsample <= 0 || tree_nin.size() != static_cast<size_t>(ntree) ||
+      tree_nodextr.size() != static_cast<size_t>(ntree))
+    throw std::runtime_error("RF-GAP: per-tree arrays do not match ntree");
+  const size_t n = static_cast<size_t>(nsample);
+  std::vector<integer_t> nin(n * ntree), ndx(n * ntree);
+  for (integer_t t = 0; t < ntree; ++t) {
+    if (tree_nin[t].size() != n || tree_nodextr[t].size() != n)
+      throw std::runtime_error("RF-GAP: per-tree arrays do not match nsample");
+    std::copy(tree_nin[t].begin(), tree_nin[t].end(), nin.begin() + static_cast<size_t>(t) * n);
+    std::copy(tree_nodextr[t].begin(), tree_nodextr[t].end(), ndx.begin() + static_cast<size_t>(t) * n);
+  }
+  gpu_proximity_rfgap(ntree, nsample, nin.data(), ndx.data(), prox);
 }
 
 // ---------------------------------------------------------------------------

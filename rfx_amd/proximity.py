@@ -108,6 +108,25 @@ class Accumulator:
                                       int(accumulate), _dptr(out), _dptr(ws), ws.numel()), "rfx_prox_casewise")
         return out
 
+    def rfgap(self, nin: torch.Tensor, nodextr: torch.Tensor, node_off: torch.Tensor,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+        """RF-GAP proximity (the reference's cpu_proximity_rfgap /
+        gpu_proximity_rfgap): FP64 n x n, symmetric, as a flat tensor.  nin,
+        nodextr: device int32 [T, n]; node_off: device int64 [T + 1]."""
+        if nin.dtype != torch.int32 or nin.numel() != self.n * self.ntree:
+            raise ValueError("nin must be an int32 [T, n] tensor")
+        if nodextr.dtype != torch.int32 or nodextr.numel() != self.n * self.ntree:
+            raise ValueError("nodextr must be an int32 [T, n] tensor")
+        if node_off.dtype != torch.int64 or node_off.numel() != self.ntree + 1:
+            raise ValueError("node_off must be an int64 [T + 1] tensor")
+        if out is None:
+            out = torch.empty(self.n * self.n, dtype=torch.float64, device=self.device)
+        need = lib().rfx_prox_rfgap_workspace_bytes(self.n, self.ntree, self.total_nodes)
+        ws = self.ws if self.ws.numel() >= need else torch.empty(need, dtype=torch.uint8, device=self.device)
+        check(lib().rfx_prox_rfgap(_stream(), _dptr(nin), _dptr(nodextr), _dptr(node_off), self.n, self.ntree,
+                                   self.total_nodes, _dptr(out), _dptr(ws), ws.numel()), "rfx_prox_rfgap")
+        return out
+
     def accumulate(self, keys: torch.Tensor, variant: int, layout: int, algo: int = _lib.ALGO_AUTO,
                    out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
         if out is None:
@@ -178,6 +197,22 @@ def mds_dense_matrix(prox: torch.Tensor, n: int, nout: torch.Tensor | None = Non
     check(lib().rfx_mds_dense_matrix(_stream(), _dptr(prox), n, _dptr(nout) if nout is not None else None, precision,
                                      max_iter, tol, _dptr(ws), ws.numel(), _dptr(X), eig.ctypes.data, C.byref(it)),
           "rfx_mds_dense_matrix")
+    return X, eig, it.value
+
+
+def mds_dense_rfgap(prox: torch.Tensor, n: int, nout: torch.Tensor, precision: int = _lib.PREC_FP64,
+                    max_iter: int = 1000, tol: float = 1e-10) -> tuple[torch.Tensor, np.ndarray, int]:
+    """3-D classical MDS of a given FP64 RF-GAP proximity with the
+    reference's use_rfgap renormalisation by nout."""
+    import ctypes as C
+
+    nb = lib().rfx_mds_dense_workspace_bytes(n, precision)
+    ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=prox.device)
+    X = torch.empty((n, 3), dtype=torch.float64, device=prox.device)
+    eig = np.zeros(16, np.float64)
+    it = C.c_int32()
+    check(lib().rfx_mds_dense_rfgap(_stream(), _dptr(prox), n, _dptr(nout), precision, max_iter, tol, _dptr(ws),
+                                    ws.numel(), _dptr(X), eig.ctypes.data, C.byref(it)), "rfx_mds_dense_rfgap")
     return X, eig, it.value
 
 

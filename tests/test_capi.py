@@ -69,7 +69,8 @@ def test_workspace_size_formula():
 def test_reference_signature_module_loads():
     import rfx_amd._rfxref as R
 
-    for name in ("gpu_proximity", "upper_triangle_fp16", "finishprox", "compute_mds_3d", "LowRankProximityMatrix"):
+    for name in ("gpu_proximity", "upper_triangle_fp16", "finishprox", "compute_mds_3d", "LowRankProximityMatrix",
+                 "proximity_rfgap"):
         assert hasattr(R, name)
 
 
@@ -86,7 +87,8 @@ def test_classifier_options_mirror_reference():
     assert clf.rank == 32 and clf.quant_mode == "nf4" and clf.iseed == 12345
     assert set(QUANT_FORMATS) >= {"nf4", "int8", "fp16", "fp32"}
     with pytest.raises(NotImplementedError):
-        RandomForestClassifier(use_rfgap=True)
+        RandomForestClassifier(use_rfgap=True, use_qlora=True)
+    assert RandomForestClassifier(use_rfgap=True).use_rfgap
     with pytest.raises(NotImplementedError):
         RandomForestClassifier(use_casewise=True, use_qlora=True)
     assert RandomForestClassifier(use_casewise=True).use_casewise

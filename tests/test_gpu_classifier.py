@@ -79,3 +79,27 @@ def test_casewise_proximity_and_mds(gpu):
         assert O.embedding_errors(X, Xref)["rel_frobenius"] < 1e-5
     else:
         assert np.allclose(np.sort(np.linalg.norm(X, axis=0) ** 2), np.sort(w[:3]), rtol=1e-6)
+
+
+def test_rfgap_proximity_and_mds(gpu):
+    # use_rfgap: the RF-GAP matrix as float32 and the use_rfgap MDS chain,
+    # against the oracle (pinned bit for bit to the reference's
+    # cpu_proximity_rfgap) on the same forest
+    print("classifier rf-gap", flush=True)
+    g = load_golden("gauss_n200.npz")
+    x, y = g["x"], g["y"].astype(np.int32)
+    clf = RandomForestClassifier(ntree=30, mtry=4, iseed=2, compute_proximity=True, use_gpu=True, use_rfgap=True,
+                                 use_casewise=True)
+    clf.fit(x, y)
+    f = clf.forest
+    n = len(y)
+    Pref = O.prox_rfgap(f.nin.reshape(f.ntree, n), O.leaf_assign(f, x)).reshape(n, n, order="F")
+    P = clf.get_proximity_matrix()
+    assert P.dtype == np.float32 and P.shape == (n, n)
+    assert np.allclose(P, Pref.astype(np.float32), rtol=1e-6, atol=1e-7)
+    X = clf.compute_mds_3d_cpu()
+    Xref, w = O.classical_mds(O.mds_input_rfgap(Pref, f.nout(2)), 3)
+    if w[2] / w[3] > 1.02:
+        assert O.embedding_errors(X, Xref)["rel_frobenius"] < 1e-5
+    else:
+        assert np.allclose(np.sort(np.linalg.norm(X, axis=0) ** 2), np.sort(w[:3]), rtol=1e-6)

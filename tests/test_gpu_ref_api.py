@@ -86,8 +86,39 @@ def test_matrix_mds_matches_oracle(R):
     _, w2 = O.classical_mds(Sf, 3)
     assert np.allclose(lam2, w2[:3], rtol=1e-9)
     assert eig_residuals(centred(Sf), X2, lam2).max() < 1e-5
+    # use_rfgap keeps the diagonal of the divided rows instead of a unit one
+    X3 = R.compute_mds_3d(Sf, nout, True)
+    M3 = M.copy()
+    np.fill_diagonal(M3, np.where(nout > 0, np.diag(Sf) / np.where(nout > 0, nout, 1), np.diag(Sf)))
+    lam3 = np.sum(X3 * X3, axis=0)
+    _, w3 = O.classical_mds(M3, 3)
+    assert np.allclose(lam3, w3[:3], rtol=1e-9)
+    assert eig_residuals(centred(M3), X3, lam3).max() < 1e-5
+    # points on a line: D^2 double-centred has rank 1, fewer than three
+    # eigenvalues above 1e-10, and the reference throws
+    z = np.linspace(0.0, 1.0, 40)
+    D1 = np.abs(z[:, None] - z[None, :])
     with pytest.raises(RuntimeError):
-        R.compute_mds_3d(Sf, nout, True)  # RF-GAP is not part of this path
+        R.compute_mds_3d(np.asfortranarray(D1.max() - D1), None, False)
+
+
+@pytest.mark.parametrize("name", ["covtype_cat_n240.npz", "gauss_n200.npz"])
+def test_rfgap_reference_signature(R, name):
+    # rf::gpu_proximity_rfgap's signature (flattened per-tree nin and reached
+    # nodes, no node counts), against the oracle pinned to the reference's
+    # cpu_proximity_rfgap; then the use_rfgap MDS renormalisation
+    _say(f"rfgap reference signature {name}")
+    g = load_golden(name)
+    n = g["nin"].shape[1]
+    P = R.proximity_rfgap(g["nin"], g["nodextr"])
+    want = O.prox_rfgap(g["nin"], g["nodextr"]).reshape(n, n, order="F")
+    assert np.allclose(P, want, rtol=1e-12, atol=1e-14)
+    # the float32 matrix of the pybind get_proximity_matrix, as the classifier's MDS receives it
+    X = R.compute_mds_3d(np.asfortranarray(P.astype(np.float32).astype(np.float64)), g["nout"], True)
+    M = O.mds_input_rfgap(want, g["nout"])
+    _, w = O.classical_mds(M, 3)
+    assert np.allclose(np.sort(np.linalg.norm(X, axis=0) ** 2)[::-1], w[:3], rtol=1e-7)
+    assert eig_residuals(centred(M), X, np.linalg.norm(X, axis=0) ** 2).max() < 1e-5
 
 
 def test_lowrank_object_and_host_helpers(R):

@@ -179,6 +179,61 @@ def test_casewise_weights_sum_to_one_per_tree():
     assert np.all(pw[oob_trees == 0, :] == 0)
 
 
+@pytest.mark.parametrize("name", GOLDEN)
+def test_oracle_rfgap_matches_reference(name):
+    # the reference's own cpu_proximity_rfgap compiled from its sources, on the
+    # golden forests' nin and reached nodes (the categorical forest has cases
+    # that stop on inner nodes), against the restatement, bit for bit
+    if not O.ref_available():
+        pytest.skip("reference build oracle/_ref not present")
+    g = load_golden(name)
+    got = O.prox_rfgap(g["nin"], g["nodextr"])
+    assert np.array_equal(got, O.ref_proximity_rfgap(g["nin"], g["nodextr"]))
+
+
+def _rfgap_by_definition(nin: np.ndarray, nodextr: np.ndarray) -> np.ndarray:
+    # pGAP(i, j) = 1/|S_i| sum_{t in S_i} c_j(t) [j in J_i(t)] / |M_i(t)|
+    # (Rhodes, Cutler and Moon, 2023), evaluated densely per tree, then
+    # symmetrised; cases never out of bag keep a unit diagonal
+    T, n = nin.shape
+    p = np.zeros((n, n))
+    for t in range(T):
+        same = (nodextr[t][:, None] == nodextr[t][None, :]) & (nodextr[t][:, None] >= 0)
+        W = same * (nin[t] > 0)[None, :] * nin[t][None, :].astype(np.float64)
+        M = W.sum(axis=1)
+        rows = (nin[t] == 0) & (M > 0)
+        p[rows] += W[rows] / M[rows, None]
+    S = (nin == 0).sum(axis=0)
+    p[S > 0] /= S[S > 0, None]
+    P = 0.5 * (p + p.T)
+    P[np.diag_indices(n)] = np.where(S > 0, np.diag(p), 1.0)
+    return P, S
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_rfgap_definition_and_invariants(name):
+    g = load_golden(name)
+    nin, nd = g["nin"], g["nodextr"]
+    n = nin.shape[1]
+    got = O.prox_rfgap(nin, nd).reshape(n, n, order="F")
+    want, S = _rfgap_by_definition(nin, nd)
+    assert np.allclose(got, want, rtol=0, atol=1e-14)
+    assert np.array_equal(got, got.T)
+    assert np.all(np.diag(got)[S == 0] == 1.0) and np.all(np.diag(got)[S > 0] == 0.0)
+    # the symmetric average keeps the total: every out-of-bag case's weights
+    # over a node with in-bag cases sum to 1, so the total is sum_i (share of
+    # i's out-of-bag trees whose node holds in-bag cases) + #never out of bag
+    share = np.zeros(n)
+    for t in range(nin.shape[0]):
+        has = np.zeros(nd[t].max() + 1, bool)
+        has[nd[t][(nin[t] > 0) & (nd[t] >= 0)]] = True
+        ok = (nin[t] == 0) & (nd[t] >= 0)
+        ok[ok] &= has[nd[t][ok]]
+        share += ok
+    total = np.sum(share[S > 0] / S[S > 0]) + np.sum(S == 0)
+    assert abs(got.sum() - total) < 1e-9
+
+
 def test_mds_oracle_recovers_euclidean_configuration():
     # a proximity built from a Euclidean 3-D configuration: P = max - D  ->
     # classical MDS recovers the centred points up to an orthogonal map
