@@ -106,6 +106,8 @@ def main() -> int:
     ap.add_argument("--no-mds", action="store_true", help="c3: counts only, no factor / MDS section")
     ap.add_argument("--casewise", action="store_true",
                     help="c1/c2: also time the case-wise weighted FP64 proximity on the same keys")
+    ap.add_argument("--rfgap", action="store_true",
+                    help="c1/c2: also time the RF-GAP FP64 proximity of the same forest")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one device-to-device copy of 1 GiB (a known-size kernel that scales the HBM "
                          "byte counters of a profiler run)")
@@ -241,6 +243,8 @@ def main() -> int:
         line["cpu_baseline"] = cb
     if args.casewise and n <= 50000:
         line["casewise"] = casewise_section(acc, keys, forest, x, n, T, not args.no_cpu)
+    if args.rfgap and n <= 50000:
+        line["rfgap"] = rfgap_section(acc, df, forest, x, n, T, not args.no_cpu)
     if args.workload == "c3" and not args.no_mds:
         del keys
         out_all = out_oob = None
@@ -288,6 +292,49 @@ def casewise_section(acc, keys, forest, x, n: int, T: int, cpu: bool) -> dict:
                                "cores": int(O.num_threads()), "kind": "port",
                                "sample": f"first {trees} of {T} trees: oracle leaf assignment + the case-wise "
                                          f"rf::cpu_proximity branch (src/rf_proximity.cpp:186-223); {dt:.2f} s"}
+    return res
+
+
+def rfgap_section(acc, df, forest, x, n: int, T: int, cpu: bool) -> dict:
+    """The RF-GAP FP64 proximity (full n x n, symmetric) of the same forest:
+    one call (keys from the reached nodes, leaf buckets, leaf draws, column
+    gather, the mirrored-tile scaling and average) timed with HIP events, and
+    the oracle's restatement of the reference's cpu_proximity_rfgap on the
+    host cores for a bounded tree sample."""
+    import numpy as np
+    import torch
+
+    _, nodextr = df.leaf_assign(want_nodextr=True)
+    nin = torch.from_numpy(np.ascontiguousarray(forest.nin.reshape(T, n), np.int32)).to(df.device)
+    out = torch.empty(n * n, dtype=torch.float64, device=df.device)
+    acc.rfgap(nin, nodextr, df.node_off, out)
+    torch.cuda.synchronize()
+    reps = 5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        acc.rfgap(nin, nodextr, df.node_off, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    res = {"call_ms": round(ms, 4), "output": "FP64 n x n, symmetric",
+           "tree_pair_updates_per_s": T * float(n) * n / (ms / 1e3), "written_bytes": 2 * 8 * n * n}
+    del out
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import oracle as O
+
+        trees = max(1, min(T, 20))
+        sub = forest.subset(slice(0, trees))
+        t0 = time.perf_counter()
+        nd = O.leaf_assign(sub, x)
+        O.prox_rfgap(sub.nin.reshape(trees, n), nd)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": trees * float(n) * n / dt, "unit": "tree·pair updates/s",
+                               "cores": int(O.num_threads()), "kind": "port",
+                               "sample": f"first {trees} of {T} trees: oracle leaf assignment + "
+                                         f"rf::cpu_proximity_rfgap (src/rf_proximity.cpp:238-391); {dt:.2f} s"}
     return res
 
 

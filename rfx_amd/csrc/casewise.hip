@@ -9,13 +9,15 @@
 //
 // Built on the leaf buckets of the integer counts: the leaf's bootstrap draws
 // sum_k nin_t(k) come from one integer-atomic pass over the slots, then one
-// workgroup per output column j (in-bag j only contributes) gathers, chunk by
-// chunk of kCwChunk rows, the out-of-bag members of j's leaf in every tree
+// workgroup per (output column j, row chunk of at most 10,240 rows; in-bag j
+// only contributes) gathers the out-of-bag members of j's leaf in every tree
 // into FP64 LDS counters, with j's weight nin_t(j) / draws, and writes the
-// column once.  The member runs of a wavefront's 64 trees are flattened
-// across its lanes as in the integer gather.  FP64 LDS atomics add the
-// weights of different trees in no fixed order, so an entry may differ from
-// the reference's tree-ordered sum in its last bits (about 1e-16 relative).
+// chunk of the column once.  The member runs of a wavefront's 64 trees are
+// flattened across its lanes in 16-byte blocks of the member array (16-bit
+// members up to 32,768 cases), as in the integer gather.  FP64 LDS atomics
+// add the weights of different trees in no fixed order, so an entry may differ
+// from the reference's tree-ordered sum in its last bits (about 1e-16
+// relative).
 //
 // RF-GAP (src/rf_proximity.cpp:238-391, cuda/rf_proximity.cu:541-786) is the
 // same leaf sum with the weight formed as nin_t(j) * (1 / draws), followed by
@@ -31,7 +33,6 @@
 
 namespace {
 
-constexpr int kCwChunk = 4096;  // FP64 LDS counters per workgroup (32 KiB)
 constexpr int kCwUnroll = 4;
 
 // draws[leaf] = sum over the leaf's in-bag slots of nin
@@ -43,107 +44,141 @@ __global__ void cw_leaf_draws_kernel(const int32_t* __restrict__ keys, const int
   if (k >= 0 && (k & 1)) atomicAdd(&draws[k >> 1], static_cast<uint32_t>(nin[s]));
 }
 
-// kRecip: the weight as nin * (1 / draws) (RF-GAP) rather than nin / draws
-template <bool kRecip>
-__global__ __launch_bounds__(256) void cw_column_kernel(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
-                                                        const uint32_t* __restrict__ mbucket,
-                                                        const uint32_t* __restrict__ members,
-                                                        const uint32_t* __restrict__ draws,
-                                                        const int32_t* __restrict__ nin, int64_t n, int32_t ntree,
-                                                        int32_t accumulate, double* __restrict__ out) {
-  __shared__ double acc[kCwChunk];
-  const int64_t j = xcd_row(blockIdx.x);
-  if (j >= n) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int64_t lo = 0; lo < n; lo += kCwChunk) {
-    const int64_t hi = min(n, lo + kCwChunk);
-    const int len_chunk = static_cast<int>(hi - lo);
-    __syncthreads();
-    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0.0;
-    __syncthreads();
-    for (int32_t tb = wave * 64; tb < ntree; tb += 256) {
-      const int32_t t = tb + lane;
-      uint32_t s = 0, len = 0;
-      double w = 0.0;
-      if (t < ntree) {
-        const uint2 q = pe[static_cast<int64_t>(t) * n + j];
-        const uint32_t end = q.y & 0x7fffffffu;
-        if (end > 0 && (q.y >> 31)) {  // j in bag in tree t
-          const uint32_t leaf = mbucket[q.x];
-          const double c = static_cast<double>(nin[static_cast<int64_t>(t) * n + j]);
-          const double m = static_cast<double>(draws[leaf]);
-          w = kRecip ? c * (1.0 / m) : c / m;
-          const uint32_t bs = bstart[leaf];
-          const uint32_t b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
-          const uint32_t b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
-          s = b0;
-          len = b1 > b0 ? b1 - b0 : 0u;
-        }
+#ifndef RFX_CW_MAX_ROWS
+#define RFX_CW_MAX_ROWS 10240  // FP64 LDS counters per workgroup: 80 KiB
+#endif
+constexpr int kCwThreads = 512;
+constexpr int64_t kCwMaxRows = RFX_CW_MAX_ROWS;
+
+// The wavefront's 64 member runs (lane l: run [s, s + len) of its tree, with
+// that tree's weight w) flattened across the lanes in aligned 16-byte blocks
+// of the member array: one owner search and one 16-byte load per block, and
+// w added to acc[row - lo] for each out-of-bag member of the block inside the
+// run.  All lanes of the wavefront must call.
+template <class M>
+__device__ __forceinline__ void cw_visit(uint32_t s, uint32_t len, double w, int lane, const M* __restrict__ members,
+                                         double* acc, int64_t lo) {
+  constexpr uint32_t P = 16 / sizeof(M);
+  constexpr int U = kCwUnroll;
+  const uint32_t e = s + len;
+  const uint32_t nb = len ? (e - 1) / P - s / P + 1 : 0u;
+  uint32_t pre = nb;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(pre, d, 64);
+    if (lane >= d) pre += y;
+  }
+  const uint32_t excl = pre - nb;
+  const uint32_t total = __shfl(pre, 63, 64);
+  for (uint32_t g0 = 0; g0 < total; g0 += 64 * U) {
+    uint4 blk[U];
+    uint32_t blo[U], bhi[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = g0 + u * 64 + lane;
+      int o = 0;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t v = __shfl(pre, o + d - 1, 64);
+        if (v <= g) o += d;
       }
-      // the 64 runs flattened across the lanes: owner lane of element g by a
-      // 6-step search over the prefix sums of the run lengths
-      uint32_t pre = len;
+      o = min(o, 63);
+      const uint32_t so = __shfl(s, o, 64), eo = __shfl(e, o, 64), xo = __shfl(excl, o, 64);
+      wv[u] = __shfl(w, o, 64);
+      const bool ok = g < total;
+      const uint32_t bb = (so / P + (g - xo)) * P;  // first member index of the block
+      blo[u] = ok ? (so > bb ? so - bb : 0u) : P;
+      bhi[u] = ok ? min(P, eo - bb) : 0u;
+      blk[u] = ok ? *reinterpret_cast<const uint4*>(members + bb) : make_uint4(0u, 0u, 0u, 0u);
+    }
 #pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(pre, d, 64);
-        if (lane >= d) pre += y;
-      }
-      const uint32_t excl = pre - len;
-      const uint32_t total = __shfl(pre, 63, 64);
-      for (uint32_t g0 = 0; g0 < total; g0 += 64 * kCwUnroll) {
-        uint32_t mv[kCwUnroll];
-        double wv[kCwUnroll];
-        bool ok[kCwUnroll];
+    for (int u = 0; u < U; ++u) {
+      const uint32_t bw[4] = {blk[u].x, blk[u].y, blk[u].z, blk[u].w};
 #pragma unroll
-        for (int u = 0; u < kCwUnroll; ++u) {
-          const uint32_t g = g0 + u * 64 + lane;
-          int o = 0;
-#pragma unroll
-          for (int d = 32; d >= 1; d >>= 1) {
-            const uint32_t v = __shfl(pre, o + d - 1, 64);
-            if (v <= g) o += d;
-          }
-          o = min(o, 63);
-          const uint32_t so = __shfl(s, o, 64), eo = __shfl(excl, o, 64);
-          wv[u] = __shfl(w, o, 64);
-          ok[u] = g < total;
-          mv[u] = ok[u] ? members[so + (g - eo)] : 1u;
-        }
-#pragma unroll
-        for (int u = 0; u < kCwUnroll; ++u)
-          if (ok[u] && !(mv[u] & 1u)) atomicAdd(&acc[static_cast<int64_t>(mv[u] >> 1) - lo], wv[u]);
+      for (uint32_t k = 0; k < P; ++k) {
+        if (k < blo[u] || k >= bhi[u]) continue;
+        const uint32_t m = sizeof(M) == 2 ? ((bw[k / 2] >> (16 * (k % 2))) & 0xffffu) : bw[k];
+        if (!(m & 1u)) atomicAdd(&acc[static_cast<int64_t>(m >> 1) - lo], wv[u]);
       }
     }
-    __syncthreads();
-    double* col = out + j * n + lo;
-    for (int q = tid; q < len_chunk; q += 256) col[q] = accumulate ? col[q] + acc[q] : acc[q];
   }
+}
+
+// One workgroup per (in-bag line j, row chunk): the out-of-bag members of j's
+// leaf in every tree where j is in bag add j's weight in that tree to the FP64
+// LDS counters of the chunk's rows; the chunk of column j is written once.
+// kRecip: the weight as nin * (1 / draws) (RF-GAP) rather than nin / draws.
+template <bool kRecip, class M>
+__global__ __launch_bounds__(kCwThreads) void cw_line_kernel(const uint2* __restrict__ pe,
+                                                             const uint32_t* __restrict__ bstart,
+                                                             const uint32_t* __restrict__ mbucket,
+                                                             const M* __restrict__ members,
+                                                             const uint32_t* __restrict__ draws,
+                                                             const int32_t* __restrict__ nin, int64_t n, int32_t ntree,
+                                                             int64_t chunk_len, int32_t accumulate,
+                                                             double* __restrict__ out) {
+  extern __shared__ double cw_lds[];
+  double* acc = cw_lds;
+  const int64_t j = xcd_row(blockIdx.x);
+  const int64_t lo = static_cast<int64_t>(blockIdx.y) * chunk_len;
+  if (j >= n || lo >= n) return;
+  const int64_t hi = min(n, lo + chunk_len);
+  const int len_chunk = static_cast<int>(hi - lo);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int q = tid; q < len_chunk; q += kCwThreads) acc[q] = 0.0;
+  __syncthreads();
+  for (int32_t tb = wave * 64; tb < ntree; tb += kCwThreads) {
+    const int32_t t = tb + lane;
+    uint32_t s = 0, len = 0;
+    double w = 0.0;
+    if (t < ntree) {
+      const uint2 q = pe[static_cast<int64_t>(t) * n + j];
+      const uint32_t end = q.y & 0x7fffffffu;
+      if (end > 0 && (q.y >> 31)) {  // j in bag in tree t
+        const uint32_t leaf = mbucket[q.x];
+        const double c = static_cast<double>(nin[static_cast<int64_t>(t) * n + j]);
+        const double m = static_cast<double>(draws[leaf]);
+        w = kRecip ? c * (1.0 / m) : c / m;
+        const uint32_t bs = bstart[leaf];
+        const uint32_t b0 = lo > 0 ? member_lb(members, bs, end, static_cast<uint32_t>(lo)) : bs;
+        const uint32_t b1 = hi < n ? member_lb(members, b0, end, static_cast<uint32_t>(hi)) : end;
+        s = b0;
+        len = b1 > b0 ? b1 - b0 : 0u;
+      }
+    }
+    cw_visit(s, len, w, lane, members, acc, lo);
+  }
+  __syncthreads();
+  double* col = out + j * n + lo;
+  for (int q = tid; q < len_chunk; q += kCwThreads) col[q] = accumulate ? col[q] + acc[q] : acc[q];
 }
 
 // RF-GAP keys from the node each case reached, terminal or not: the
 // reference matches cases by equal node index (src/rf_proximity.cpp:
 // 320-355), so a case whose traversal stopped early still meets the cases
 // that stopped at the same node.  key = (node_off[t] + node) << 1 | in-bag,
-// -1 for a negative node index, which the reference skips.
-__global__ void rg_keys_kernel(const int32_t* __restrict__ nin, const int32_t* __restrict__ nodextr,
-                               const int64_t* __restrict__ node_off, int64_t n, int32_t* __restrict__ keys) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int32_t t = blockIdx.y;
-  if (i >= n) return;
-  const int64_t s = static_cast<int64_t>(t) * n + i;
-  const int32_t k = nodextr[s];
-  const int64_t nn = node_off[t + 1] - node_off[t];
-  keys[s] = (k >= 0 && k < nn) ? static_cast<int32_t>(((node_off[t] + k) << 1) | (nin[s] > 0 ? 1 : 0)) : -1;
-}
+// -1 for a negative node index, which the reference skips.  The same pass
+// counts the trees in which each case is out of bag (nin == 0) into oob[i]
+// (zeroed beforehand): thread (i, tree group) covers kRgTreeGroup trees.
+constexpr int kRgTreeGroup = 16;
 
-// oob[i] = number of trees in which case i is out of bag (nin == 0)
-__global__ void rg_oob_trees_kernel(const int32_t* __restrict__ nin, int64_t n, int32_t ntree,
-                                    uint32_t* __restrict__ oob) {
+__global__ void rg_keys_kernel(const int32_t* __restrict__ nin, const int32_t* __restrict__ nodextr,
+                               const int64_t* __restrict__ node_off, int64_t n, int32_t ntree,
+                               int32_t* __restrict__ keys, uint32_t* __restrict__ oob) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t c = 0;
-  for (int32_t t = 0; t < ntree; ++t) c += nin[static_cast<int64_t>(t) * n + i] == 0 ? 1u : 0u;
-  oob[i] = c;
+  const int32_t t0 = static_cast<int32_t>(blockIdx.y) * kRgTreeGroup;
+  const int32_t t1 = min(ntree, t0 + kRgTreeGroup);
+  uint32_t zeros = 0;
+  for (int32_t t = t0; t < t1; ++t) {
+    const int64_t s = static_cast<int64_t>(t) * n + i;
+    const int32_t k = nodextr[s], c = nin[s];
+    const int64_t nn = node_off[t + 1] - node_off[t];
+    keys[s] = (k >= 0 && k < nn) ? static_cast<int32_t>(((node_off[t] + k) << 1) | (c > 0 ? 1 : 0)) : -1;
+    zeros += c == 0 ? 1u : 0u;
+  }
+  if (zeros) atomicAdd(&oob[i], zeros);
 }
 
 constexpr int kRgTile = 64;
@@ -206,8 +241,16 @@ int weighted_leaf_sums(hipStream_t st, const int32_t* keys, const int32_t* nin, 
   if (rc) return rc;
   const int64_t slots = n * static_cast<int64_t>(ntree);
   hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, keys, nin, slots, draws);
-  hipLaunchKernelGGL(cw_column_kernel<kRecip>, dim3(static_cast<unsigned>(xcd_rows_padded(n))), dim3(256), 0, st,
-                     w.pe, w.bstart, w.mbucket, w.members, draws, nin, n, ntree, accumulate, out);
+  const int64_t nchunks = (n + kCwMaxRows - 1) / kCwMaxRows;
+  const int64_t chunk_len = (n + nchunks - 1) / nchunks;
+  const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), static_cast<unsigned>(nchunks));
+  const size_t lds = 8 * static_cast<size_t>(chunk_len);
+  if (w.members16)
+    hipLaunchKernelGGL((cw_line_kernel<kRecip, uint16_t>), grid, dim3(kCwThreads), lds, st, w.pe, w.bstart, w.mbucket,
+                       w.members16, draws, nin, n, ntree, chunk_len, accumulate, out);
+  else
+    hipLaunchKernelGGL((cw_line_kernel<kRecip, uint32_t>), grid, dim3(kCwThreads), lds, st, w.pe, w.bstart, w.mbucket,
+                       w.members, draws, nin, n, ntree, chunk_len, accumulate, out);
   return rfx_launch_status();
 }
 
@@ -241,17 +284,17 @@ extern "C" size_t rfx_prox_rfgap_workspace_bytes(int64_t n, int32_t ntree, int64
 extern "C" int rfx_prox_rfgap(hipStream_t st, const int32_t* nin, const int32_t* nodextr, const int64_t* node_off,
                               int64_t n, int32_t ntree, int64_t total_nodes, double* out, void* ws, size_t ws_bytes) {
   if (!nodextr || !node_off || !weighted_args_ok(nin, nin, out, ws, n, ntree, total_nodes)) return RFX_ERR_INVALID;
-  // grid y of the key kernel is the tree index
-  if (n * static_cast<int64_t>(ntree) >= (1LL << 31) || ntree > 65535) return RFX_ERR_UNSUPPORTED;
+  if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
   if (ws_bytes < rfx_prox_rfgap_workspace_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
   char* tail = static_cast<char*>(ws) + rfx_prox_casewise_workspace_bytes(n, ntree, total_nodes);
   uint32_t* oob = reinterpret_cast<uint32_t*>(tail);
   int32_t* keys = reinterpret_cast<int32_t*>(tail + align256(4 * static_cast<size_t>(n)));
-  hipLaunchKernelGGL(rg_keys_kernel, dim3(rfx_ceil_div(n, 256), static_cast<unsigned>(ntree)), dim3(256), 0, st, nin,
-                     nodextr, node_off, n, keys);
-  int rc = weighted_leaf_sums<true>(st, keys, nin, n, ntree, total_nodes, 0, out, ws);
+  int rc = fill_u32(st, oob, n, 0u);
   if (rc) return rc;
-  hipLaunchKernelGGL(rg_oob_trees_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, nin, n, ntree, oob);
+  hipLaunchKernelGGL(rg_keys_kernel, dim3(rfx_ceil_div(n, 256), rfx_ceil_div(ntree, kRgTreeGroup)), dim3(256), 0, st,
+                     nin, nodextr, node_off, n, ntree, keys, oob);
+  rc = weighted_leaf_sums<true>(st, keys, nin, n, ntree, total_nodes, 0, out, ws);
+  if (rc) return rc;
   const unsigned nt = static_cast<unsigned>(rfx_ceil_div(n, kRgTile));
   hipLaunchKernelGGL(rg_symmetric_kernel, dim3(nt, nt), dim3(256), 0, st, out, oob, n);
   return rfx_launch_status();
