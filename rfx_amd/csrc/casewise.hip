@@ -9,7 +9,7 @@
 //
 // Built on the leaf buckets of the integer counts: the leaf's bootstrap draws
 // sum_k nin_t(k) come from one integer-atomic pass over the slots, then one
-// workgroup per (output column j, row chunk of at most 10,240 rows; in-bag j
+// workgroup per (output column j, row chunk of at most 20,480 rows; in-bag j
 // only contributes) gathers the out-of-bag members of j's leaf in every tree
 // into FP64 LDS counters, with j's weight nin_t(j) / draws, and writes the
 // chunk of the column once.  The member runs of a wavefront's 64 trees are
@@ -35,19 +35,42 @@ namespace {
 
 constexpr int kCwUnroll = 4;
 
-// draws[leaf] = sum over the leaf's in-bag slots of nin
-__global__ void cw_leaf_draws_kernel(const int32_t* __restrict__ keys, const int32_t* __restrict__ nin, int64_t slots,
+// draws[leaf] = sum of nin over the leaf's in-bag slots, read from the sorted
+// slots of the bucket build (position p: leaf id K[p], V[p] = slot << 1 |
+// in-bag).  The positions of one leaf are contiguous, so a segmented scan
+// across the wavefront sums each leaf's run and the run's last lane adds it
+// with one atomic (two when a run crosses a wavefront boundary).  draws must
+// be zeroed; the off-leaf sentinel id (total_nodes) is skipped.
+__global__ void cw_leaf_draws_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V,
+                                     const int32_t* __restrict__ nin, int64_t slots, uint32_t total_nodes,
                                      uint32_t* __restrict__ draws) {
-  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= slots) return;
-  const int32_t k = keys[s];
-  if (k >= 0 && (k & 1)) atomicAdd(&draws[k >> 1], static_cast<uint32_t>(nin[s]));
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool in = p < slots;
+  const uint32_t k = in ? K[p] : total_nodes;
+  uint32_t sum = 0;
+  if (in) {
+    const uint32_t v = V[p];
+    if (v & 1u) sum = static_cast<uint32_t>(nin[v >> 1]);
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(sum, d, 64);
+    const uint32_t ky = __shfl_up(k, d, 64);
+    if (lane >= d && ky == k) sum += y;
+  }
+  const uint32_t knext = __shfl_down(k, 1, 64);
+  const bool last = lane == 63 || knext != k;
+  if (in && last && k < total_nodes && sum) atomicAdd(&draws[k], sum);
 }
 
 #ifndef RFX_CW_MAX_ROWS
-#define RFX_CW_MAX_ROWS 10240  // FP64 LDS counters per workgroup: 80 KiB
+#define RFX_CW_MAX_ROWS 20480  // FP64 LDS counters per workgroup: up to 160 KiB, a whole column to n = 20,480
 #endif
-constexpr int kCwThreads = 512;
+#ifndef RFX_CW_THREADS
+#define RFX_CW_THREADS 512
+#endif
+constexpr int kCwThreads = RFX_CW_THREADS;
 constexpr int64_t kCwMaxRows = RFX_CW_MAX_ROWS;
 
 // The wavefront's 64 member runs (lane l: run [s, s + len) of its tree, with
@@ -202,31 +225,45 @@ __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ 
   __shared__ double L[kRgTile][kRgTile + 1];  // L[b][a] = element (J0 + b, I0 + a)
   const int64_t i0 = I * kRgTile, j0 = J * kRgTile;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  constexpr int R = kRgTile / 4;  // tile rows per thread; each pass loads all of its rows first
   const double inv_b = (j0 + tx < n && oob[j0 + tx] > 0) ? 1.0 / static_cast<double>(oob[j0 + tx]) : 1.0;
   const double inv_a = (i0 + tx < n && oob[i0 + tx] > 0) ? 1.0 / static_cast<double>(oob[i0 + tx]) : 1.0;
+  const bool row_b = j0 + tx < n, row_a = i0 + tx < n;
+  double v[R];
   // (J, I) tile: row J0 + b (b = tx, contiguous), column I0 + a; scaled by its row's 1 / S
-  for (int a = ty; a < kRgTile; a += 4) {
-    const int64_t r = j0 + tx, c = i0 + a;
-    L[tx][a] = (r < n && c < n) ? w[r + c * n] * inv_b : 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int64_t c = i0 + ty + 4 * k;
+    v[k] = (row_b && c < n) ? w[(j0 + tx) + c * n] : 0.0;
   }
+#pragma unroll
+  for (int k = 0; k < R; ++k) L[tx][ty + 4 * k] = v[k] * inv_b;
   __syncthreads();
   // (I, J) tile: row I0 + a (a = tx, contiguous), column J0 + b
-  for (int b = ty; b < kRgTile; b += 4) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int64_t c = j0 + ty + 4 * k;
+    v[k] = (row_a && c < n) ? w[(i0 + tx) + c * n] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int b = ty + 4 * k;
     const int64_t r = i0 + tx, c = j0 + b;
-    if (r >= n || c >= n) continue;
-    double v;
+    if (!row_a || c >= n) continue;
+    double x;
     if (r == c)
-      v = oob[r] > 0 ? w[r + c * n] * inv_a : 1.0;
+      x = oob[r] > 0 ? v[k] * inv_a : 1.0;
     else
-      v = (w[r + c * n] * inv_a + L[b][tx]) / 2.0;
-    w[r + c * n] = v;
-    L[b][tx] = v;
+      x = (v[k] * inv_a + L[b][tx]) / 2.0;
+    w[r + c * n] = x;
+    L[b][tx] = x;
   }
   __syncthreads();
   if (I == J) return;  // the diagonal tile is its own mirror: written above
-  for (int a = ty; a < kRgTile; a += 4) {
-    const int64_t r = j0 + tx, c = i0 + a;
-    if (r < n && c < n) w[r + c * n] = L[tx][a];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int64_t c = i0 + ty + 4 * k;
+    if (row_b && c < n) w[(j0 + tx) + c * n] = L[tx][ty + 4 * k];
   }
 }
 
@@ -240,7 +277,8 @@ int weighted_leaf_sums(hipStream_t st, const int32_t* keys, const int32_t* nin, 
   if (!rc) rc = fill_u32(st, draws, total_nodes, 0u);
   if (rc) return rc;
   const int64_t slots = n * static_cast<int64_t>(ntree);
-  hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, keys, nin, slots, draws);
+  hipLaunchKernelGGL(cw_leaf_draws_kernel, dim3(rfx_ceil_div(slots, 256)), dim3(256), 0, st, w.mbucket, w.V[0], nin,
+                     slots, static_cast<uint32_t>(total_nodes), draws);
   const int64_t nchunks = (n + kCwMaxRows - 1) / kCwMaxRows;
   const int64_t chunk_len = (n + nchunks - 1) / nchunks;
   const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n)), static_cast<unsigned>(nchunks));
