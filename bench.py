@@ -65,29 +65,39 @@ def measured_traffic(key: str):
     return int(b), e["summary"]
 
 
-def cpu_baseline(forest, x, sample_trees: int) -> dict:
-    """The oracle port of the reference CPU path (per tree: leaf assignment,
-    then rf::cpu_proximity's accumulating OpenMP branch) on sample_trees trees."""
-    import numpy as np
+def _repeat_for(fn, min_seconds: float) -> tuple[int, float]:
+    """Run fn until at least min_seconds have elapsed; (passes, seconds)."""
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        passes += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            return passes, dt
 
+
+def cpu_baseline(forest, x, sample_trees: int, min_seconds: float = 10.0) -> dict:
+    """The oracle port of the reference CPU path (per tree: leaf assignment,
+    then rf::cpu_proximity's accumulating OpenMP branch) on sample_trees trees,
+    repeated until about min_seconds of host work have been timed."""
     from oracle import oracle as O
 
     sub = forest.subset(slice(0, sample_trees))
     n = forest.n
-    t0 = time.perf_counter()
-    nd = O.leaf_assign(sub, x)
-    prox = O.prox_v1(sub, nd)
-    dt = time.perf_counter() - t0
-    del prox, nd
+
+    def one():
+        O.prox_v1(sub, O.leaf_assign(sub, x))
+
+    passes, dt = _repeat_for(one, min_seconds)
     P = n * (n + 1) // 2
     return {
-        "value": sample_trees * P / dt,
+        "value": passes * sample_trees * P / dt,
         "unit": "tree·pair updates/s",
         "cores": int(O.num_threads()),
         "kind": "port",
-        "sample": f"first {sample_trees} of {forest.ntree} trees at n={n}: oracle leaf assignment "
-                  f"(src/rf_testreebag.cpp, find_terminal_node) + rf::cpu_proximity dense OpenMP branch "
-                  f"(src/rf_proximity.cpp:99-231) into an n x n float64 matrix; {dt:.2f} s",
+        "sample": f"{passes} passes over the first {sample_trees} of {forest.ntree} trees at n={n}: oracle leaf "
+                  f"assignment (src/rf_testreebag.cpp, find_terminal_node) + rf::cpu_proximity dense OpenMP "
+                  f"branch (src/rf_proximity.cpp:99-231) into an n x n float64 matrix; {dt:.2f} s",
         "seconds": dt,
     }
 
@@ -282,16 +292,12 @@ def casewise_section(acc, keys, forest, x, n: int, T: int, cpu: bool) -> dict:
     if cpu:
         from oracle import oracle as O
 
-        trees = max(1, min(T, 20))
-        sub = forest.subset(slice(0, trees))
-        t0 = time.perf_counter()
-        nd = O.leaf_assign(sub, x)
-        O.prox_v1(sub, nd, casewise=True)
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": trees * float(n) * n / dt, "unit": "tree·pair updates/s",
+        passes, dt = _repeat_for(lambda: O.prox_v1(forest, O.leaf_assign(forest, x), casewise=True), 5.0)
+        res["cpu_baseline"] = {"value": passes * T * float(n) * n / dt, "unit": "tree·pair updates/s",
                                "cores": int(O.num_threads()), "kind": "port",
-                               "sample": f"first {trees} of {T} trees: oracle leaf assignment + the case-wise "
-                                         f"rf::cpu_proximity branch (src/rf_proximity.cpp:186-223); {dt:.2f} s"}
+                               "sample": f"{passes} passes over all {T} trees: oracle leaf assignment + the "
+                                         f"case-wise rf::cpu_proximity branch (src/rf_proximity.cpp:186-223); "
+                                         f"{dt:.2f} s"}
     return res
 
 
@@ -325,15 +331,10 @@ def rfgap_section(acc, df, forest, x, n: int, T: int, cpu: bool) -> dict:
     if cpu:
         from oracle import oracle as O
 
-        trees = max(1, min(T, 20))
-        sub = forest.subset(slice(0, trees))
-        t0 = time.perf_counter()
-        nd = O.leaf_assign(sub, x)
-        O.prox_rfgap(sub.nin.reshape(trees, n), nd)
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": trees * float(n) * n / dt, "unit": "tree·pair updates/s",
+        passes, dt = _repeat_for(lambda: O.prox_rfgap(forest.nin.reshape(T, n), O.leaf_assign(forest, x)), 5.0)
+        res["cpu_baseline"] = {"value": passes * T * float(n) * n / dt, "unit": "tree·pair updates/s",
                                "cores": int(O.num_threads()), "kind": "port",
-                               "sample": f"first {trees} of {T} trees: oracle leaf assignment + "
+                               "sample": f"{passes} passes over all {T} trees: oracle leaf assignment + "
                                          f"rf::cpu_proximity_rfgap (src/rf_proximity.cpp:238-391); {dt:.2f} s"}
     return res
 

@@ -232,6 +232,18 @@ int rfx_finishprox(hipStream_t stream, const int32_t* counts, int32_t in_layout,
 int rfx_finishprox_f64(hipStream_t stream, const double* prox, const int32_t* nout, int64_t n, double* proxsym,
                        int32_t out_layout);
 
+/* Outlier scores of a finished proximity matrix.  Replaces
+ * RandomForest::compute_outlier_scores (src/rf_random_forest.cpp:2489-2520;
+ * results read by get_outlier_scores, include/rf_random_forest.hpp:210):
+ * s_i = 1 / sum_j prox[i n + j]^2 (0 if the sum is not positive), summed in
+ * float in j order with each step rounding fma(p, p, (double)sum) to float, as
+ * the reference's GNU C++ -O3 build evaluates it; then scores_i = s_i / m with
+ * m = the (n/2)-th smallest s (no division when m <= 0).  prox: device FP64
+ * n x n (the finishprox output or the RF-GAP matrix); scores: device float32
+ * [n]; median_host may be NULL.  Bit-identical to the reference.  Synchronises
+ * the stream (the median is selected on the host). */
+int rfx_outlier_scores(hipStream_t stream, const double* prox, int64_t n, float* scores, float* median_host);
+
 /* ---- MDS ---------------------------------------------------------------- */
 #define RFX_MDS_COUNTS 0    /* M_ij = c_ij * scale (e.g. ALL counts / T)       */
 #define RFX_MDS_REFERENCE 1 /* OOB_UPPER counts -> finishprox -> float32 -> nout renormalisation */

@@ -79,6 +79,10 @@ void cpu_proximity_rfgap(integer_t ntree, integer_t nsample, const std::vector<s
 void gpu_proximity_rfgap(integer_t ntree, integer_t nsample, const integer_t* tree_nin_flat,
                          const integer_t* tree_nodextr_flat, dp_t* prox);
 
+// RandomForest::compute_outlier_scores (src/rf_random_forest.cpp:2489-2520)
+// on a host float64 n x n finished proximity: the n float scores, bit-identical.
+std::vector<float> compute_outlier_scores(const dp_t* proximity_matrix, integer_t nsample);
+
 // Storage levels of the factors (the reference's QuantizationLevel names
 // FP32/FP16/INT8/NF4; here the stored formats of rfx_quantize_factors).
 enum class FactorFormat : int32_t { FP32 = RFX_PREC_FP32, BF16 = RFX_PREC_BF16, INT8 = RFX_PREC_INT8,

@@ -25,6 +25,10 @@
  *                       nin(j) * (1 / node draws) summed per (i, j) in tree
  *                       order, rows scaled by 1 / |S_i|, then the symmetric
  *                       average; cases matched by equal reached node
+ *   orc_outlier_scores  src/rf_random_forest.cpp:2489-2520: float row sums
+ *                       of squares (each step float(fma(p, p, (double)sum)),
+ *                       the contraction the reference's GNU C++ -O3 build
+ *                       makes), 1/sum, divided by the (n/2)-th smallest
  *   orc_counts_bucket   the ALL / OOB_ASYM / OOB_UPPER integer counts by leaf
  *                       buckets (ALL has no reference implementation; it is
  *                       pinned by brute force and closed-form invariants)
@@ -32,6 +36,7 @@
  *                       OOB_ROW (variant 4) follows cuda/rf_proximity.cu:32-67
  *   orc_finishprox      src/rf_finishprox.cpp:12-74
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -280,6 +285,33 @@ int orc_prox_rfgap(const int32_t* nin, const int32_t* nodextr, int64_t n, int32_
   free(memb);
   free(start);
   free(draws);
+  return 0;
+}
+
+static int cmp_float(const void* a, const void* b) {
+  const float x = *(const float*)a, y = *(const float*)b;
+  return (x > y) - (x < y);
+}
+
+/* Outlier scores of a finished n x n FP64 proximity, read as the reference
+ * reads it (prox[i * n + j]); out: n floats. */
+int orc_outlier_scores(const double* prox, int64_t n, float* out) {
+  float* sorted = (float*)malloc(sizeof(float) * (size_t)n);
+  if (!sorted) return -2;
+  for (int64_t i = 0; i < n; ++i) {
+    float sum = 0.0f;
+    for (int64_t j = 0; j < n; ++j) {
+      const double v = prox[i * n + j];
+      sum = (float)fma(v, v, (double)sum);
+    }
+    out[i] = sum > 0.0f ? 1.0f / sum : 0.0f;
+    sorted[i] = out[i];
+  }
+  qsort(sorted, (size_t)n, sizeof(float), cmp_float);
+  const float median = sorted[n / 2];
+  if (median > 0.0f)
+    for (int64_t i = 0; i < n; ++i) out[i] /= median;
+  free(sorted);
   return 0;
 }
 

@@ -50,10 +50,12 @@ def orc() -> C.CDLL:
         L.orc_prox_v1.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
         L.orc_prox_v1_casewise.argtypes = [_p, _p, _p, _p, _i64, _i32, _p]
         L.orc_prox_rfgap.argtypes = [_p, _p, _i64, _i32, _p]
+        L.orc_outlier_scores.argtypes = [_p, _i64, _p]
         L.orc_counts_bucket.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]
         L.orc_counts_brute.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _p]
         L.orc_finishprox.argtypes = [_i64, _p, _p, _p]
-        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_prox_rfgap, L.orc_counts_bucket,
+        for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_prox_rfgap, L.orc_outlier_scores,
+                  L.orc_counts_bucket,
                   L.orc_counts_brute, L.orc_finishprox):
             f.restype = _i32
         L.orc_num_threads.restype = _i32
@@ -123,6 +125,16 @@ def prox_rfgap(nin: np.ndarray, nodextr: np.ndarray) -> np.ndarray:
     prox = np.empty(n * n, np.float64)
     _ok(orc().orc_prox_rfgap(_a(nin), _a(nodextr), n, T, _a(prox)), "orc_prox_rfgap")
     return prox
+
+
+def outlier_scores(prox: np.ndarray) -> np.ndarray:
+    """Outlier scores (float32 [n]) of a finished FP64 n x n proximity, given
+    as the flat buffer the reference holds."""
+    prox = np.ascontiguousarray(prox, np.float64).reshape(-1)
+    n = int(round(np.sqrt(prox.size)))
+    out = np.empty(n, np.float32)
+    _ok(orc().orc_outlier_scores(_a(prox), n, _a(out)), "orc_outlier_scores")
+    return out
 
 
 def counts(forest, nodextr: np.ndarray, variant: int, layout: int) -> np.ndarray:

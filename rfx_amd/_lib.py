@@ -102,6 +102,7 @@ _SIGS = {
                                      _p, _p, _p]),
     "rfx_mds_dense_matrix": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
     "rfx_mds_dense_rfgap": (_i32, [_p, _p, _i64, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
+    "rfx_outlier_scores": (_i32, [_p, _p, _i64, _p, _p]),
     "rfx_counts_convert": (_i32, [_p, _p, _i64, _i32, _p]),
     "rfx_batch_create": (_i32, [_p, _i32, _i32, C.POINTER(_p)]),
     "rfx_batch_workspace_bytes": (_sz, [_p]),

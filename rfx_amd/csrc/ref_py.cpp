@@ -73,6 +73,15 @@ F64F proximity_rfgap(I32 nin, I32 nodextr) {
   return out;
 }
 
+py::array_t<float> outlier_scores(F64F prox) {
+  need(prox.ndim() == 2 && prox.shape(0) == prox.shape(1), "prox must be square");
+  const integer_t n = static_cast<integer_t>(prox.shape(0));
+  std::vector<float> s = rfx::ref::compute_outlier_scores(prox.data(), n);
+  py::array_t<float> out(static_cast<py::ssize_t>(n));
+  std::copy(s.begin(), s.end(), out.mutable_data());
+  return out;
+}
+
 py::array_t<double> compute_mds_3d(F64F prox, py::object oob_counts, bool use_rfgap) {
   need(prox.ndim() == 2 && prox.shape(0) == prox.shape(1), "prox must be square");
   const integer_t n = static_cast<integer_t>(prox.shape(0));
@@ -135,6 +144,7 @@ PYBIND11_MODULE(_rfxref, m) {
   m.def("upper_triangle_fp16", &upper_triangle_fp16, "one tree's OOB fold as IEEE half bits, packed upper");
   m.def("finishprox", &finishprox, "rf::cpu_finishprox semantics");
   m.def("proximity_rfgap", &proximity_rfgap, "rf::gpu_proximity_rfgap semantics over all trees");
+  m.def("outlier_scores", &outlier_scores, "RandomForest::compute_outlier_scores semantics");
   m.def("compute_mds_3d", &compute_mds_3d, py::arg("prox"), py::arg("oob_counts") = py::none(),
         py::arg("use_rfgap") = false);
   py::class_<LowRank>(m, "LowRankProximityMatrix")

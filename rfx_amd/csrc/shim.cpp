@@ -204,6 +204,17 @@ void cpu_proximity_rfgap(integer_t ntree, integer_t nsample, const std::vector<s
   gpu_proximity_rfgap(ntree, nsample, nin.data(), ndx.data(), prox);
 }
 
+std::vector<float> compute_outlier_scores(const dp_t* proximity_matrix, integer_t nsample) {
+  if (!proximity_matrix || nsample <= 0) throw std::runtime_error("invalid outlier-score arguments");
+  const size_t nn = static_cast<size_t>(nsample) * nsample;
+  DevBuf pd(8 * nn), sd(4 * static_cast<size_t>(nsample));
+  upload(pd.p, proximity_matrix, nn);
+  rfx_ok(rfx_outlier_scores(nullptr, pd.as<double>(), nsample, sd.as<float>(), nullptr), "rfx_outlier_scores");
+  std::vector<float> out(static_cast<size_t>(nsample));
+  download(out.data(), sd.p, out.size());
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // LowRankProximityMatrix
 // ---------------------------------------------------------------------------

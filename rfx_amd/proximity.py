@@ -216,6 +216,19 @@ def mds_dense_rfgap(prox: torch.Tensor, n: int, nout: torch.Tensor, precision: i
     return X, eig, it.value
 
 
+def outlier_scores(prox: torch.Tensor, n: int) -> tuple[torch.Tensor, float]:
+    """Outlier scores of a finished FP64 n x n proximity (the reference's
+    RandomForest::compute_outlier_scores): float32 [n] and the median used."""
+    import ctypes as C
+
+    if prox.dtype != torch.float64 or prox.numel() != n * n:
+        raise ValueError("prox must be a float64 n x n tensor")
+    out = torch.empty(n, dtype=torch.float32, device=prox.device)
+    med = C.c_float()
+    check(lib().rfx_outlier_scores(_stream(), _dptr(prox), n, _dptr(out), C.byref(med)), "rfx_outlier_scores")
+    return out, med.value
+
+
 def unpack_upper(u: np.ndarray, n: int) -> np.ndarray:
     """Packed row-major upper triangle -> symmetric dense (host helper)."""
     m = np.zeros((n, n), dtype=u.dtype)

@@ -70,7 +70,7 @@ def test_reference_signature_module_loads():
     import rfx_amd._rfxref as R
 
     for name in ("gpu_proximity", "upper_triangle_fp16", "finishprox", "compute_mds_3d", "LowRankProximityMatrix",
-                 "proximity_rfgap"):
+                 "proximity_rfgap", "outlier_scores"):
         assert hasattr(R, name)
 
 

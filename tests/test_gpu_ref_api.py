@@ -114,6 +114,7 @@ def test_rfgap_reference_signature(R, name):
     want = O.prox_rfgap(g["nin"], g["nodextr"]).reshape(n, n, order="F")
     assert np.allclose(P, want, rtol=1e-12, atol=1e-14)
     # the float32 matrix of the pybind get_proximity_matrix, as the classifier's MDS receives it
+    assert np.array_equal(R.outlier_scores(P), O.outlier_scores(np.asfortranarray(P).reshape(-1, order="F")))
     X = R.compute_mds_3d(np.asfortranarray(P.astype(np.float32).astype(np.float64)), g["nout"], True)
     M = O.mds_input_rfgap(want, g["nout"])
     _, w = O.classical_mds(M, 3)

@@ -13,7 +13,7 @@ import torch
 from oracle import oracle as O
 from rfx_amd import _lib
 from rfx_amd.forest import Forest, make_dataset
-from rfx_amd.proximity import Accumulator, DeviceForest, mds_dense_rfgap
+from rfx_amd.proximity import Accumulator, DeviceForest, finishprox_f64, mds_dense_rfgap, outlier_scores
 
 from helpers import centred, eig_residuals, load_golden
 
@@ -128,3 +128,26 @@ def test_rfgap_rejects_bad_shapes(gpu):
         acc.rfgap(torch.zeros(10, dtype=torch.int32, device=df.device), nodextr, df.node_off)
     with pytest.raises(ValueError):
         acc.rfgap(nin, nodextr, df.node_off[:2])
+
+
+@pytest.mark.parametrize("kind,n,p,ncl,T,seed", [(_lib.DATA_GAUSS_MIXTURE, 2500, 12, 4, 21, 3),
+                                                 (_lib.DATA_COVTYPE_LIKE, 4100, 54, 7, 30, 5)])
+def test_outlier_scores_bitwise(gpu, kind, n, p, ncl, T, seed):
+    # RandomForest::compute_outlier_scores on the RF-GAP matrix and on the
+    # finishprox output of the OOB counts, bit for bit against the oracle
+    print(f"outlier scores n={n} T={T}", flush=True)
+    x, y = make_dataset(kind, n, p, ncl, seed)
+    f = Forest.grow(x, y, T, iseed=seed)
+    P, _ = _device_rfgap(f, x)
+    got, med = outlier_scores(P, n)
+    want = O.outlier_scores(P.cpu().numpy())
+    assert np.array_equal(got.cpu().numpy(), want)
+    assert med > 0
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(n, T, f.total_nodes)
+    C = acc.accumulate(keys, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL)
+    nout = torch.from_numpy(f.nout(2)).to(gpu)
+    S = finishprox_f64(C.double(), nout, n, _lib.LAYOUT_FULL)
+    got, _ = outlier_scores(S, n)
+    assert np.array_equal(got.cpu().numpy(), O.outlier_scores(S.cpu().numpy()))

@@ -234,6 +234,26 @@ def test_rfgap_definition_and_invariants(name):
     assert abs(got.sum() - total) < 1e-9
 
 
+def test_outlier_scores_closed_forms():
+    # diagonal proximities of powers of two keep every step exact: raw score
+    # 1/d^2, divided by the (n/2)-th smallest (index 2 of 4, not 1)
+    P = np.diag([1.0, 2.0, 4.0, 8.0])
+    assert np.array_equal(O.outlier_scores(P), np.array([4.0, 1.0, 0.25, 0.0625], np.float32))
+    # a row of zeros scores 0, and its 0 takes part in the median: sorted
+    # raw scores [0, 1/64, 1/16, 1/4], median 1/16
+    P = np.diag([0.0, 2.0, 4.0, 8.0])
+    assert np.array_equal(O.outlier_scores(P), np.array([0.0, 4.0, 1.0, 0.25], np.float32))
+    # a power-of-two scale of the matrix leaves the scores unchanged bit for bit
+    rng = np.random.default_rng(3)
+    A = rng.random((301, 301))
+    S = 0.5 * (A + A.T)
+    assert np.array_equal(O.outlier_scores(S), O.outlier_scores(4.0 * S))
+    # against the float64 definition: 1 / sum_j P_ij^2 over the median
+    raw = 1.0 / np.sum(S * S, axis=1)
+    want = raw / np.sort(raw)[301 // 2]
+    assert np.allclose(O.outlier_scores(S), want, rtol=2e-5, atol=0)
+
+
 def test_mds_oracle_recovers_euclidean_configuration():
     # a proximity built from a Euclidean 3-D configuration: P = max - D  ->
     # classical MDS recovers the centred points up to an orthogonal map
