@@ -623,11 +623,288 @@ int bucket_sort_trees(hipStream_t st, const int32_t* keys, int64_t n, int32_t nt
   return rfx_launch_status();
 }
 
+// ---------------------------------------------------------------------------
+// One-workgroup-per-tree build for n <= kTreeLdsMaxN: a stable counting sort
+// of the tree's n slots by local leaf id in LDS, writing the same arrays as
+// the radix path (mbucket, V[0], members, members16, pe, bstart, bend,
+// nvalid) bit for bit.
+//
+// local = leaf id - tmin for a slot on a leaf, and nleaf (one past the
+// largest) for an off-leaf slot, so those end the tree's segment.  A counting
+// pass sorts by a digit: the samples are cut into B contiguous blocks
+// (B <= 8 wavefronts) and 16-bit counters c[digit][block] live in LDS, two to
+// a 32-bit word, beside the 16-bit permutation perm[position] = slot << 1 |
+// bag that the pass fills:
+//   A  each wavefront counts its block, 64 samples a step: the lanes of a
+//      step with the same digit are found with one ballot per digit bit, and
+//      the lowest such lane adds their number with one LDS atomic;
+//   B  one exclusive scan over the counters in digit-major order turns them
+//      into first positions (< n, so still 16-bit);
+//   C  each wavefront walks its block again: a slot's position is its
+//      counter plus its rank among the step's lanes of the same digit, and
+//      the lowest lane advances the counter.
+// Samples are taken in order throughout, so a pass is stable.  The sorted
+// arrays are then written along the positions (coalesced) from the
+// permutation, and pe along the slots.  A tree's ids span at most its node
+// count; when the counters cannot hold that many digits (trees grown on more
+// samples than are assigned here), a first pass on the low 15 bits leaves
+// its order in the spare key buffer and a second pass sorts by the rest.
+constexpr int64_t kTreeLdsMaxN = 32767;
+constexpr int kTreeLdsThreads = 512;
+constexpr int kTreeLdsWaves = kTreeLdsThreads / 64;
+constexpr uint32_t kTreeLdsBytes = 159 * 1024;  // counters + the 16-bit permutation of the tree's slots
+constexpr int kTreeLowBits = 15;                 // low digit of the two-pass form
+
+__device__ __forceinline__ uint32_t lds_half(const uint32_t* words, uint32_t e) {
+  return (words[e >> 1] >> (16 * (e & 1u))) & 0xffffu;
+}
+
+// Lanes of the wavefront (among `active`) whose digit equals this lane's,
+// comparing the low nbits bits.
+__device__ __forceinline__ uint64_t same_digit_lanes(uint32_t d, int nbits, uint64_t active) {
+  uint64_t m = active;
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t on = __ballot(bit);
+    m &= bit ? on : ~on;
+  }
+  return m;
+}
+
+// One stable counting pass over elements q = 0 .. n - 1: get(q) fetches an
+// element's token (kTlPrefetch steps of tokens are fetched before they are
+// used, so the loads overlap), digit(token) < ndig, and place(q, token, pos)
+// receives its position.  On return the counter of (digit d, last block)
+// holds the end of d's positions.  All threads must call.
+constexpr int kTlPrefetch = 8;
+
+template <class GetF, class DigitF, class PlaceF>
+__device__ void tl_count_pass(uint32_t* words, uint32_t* wsum, int64_t n, uint32_t ndig, int B, GetF get,
+                              DigitF digit, PlaceF place) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int nbits = 0;
+  while ((1u << nbits) < ndig) ++nbits;
+  const uint32_t nwords = (ndig * static_cast<uint32_t>(B) + 1) >> 1;
+  const int64_t steps = (n + 63) / 64;
+  const int64_t steps_per = (steps + B - 1) / B;
+  const int64_t st0 = wave * steps_per, st1 = min(steps, (wave + 1) * steps_per);
+  __syncthreads();
+  for (uint32_t q = tid; q < nwords; q += kTreeLdsThreads) words[q] = 0u;
+  __syncthreads();
+  if (wave < B) {  // A
+    for (int64_t sb = st0; sb < st1; sb += kTlPrefetch) {
+      uint32_t tok[kTlPrefetch];
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        const int64_t q = (sb + u) * 64 + lane;
+        tok[u] = (sb + u < st1 && q < n) ? get(q) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        if (sb + u >= st1) break;  // uniform across the wavefront
+        const int64_t q = (sb + u) * 64 + lane;
+        const bool in = q < n;
+        const uint32_t d = in ? digit(tok[u]) : 0u;
+        const uint64_t m = same_digit_lanes(d, nbits, __ballot(in));
+        if (in && (m & ((1ull << lane) - 1)) == 0) {
+          const uint32_t e = d * B + wave;
+          atomicAdd(&words[e >> 1], static_cast<uint32_t>(__popcll(m)) << (16 * (e & 1u)));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // B: each thread owns a contiguous run of whole words
+  const uint32_t per = (nwords + kTreeLdsThreads - 1) / kTreeLdsThreads;
+  const uint32_t w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
+  uint32_t local = 0;
+  for (uint32_t q = w0; q < w1; ++q) local += (words[q] & 0xffffu) + (words[q] >> 16);
+  uint32_t incl = local;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t run = incl - local;
+  for (int q = 0; q < wave; ++q) run += wsum[q];
+  for (uint32_t q = w0; q < w1; ++q) {
+    const uint32_t a = words[q] & 0xffffu, b = words[q] >> 16;
+    words[q] = run | ((run + a) << 16);
+    run += a + b;
+  }
+  __syncthreads();
+  if (wave < B) {  // C
+    for (int64_t sb = st0; sb < st1; sb += kTlPrefetch) {
+      uint32_t tok[kTlPrefetch];
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        const int64_t q = (sb + u) * 64 + lane;
+        tok[u] = (sb + u < st1 && q < n) ? get(q) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        if (sb + u >= st1) break;
+        const int64_t q = (sb + u) * 64 + lane;
+        const bool in = q < n;
+        const uint32_t d = in ? digit(tok[u]) : 0u;
+        const uint64_t m = same_digit_lanes(d, nbits, __ballot(in));
+        const uint32_t e = d * B + wave;
+        const uint32_t first = in ? lds_half(words, e) : 0u;
+        const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1)));
+        if (in && rank == 0) atomicAdd(&words[e >> 1], static_cast<uint32_t>(__popcll(m)) << (16 * (e & 1u)));
+        if (in) place(q, tok[u], first + rank);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
+    const int32_t* __restrict__ keys, int64_t n, uint32_t total_nodes, uint32_t* __restrict__ order,
+    uint32_t* __restrict__ mbucket, uint32_t* __restrict__ V0, uint32_t* __restrict__ members,
+    uint16_t* __restrict__ members16, uint2* __restrict__ pe, uint32_t* __restrict__ bstart,
+    uint32_t* __restrict__ bend, uint32_t* __restrict__ nvalid) {
+  extern __shared__ uint32_t tl_words[];
+  __shared__ uint32_t red_lo[kTreeLdsWaves], red_hi[kTreeLdsWaves], wsum[kTreeLdsWaves];
+  const int64_t t = blockIdx.x;
+  const int64_t base = t * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t* kt = keys + base;
+  // LDS: perm[pos] = i << 1 | bag (16 bits, n <= 32767), then the counters
+  const uint32_t perm_words = static_cast<uint32_t>((n + 1) / 2);
+  uint16_t* perm = reinterpret_cast<uint16_t*>(tl_words);
+  uint32_t* words = tl_words + perm_words;
+  const uint32_t emax = 2 * (kTreeLdsBytes / 4 - perm_words);  // counter entries that fit
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  for (int64_t i = tid; i < n; i += kTreeLdsThreads) {
+    const int32_t k = kt[i];
+    if (k >= 0) {
+      lo = min(lo, static_cast<uint32_t>(k) >> 1);
+      hi = max(hi, static_cast<uint32_t>(k) >> 1);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = min(lo, static_cast<uint32_t>(__shfl_xor(lo, d, 64)));
+    hi = max(hi, static_cast<uint32_t>(__shfl_xor(hi, d, 64)));
+  }
+  if (lane == 0) {
+    red_lo[wave] = lo;
+    red_hi[wave] = hi;
+  }
+  __syncthreads();
+  lo = red_lo[0];
+  hi = red_hi[0];
+  for (int q = 1; q < kTreeLdsWaves; ++q) {
+    lo = min(lo, red_lo[q]);
+    hi = max(hi, red_hi[q]);
+  }
+  if (t == 0 && tid == 0) *nvalid = static_cast<uint32_t>(n * gridDim.x);
+  const uint32_t nleaf = lo <= hi ? hi - lo + 1 : 0u;  // local ids 0 .. nleaf - 1; nleaf = off-leaf
+  const auto local_of = [&](int32_t k) -> uint32_t { return k >= 0 ? (static_cast<uint32_t>(k) >> 1) - lo : nleaf; };
+  // a slot's position: pe.x now (written along the slots), the permutation in LDS
+  const auto place_final = [&](uint32_t i, int32_t k, uint32_t pos) {
+    if (pos >= n) return;  // cannot happen: positions come from a scan of n counts
+    const uint32_t bag = k >= 0 ? (static_cast<uint32_t>(k) & 1u) : 0u;
+    perm[pos] = static_cast<uint16_t>((i << 1) | bag);
+    pe[base + i].x = static_cast<uint32_t>(base + pos);
+  };
+  const auto get_key = [&](int64_t q) -> uint32_t { return static_cast<uint32_t>(kt[q]); };
+  const bool one_pass = nleaf + 1 <= emax;
+  int B = 1;
+  if (one_pass) {
+    const uint32_t ndig = nleaf + 1;
+    B = static_cast<int>(min<uint32_t>(kTreeLdsWaves, emax / ndig));
+    tl_count_pass(
+        words, wsum, n, ndig, B, get_key, [&](uint32_t k) { return local_of(static_cast<int32_t>(k)); },
+        [&](int64_t q, uint32_t k, uint32_t pos) { place_final(static_cast<uint32_t>(q), static_cast<int32_t>(k), pos); });
+  } else {
+    // leaf ids spanning more values than the counters hold: the low bits,
+    // with the order kept in the spare key buffer, then the rest
+    uint32_t* ord = order + base;
+    constexpr uint32_t kLow = 1u << kTreeLowBits;
+    tl_count_pass(
+        words, wsum, n, kLow, 1, get_key,
+        [&](uint32_t k) { return local_of(static_cast<int32_t>(k)) & (kLow - 1); },
+        [&](int64_t q, uint32_t, uint32_t pos) {
+          if (pos < n) ord[pos] = static_cast<uint32_t>(q);
+        });
+    const uint32_t ndig2 = (nleaf >> kTreeLowBits) + 1;
+    tl_count_pass(
+        words, wsum, n, ndig2, static_cast<int>(min<uint32_t>(kTreeLdsWaves, emax / ndig2)),
+        [&](int64_t q) -> uint32_t { return ord[q]; },
+        [&](uint32_t i) { return local_of(kt[i]) >> kTreeLowBits; },
+        [&](int64_t, uint32_t i, uint32_t pos) { place_final(i, kt[i], pos); });
+  }
+  // the sorted segment, written along the positions
+  for (int64_t p = tid; p < n; p += kTreeLdsThreads) {
+    const uint32_t v = perm[p];
+    const uint32_t i = v >> 1, bag = v & 1u;
+    if (i >= n) continue;  // cannot happen: every position is filled by the pass
+    const int32_t k = kt[i];
+    mbucket[base + p] = k >= 0 ? (static_cast<uint32_t>(k) >> 1) : total_nodes;
+    V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
+    members[base + p] = v;
+    if (members16) members16[base + p] = static_cast<uint16_t>(v);
+  }
+  if (one_pass) {
+    // leaf d holds [end(d - 1), end(d)), end(d) = the counter of (d, last block)
+    for (uint32_t d = tid; d < nleaf; d += kTreeLdsThreads) {
+      const uint32_t s0 = d > 0 ? lds_half(words, (d - 1) * B + (B - 1)) : 0u;
+      const uint32_t s1 = lds_half(words, d * B + (B - 1));
+      if (s1 > s0) {
+        bstart[lo + d] = static_cast<uint32_t>(base + s0);
+        bend[lo + d] = static_cast<uint32_t>(base + s1);
+      }
+    }
+    for (int64_t i = tid; i < n; i += kTreeLdsThreads) {
+      const int32_t k = kt[i];
+      if (k < 0)
+        pe[base + i] = make_uint2(0u, 0u);
+      else
+        pe[base + i].y = static_cast<uint32_t>(base + lds_half(words, local_of(k) * B + (B - 1))) |
+                         ((static_cast<uint32_t>(k) & 1u) << 31);
+    }
+    return;
+  }
+  // two-pass form: leaf ranges and pe ends from the sorted positions
+  const auto leaf_at = [&](int64_t p) -> int64_t {
+    const uint32_t i = perm[p] >> 1u;
+    if (i >= n) return -1;  // cannot happen, as above
+    const int32_t k = kt[i];
+    return k >= 0 ? static_cast<int64_t>(static_cast<uint32_t>(k) >> 1) : -1;
+  };
+  for (int64_t p = tid; p < n; p += kTreeLdsThreads) {
+    const int64_t g = leaf_at(p);
+    if (g < 0) continue;
+    if (p == 0 || leaf_at(p - 1) != g) bstart[g] = static_cast<uint32_t>(base + p);
+    if (p + 1 == n || leaf_at(p + 1) != g) bend[g] = static_cast<uint32_t>(base + p + 1);
+  }
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += kTreeLdsThreads) {
+    const int32_t k = kt[i];
+    if (k < 0)
+      pe[base + i] = make_uint2(0u, 0u);
+    else
+      pe[base + i].y = bend[static_cast<uint32_t>(k) >> 1] | ((static_cast<uint32_t>(k) & 1u) << 31);
+  }
+}
+
 // Full bucket build from the leaf keys [T][n] (see the file comment).
 int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                  const BucketWs& w) {
   const int64_t N = n * static_cast<int64_t>(ntree);
   if (!w.per_tree) return RFX_ERR_INVALID;
+  if (n <= kTreeLdsMaxN) {
+    int rc = fill_u32(st, w.bstart, total_nodes, 0u);
+    if (!rc) rc = fill_u32(st, w.bend, total_nodes, 0u);
+    if (rc) return rc;
+    hipLaunchKernelGGL(rs_tree_lds_kernel, dim3(static_cast<unsigned>(ntree)), dim3(kTreeLdsThreads), kTreeLdsBytes, st, keys, n, static_cast<uint32_t>(total_nodes), w.K[0], w.mbucket,
+                       w.V[0], w.members, w.members16, w.pe, w.bstart, w.bend, w.nvalid);
+    return rfx_launch_status();
+  }
   int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.V[0], N,

@@ -1,0 +1,59 @@
+"""GPU leaf-bucket build: the one-workgroup-per-tree counting sort (n up to
+32,767) checked through the counts of the BUCKET and GATHER kernels, which
+read the buckets, against the oracle and against the DENSE kernel, which does
+not.  Covers the case of trees with more than 65,535 nodes (trees grown on
+more samples than are assigned), whose leaf ids need two counting passes,
+and both sides of the size limit of that build."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rfx_amd import _lib
+from rfx_amd.forest import Forest, make_dataset
+from rfx_amd.proximity import Accumulator, DeviceForest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_wide_leaf_range_two_passes(gpu):
+    # 2 trees grown on 200,000 noisy samples (> 65,535 nodes each), assigned
+    # to the first 3,000 samples: each tree's reached leaf ids span more than
+    # 65,536 values
+    print("wide leaf range: grow 200k", flush=True)
+    x, y = make_dataset(_lib.DATA_OVERLAP_MIXTURE, 200000, 8, 3, 11)
+    g = Forest.grow(x, y, 2, mtry=3, iseed=11)
+    n = 3000
+    f = Forest(n, 8, g.node_offset, g.nodestatus, g.bestvar, g.xbestsplit, g.treemap, g.nodeclass,
+               np.ascontiguousarray(g.nin.reshape(2, -1)[:, :n]))
+    xs = np.ascontiguousarray(x[:n])
+    nd = O.leaf_assign(f, xs)
+    assert all(int(nd[t].max()) - int(nd[t].min()) > 65536 for t in range(2))
+    df = DeviceForest(f, xs)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(n, 2, f.total_nodes)
+    for variant in (_lib.PROX_ALL, _lib.PROX_OOB_ASYM):
+        want = O.counts(f, nd, variant, _lib.LAYOUT_FULL)
+        for algo in (_lib.ALGO_BUCKET, _lib.ALGO_GATHER):
+            got = acc.accumulate(keys, variant, _lib.LAYOUT_FULL, algo)
+            assert np.array_equal(got.cpu().numpy(), want), (variant, algo)
+
+
+@pytest.mark.parametrize("n", [32767, 32768])
+def test_build_size_limit(gpu, n):
+    # the last size of the per-tree counting sort and the first of the radix
+    # sort: bucket-based counts equal the DENSE kernel's
+    print(f"build size limit n={n}", flush=True)
+    x, y = make_dataset(_lib.DATA_COVTYPE_LIKE, n, 54, 7, 2)
+    f = Forest.grow(x, y, 3, mtry=7, iseed=2)
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(n, 3, f.total_nodes)
+    a_d, o_d = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_DENSE)
+    a_b, o_b = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET)
+    assert torch.equal(a_b, a_d) and torch.equal(o_b, o_d)
+    del a_b, o_b
+    a_g, o_g = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+    assert torch.equal(a_g, a_d) and torch.equal(o_g, o_d)
