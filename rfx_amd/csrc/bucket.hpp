@@ -635,14 +635,13 @@ int bucket_sort_trees(hipStream_t st, const int32_t* keys, int64_t n, int32_t nt
 // (B <= 8 wavefronts) and 16-bit counters c[digit][block] live in LDS, two to
 // a 32-bit word, beside the 16-bit permutation perm[position] = slot << 1 |
 // bag that the pass fills:
-//   A  each wavefront counts its block, 64 samples a step: the lanes of a
-//      step with the same digit are found with one ballot per digit bit, and
-//      the lowest such lane adds their number with one LDS atomic;
+//   A  each wavefront counts its block with one LDS atomic per sample;
 //   B  one exclusive scan over the counters in digit-major order turns them
 //      into first positions (< n, so still 16-bit);
-//   C  each wavefront walks its block again: a slot's position is its
-//      counter plus its rank among the step's lanes of the same digit, and
-//      the lowest lane advances the counter.
+//   C  each wavefront walks its block again, 64 samples a step: the lanes
+//      of a step with the same digit are found with one ballot per digit bit;
+//      a slot's position is its counter plus its rank among them, and the
+//      lowest such lane advances the counter.
 // Samples are taken in order throughout, so a pass is stable.  The sorted
 // arrays are then written along the positions (coalesced) from the
 // permutation, and pe along the slots.  A tree's ids span at most its node
@@ -703,12 +702,9 @@ __device__ void tl_count_pass(uint32_t* words, uint32_t* wsum, int64_t n, uint32
       for (int u = 0; u < kTlPrefetch; ++u) {
         if (sb + u >= st1) break;  // uniform across the wavefront
         const int64_t q = (sb + u) * 64 + lane;
-        const bool in = q < n;
-        const uint32_t d = in ? digit(tok[u]) : 0u;
-        const uint64_t m = same_digit_lanes(d, nbits, __ballot(in));
-        if (in && (m & ((1ull << lane) - 1)) == 0) {
-          const uint32_t e = d * B + wave;
-          atomicAdd(&words[e >> 1], static_cast<uint32_t>(__popcll(m)) << (16 * (e & 1u)));
+        if (q < n) {
+          const uint32_t e = digit(tok[u]) * B + wave;
+          atomicAdd(&words[e >> 1], 1u << (16 * (e & 1u)));  // counts need no ranks
         }
       }
     }
@@ -838,16 +834,27 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
         [&](uint32_t i) { return local_of(kt[i]) >> kTreeLowBits; },
         [&](int64_t, uint32_t i, uint32_t pos) { place_final(i, kt[i], pos); });
   }
-  // the sorted segment, written along the positions
-  for (int64_t p = tid; p < n; p += kTreeLdsThreads) {
-    const uint32_t v = perm[p];
-    const uint32_t i = v >> 1, bag = v & 1u;
-    if (i >= n) continue;  // cannot happen: every position is filled by the pass
-    const int32_t k = kt[i];
-    mbucket[base + p] = k >= 0 ? (static_cast<uint32_t>(k) >> 1) : total_nodes;
-    V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
-    members[base + p] = v;
-    if (members16) members16[base + p] = static_cast<uint16_t>(v);
+  // the sorted segment, written along the positions (kTlPrefetch positions
+  // per thread at a time, so the key gathers overlap)
+  for (int64_t p0 = tid; p0 < n; p0 += static_cast<int64_t>(kTreeLdsThreads) * kTlPrefetch) {
+    uint32_t v[kTlPrefetch];
+    int32_t k[kTlPrefetch];
+#pragma unroll
+    for (int u = 0; u < kTlPrefetch; ++u) {
+      const int64_t p = p0 + static_cast<int64_t>(u) * kTreeLdsThreads;
+      v[u] = p < n ? perm[p] : 0u;
+      k[u] = (p < n && (v[u] >> 1) < n) ? kt[v[u] >> 1] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kTlPrefetch; ++u) {
+      const int64_t p = p0 + static_cast<int64_t>(u) * kTreeLdsThreads;
+      const uint32_t i = v[u] >> 1, bag = v[u] & 1u;
+      if (p >= n || i >= n) continue;  // i < n always: every position is filled by the pass
+      mbucket[base + p] = k[u] >= 0 ? (static_cast<uint32_t>(k[u]) >> 1) : total_nodes;
+      V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
+      members[base + p] = v[u];
+      if (members16) members16[base + p] = static_cast<uint16_t>(v[u]);
+    }
   }
   if (one_pass) {
     // leaf d holds [end(d - 1), end(d)), end(d) = the counter of (d, last block)
@@ -859,13 +866,23 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
         bend[lo + d] = static_cast<uint32_t>(base + s1);
       }
     }
-    for (int64_t i = tid; i < n; i += kTreeLdsThreads) {
-      const int32_t k = kt[i];
-      if (k < 0)
-        pe[base + i] = make_uint2(0u, 0u);
-      else
-        pe[base + i].y = static_cast<uint32_t>(base + lds_half(words, local_of(k) * B + (B - 1))) |
-                         ((static_cast<uint32_t>(k) & 1u) << 31);
+    for (int64_t i0 = tid; i0 < n; i0 += static_cast<int64_t>(kTreeLdsThreads) * kTlPrefetch) {
+      int32_t k[kTlPrefetch];
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        const int64_t i = i0 + static_cast<int64_t>(u) * kTreeLdsThreads;
+        k[u] = i < n ? kt[i] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kTlPrefetch; ++u) {
+        const int64_t i = i0 + static_cast<int64_t>(u) * kTreeLdsThreads;
+        if (i >= n) continue;
+        if (k[u] < 0)
+          pe[base + i] = make_uint2(0u, 0u);
+        else
+          pe[base + i].y = static_cast<uint32_t>(base + lds_half(words, local_of(k[u]) * B + (B - 1))) |
+                           ((static_cast<uint32_t>(k[u]) & 1u) << 31);
+      }
     }
     return;
   }
