@@ -147,9 +147,10 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 // LDS-staged form: the workgroup first copies its samples' feature rows
 // into LDS ([p][LB] floats, coalesced column reads), then every traversal
 // step reads the feature from LDS, so the divergent global gathers of a step
-// are the node records alone.  Used while p * LB * 4 bytes fit kStageMaxLds;
-// at n = 200,000 (p = 54, 100 trees) it takes 1.70 ms against 2.49 ms for
-// the global-feature kernel.
+// are the node records alone.  Used while p * LB * 4 bytes fit kStageMaxLds.
+// At n = 20,000, p = 54, T = 500: 8 wavefronts of 8 trees each (4 in
+// lockstep) per tile take 0.33 ms, one wavefront of 16 trees (8 in lockstep)
+// per tile 0.51 ms -- the single-wavefront tiles bounded occupancy by LDS.
 #ifndef RFX_LEAF_STAGE
 #define RFX_LEAF_STAGE 1
 #endif
@@ -157,17 +158,24 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 #define RFX_LEAF_STAGE_LB 64
 #endif
 #ifndef RFX_LEAF_STAGE_TREES
-#define RFX_LEAF_STAGE_TREES 16
+#define RFX_LEAF_STAGE_TREES 8
 #endif
 #ifndef RFX_LEAF_STAGE_ILP
-#define RFX_LEAF_STAGE_ILP 8
+#define RFX_LEAF_STAGE_ILP 4
+#endif
+#ifndef RFX_LEAF_STAGE_WAVES
+#define RFX_LEAF_STAGE_WAVES 8
 #endif
 constexpr int kStageLb = RFX_LEAF_STAGE_LB;
-constexpr int kStageTrees = RFX_LEAF_STAGE_TREES;
+constexpr int kStageTrees = RFX_LEAF_STAGE_TREES;  // trees per wavefront
 constexpr int kStageIlp = RFX_LEAF_STAGE_ILP;
+// Wavefronts per workgroup: they share the workgroup's LDS feature tile and
+// take consecutive groups of kStageTrees trees, so the tile's LDS is spread
+// over several wavefronts (occupancy is otherwise bounded by LDS).
+constexpr int kStageWaves = RFX_LEAF_STAGE_WAVES;
 constexpr int64_t kStageMaxLds = 64 * 1024;
 
-__global__ __launch_bounds__(kStageLb) void leaf_assign_staged_kernel(const float* __restrict__ x, int64_t n,
+__global__ __launch_bounds__(kStageLb * kStageWaves) void leaf_assign_staged_kernel(const float* __restrict__ x, int64_t n,
                                                                       int32_t p, const uint2* __restrict__ nodes,
                                                                       const int64_t* __restrict__ node_off,
                                                                       int32_t ntree,
@@ -177,14 +185,14 @@ __global__ __launch_bounds__(kStageLb) void leaf_assign_staged_kernel(const floa
   extern __shared__ float xs[];
   int64_t sb, tg;
   if (!leaf_grid_map(g, blockIdx.x, sb, tg)) return;  // uniform over the workgroup
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x % kStageLb, wave = threadIdx.x / kStageLb;
   const int64_t i = sb * kStageLb + tid;
   const bool in = i < n;
-  for (int32_t f = 0; f < p; ++f) xs[f * kStageLb + tid] = in ? x[static_cast<int64_t>(f) * n + i] : 0.0f;
+  for (int32_t f = wave; f < p; f += kStageWaves) xs[f * kStageLb + tid] = in ? x[static_cast<int64_t>(f) * n + i] : 0.0f;
   __syncthreads();
-  if (!in) return;
-  const int32_t tb0 = static_cast<int32_t>(tg) * kStageTrees;
+  const int32_t tb0 = (static_cast<int32_t>(tg) * kStageWaves + wave) * kStageTrees;
   const int32_t tb1 = min(tb0 + kStageTrees, ntree);
+  if (!in || tb0 >= tb1) return;
   const float* xr = xs + tid;
   leaf_walk_trees<kStageIlp>([&](uint32_t f) { return xr[f * kStageLb]; }, i, n, nodes, node_off, tb0, tb1, inbag,
                              nodextr, keys, 0);
@@ -202,10 +210,10 @@ extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n
   if (cat == nullptr) {
     const int64_t lds = static_cast<int64_t>(p) * kStageLb * 4;
     if (RFX_LEAF_STAGE && lds <= kStageMaxLds) {
-      const LeafGrid g = leaf_grid(n, p, ntree, kStageLb, kStageTrees);
+      const LeafGrid g = leaf_grid(n, p, ntree, kStageLb, kStageTrees * kStageWaves);
       const int64_t blocks = 8 * g.nsbx * g.ntg;
       if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
-      hipLaunchKernelGGL(leaf_assign_staged_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kStageLb),
+      hipLaunchKernelGGL(leaf_assign_staged_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kStageLb * kStageWaves),
                          static_cast<unsigned>(lds), stream, x_col, n, p, reinterpret_cast<const uint2*>(nodes),
                          node_off, ntree, inbag, nodextr, keys, g);
       return rfx_launch_status();
