@@ -151,51 +151,68 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 // At n = 20,000, p = 54, T = 500: 8 wavefronts of 8 trees each (4 in
 // lockstep) per tile take 0.33 ms, one wavefront of 16 trees (8 in lockstep)
 // per tile 0.51 ms -- the single-wavefront tiles bounded occupancy by LDS.
+// At n = 200,000 (about 54,000 nodes per tree) 4 wavefronts of 2 trees take
+// 13.1 ms per 1,000 trees, one wavefront of 16 trees 17.4 ms and 8 x 8 trees
+// 23.7 ms: there the node records of a tree group must stay in L2.
 #ifndef RFX_LEAF_STAGE
 #define RFX_LEAF_STAGE 1
 #endif
 #ifndef RFX_LEAF_STAGE_LB
 #define RFX_LEAF_STAGE_LB 64
 #endif
-#ifndef RFX_LEAF_STAGE_TREES
-#define RFX_LEAF_STAGE_TREES 8
-#endif
-#ifndef RFX_LEAF_STAGE_ILP
-#define RFX_LEAF_STAGE_ILP 4
-#endif
-#ifndef RFX_LEAF_STAGE_WAVES
-#define RFX_LEAF_STAGE_WAVES 8
-#endif
 constexpr int kStageLb = RFX_LEAF_STAGE_LB;
-constexpr int kStageTrees = RFX_LEAF_STAGE_TREES;  // trees per wavefront
-constexpr int kStageIlp = RFX_LEAF_STAGE_ILP;
-// Wavefronts per workgroup: they share the workgroup's LDS feature tile and
-// take consecutive groups of kStageTrees trees, so the tile's LDS is spread
-// over several wavefronts (occupancy is otherwise bounded by LDS).
-constexpr int kStageWaves = RFX_LEAF_STAGE_WAVES;
 constexpr int64_t kStageMaxLds = 64 * 1024;
+// Wavefronts per workgroup W share the workgroup's LDS feature tile and take
+// consecutive groups of TPW trees each, ILP of them in lockstep.  Small trees
+// (the forest grown on up to kStageSmallN samples) use 8 x 8 trees per
+// workgroup; larger trees fewer per workgroup, so the node records that the
+// sample blocks of an XCD chunk share stay within its L2.
+#ifndef RFX_LEAF_SMALL_CFG
+#define RFX_LEAF_SMALL_CFG 8, 8, 4
+#endif
+#ifndef RFX_LEAF_BIG_CFG
+#define RFX_LEAF_BIG_CFG 4, 2, 2
+#endif
+#ifndef RFX_LEAF_SMALL_N
+#define RFX_LEAF_SMALL_N 65536
+#endif
+constexpr int64_t kStageSmallN = RFX_LEAF_SMALL_N;
 
-__global__ __launch_bounds__(kStageLb * kStageWaves) void leaf_assign_staged_kernel(const float* __restrict__ x, int64_t n,
-                                                                      int32_t p, const uint2* __restrict__ nodes,
-                                                                      const int64_t* __restrict__ node_off,
-                                                                      int32_t ntree,
-                                                                      const uint32_t* __restrict__ inbag,
-                                                                      int32_t* __restrict__ nodextr,
-                                                                      int32_t* __restrict__ keys, LeafGrid g) {
+template <int W, int TPW, int ILP>
+__global__ __launch_bounds__(kStageLb * W) void leaf_assign_staged_kernel(const float* __restrict__ x, int64_t n,
+                                                                          int32_t p, const uint2* __restrict__ nodes,
+                                                                          const int64_t* __restrict__ node_off,
+                                                                          int32_t ntree,
+                                                                          const uint32_t* __restrict__ inbag,
+                                                                          int32_t* __restrict__ nodextr,
+                                                                          int32_t* __restrict__ keys, LeafGrid g) {
   extern __shared__ float xs[];
   int64_t sb, tg;
   if (!leaf_grid_map(g, blockIdx.x, sb, tg)) return;  // uniform over the workgroup
   const int tid = threadIdx.x % kStageLb, wave = threadIdx.x / kStageLb;
   const int64_t i = sb * kStageLb + tid;
   const bool in = i < n;
-  for (int32_t f = wave; f < p; f += kStageWaves) xs[f * kStageLb + tid] = in ? x[static_cast<int64_t>(f) * n + i] : 0.0f;
+  for (int32_t f = wave; f < p; f += W) xs[f * kStageLb + tid] = in ? x[static_cast<int64_t>(f) * n + i] : 0.0f;
   __syncthreads();
-  const int32_t tb0 = (static_cast<int32_t>(tg) * kStageWaves + wave) * kStageTrees;
-  const int32_t tb1 = min(tb0 + kStageTrees, ntree);
+  const int32_t tb0 = (static_cast<int32_t>(tg) * W + wave) * TPW;
+  const int32_t tb1 = min(tb0 + TPW, ntree);
   if (!in || tb0 >= tb1) return;
   const float* xr = xs + tid;
-  leaf_walk_trees<kStageIlp>([&](uint32_t f) { return xr[f * kStageLb]; }, i, n, nodes, node_off, tb0, tb1, inbag,
-                             nodextr, keys, 0);
+  leaf_walk_trees<ILP>([&](uint32_t f) { return xr[f * kStageLb]; }, i, n, nodes, node_off, tb0, tb1, inbag, nodextr,
+                       keys, 0);
+}
+
+template <int W, int TPW, int ILP>
+int launch_staged(hipStream_t stream, const float* x_col, int64_t n, int32_t p, const uint32_t* nodes,
+                  const int64_t* node_off, int32_t ntree, const uint32_t* inbag, int32_t* nodextr, int32_t* keys) {
+  const int64_t lds = static_cast<int64_t>(p) * kStageLb * 4;
+  const LeafGrid g = leaf_grid(n, p, ntree, kStageLb, TPW * W);
+  const int64_t blocks = 8 * g.nsbx * g.ntg;
+  if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((leaf_assign_staged_kernel<W, TPW, ILP>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(kStageLb * W), static_cast<unsigned>(lds), stream, x_col, n, p,
+                     reinterpret_cast<const uint2*>(nodes), node_off, ntree, inbag, nodextr, keys, g);
+  return rfx_launch_status();
 }
 
 }  // namespace
@@ -210,13 +227,9 @@ extern "C" int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n
   if (cat == nullptr) {
     const int64_t lds = static_cast<int64_t>(p) * kStageLb * 4;
     if (RFX_LEAF_STAGE && lds <= kStageMaxLds) {
-      const LeafGrid g = leaf_grid(n, p, ntree, kStageLb, kStageTrees * kStageWaves);
-      const int64_t blocks = 8 * g.nsbx * g.ntg;
-      if (blocks >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
-      hipLaunchKernelGGL(leaf_assign_staged_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kStageLb * kStageWaves),
-                         static_cast<unsigned>(lds), stream, x_col, n, p, reinterpret_cast<const uint2*>(nodes),
-                         node_off, ntree, inbag, nodextr, keys, g);
-      return rfx_launch_status();
+      if (n <= kStageSmallN)
+        return launch_staged<RFX_LEAF_SMALL_CFG>(stream, x_col, n, p, nodes, node_off, ntree, inbag, nodextr, keys);
+      return launch_staged<RFX_LEAF_BIG_CFG>(stream, x_col, n, p, nodes, node_off, ntree, inbag, nodextr, keys);
     }
     const LeafGrid g = leaf_grid(n, p, ntree, kBlock, kFastTrees);
     const int64_t blocks = 8 * g.nsbx * g.ntg;

@@ -258,3 +258,15 @@ def test_long_rows_many_trees_match_bucket_form(gpu):
     P = acc.accumulate(keys, _lib.PROX_OOB_ROW, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
     Q = acc.accumulate(keys, _lib.PROX_OOB_ROW, _lib.LAYOUT_FULL, _lib.ALGO_BUCKET)
     assert torch.equal(P, Q)
+
+
+@pytest.mark.parametrize("n", [65536, 70001])
+def test_leaf_assign_both_tile_configurations(gpu, n):
+    """The staged leaf kernel takes 8 x 8 trees per tile up to n = 65,536 and
+    4 x 2 trees beyond: both against the oracle, keys and nodextr."""
+    _say(f"leaf tiles n={n}")
+    x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, n, 54, 7, 13, 21, mtry=7)
+    _, keys, nodextr = run_keys(f, x)
+    nd = O.leaf_assign(f, x)
+    assert np.array_equal(nodextr.cpu().numpy(), nd)
+    assert np.array_equal(keys.cpu().numpy(), keys_from_nodextr(f, nd))
