@@ -297,19 +297,23 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 // GATHER, long rows (n > 32768): one workgroup per output line, its column
 // chunks walked with per-tree cursors (gather_gallop_body in gather.hpp).
 // ---------------------------------------------------------------------------
+#ifndef RFX_GALLOP_CHUNK
+#define RFX_GALLOP_CHUNK 8192
+#endif
+constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;  // LDS counters per workgroup
 template <int MODE, int LAYOUT, int G>
 __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restrict__ pe,
                                                             const uint32_t* __restrict__ bstart,
                                                             const uint32_t* __restrict__ mbucket, int64_t n,
                                                             int32_t ntree, const uint32_t* __restrict__ members,
                                                             int32_t accumulate, int32_t* __restrict__ out,
-                                                            int32_t* __restrict__ out2) {
-  __shared__ uint32_t acc[kGatherChunk];
+                                                            int32_t* __restrict__ out2, int64_t chunk) {
+  extern __shared__ uint32_t gallop_acc[];
   __shared__ uint32_t diag_valid;
   const int64_t line = xcd_row(blockIdx.x);
   if (line >= n) return;
-  gather_gallop_body<MODE, LAYOUT, G>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line, acc,
-                                      diag_valid);
+  gather_gallop_body<MODE, LAYOUT, G>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line,
+                                      gallop_acc, diag_valid, chunk);
 }
 
 // Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
@@ -402,21 +406,23 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
     return rfx_launch_status();
   }
   const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n))), blk(256);
+  const int64_t chunk = kGallopChunk;
+  const size_t lds = 4 * static_cast<size_t>(chunk);
   if (ntree <= 256)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 1>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 1>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2, chunk);
   else if (ntree <= 512)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 2>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 2>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2, chunk);
   else if (ntree <= 1024)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 4>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 4>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2, chunk);
   else if (ntree <= 2048)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 8>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 8>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2, chunk);
   else if (ntree <= 4096)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 16>), grid, blk, 0, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2);
+    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 16>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
+                       ntree, w.members, accumulate, out, out2, chunk);
   else
     return RFX_ERR_UNSUPPORTED;
   return rfx_launch_status();

@@ -235,7 +235,7 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
 }
 
 
-// Long line in chunks of kGatherChunk LDS counters, in order, for a
+// Long line in chunks of `chunk` LDS counters, in order, for a
 // workgroup of 256 threads: lane l of wave w holds the cursors of trees
 // g*256 + w*64 + l (g < G, so T <= 256 G) into their member runs.  For every
 // chunk a lane finds where each of its runs crosses the chunk end by a
@@ -249,7 +249,8 @@ __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe,
                                                    const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
                                                    const M* __restrict__ members, int32_t accumulate,
                                                    int32_t* __restrict__ out, int32_t* __restrict__ out2,
-                                                   int64_t line, uint32_t* acc, uint32_t& diag_valid) {
+                                                   int64_t line, uint32_t* acc, uint32_t& diag_valid,
+                                                   int64_t chunk = kGatherChunk) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   uint32_t cur[G], end[G];
   uint32_t bagm = 0, valid = 0;
@@ -278,8 +279,8 @@ __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe,
     if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
   }
   const int64_t first = LAYOUT == RFX_LAYOUT_UPPER ? line : 0;
-  for (int64_t lo = first; lo < n; lo += kGatherChunk) {
-    const int64_t hi = min(n, lo + kGatherChunk);
+  for (int64_t lo = first; lo < n; lo += chunk) {
+    const int64_t hi = min(n, lo + chunk);
     const int len_chunk = static_cast<int>(hi - lo);
     __syncthreads();
     for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
