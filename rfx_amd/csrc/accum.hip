@@ -298,9 +298,11 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 // chunks walked with per-tree cursors (gather_gallop_body in gather.hpp).
 // ---------------------------------------------------------------------------
 #ifndef RFX_GALLOP_CHUNK
-#define RFX_GALLOP_CHUNK 8192
+#define RFX_GALLOP_CHUNK 6144
 #endif
-constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;  // LDS counters per workgroup
+// LDS counters per workgroup.  At n = 200,000, T = 300: 2,048 columns 87.2 ms
+// per call, 4,096 65.7, 6,144 62.8, 8,192 64.8, 16,384 76.4, 32,768 107.5.
+constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;
 template <int MODE, int LAYOUT, int G>
 __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restrict__ pe,
                                                             const uint32_t* __restrict__ bstart,
