@@ -298,10 +298,11 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 // chunks walked with per-tree cursors (gather_gallop_body in gather.hpp).
 // ---------------------------------------------------------------------------
 #ifndef RFX_GALLOP_CHUNK
-#define RFX_GALLOP_CHUNK 6144
+#define RFX_GALLOP_CHUNK 8192
 #endif
 // LDS counters per workgroup.  At n = 200,000, T = 300: 2,048 columns 87.2 ms
-// per call, 4,096 65.7, 6,144 62.8, 8,192 64.8, 16,384 76.4, 32,768 107.5.
+// per call, 4,096 65.7, 6,144 62.8, 8,192 64.8, 16,384 76.4, 32,768 107.5;
+// at T = 1000 (four cursors per lane) 6,144 takes 143.8 ms against 137.0.
 constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;
 template <int MODE, int LAYOUT, int G>
 __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restrict__ pe,
