@@ -381,7 +381,9 @@ template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  int rc = prebuilt ? RFX_OK : bucket_build(st, keys, n, ntree, total_nodes, w);
+  // the line kernels (n <= kGatherLineMaxN) read no 32-bit members: a lean build
+  int rc = prebuilt ? RFX_OK
+                    : bucket_build(st, keys, n, ntree, total_nodes, w, n <= kGatherLineMaxN && w.members16 != nullptr);
   if (rc) return rc;
   return launch_gather_kernels<MODE, LAYOUT>(st, w, n, ntree, accumulate, out, out2);
 }

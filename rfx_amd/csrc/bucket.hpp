@@ -851,8 +851,8 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
       const uint32_t i = v[u] >> 1, bag = v[u] & 1u;
       if (p >= n || i >= n) continue;  // i < n always: every position is filled by the pass
       mbucket[base + p] = k[u] >= 0 ? (static_cast<uint32_t>(k[u]) >> 1) : total_nodes;
-      V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
-      members[base + p] = v[u];
+      if (V0) V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
+      if (members) members[base + p] = v[u];
       if (members16) members16[base + p] = static_cast<uint16_t>(v[u]);
     }
   }
@@ -910,16 +910,21 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
 }
 
 // Full bucket build from the leaf keys [T][n] (see the file comment).
+// lean: for a caller that reads only pe, bstart, mbucket and members16 (the
+// whole-line and line-chunk GATHER kernels), the per-tree build skips the
+// 32-bit members and the sorted slot values.
 int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-                 const BucketWs& w) {
+                 const BucketWs& w, bool lean = false) {
   const int64_t N = n * static_cast<int64_t>(ntree);
   if (!w.per_tree) return RFX_ERR_INVALID;
   if (n <= kTreeLdsMaxN) {
+    if (lean && !w.members16) return RFX_ERR_INVALID;
     int rc = fill_u32(st, w.bstart, total_nodes, 0u);
     if (!rc) rc = fill_u32(st, w.bend, total_nodes, 0u);
     if (rc) return rc;
     hipLaunchKernelGGL(rs_tree_lds_kernel, dim3(static_cast<unsigned>(ntree)), dim3(kTreeLdsThreads), kTreeLdsBytes, st, keys, n, static_cast<uint32_t>(total_nodes), w.K[0], w.mbucket,
-                       w.V[0], w.members, w.members16, w.pe, w.bstart, w.bend, w.nvalid);
+                       lean ? nullptr : w.V[0], lean ? nullptr : w.members, w.members16, w.pe, w.bstart, w.bend,
+                       w.nvalid);
     return rfx_launch_status();
   }
   int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
