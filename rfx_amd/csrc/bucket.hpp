@@ -15,6 +15,9 @@
 // maximum and the surplus passes return at once.  Off-leaf slots (sentinel
 // key) sort to the end of their tree's segment.  The batched build sorts the
 // concatenated key space in one segment, off-leaf slots at the very end.
+// Up to n = 32,767 the one-problem build is instead one counting-sort
+// workgroup per tree in LDS (rs_tree_lds_kernel below), with the same
+// results.
 // From the sorted arrays:
 //   bstart[b], bend[b]  the member range of leaf b (0, 0 when empty)
 //   members[p]          (i << 1) | in-bag of the p-th sorted slot
