@@ -54,3 +54,22 @@ def test_casewise_rejects_bad_nin(gpu):
     acc = Accumulator(200, 3, f.total_nodes)
     with pytest.raises(ValueError):
         acc.casewise(keys, torch.zeros(10, dtype=torch.int32, device=df.device))
+
+
+@pytest.mark.parametrize("n,T", [(25000, 3), (33000, 2)])
+def test_casewise_row_chunks(gpu, n, T):
+    # two row chunks per column (16-bit members at n = 25,000, 32-bit beyond
+    # 32,768), case-wise and RF-GAP against the oracle
+    print(f"casewise chunks n={n} T={T}", flush=True)
+    x, y = make_dataset(_lib.DATA_COVTYPE_LIKE, n, 54, 7, 8)
+    f = Forest.grow(x, y, T, iseed=8)
+    df = DeviceForest(f, x)
+    keys, nodextr = df.leaf_assign(want_nodextr=True)
+    nin = torch.from_numpy(np.ascontiguousarray(f.nin.reshape(T, n), np.int32)).to(df.device)
+    acc = Accumulator(n, T, f.total_nodes)
+    nd = O.leaf_assign(f, x)
+    got = acc.casewise(keys, nin).cpu().numpy()
+    assert np.allclose(got, O.prox_v1(f, nd, casewise=True), rtol=1e-12, atol=1e-14)
+    del got
+    rg = acc.rfgap(nin, nodextr, df.node_off).cpu().numpy()
+    assert np.allclose(rg, O.prox_rfgap(f.nin.reshape(T, n), nd), rtol=1e-12, atol=1e-14)
