@@ -389,7 +389,9 @@ typedef struct rfx_batch rfx_batch;
  * no multi-forest batching; it runs each forest tree by tree from the host
  * (fit_batch_gpu, src/rf_random_forest.cpp:1364-1570; batch sizing
  * get_recommended_batch_size, src/rf_cuda_config.cpp:225-295).  The plan
- * copies the descriptors; the device buffers they name must outlive it.
+ * copies the descriptors and reads each forest's node offsets once (a
+ * synchronous copy: forests whose trees have at most 16,384 nodes get the
+ * packed 16-bit DENSE compare); the device buffers they name must outlive it.
  * Needs the current device to be the one the batch runs on. */
 int rfx_batch_create(const rfx_batch_problem* problems, int32_t count, int32_t variant, rfx_batch** out);
 size_t rfx_batch_workspace_bytes(const rfx_batch* b);

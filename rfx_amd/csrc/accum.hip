@@ -59,6 +59,18 @@ __global__ __launch_bounds__(kThreads) void dense_tile_kernel(const int32_t* __r
   dense_tile_body<MODE, LAYOUT>(keys, n, ntree, bi, bj, accumulate, out, out2, sR, sS);
 }
 
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void dense16_tile_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                                int32_t ntree, int64_t nb,
+                                                                int32_t* __restrict__ out) {
+  __shared__ uint32_t sR[kTreeChunk16 / 2][kTile];
+  __shared__ uint32_t sS[kTreeChunk16 / 2][kTile];
+  const int64_t lin = xcd_linear(blockIdx.x, gridDim.x);
+  int64_t bi, bj;
+  upper_tile(lin, nb, bi, bj);
+  dense16_tile_body<MODE>(keys, n, ntree, bi, bj, out, sR, sS);
+}
+
 // Rows [r0, r1) of the full matrix, written row-major (r - r0) * n + j.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void dense_rows_kernel(const int32_t* __restrict__ keys, int64_t n,
@@ -342,6 +354,18 @@ __global__ void sum_squares_kernel(const uint32_t* __restrict__ bstart, const ui
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
 }
 
+// Packed 16-bit DENSE (ALL or OOB_UPPER, UPPER, overwrite): only for forests
+// whose trees have at most kDense16MaxNodes nodes, which the caller knows.
+template <int MODE>
+int run_dense16(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int32_t* out) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  const int64_t tiles = nb * (nb + 1) / 2;
+  if (tiles > 0x7fffffffLL || ntree > 131070) return RFX_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((dense16_tile_kernel<MODE>), dim3(static_cast<unsigned>(tiles)), dim3(kThreads), 0, st, keys, n,
+                     ntree, nb, out);
+  return rfx_launch_status();
+}
+
 template <int MODE, int LAYOUT>
 int run_dense(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
               int32_t* out2) {
@@ -504,6 +528,11 @@ int check_common(const int32_t* keys, int64_t n, int32_t ntree, int64_t total_no
 int count_problem_slice(hipStream_t st, int32_t variant, int32_t algo, const int32_t* keys, int64_t n,
                         int32_t ntree, const uint2* pe, const uint32_t* bstart, const uint32_t* mbucket,
                         const uint32_t* members, const uint16_t* members16, int32_t* out, int32_t* out2) {
+  if (algo == (RFX_ALGO_DENSE | kAlgoDense16)) {
+    if (variant == RFX_PROX_ALL) return run_dense16<kMode_ALL>(st, keys, n, ntree, out);
+    if (variant == RFX_PROX_OOB_UPPER) return run_dense16<kMode_OOB_UPPER>(st, keys, n, ntree, out);
+    algo = RFX_ALGO_DENSE;  // PAIR: the 32-bit form
+  }
   if (algo == RFX_ALGO_DENSE) {
     switch (variant) {
       case RFX_PROX_ALL:

@@ -167,6 +167,7 @@ struct rfx_batch {
   std::vector<int32_t> algo;    // RFX_ALGO_GATHER or RFX_ALGO_DENSE per problem
   std::vector<double> sumsq;    // measured sum of |leaf|^2 per problem (0 until measured)
   std::vector<uint8_t> direct;  // 1: counted by its own one-problem kernel launch, not the queue
+  std::vector<uint8_t> dense16; // 1: every tree has <= kDense16MaxNodes nodes (packed DENSE form)
   int32_t lines_per_item = 4;
   int32_t blocks_per_cu = 0;  // 0: as many as fit
   // device layout inside the workspace
@@ -365,6 +366,7 @@ int max_blocks_per_cu(int32_t variant, int32_t max_ntree) {
 int launch_direct(hipStream_t st, const rfx_batch* b, char* ws, const BucketWs& w, size_t i, int32_t algo) {
   const auto& p = b->prob[i];
   const int32_t* keys = reinterpret_cast<const int32_t*>(ws + b->off_keys) + b->seg_start[i];
+  if (algo == RFX_ALGO_DENSE && b->dense16[i]) algo |= kAlgoDense16;
   return count_problem_slice(st, b->variant, algo, keys, p.n, p.ntree, w.pe + b->seg_start[i], w.bstart, w.mbucket,
                              w.members, reinterpret_cast<const uint16_t*>(ws + b->off_m16), p.out, p.out2);
 }
@@ -477,6 +479,19 @@ extern "C" int rfx_batch_create(const rfx_batch_problem* problems, int32_t count
   b->algo.assign(count, RFX_ALGO_GATHER);
   b->sumsq.assign(count, 0.0);
   b->direct.assign(count, 0);
+  // node counts per tree (planning reads the device offsets once)
+  b->dense16.assign(count, 0);
+  for (int32_t i = 0; i < count; ++i) {
+    const auto& p = b->prob[i];
+    std::vector<int64_t> off(static_cast<size_t>(p.ntree) + 1);
+    if (hipMemcpy(off.data(), p.node_off, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      delete b;
+      return RFX_ERR_HIP;
+    }
+    int64_t most = 0;
+    for (int32_t t = 0; t < p.ntree; ++t) most = std::max(most, off[t + 1] - off[t]);
+    b->dense16[i] = most <= kDense16MaxNodes ? 1 : 0;
+  }
   order_problems(b);
   b->total_bytes = layout_bytes(b);
   int dev = 0;

@@ -143,4 +143,101 @@ __device__ __forceinline__ void dense_tile_body(const int32_t* __restrict__ keys
   }
 }
 
+// ---------------------------------------------------------------------------
+// Packed 16-bit form for ALL and OOB_UPPER, UPPER layout, for forests whose
+// trees have at most kDense16MaxNodes nodes: a tree's leaf ids then differ
+// modulo 16,384, so a key shrinks to (leaf mod 16,384) << 1 | bag (15 bits)
+// with bit 15 free for the pads (row 0xFFFF, column 0xFFFE, which never
+// compare equal to anything).  Two trees share a 32-bit word; per word and
+// output pair, d = a ^ b, min(d, 1) per 16-bit half counts the non-matching
+// slots (v_xor_b32, v_pk_min_u16, v_pk_add_u16): 3 operations for two
+// tree-pairs against 2 per tree-pair.  count = slots - non-matches.
+constexpr int64_t kDense16MaxNodes = 16384;
+// algo flag of count_problem_slice (accum.hip): DENSE in the packed 16-bit
+// form, for a caller that has checked every tree's node count
+constexpr int32_t kAlgoDense16 = 0x1000;
+constexpr int kTreeChunk16 = 64;  // trees per LDS stage (32 words per column)
+typedef unsigned short rfx_u16x2 __attribute__((ext_vector_type(2)));
+
+template <int MODE>
+__device__ __forceinline__ uint32_t row_key16(int32_t k) {
+  if (k < 0) return 0xFFFFu;
+  if (MODE == kMode_ALL) return (static_cast<uint32_t>(k) >> 1) & 0x3FFFu;
+  return static_cast<uint32_t>(k) & 0x7FFFu;  // OOB_UPPER
+}
+template <int MODE>
+__device__ __forceinline__ uint32_t col_key16(int32_t k) {
+  if (k < 0) return 0xFFFEu;
+  if (MODE == kMode_ALL) return (static_cast<uint32_t>(k) >> 1) & 0x3FFFu;
+  return (static_cast<uint32_t>(k) & 0x7FFFu) ^ 1u;  // OOB_UPPER: same leaf, other bag
+}
+
+template <int MODE>
+__device__ __forceinline__ void dense16_tile_body(const int32_t* __restrict__ keys, int64_t n, int32_t ntree,
+                                                  int64_t bi, int64_t bj, int32_t* __restrict__ out,
+                                                  uint32_t (*sR)[kTile], uint32_t (*sS)[kTile]) {
+  constexpr int kWords = kTreeChunk16 / 2;
+  const int64_t i0 = bi * kTile, j0 = bj * kTile;
+  const int tid = threadIdx.x;
+  const int rb = (tid >> 4) * kSub, cb = (tid & 15) * kSub;  // UPPER: columns contiguous per thread
+  rfx_u16x2 cnt[kSub][kSub];
+#pragma unroll
+  for (int r = 0; r < kSub; ++r)
+#pragma unroll
+    for (int c = 0; c < kSub; ++c) cnt[r][c] = rfx_u16x2{0, 0};
+  uint32_t slots = 0;  // tree slots compared per output pair, pads included
+  const uint32_t ones = 0x00010001u;
+  for (int32_t t0 = 0; t0 < ntree; t0 += kTreeChunk16) {
+    const int32_t tc = min(kTreeChunk16, ntree - t0);
+    const int words = (tc + 1) / 2;
+    __syncthreads();
+    for (int e = tid; e < kWords * kTile; e += kThreads) {
+      const int w = e / kTile, q = e % kTile;
+      const int64_t i = i0 + q, j = j0 + q;
+      const int32_t ta = t0 + 2 * w, tb = ta + 1;
+      const bool va = 2 * w < tc, vb = 2 * w + 1 < tc;
+      const uint32_t ra = (va && i < n) ? row_key16<MODE>(keys[static_cast<int64_t>(ta) * n + i]) : 0xFFFFu;
+      const uint32_t rb2 = (vb && i < n) ? row_key16<MODE>(keys[static_cast<int64_t>(tb) * n + i]) : 0xFFFFu;
+      const uint32_t ca = (va && j < n) ? col_key16<MODE>(keys[static_cast<int64_t>(ta) * n + j]) : 0xFFFEu;
+      const uint32_t cb2 = (vb && j < n) ? col_key16<MODE>(keys[static_cast<int64_t>(tb) * n + j]) : 0xFFFEu;
+      sR[w][q] = ra | (rb2 << 16);
+      sS[w][q] = ca | (cb2 << 16);
+    }
+    __syncthreads();
+    for (int w = 0; w < words; ++w) {
+      uint32_t a[kSub], b[kSub];
+      const uint4* pa = reinterpret_cast<const uint4*>(&sR[w][rb]);
+      const uint4* pb = reinterpret_cast<const uint4*>(&sS[w][cb]);
+      const uint4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+      a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
+      a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
+      b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+      b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+#pragma unroll
+      for (int r = 0; r < kSub; ++r)
+#pragma unroll
+        for (int c = 0; c < kSub; ++c) {
+          // min per 16-bit half, written out: the compiler would turn the
+          // vector min into two compares and selects
+          uint32_t m;
+          asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(a[r] ^ b[c]), "v"(ones));
+          cnt[r][c] += __builtin_bit_cast(rfx_u16x2, m);
+        }
+    }
+    slots += 2 * static_cast<uint32_t>(words);
+  }
+#pragma unroll
+  for (int r = 0; r < kSub; ++r) {
+    const int64_t i = i0 + rb + r;
+    if (i >= n) continue;
+#pragma unroll
+    for (int c = 0; c < kSub; ++c) {
+      const int64_t j = j0 + cb + c;
+      if (j >= n || j < i) continue;
+      const uint32_t miss = static_cast<uint32_t>(cnt[r][c].x) + static_cast<uint32_t>(cnt[r][c].y);
+      out[rfx_upper_index(i, j, n)] = static_cast<int32_t>(slots - miss);
+    }
+  }
+}
+
 }  // namespace

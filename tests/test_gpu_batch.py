@@ -210,3 +210,29 @@ def test_batch_rejects_bad_input(gpu, problems):
         ForestBatch(dfs, "oob_asym")
     with pytest.raises(RuntimeError):
         ForestBatch([], "all")
+
+
+@pytest.mark.parametrize("variant", ["oob_upper", "all", "pair"])
+def test_batch_own_launch_dense_packed_form(gpu, variant):
+    # own-launched DENSE forests whose trees have at most 16,384 nodes run the
+    # packed 16-bit compare (two trees per word, odd tree counts padded);
+    # against the oracle for ALL and OOB_UPPER, PAIR keeps the 32-bit form
+    shapes = [(6000, 10, 4, 61, 31), (4000, 8, 3, 37, 32), (9000, 12, 5, 20, 33)]
+    probs = _problems(shapes)
+    assert all(max(np.diff(f.node_offset)) <= 16384 for f, _ in probs)
+    dfs = [DeviceForest(f, x) for f, x in probs]
+    _say(f"batch packed DENSE, variant {variant}")
+    b = ForestBatch(dfs, variant)
+    b.autotune(reps=1)
+    b.set_algos([_lib.ALGO_DENSE] * len(dfs))
+    own = b.own_launch()
+    assert own[0] and own[2], "the large forests launch on their own"
+    b.run(use_graph=False)
+    torch.cuda.synchronize()
+    v = {"oob_upper": _lib.PROX_OOB_UPPER, "all": _lib.PROX_ALL, "pair": _lib.PROX_ALL}[variant]
+    for k, (f, x) in enumerate(probs):
+        nd = O.leaf_assign(f, x)
+        assert np.array_equal(b.out[k].cpu().numpy(), O.counts(f, nd, v, _lib.LAYOUT_UPPER)), k
+        if variant == "pair":
+            assert np.array_equal(b.out2[k].cpu().numpy(), O.counts(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)), k
+    b.close()
