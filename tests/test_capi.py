@@ -42,6 +42,30 @@ def test_invalid_arguments_return_status():
     assert L.rfx_forest_grow(x.ctypes.data, y.ctypes.data, 4, 2, 1, 1, 1, 1, 0, 0, 0, 0, 1, C.byref(h)) == -1
 
 
+def test_weighted_proximity_and_outlier_arguments():
+    # case-wise, RF-GAP, the RF-GAP MDS and the outlier scores reject bad
+    # arguments and short workspaces before any GPU work; dummy non-null
+    # pointers are never dereferenced on these paths
+    L = _lib.lib()
+    d = 256  # a non-null dummy address
+    n, T, tn = 1000, 10, 500
+    cw = L.rfx_prox_casewise_workspace_bytes(n, T, tn)
+    rg = L.rfx_prox_rfgap_workspace_bytes(n, T, tn)
+    assert cw >= L.rfx_prox_workspace_bytes(n, T, tn, _lib.ALGO_BUCKET) and rg > cw
+    assert L.rfx_prox_casewise_workspace_bytes(0, T, tn) == 0 and L.rfx_prox_rfgap_workspace_bytes(n, 0, tn) == 0
+    assert L.rfx_prox_casewise(None, None, d, n, T, tn, 0, d, d, cw) == -1
+    assert L.rfx_prox_casewise(None, d, d, n, T, tn, 0, d, d, cw - 1) == -4
+    assert L.rfx_prox_rfgap(None, d, None, d, n, T, tn, d, d, rg) == -1
+    assert L.rfx_prox_rfgap(None, d, d, d, n, T, tn, d, d, rg - 1) == -4
+    assert L.rfx_prox_rfgap(None, d, d, d, n, T, 0, d, d, rg) == -1
+    assert L.rfx_outlier_scores(None, None, n, d, None) == -1
+    assert L.rfx_outlier_scores(None, d, 0, d, None) == -1
+    mb = L.rfx_mds_dense_workspace_bytes(n, _lib.PREC_FP64)
+    assert L.rfx_mds_dense_rfgap(None, d, n, None, _lib.PREC_FP64, 10, 1e-8, d, mb, d, None, None) == -1
+    assert L.rfx_mds_dense_rfgap(None, d, n, d, _lib.PREC_FP64, 10, 1e-8, d, mb - 1, d, None, None) == -4
+    assert L.rfx_mds_dense_rfgap(None, d, 3, d, _lib.PREC_FP64, 10, 1e-8, d, mb, d, None, None) == -1
+
+
 def test_pack_nodes_rejects_malformed_forest():
     L = _lib.lib()
     off = np.array([0, 3], np.int64)
