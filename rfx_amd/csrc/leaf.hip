@@ -148,9 +148,10 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 // into LDS ([p][LB] floats, coalesced column reads), then every traversal
 // step reads the feature from LDS, so the divergent global gathers of a step
 // are the node records alone.  Used while p * LB * 4 bytes fit kStageMaxLds.
-// At n = 20,000, p = 54, T = 500: 8 wavefronts of 8 trees each (4 in
-// lockstep) per tile take 0.33 ms, one wavefront of 16 trees (8 in lockstep)
-// per tile 0.51 ms -- the single-wavefront tiles bounded occupancy by LDS.
+// At n = 20,000, p = 54, T = 500: 8 wavefronts of 4 trees each (4 in
+// lockstep) per tile take 0.31 ms, 8 of 8 trees 0.33 ms, 16 of 8 trees
+// 0.34 ms, one wavefront of 16 trees (8 in lockstep) per tile 0.51 ms -- the
+// single-wavefront tiles bounded occupancy by LDS.
 // At n = 200,000 (about 54,000 nodes per tree) 4 wavefronts of 2 trees take
 // 13.1 ms per 1,000 trees, one wavefront of 16 trees 17.4 ms and 8 x 8 trees
 // 23.7 ms: there the node records of a tree group must stay in L2.
@@ -164,11 +165,11 @@ constexpr int kStageLb = RFX_LEAF_STAGE_LB;
 constexpr int64_t kStageMaxLds = 64 * 1024;
 // Wavefronts per workgroup W share the workgroup's LDS feature tile and take
 // consecutive groups of TPW trees each, ILP of them in lockstep.  Small trees
-// (the forest grown on up to kStageSmallN samples) use 8 x 8 trees per
+// (the forest grown on up to kStageSmallN samples) use 8 x 4 trees per
 // workgroup; larger trees fewer per workgroup, so the node records that the
 // sample blocks of an XCD chunk share stay within its L2.
 #ifndef RFX_LEAF_SMALL_CFG
-#define RFX_LEAF_SMALL_CFG 8, 8, 4
+#define RFX_LEAF_SMALL_CFG 8, 4, 4
 #endif
 #ifndef RFX_LEAF_BIG_CFG
 #define RFX_LEAF_BIG_CFG 4, 2, 2

@@ -262,7 +262,7 @@ def test_long_rows_many_trees_match_bucket_form(gpu):
 
 @pytest.mark.parametrize("n", [65536, 70001])
 def test_leaf_assign_both_tile_configurations(gpu, n):
-    """The staged leaf kernel takes 8 x 8 trees per tile up to n = 65,536 and
+    """The staged leaf kernel takes 8 x 4 trees per tile up to n = 65,536 and
     4 x 2 trees beyond: both against the oracle, keys and nodextr."""
     _say(f"leaf tiles n={n}")
     x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, n, 54, 7, 13, 21, mtry=7)
