@@ -29,6 +29,7 @@
 
 #include "rfx_common.hpp"
 #include "bucket.hpp"
+#include "dense.hpp"
 #include "gather.hpp"
 
 namespace {
@@ -215,13 +216,13 @@ constexpr int kRgTile = 64;
 // the averages are written back to (J, I) from LDS.  Both global passes run
 // along the contiguous (row) direction.
 __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ w, const uint32_t* __restrict__ oob,
-                                                           int64_t n) {
+                                                           int64_t n, int64_t nt) {
   // p_ij is rounded before the sum, as in the reference: a fused
   // multiply-add of w_ij * (1 / S_i) + p_ji would differ between the two
   // mirrored entries of the diagonal tile, which are computed separately
 #pragma clang fp contract(off)
-  const int64_t I = blockIdx.y, J = blockIdx.x;
-  if (I > J) return;
+  int64_t I, J;  // workgroup -> tile pair I <= J of the upper triangle of nt x nt tiles
+  upper_tile(blockIdx.x, nt, I, J);
   __shared__ double L[kRgTile][kRgTile + 1];  // L[b][a] = element (J0 + b, I0 + a)
   const int64_t i0 = I * kRgTile, j0 = J * kRgTile;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -334,6 +335,10 @@ extern "C" int rfx_prox_rfgap(hipStream_t st, const int32_t* nin, const int32_t*
   rc = weighted_leaf_sums<true>(st, keys, nin, n, ntree, total_nodes, 0, out, ws);
   if (rc) return rc;
   const unsigned nt = static_cast<unsigned>(rfx_ceil_div(n, kRgTile));
-  hipLaunchKernelGGL(rg_symmetric_kernel, dim3(nt, nt), dim3(256), 0, st, out, oob, n);
+  // one workgroup per tile pair I <= J
+  const int64_t pairs = static_cast<int64_t>(nt) * (nt + 1) / 2;
+  if (pairs >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(rg_symmetric_kernel, dim3(static_cast<unsigned>(pairs)), dim3(256), 0, st, out, oob, n,
+                     static_cast<int64_t>(nt));
   return rfx_launch_status();
 }
