@@ -20,7 +20,9 @@ from helpers import centred, eig_residuals, load_golden
 pytestmark = pytest.mark.gpu
 
 # (kind, n, p, classes, T, seed): one tile, many tiles with a ragged edge
-SHAPES = [(_lib.DATA_OVERLAP_MIXTURE, 300, 7, 3, 6, 9), (_lib.DATA_GAUSS_MIXTURE, 2500, 12, 4, 21, 3),
+# n = 50: one 64 x 64 tile pair; n = 640: whole tiles only; the others end in a partial tile
+SHAPES = [(_lib.DATA_OVERLAP_MIXTURE, 50, 7, 3, 6, 9), (_lib.DATA_GAUSS_MIXTURE, 640, 12, 4, 11, 2),
+          (_lib.DATA_OVERLAP_MIXTURE, 300, 7, 3, 6, 9), (_lib.DATA_GAUSS_MIXTURE, 2500, 12, 4, 21, 3),
           (_lib.DATA_COVTYPE_LIKE, 9000, 54, 7, 8, 5)]
 
 
