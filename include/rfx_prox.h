@@ -72,7 +72,9 @@ extern "C" {
 #define RFX_ALGO_GATHER 3 /* leaf buckets gathered per output line in LDS  */
 /* Flag or-ed into a BUCKET or GATHER algo: the workspace already holds the
  * leaf buckets of these keys (rfx_prox_build_buckets), so only the counting
- * kernel runs. */
+ * kernel runs.  A call without it builds in the workspace only what its own
+ * kernel reads and leaves the rest of the buckets unspecified: build again
+ * with rfx_prox_build_buckets before the next RFX_ALGO_PREBUILT call. */
 #define RFX_ALGO_PREBUILT 0x100
 
 /* ========================================================================

@@ -405,9 +405,11 @@ template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
   const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
-  // the line kernels (n <= kGatherLineMaxN) read no 32-bit members: a lean build
+  // the line kernels (n <= kGatherLineMaxN) read no 32-bit members: a lean
+  // build; for UPPER they read only pe and the 16-bit members
+  const bool lean = n <= kGatherLineMaxN && w.members16 != nullptr;
   int rc = prebuilt ? RFX_OK
-                    : bucket_build(st, keys, n, ntree, total_nodes, w, n <= kGatherLineMaxN && w.members16 != nullptr);
+                    : bucket_build(st, keys, n, ntree, total_nodes, w, lean, lean && LAYOUT == RFX_LAYOUT_UPPER);
   if (rc) return rc;
   return launch_gather_kernels<MODE, LAYOUT>(st, w, n, ntree, accumulate, out, out2);
 }

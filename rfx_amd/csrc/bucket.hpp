@@ -846,14 +846,14 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
     for (int u = 0; u < kTlPrefetch; ++u) {
       const int64_t p = p0 + static_cast<int64_t>(u) * kTreeLdsThreads;
       v[u] = p < n ? perm[p] : 0u;
-      k[u] = (p < n && (v[u] >> 1) < n) ? kt[v[u] >> 1] : -1;
+      k[u] = (mbucket && p < n && (v[u] >> 1) < n) ? kt[v[u] >> 1] : -1;
     }
 #pragma unroll
     for (int u = 0; u < kTlPrefetch; ++u) {
       const int64_t p = p0 + static_cast<int64_t>(u) * kTreeLdsThreads;
       const uint32_t i = v[u] >> 1, bag = v[u] & 1u;
       if (p >= n || i >= n) continue;  // i < n always: every position is filled by the pass
-      mbucket[base + p] = k[u] >= 0 ? (static_cast<uint32_t>(k[u]) >> 1) : total_nodes;
+      if (mbucket) mbucket[base + p] = k[u] >= 0 ? (static_cast<uint32_t>(k[u]) >> 1) : total_nodes;
       if (V0) V0[base + p] = (static_cast<uint32_t>(base + i) << 1) | bag;
       if (members) members[base + p] = v[u];
       if (members16) members16[base + p] = static_cast<uint16_t>(v[u]);
@@ -864,7 +864,7 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
     for (uint32_t d = tid; d < nleaf; d += kTreeLdsThreads) {
       const uint32_t s0 = d > 0 ? lds_half(words, (d - 1) * B + (B - 1)) : 0u;
       const uint32_t s1 = lds_half(words, d * B + (B - 1));
-      if (s1 > s0) {
+      if (s1 > s0 && bstart) {
         bstart[lo + d] = static_cast<uint32_t>(base + s0);
         bend[lo + d] = static_cast<uint32_t>(base + s1);
       }
@@ -899,7 +899,7 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
   for (int64_t p = tid; p < n; p += kTreeLdsThreads) {
     const int64_t g = leaf_at(p);
     if (g < 0) continue;
-    if (p == 0 || leaf_at(p - 1) != g) bstart[g] = static_cast<uint32_t>(base + p);
+    if (bstart && (p == 0 || leaf_at(p - 1) != g)) bstart[g] = static_cast<uint32_t>(base + p);
     if (p + 1 == n || leaf_at(p + 1) != g) bend[g] = static_cast<uint32_t>(base + p + 1);
   }
   __syncthreads();
@@ -915,19 +915,27 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
 // Full bucket build from the leaf keys [T][n] (see the file comment).
 // lean: for a caller that reads only pe, bstart, mbucket and members16 (the
 // whole-line and line-chunk GATHER kernels), the per-tree build skips the
-// 32-bit members and the sorted slot values.
+// 32-bit members and the sorted slot values.  rows_upper (with lean): for the
+// UPPER line kernels, which read only pe and members16, it also skips the
+// leaf bounds (and their clears) and mbucket, with the key gather that feeds
+// it; bend stays the two-pass form's scratch, written for every leaf present
+// before it is read.  The skipped arrays are left as they were.
 int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
-                 const BucketWs& w, bool lean = false) {
+                 const BucketWs& w, bool lean = false, bool rows_upper = false) {
   const int64_t N = n * static_cast<int64_t>(ntree);
   if (!w.per_tree) return RFX_ERR_INVALID;
   if (n <= kTreeLdsMaxN) {
     if (lean && !w.members16) return RFX_ERR_INVALID;
-    int rc = fill_u32(st, w.bstart, total_nodes, 0u);
-    if (!rc) rc = fill_u32(st, w.bend, total_nodes, 0u);
-    if (rc) return rc;
-    hipLaunchKernelGGL(rs_tree_lds_kernel, dim3(static_cast<unsigned>(ntree)), dim3(kTreeLdsThreads), kTreeLdsBytes, st, keys, n, static_cast<uint32_t>(total_nodes), w.K[0], w.mbucket,
-                       lean ? nullptr : w.V[0], lean ? nullptr : w.members, w.members16, w.pe, w.bstart, w.bend,
-                       w.nvalid);
+    rows_upper = rows_upper && lean;
+    if (!rows_upper) {
+      int rc = fill_u32(st, w.bstart, total_nodes, 0u);
+      if (!rc) rc = fill_u32(st, w.bend, total_nodes, 0u);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(rs_tree_lds_kernel, dim3(static_cast<unsigned>(ntree)), dim3(kTreeLdsThreads), kTreeLdsBytes,
+                       st, keys, n, static_cast<uint32_t>(total_nodes), w.K[0], rows_upper ? nullptr : w.mbucket,
+                       lean ? nullptr : w.V[0], lean ? nullptr : w.members, w.members16, w.pe,
+                       rows_upper ? nullptr : w.bstart, w.bend, w.nvalid);
     return rfx_launch_status();
   }
   int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
