@@ -39,6 +39,13 @@ def test_wide_leaf_range_two_passes(gpu):
         for algo in (_lib.ALGO_BUCKET, _lib.ALGO_GATHER):
             got = acc.accumulate(keys, variant, _lib.LAYOUT_FULL, algo)
             assert np.array_equal(got.cpu().numpy(), want), (variant, algo)
+    # the UPPER whole-row path builds only pe and the 16-bit members; its
+    # two-pass sort uses the leaf ends, still holding the calls above, as
+    # scratch without a clear
+    print("wide leaf range: UPPER rows", flush=True)
+    a, o = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
+    assert np.array_equal(a.cpu().numpy(), O.counts(f, nd, _lib.PROX_ALL, _lib.LAYOUT_UPPER))
+    assert np.array_equal(o.cpu().numpy(), O.counts(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER))
 
 
 @pytest.mark.parametrize("n", [32767, 32768])
