@@ -777,11 +777,20 @@ __global__ __launch_bounds__(kTreeLdsThreads) void rs_tree_lds_kernel(
   uint32_t* words = tl_words + perm_words;
   const uint32_t emax = 2 * (kTreeLdsBytes / 4 - perm_words);  // counter entries that fit
   uint32_t lo = 0xffffffffu, hi = 0u;
-  for (int64_t i = tid; i < n; i += kTreeLdsThreads) {
-    const int32_t k = kt[i];
-    if (k >= 0) {
-      lo = min(lo, static_cast<uint32_t>(k) >> 1);
-      hi = max(hi, static_cast<uint32_t>(k) >> 1);
+  // kTlPrefetch keys in flight per thread (a plain loop waits for each load)
+  for (int64_t i0 = tid; i0 < n; i0 += static_cast<int64_t>(kTreeLdsThreads) * kTlPrefetch) {
+    int32_t k[kTlPrefetch];
+#pragma unroll
+    for (int u = 0; u < kTlPrefetch; ++u) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kTreeLdsThreads;
+      k[u] = i < n ? kt[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kTlPrefetch; ++u) {
+      if (k[u] >= 0) {
+        lo = min(lo, static_cast<uint32_t>(k[u]) >> 1);
+        hi = max(hi, static_cast<uint32_t>(k[u]) >> 1);
+      }
     }
   }
 #pragma unroll
