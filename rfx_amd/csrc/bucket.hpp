@@ -678,7 +678,12 @@ __device__ __forceinline__ uint64_t same_digit_lanes(uint32_t d, int nbits, uint
 // used, so the loads overlap), digit(token) < ndig, and place(q, token, pos)
 // receives its position.  On return the counter of (digit d, last block)
 // holds the end of d's positions.  All threads must call.
-constexpr int kTlPrefetch = 8;
+// keys in flight per thread in the passes over a tree's keys; the full
+// build at n = 20,000, T = 500 takes 0.207 ms with 4, 0.190 with 8, 0.234 with 16
+#ifndef RFX_TL_PREFETCH
+#define RFX_TL_PREFETCH 8
+#endif
+constexpr int kTlPrefetch = RFX_TL_PREFETCH;
 
 template <class GetF, class DigitF, class PlaceF>
 __device__ void tl_count_pass(uint32_t* words, uint32_t* wsum, int64_t n, uint32_t ndig, int B, GetF get,
