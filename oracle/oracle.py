@@ -78,6 +78,10 @@ def ref() -> C.CDLL:
         L.ref_finishprox.restype = _i32
         L.ref_proximity_rfgap.argtypes = [_i32, _i32, _p, _p, _p]
         L.ref_proximity_rfgap.restype = _i32
+        L.ref_rf_fit.argtypes = [_p, _p, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _p, _i32, _i32, _i32, _p, _p, _p]
+        L.ref_rf_fit.restype = _i32
+        L.ref_mds_3d.argtypes = [_p, _i32, _p, _i32, _p]
+        L.ref_mds_3d.restype = _i32
         _ref = L
     return _ref
 
@@ -280,3 +284,43 @@ def ref_proximity_rfgap(nin: np.ndarray, nodextr: np.ndarray) -> np.ndarray:
     prox = np.empty(n * n, np.float64)
     _ok(ref().ref_proximity_rfgap(T, n, _a(nin), _a(nodextr), _a(prox)), "ref_proximity_rfgap")
     return prox
+
+
+def ref_rf_fit(x: np.ndarray, y: np.ndarray | None, ntree: int, mtry: int, minndsize: int = 1, iseed: int = 12345,
+               cat: np.ndarray | None = None, maxcat: int = 1, unsupervised: bool = False,
+               casewise: bool = False) -> dict:
+    """The reference's RandomForest class fitted on the CPU
+    (src/rf_random_forest.cpp: fit -> grow_tree_single -> finalize_training):
+    its finished proximity (float64, as get_proximity_matrix returns it), nout
+    and, for the unsupervised task, compute_outlier_scores."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, p = x.shape
+    if unsupervised:
+        yy, nclass = None, 2
+    else:
+        # RandomForest::setup_classification_task sets nclass to the number of
+        # distinct labels but keeps the labels themselves
+        # (src/rf_random_forest.cpp:1855-1864), so labels with gaps index past
+        # the grower's class arrays (src/rf_growtree.cpp:638, 717): hand it
+        # contiguous labels (a monotone relabelling leaves the trees unchanged).
+        _, yy = np.unique(np.asarray(y), return_inverse=True)
+        yy = np.ascontiguousarray(yy, np.int32)
+        nclass = max(2, int(yy.max()) + 1)
+    cat = None if cat is None else np.ascontiguousarray(cat, np.int32)
+    r = dict(prox=np.empty(n * n, np.float64), nout=np.empty(n, np.int32),
+             outlier=np.zeros(n, np.float32) if unsupervised else None)
+    _ok(ref().ref_rf_fit(_a(x), _a(yy), n, p, nclass, ntree, mtry, minndsize, iseed, _a(cat), maxcat,
+                         2 if unsupervised else 0, int(casewise), _a(r["prox"]), _a(r["nout"]), _a(r["outlier"])),
+        "ref_rf_fit")
+    return r
+
+
+def ref_mds_3d(prox: np.ndarray, nout: np.ndarray | None, use_rfgap: bool = False) -> np.ndarray:
+    """The reference's rf::compute_mds_3d_cpu (src/rf_mds_cpu.cpp:88-341) on a
+    column-major float64 n x n proximity, with dsyevd from this image's LAPACK."""
+    P = np.ascontiguousarray(np.asarray(prox, np.float64).ravel())
+    n = int(round(np.sqrt(P.size)))
+    out = np.empty((n, 3), np.float64)
+    nv = None if nout is None else np.ascontiguousarray(nout, np.int32)
+    _ok(ref().ref_mds_3d(_a(P), n, _a(nv), int(use_rfgap), _a(out)), "ref_mds_3d")
+    return out

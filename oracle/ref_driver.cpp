@@ -2,24 +2,41 @@
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile (target `ref`) compiles this
 // file together with the reference sources it calls (read in place from the
-// reference tree, never copied) into oracle/_ref/librfxref.so.  The driver
-// repeats what RandomForest::grow_tree_single does per tree
-// (src/rf_random_forest.cpp:928-1308): rf::cpu_bootstrap, rf::cpu_growtree
-// (through rf::cpu_growtree_wrapper), rf::cpu_testreebag for out-of-bag
-// leaves, an in-bag traversal written here after
-// RandomForest::find_terminal_node (src/rf_random_forest.cpp:2194-2244, a
-// member function whose class cannot be compiled standalone), the second
-// nout increment (src/rf_random_forest.cpp:1108-1110), and rf::cpu_proximity;
-// rf::cpu_finishprox and rf::cpu_proximity_rfgap are exported separately.  rf_mds_cpu.cpp is not built:
-// it needs a LAPACK dsyevd_ that this image does not provide.
+// reference tree, never copied) into oracle/_ref/librfxref.so.
+//
+//   ref_run_forest  repeats what RandomForest::grow_tree_single does per tree
+//                   (src/rf_random_forest.cpp:928-1308) to expose the per-tree
+//                   arrays the class keeps private: rf::cpu_bootstrap,
+//                   rf::cpu_growtree (through rf::cpu_growtree_wrapper),
+//                   rf::cpu_testreebag for out-of-bag leaves, an in-bag
+//                   traversal written here after RandomForest::find_terminal_node
+//                   (src/rf_random_forest.cpp:2194-2244), the second nout
+//                   increment (src/rf_random_forest.cpp:1108-1110), and
+//                   rf::cpu_proximity.  The tests pin that in-bag traversal
+//                   against the class itself (ref_rf_fit gives the same
+//                   proximity only if every in-bag leaf agrees).
+//   ref_rf_fit      the reference's RandomForest class end to end on the CPU
+//                   (fit -> grow_tree_single per tree -> finalize_training):
+//                   finished proximity, nout and, for the unsupervised task,
+//                   RandomForest::compute_outlier_scores.
+//   ref_mds_3d      rf::compute_mds_3d_cpu (src/rf_mds_cpu.cpp:88-341).  Its
+//                   LAPACK dsyevd_ is bound at link time to the LAPACK in this
+//                   image (scipy's bundled OpenBLAS, scipy_dsyevd_) by renaming
+//                   the undefined symbol of the compiled object.
+//   ref_finishprox, ref_proximity_rfgap  rf::cpu_finishprox and
+//                   rf::cpu_proximity_rfgap.
 #include <cstdint>
+#include <cstring>
+#include <exception>
 #include <vector>
 
 #include "rf_bootstrap.hpp"
 #include "rf_config.hpp"
 #include "rf_finishprox.hpp"
 #include "rf_growtree.hpp"
+#include "rf_mds_cpu.hpp"
 #include "rf_proximity.hpp"
+#include "rf_random_forest.hpp"
 #include "rf_testreebag.hpp"
 
 using rf::integer_t;
@@ -126,4 +143,62 @@ extern "C" int ref_proximity_rfgap(int32_t ntree, int32_t n, const int32_t* nin,
   }
   rf::cpu_proximity_rfgap(ntree, n, tn, tx, prox);
   return 0;
+}
+
+// RandomForest fit on the CPU.  task: 0 classification (y int32 labels),
+// 2 unsupervised (y ignored).  cat: per-feature category counts or NULL (all
+// numeric), set through RandomForest::set_categorical_features as the
+// classifier does.  Outputs (any may be NULL): prox n*n float64 as
+// get_proximity_matrix returns it, nout [n] (get_nout_ptr), outlier [n]
+// float32 (get_outlier_scores, unsupervised only).
+extern "C" int ref_rf_fit(const float* x, const int32_t* y, int32_t n, int32_t p, int32_t nclass, int32_t ntree,
+                          int32_t mtry, int32_t minndsize, int32_t iseed, const int32_t* cat, int32_t maxcat,
+                          int32_t task, int32_t casewise, double* prox_out, int32_t* nout_out, float* outlier_out) {
+  try {
+    rf::RandomForestConfig cfg;
+    cfg.nsample = n;
+    cfg.ntree = ntree;
+    cfg.mdim = p;
+    cfg.nclass = nclass;
+    cfg.maxcat = maxcat < 1 ? 1 : maxcat;
+    cfg.mtry = mtry;
+    cfg.maxnode = 2 * n + 1;
+    cfg.minndsize = minndsize;
+    cfg.iseed = iseed;
+    cfg.task_type = task == 2 ? rf::TaskType::UNSUPERVISED : rf::TaskType::CLASSIFICATION;
+    cfg.compute_proximity = true;
+    cfg.compute_importance = false;
+    cfg.use_casewise = casewise != 0;
+    cfg.use_gpu = false;
+    cfg.execution_mode = "cpu";
+    rf::RandomForest forest(cfg);
+    if (cat) forest.set_categorical_features(cat, p);
+    if (task == 2)
+      forest.fit_unsupervised(x);
+    else
+      forest.fit(x, y);
+    const rf::dp_t* pm = forest.get_proximity_matrix();
+    if (prox_out) {
+      if (!pm) return -2;
+      std::memcpy(prox_out, pm, sizeof(double) * static_cast<size_t>(n) * n);
+    }
+    if (nout_out) std::memcpy(nout_out, forest.get_nout_ptr(), sizeof(int32_t) * n);
+    if (outlier_out) std::memcpy(outlier_out, forest.get_outlier_scores(), sizeof(float) * n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// rf::compute_mds_3d_cpu(prox, n, false, nout, use_rfgap) -> coords n x 3
+// row-major.  prox is column-major float64 as the pybind wrapper passes it.
+extern "C" int ref_mds_3d(const double* prox, int32_t n, const int32_t* nout, int32_t use_rfgap, double* coords) {
+  try {
+    const std::vector<double> c = rf::compute_mds_3d_cpu(prox, n, false, nout, use_rfgap != 0);
+    if (c.size() != static_cast<size_t>(n) * 3) return -2;
+    std::memcpy(coords, c.data(), sizeof(double) * c.size());
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
 }

@@ -177,6 +177,8 @@ static_assert(kGatherLineMaxN <= kMembers16MaxN, "line kernel reads the 16-bit m
 #endif
 constexpr int kLineThreads = RFX_LINE_THREADS;
 constexpr int64_t kLineMaxLds = RFX_LINE_MAX_LDS;  // bytes of LDS counters per workgroup
+// the line kernels have no static LDS; their requests are at most kLineMaxLds
+static_assert(kLineMaxLds <= kLdsLimitBytes, "line counters exceed the LDS");
 
 // Counters of one line chunk in dynamic LDS: the whole line when it fits
 // kLineMaxLds (no chunk searches, every pe entry read once), else chunks of
@@ -316,8 +318,15 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 // per call, 4,096 65.7, 6,144 62.8, 8,192 64.8, 16,384 76.4, 32,768 107.5;
 // at T = 1000 (four cursors per lane) 6,144 takes 143.8 ms against 137.0.
 constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;
+// dynamic counters + the kernel's static diag_valid word
+static_assert(4 * kGallopChunk + 4 <= kLdsLimitBytes, "gallop chunk exceeds the LDS");
+#ifndef RFX_GALLOP_THREADS
+#define RFX_GALLOP_THREADS 256
+#endif
+constexpr int kGallopThreads = RFX_GALLOP_THREADS;  // one tree cursor per thread and group
+static_assert(kGallopThreads % 64 == 0 && kGallopThreads <= 1024, "gallop workgroup size");
 template <int MODE, int LAYOUT, int G>
-__global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restrict__ pe,
+__global__ __launch_bounds__(kGallopThreads) void gather_gallop_kernel(const uint2* __restrict__ pe,
                                                             const uint32_t* __restrict__ bstart,
                                                             const uint32_t* __restrict__ mbucket, int64_t n,
                                                             int32_t ntree, const uint32_t* __restrict__ members,
@@ -327,8 +336,18 @@ __global__ __launch_bounds__(256) void gather_gallop_kernel(const uint2* __restr
   __shared__ uint32_t diag_valid;
   const int64_t line = xcd_row(blockIdx.x);
   if (line >= n) return;
-  gather_gallop_body<MODE, LAYOUT, G>(pe, bstart, mbucket, n, ntree, members, accumulate, out, out2, line,
-                                      gallop_acc, diag_valid, chunk);
+  gather_gallop_body<MODE, LAYOUT, G, uint32_t, kGallopThreads>(pe, bstart, mbucket, n, ntree, members, accumulate,
+                                                                out, out2, line, gallop_acc, diag_valid, chunk);
+}
+
+template <int MODE, int LAYOUT, int G>
+int launch_gallop(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
+                  int32_t* out2) {
+  const size_t lds = 4 * static_cast<size_t>(kGallopChunk);
+  hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, G>), dim3(static_cast<unsigned>(xcd_rows_padded(n))),
+                     dim3(kGallopThreads), lds, st, w.pe, w.bstart, w.mbucket, n, ntree, w.members, accumulate, out,
+                     out2, kGallopChunk);
+  return rfx_launch_status();
 }
 
 // Diagonal of ALL: C_ii = number of trees in which i ends on a terminal node.
@@ -436,27 +455,13 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
                        w.members16, accumulate, out, out2, chunk_len);
     return rfx_launch_status();
   }
-  const dim3 grid(static_cast<unsigned>(xcd_rows_padded(n))), blk(256);
-  const int64_t chunk = kGallopChunk;
-  const size_t lds = 4 * static_cast<size_t>(chunk);
-  if (ntree <= 256)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 1>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2, chunk);
-  else if (ntree <= 512)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 2>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2, chunk);
-  else if (ntree <= 1024)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 4>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2, chunk);
-  else if (ntree <= 2048)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 8>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2, chunk);
-  else if (ntree <= 4096)
-    hipLaunchKernelGGL((gather_gallop_kernel<MODE, LAYOUT, 16>), grid, blk, lds, st, w.pe, w.bstart, w.mbucket, n,
-                       ntree, w.members, accumulate, out, out2, chunk);
-  else
-    return RFX_ERR_UNSUPPORTED;
-  return rfx_launch_status();
+  constexpr int TH = kGallopThreads;
+  if (ntree <= TH) return launch_gallop<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
+  if (ntree <= 2 * TH) return launch_gallop<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
+  if (ntree <= 4 * TH) return launch_gallop<MODE, LAYOUT, 4>(st, w, n, ntree, accumulate, out, out2);
+  if (TH < 512 && ntree <= 8 * TH) return launch_gallop<MODE, LAYOUT, 8>(st, w, n, ntree, accumulate, out, out2);
+  if (TH < 256 && ntree <= 16 * TH) return launch_gallop<MODE, LAYOUT, 16>(st, w, n, ntree, accumulate, out, out2);
+  return RFX_ERR_UNSUPPORTED;
 }
 
 template <int MODE>
@@ -682,3 +687,4 @@ extern "C" int rfx_prox_rows(hipStream_t st, const int32_t* keys, int64_t n, int
   }
   return rfx_launch_status();
 }
+

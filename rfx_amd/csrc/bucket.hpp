@@ -655,6 +655,8 @@ constexpr int64_t kTreeLdsMaxN = 32767;
 constexpr int kTreeLdsThreads = 512;
 constexpr int kTreeLdsWaves = kTreeLdsThreads / 64;
 constexpr uint32_t kTreeLdsBytes = 159 * 1024;  // counters + the 16-bit permutation of the tree's slots
+// dynamic request + rs_tree_lds_kernel's static red_lo / red_hi / wsum
+static_assert(kTreeLdsBytes + 3 * 4 * kTreeLdsWaves <= kLdsLimitBytes, "per-tree sort exceeds the LDS");
 constexpr int kTreeLowBits = 15;                 // low digit of the two-pass form
 
 __device__ __forceinline__ uint32_t lds_half(const uint32_t* words, uint32_t e) {

@@ -73,6 +73,8 @@ __global__ void cw_leaf_draws_kernel(const uint32_t* __restrict__ K, const uint3
 #endif
 constexpr int kCwThreads = RFX_CW_THREADS;
 constexpr int64_t kCwMaxRows = RFX_CW_MAX_ROWS;
+// cw_line_kernel has no static LDS: its dynamic request is 8 * chunk_len <= 8 * kCwMaxRows.
+static_assert(8 * kCwMaxRows <= kLdsLimitBytes, "case-wise line counters exceed the LDS");
 
 // The wavefront's 64 member runs (lane l: run [s, s + len) of its tree, with
 // that tree's weight w) flattened across the lanes in aligned 16-byte blocks

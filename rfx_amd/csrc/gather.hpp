@@ -236,27 +236,27 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
 
 
 // Long line in chunks of `chunk` LDS counters, in order, for a
-// workgroup of 256 threads: lane l of wave w holds the cursors of trees
-// g*256 + w*64 + l (g < G, so T <= 256 G) into their member runs.  For every
+// workgroup of THREADS threads: thread x holds the cursors of trees
+// g*THREADS + x (g < G, so T <= THREADS G) into their member runs.  For every
 // chunk a lane finds where each of its runs crosses the chunk end by a
 // galloping search from its cursor, and the wavefront then visits the 64 run
 // pieces flattened across its lanes -- a leaf of 10,000 members costs its
 // wavefront no more per member than a leaf of 2.  No binary search from the
 // run start is ever repeated: cursors only move forward, and every pe entry
 // of the line is read once.
-template <int MODE, int LAYOUT, int G, class M>
+template <int MODE, int LAYOUT, int G, class M, int THREADS = 256>
 __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe, const uint32_t* __restrict__ bstart,
                                                    const uint32_t* __restrict__ mbucket, int64_t n, int32_t ntree,
                                                    const M* __restrict__ members, int32_t accumulate,
                                                    int32_t* __restrict__ out, int32_t* __restrict__ out2,
                                                    int64_t line, uint32_t* acc, uint32_t& diag_valid,
                                                    int64_t chunk = kGatherChunk) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
   uint32_t cur[G], end[G];
   uint32_t bagm = 0, valid = 0;
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int32_t t = g * 256 + wave * 64 + lane;
+    const int32_t t = g * THREADS + tid;
     cur[g] = 0;
     end[g] = 0;
     if (t < ntree) {
@@ -283,7 +283,7 @@ __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe,
     const int64_t hi = min(n, lo + chunk);
     const int len_chunk = static_cast<int>(hi - lo);
     __syncthreads();
-    for (int q = tid; q < len_chunk; q += 256) acc[q] = 0u;
+    for (int q = tid; q < len_chunk; q += THREADS) acc[q] = 0u;
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -296,7 +296,7 @@ __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe,
     if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi) acc[line - lo] = diag_valid;
     __syncthreads();
     const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, lo, n) : lo + line * n;
-    for (int q = tid; q < len_chunk; q += 256) {
+    for (int q = tid; q < len_chunk; q += THREADS) {
       const uint32_t v = acc[q];
       if (MODE == kMode_PAIR) {
         const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);

@@ -163,6 +163,7 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 #endif
 constexpr int kStageLb = RFX_LEAF_STAGE_LB;
 constexpr int64_t kStageMaxLds = 64 * 1024;
+static_assert(kStageMaxLds <= kLdsLimitBytes, "staged feature tile exceeds the LDS");
 // Wavefronts per workgroup W share the workgroup's LDS feature tile and take
 // consecutive groups of TPW trees each, ILP of them in lockstep.  Small trees
 // (the forest grown on up to kStageSmallN samples) use 8 x 4 trees per

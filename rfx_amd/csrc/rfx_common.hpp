@@ -24,3 +24,9 @@ __host__ __device__ inline int64_t rfx_upper_index(int64_t i, int64_t j, int64_t
 }
 
 inline unsigned rfx_ceil_div(int64_t a, int64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// LDS of one workgroup on gfx950 (160 KiB).  Every launch with dynamic LDS
+// asserts at compile time that its largest dynamic request plus the kernel's
+// static LDS stays within it: a larger dispatch is rejected by the runtime
+// and aborts the queue.
+constexpr int64_t kLdsLimitBytes = 163840;

@@ -10,7 +10,7 @@ from rfx_amd import _lib
 from rfx_amd.classifier import detect_categorical
 from rfx_amd.forest import Forest, make_dataset
 
-from helpers import load_golden, small_forest
+from helpers import GOLDEN as GOLDEN_DIR, load_golden, small_forest
 
 GOLDEN = ["gauss_n200.npz", "covtype_cat_n240.npz"]
 
@@ -306,3 +306,118 @@ def test_oob_row_closed_forms():
     assert np.array_equal(P4, P1 + oo)
     assert np.array_equal(np.diag(P4), diag)
     assert np.array_equal(P4.sum(axis=1), rows)
+
+
+# ---------------------------------------------------------------------------
+# Pins by the reference's RandomForest class, compute_outlier_scores and
+# compute_mds_3d_cpu themselves (fixtures written by tests/golden/make_golden.py
+# from oracle/_ref), and by closed forms of the MDS input.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", GOLDEN)
+def test_reference_class_pins_the_oracle_chain(name):
+    # RandomForest::fit on the CPU (find_terminal_node for in-bag samples,
+    # cpu_testreebag for out-of-bag ones, cpu_proximity per tree, then
+    # cpu_finishprox): the oracle's leaf assignment (with the categorical
+    # in-bag quirk), V1 counts and finishprox give the same matrix bit for bit
+    g = load_golden(name)
+    f = g["forest"]
+    cat = g["cat"] if int(g["maxcat"]) > 1 else None
+    nd = O.leaf_assign(f, g["x"], cat)
+    P = O.counts(f, nd, O.OOB_ASYM, 0).astype(np.float64)
+    S = O.finishprox(f.n, f.nout(2), P)
+    assert np.array_equal(g["rf_nout"], f.nout(2))
+    assert np.array_equal(S.view(np.uint64), g["rf_prox"].view(np.uint64))
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_reference_class_pins_casewise(name):
+    # the class with use_casewise: weights nin(j) / sum of nin over the leaf,
+    # summed tree by tree in FP64, then cpu_finishprox
+    g = load_golden(name)
+    f = g["forest"]
+    cat = g["cat"] if int(g["maxcat"]) > 1 else None
+    nd = O.leaf_assign(f, g["x"], cat)
+    S = O.finishprox(f.n, f.nout(2), O.prox_v1(f, nd, casewise=True))
+    assert np.array_equal(S.view(np.uint64), g["rf_prox_casewise"].view(np.uint64))
+
+
+def test_reference_outlier_scores_bitwise():
+    # RandomForest::compute_outlier_scores of an unsupervised forest's
+    # finished proximity, bit for bit
+    with np.load(GOLDEN_DIR / "unsup_n160.npz", allow_pickle=False) as z:
+        prox, want = z["prox"], z["outlier"]
+    n = want.size
+    got = O.outlier_scores(prox.reshape(n, n))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_mds_inputs_pinned_by_reference_mds(name):
+    # rf::compute_mds_3d_cpu (dsyevd) on the classifier's inputs against the
+    # oracle's renormalisation + FP64 classical MDS: the coordinates agree to
+    # rounding, so the renormalisation, the float32 widening, D = max - M,
+    # the squaring and centring all match
+    g = load_golden(name)
+    n = g["x"].shape[0]
+    S = g["rf_prox"].reshape(n, n, order="F")
+    X, w = O.classical_mds(O.mds_input_reference(S, g["rf_nout"]))
+    assert w[2] - w[3] > 1e-3 * w[0]
+    assert O.embedding_errors(X, g["mds_ref"])["rel_frobenius"] < 1e-10
+    P = g["rfgap"].reshape(n, n, order="F")
+    X, w = O.classical_mds(O.mds_input_rfgap(P, g["rf_nout"]))
+    assert w[2] - w[3] > 1e-3 * w[0]
+    assert O.embedding_errors(X, g["mds_rfgap"])["rel_frobenius"] < 1e-10
+
+
+def test_mds_input_reference_closed_forms():
+    # dyadic entries survive the float32 widening exactly
+    rng = np.random.default_rng(5)
+    n = 6
+    A = rng.integers(0, 64, size=(n, n)) / 64.0
+    S = 0.5 * (A + A.T)
+    np.fill_diagonal(S, 1.0)
+    # uniform nout = k: M = S / k off the diagonal, 1 on it
+    M = O.mds_input_reference(S, np.full(n, 4, np.int32))
+    want = S / 4.0
+    np.fill_diagonal(want, 1.0)
+    assert np.array_equal(M, want)
+    # unequal nout and a never-out-of-bag row (nout 0: not divided)
+    S3 = np.array([[1.0, 0.5, 0.25], [0.5, 1.0, 0.75], [0.25, 0.75, 1.0]])
+    M3 = O.mds_input_reference(S3, np.array([2, 4, 0], np.int32))
+    assert np.array_equal(M3, np.array([[1.0, 0.5 * (0.25 + 0.125), 0.5 * (0.125 + 0.25)],
+                                        [0.5 * (0.25 + 0.125), 1.0, 0.5 * (0.1875 + 0.75)],
+                                        [0.5 * (0.125 + 0.25), 0.5 * (0.1875 + 0.75), 1.0]]))
+    # a non-dyadic entry is rounded to float32 before the division
+    S4 = np.array([[1.0, 0.1], [0.1, 1.0]])
+    M4 = O.mds_input_reference(S4, np.array([1, 1], np.int32))
+    assert M4[0, 1] == float(np.float32(0.1)) and M4[0, 1] != 0.1
+
+
+def test_mds_input_rfgap_closed_forms():
+    # RF-GAP keeps the divided diagonal (no unit diagonal is imposed) and
+    # leaves a zero-nout row undivided
+    P = np.array([[2.0, 0.5, 0.25], [0.5, 4.0, 0.75], [0.25, 0.75, 1.0]])
+    M = O.mds_input_rfgap(P, np.array([2, 4, 0], np.int32))
+    R = np.array([[1.0, 0.25, 0.125], [0.125, 1.0, 0.1875], [0.25, 0.75, 1.0]])
+    want = 0.5 * (R + R.T)
+    np.fill_diagonal(want, [1.0, 1.0, 1.0])
+    assert np.array_equal(M, want)
+    M2 = O.mds_input_rfgap(P, np.array([4, 4, 4], np.int32))
+    assert np.array_equal(np.diag(M2), np.diag(P) / 4.0)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_reference_mds_fresh_inputs():
+    # compute_mds_3d_cpu on inputs no fixture holds: a random symmetric
+    # proximity with random nout, both forms
+    rng = np.random.default_rng(11)
+    n = 90
+    A = rng.random((n, n))
+    S = 0.5 * (A + A.T)
+    np.fill_diagonal(S, 1.0)
+    nout = rng.integers(0, 9, size=n).astype(np.int32)
+    for rfgap in (False, True):
+        Xr = O.ref_mds_3d(S.astype(np.float32).astype(np.float64).ravel(order="F"), nout, rfgap)
+        M = O.mds_input_rfgap(S, nout) if rfgap else O.mds_input_reference(S, nout)
+        X, w = O.classical_mds(M)
+        assert O.embedding_errors(X, Xr)["rel_frobenius"] < 1e-8 * max(1.0, w[0] / (w[2] - w[3]))
