@@ -312,7 +312,7 @@ int launch_rows(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int
 // chunks walked with per-tree cursors (gather_gallop_body in gather.hpp).
 // ---------------------------------------------------------------------------
 #ifndef RFX_GALLOP_CHUNK
-#define RFX_GALLOP_CHUNK 8192
+#define RFX_GALLOP_CHUNK 16384
 #endif
 // LDS counters per workgroup.  At n = 200,000, T = 300: 2,048 columns 87.2 ms
 // per call, 4,096 65.7, 6,144 62.8, 8,192 64.8, 16,384 76.4, 32,768 107.5;
@@ -321,7 +321,7 @@ constexpr int64_t kGallopChunk = RFX_GALLOP_CHUNK;
 // dynamic counters + the kernel's static diag_valid word
 static_assert(4 * kGallopChunk + 4 <= kLdsLimitBytes, "gallop chunk exceeds the LDS");
 #ifndef RFX_GALLOP_THREADS
-#define RFX_GALLOP_THREADS 256
+#define RFX_GALLOP_THREADS 1024
 #endif
 constexpr int kGallopThreads = RFX_GALLOP_THREADS;  // one tree cursor per thread and group
 static_assert(kGallopThreads % 64 == 0 && kGallopThreads <= 1024, "gallop workgroup size");
@@ -338,6 +338,119 @@ __global__ __launch_bounds__(kGallopThreads) void gather_gallop_kernel(const uin
   if (line >= n) return;
   gather_gallop_body<MODE, LAYOUT, G, uint32_t, kGallopThreads>(pe, bstart, mbucket, n, ntree, members, accumulate,
                                                                 out, out2, line, gallop_acc, diag_valid, chunk);
+}
+
+// ---------------------------------------------------------------------------
+// GATHER, long rows split by the chunk-boundary table (n > 32,768 with at
+// most kWinMaxBounds boundaries): one workgroup of kGallopThreads threads per
+// output line, thread x holding tree g * kGallopThreads + x.  At the start a
+// thread loads its tree's pe entry and key (independent loads), then the
+// leaf's boundary positions from wtab into registers; chunk c of the line is
+// the piece [cursor, boundary c + 1) of each run, so no search is made.  The
+// column chunks are aligned to kWinChunk and the members are read as 16-bit
+// offsets within their 32,768-column window (members_w).  For UPPER the
+// first chunk starts at the line's own column.
+// ---------------------------------------------------------------------------
+template <int MODE, int LAYOUT, int G>
+__global__ __launch_bounds__(kGallopThreads) void gather_window_kernel(
+    const uint2* __restrict__ pe, const int32_t* __restrict__ keys, const uint32_t* __restrict__ wtab, int nb,
+    const uint32_t* __restrict__ bstart, int64_t n, int32_t ntree, const uint16_t* __restrict__ members,
+    int32_t accumulate, int32_t* __restrict__ out, int32_t* __restrict__ out2) {
+  extern __shared__ uint32_t win_acc[];
+  __shared__ uint32_t diag_valid;
+  const int64_t line = xcd_row(blockIdx.x);
+  if (line >= n) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int c0 = LAYOUT == RFX_LAYOUT_UPPER ? static_cast<int>(line / kWinChunk) : 0;
+  uint32_t cur[G], end[G], wb[G][kWinMaxBounds];
+  uint32_t bagm = 0, valid = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int32_t t = g * kGallopThreads + tid;
+    cur[g] = 0;
+    end[g] = 0;
+    uint32_t e = 0, bag = 0, pos = 0;
+    int32_t key = -1;
+    if (t < ntree) {
+      const int64_t s = static_cast<int64_t>(t) * n + line;
+      const uint2 q = pe[s];
+      key = keys[s];
+      e = q.y & 0x7fffffffu;
+      bag = q.y >> 31;
+      pos = q.x;
+    }
+    if (e > 0) {
+      ++valid;
+      bagm |= bag << g;
+      const uint32_t leaf = static_cast<uint32_t>(key) >> 1;
+      const uint32_t* tb = wtab + static_cast<int64_t>(leaf) * nb;
+      // boundary k of the list is the start of chunk c0 + k + 1
+#pragma unroll
+      for (int k = 0; k < kWinMaxBounds; ++k) wb[g][k] = c0 + k < nb ? tb[c0 + k] : e;
+      cur[g] = LAYOUT == RFX_LAYOUT_UPPER ? pos + 1 : bstart[leaf];
+      end[g] = (MODE == kMode_OOB_ASYM && !bag) ? cur[g] : e;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kWinMaxBounds; ++k) wb[g][k] = 0u;
+    }
+  }
+  if (tid == 0) diag_valid = 0u;
+  __syncthreads();
+  if (MODE == kMode_ALL || MODE == kMode_PAIR) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
+    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+  }
+  const int nch = nb + 1;
+  for (int c = c0; c < nch; ++c) {
+    const int64_t lo = static_cast<int64_t>(c) * kWinChunk;
+    const int64_t hi = min(n, lo + kWinChunk);
+    const int64_t from = LAYOUT == RFX_LAYOUT_UPPER ? max(lo, line) : lo;  // first column written
+    const int64_t wbase = lo & ~(kWinSpan - 1);
+    __syncthreads();
+    for (int64_t q = from - lo + tid; q < hi - lo; q += kGallopThreads) win_acc[q] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint32_t e2 = min(wb[g][0], end[g]);
+      const uint32_t s0 = cur[g];
+      cur[g] = max(cur[g], e2);
+#pragma unroll
+      for (int k = 0; k + 1 < kWinMaxBounds; ++k) wb[g][k] = wb[g][k + 1];
+      wb[g][kWinMaxBounds - 1] = end[g];
+      line_visits<MODE, LAYOUT>(s0, e2 > s0 ? e2 - s0 : 0u, (bagm >> g) & 1u, lane, members, win_acc, lo - wbase,
+                                line - wbase);
+    }
+    __syncthreads();
+    if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi)
+      win_acc[line - lo] = diag_valid;
+    __syncthreads();
+    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, from, n) : lo + line * n;
+    const int64_t off = from - lo;
+    for (int64_t q = off + tid; q < hi - lo; q += kGallopThreads) {
+      const uint32_t v = win_acc[q];
+      const int64_t o = base + (q - off);
+      if (MODE == kMode_PAIR) {
+        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+        out[o] = accumulate ? out[o] + va : va;
+        out2[o] = accumulate ? out2[o] + vb : vb;
+      } else {
+        out[o] = accumulate ? out[o] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+      }
+    }
+  }
+}
+// counters of one chunk + the static diag_valid word
+static_assert(4 * kWinChunk + 4 <= kLdsLimitBytes, "window chunk exceeds the LDS");
+
+template <int MODE, int LAYOUT, int G>
+int launch_window(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
+                  int32_t* out2) {
+  const size_t lds = 4 * static_cast<size_t>(kWinChunk);
+  hipLaunchKernelGGL((gather_window_kernel<MODE, LAYOUT, G>), dim3(static_cast<unsigned>(xcd_rows_padded(n))),
+                     dim3(kGallopThreads), lds, st, w.pe, w.keys, w.wtab, w.nb, w.bstart, n, ntree, w.members_w,
+                     accumulate, out, out2);
+  return rfx_launch_status();
 }
 
 template <int MODE, int LAYOUT, int G>
@@ -399,7 +512,7 @@ int run_dense(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int
 template <int MODE, int LAYOUT>
 int run_bucket(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
-  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes, true, true);
   const int64_t slots = n * static_cast<int64_t>(ntree);
   const int64_t elems = LAYOUT == RFX_LAYOUT_UPPER ? n * (n + 1) / 2 : n * n;
   if (!accumulate) {
@@ -423,7 +536,8 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
 template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
-  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes, true, true);
+  w.keys = keys;
   // the line kernels (n <= kGatherLineMaxN) read no 32-bit members: a lean
   // build; for UPPER they read only pe and the 16-bit members
   const bool lean = n <= kGatherLineMaxN && w.members16 != nullptr;
@@ -456,6 +570,10 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
     return rfx_launch_status();
   }
   constexpr int TH = kGallopThreads;
+  if (w.wtab && w.members_w && w.keys && w.nb == win_bounds(n)) {
+    if (ntree <= TH) return launch_window<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
+    if (ntree <= 2 * TH) return launch_window<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
+  }
   if (ntree <= TH) return launch_gallop<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
   if (ntree <= 2 * TH) return launch_gallop<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
   if (ntree <= 4 * TH) return launch_gallop<MODE, LAYOUT, 4>(st, w, n, ntree, accumulate, out, out2);
@@ -496,7 +614,7 @@ constexpr double kAtomicVsCompare = 24.0;
 
 int choose_algo(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, void* ws,
                 size_t ws_bytes, int32_t* algo) {
-  if (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) {
+  if (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes, true)) {
     *algo = RFX_ALGO_DENSE;
     return RFX_OK;
   }
@@ -573,7 +691,7 @@ int count_problem_slice(hipStream_t st, int32_t variant, int32_t algo, const int
 extern "C" size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo) {
   if (algo == RFX_ALGO_DENSE) return 0;
   if (n <= 0 || ntree <= 0 || total_nodes <= 0) return 0;
-  return bucket_ws_bytes(n, ntree, total_nodes);
+  return bucket_ws_bytes(n, ntree, total_nodes, true);
 }
 
 extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
@@ -582,8 +700,8 @@ extern "C" int rfx_prox_pair_stats(hipStream_t st, const int32_t* keys, int64_t 
       total_nodes > RFX_MAX_TOTAL_NODES)
     return RFX_ERR_INVALID;
   if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
-  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
-  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes);
+  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes, true)) return RFX_ERR_WORKSPACE;
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes, true, true);
   int rc = bucket_build(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   rc = fill_u32(st, w.stat, 2, 0u);
@@ -605,8 +723,8 @@ extern "C" int rfx_prox_build_buckets(hipStream_t st, const int32_t* keys, int64
       total_nodes > RFX_MAX_TOTAL_NODES)
     return RFX_ERR_INVALID;
   if (n * static_cast<int64_t>(ntree) >= (1LL << 31)) return RFX_ERR_UNSUPPORTED;
-  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)) return RFX_ERR_WORKSPACE;
-  return bucket_build(st, keys, n, ntree, total_nodes, bucket_ws_carve(ws, n, ntree, total_nodes));
+  if (ws_bytes < bucket_ws_bytes(n, ntree, total_nodes, true)) return RFX_ERR_WORKSPACE;
+  return bucket_build(st, keys, n, ntree, total_nodes, bucket_ws_carve(ws, n, ntree, total_nodes, true, true));
 }
 
 extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
@@ -623,7 +741,7 @@ extern "C" int rfx_prox_accumulate(hipStream_t st, const int32_t* keys, int64_t 
   }
   {
     const int32_t base = algo & ~RFX_ALGO_PREBUILT;
-    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes, true)))
       return RFX_ERR_WORKSPACE;
   }
   switch (variant) {
@@ -653,7 +771,7 @@ extern "C" int rfx_prox_accumulate_pair(hipStream_t st, const int32_t* keys, int
   }
   {
     const int32_t base = algo & ~RFX_ALGO_PREBUILT;
-    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes)))
+    if ((base == RFX_ALGO_BUCKET || base == RFX_ALGO_GATHER) && (!ws || ws_bytes < bucket_ws_bytes(n, ntree, total_nodes, true)))
       return RFX_ERR_WORKSPACE;
   }
   return dispatch<kMode_PAIR>(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out_all, out_oob, ws);

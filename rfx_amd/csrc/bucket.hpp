@@ -27,6 +27,16 @@
 //   pe[s]               (position of slot s in members, end of its leaf |
 //                       in-bag << 31); (0, 0) off-leaf
 // so a sample finds the partners that follow it in its leaf with one load.
+// For long lines (n > 32,768 and at most kWinMaxBounds column-chunk
+// boundaries) the one-problem build of the count calls also writes
+//   members_w[p]        ((i mod 32,768) << 1) | in-bag: the member's column
+//                       relative to its 32,768-column window, 16 bits
+//   wtab[b * nb + k]    for leaf b and chunk boundary k + 1 (column
+//                       (k + 1) * kWinChunk): the first position in the
+//                       leaf's run whose column is at or past it (bend[b]
+//                       when none is)
+// so a line's run splits into its column chunks by table lookups instead of
+// searches, and each chunk's members are read as 16-bit window offsets.
 #pragma once
 
 #include "rfx_common.hpp"
@@ -34,6 +44,24 @@
 namespace {
 
 constexpr int64_t kMembers16MaxN = 32768;
+
+#ifndef RFX_WIN_CHUNK
+#define RFX_WIN_CHUNK 16384
+#endif
+#ifndef RFX_WIN_MAX_BOUNDS
+#define RFX_WIN_MAX_BOUNDS 16
+#endif
+constexpr int64_t kWinChunk = RFX_WIN_CHUNK;      // columns per LDS chunk of a long line
+constexpr int kWinMaxBounds = RFX_WIN_MAX_BOUNDS;  // chunk boundaries held per (line, tree)
+constexpr int64_t kWinSpan = 32768;                // members_w offsets are relative to these windows
+static_assert((kWinChunk & (kWinChunk - 1)) == 0 && kWinSpan % kWinChunk == 0, "chunks tile the windows");
+inline int win_bounds(int64_t n) { return static_cast<int>((n + kWinChunk - 1) / kWinChunk - 1); }
+inline bool win_enabled(int64_t n) { return n > kMembers16MaxN && win_bounds(n) <= kWinMaxBounds; }
+inline int win_shift() {
+  int s = 0;
+  while ((int64_t{1} << s) < kWinChunk) ++s;
+  return s;
+}
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanChunk = kScanThreads * kScanItems;
@@ -414,7 +442,7 @@ __global__ void rs_bounds_tree_kernel(const uint32_t* __restrict__ K, int64_t N,
 __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N, uint32_t n,
                                   uint32_t total_nodes, const uint32_t* __restrict__ bend,
                                   uint32_t* __restrict__ members, uint16_t* __restrict__ members16,
-                                  uint2* __restrict__ pe) {
+                                  uint2* __restrict__ pe, uint16_t* __restrict__ members_w = nullptr) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t v = V[p];
@@ -422,8 +450,30 @@ __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t
   const uint32_t i = s % n;
   members[p] = (i << 1) | bag;
   if (members16) members16[p] = static_cast<uint16_t>((i << 1) | bag);
+  if (members_w) members_w[p] = static_cast<uint16_t>(((i & (kWinSpan - 1)) << 1) | bag);
   const uint32_t k = K[p];
   pe[s] = k == total_nodes ? make_uint2(0u, 0u) : make_uint2(static_cast<uint32_t>(p), bend[k] | (bag << 31));
+}
+
+// Chunk-boundary table of the long-line counts (wtab above): slot p of leaf
+// b, column j, is the first of its run at or past boundary k + 1 for every
+// boundary after its predecessor's column and at most j; the last member of
+// the run also fills the boundaries after it with the run end.  Every
+// boundary of every non-empty leaf is written once.
+__global__ void win_table_kernel(const uint32_t* __restrict__ mbucket, const uint32_t* __restrict__ members,
+                                 const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend, int64_t N,
+                                 uint32_t total_nodes, int nb, int shift, uint32_t* __restrict__ wtab) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t b = mbucket[p];
+  if (b >= total_nodes) return;  // off-leaf slot
+  const uint32_t bs = bstart[b], be = bend[b];
+  const int cj = static_cast<int>((members[p] >> 1) >> shift);
+  const int first = static_cast<uint32_t>(p) == bs ? 1 : static_cast<int>((members[p - 1] >> 1) >> shift) + 1;
+  uint32_t* row = wtab + static_cast<int64_t>(b) * nb;
+  for (int c = first; c <= cj && c <= nb; ++c) row[c - 1] = static_cast<uint32_t>(p);
+  if (static_cast<uint32_t>(p) + 1 == be)
+    for (int c = cj + 1; c <= nb; ++c) row[c - 1] = be;
 }
 
 // Segmented form for a batch of problems laid end to end: slots
@@ -472,6 +522,10 @@ struct BucketWs {
   uint32_t* bend;
   uint32_t* members;
   uint16_t* members16;  // nullptr unless n <= kMembers16MaxN
+  uint16_t* members_w;  // nullptr unless the long-line window arrays are carved
+  uint32_t* wtab;       // chunk-boundary table, nb entries per node (or nullptr)
+  int nb;               // chunk boundaries per node in wtab
+  const int32_t* keys;  // the keys the buckets were built from (set by the count calls)
   uint32_t* mbucket;  // sorted keys (one of the two key buffers)
   uint2* pe;
   uint32_t* nvalid;
@@ -500,19 +554,23 @@ int radix_passes(int64_t total_nodes) {
 // tiles than the global layout of the same slots needs).
 int64_t rs_tiles_per_tree(int64_t n) { return (n + kRsTile - 1) / kRsTile; }
 
-size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes) {
+// windows: the count calls' workspace, which also holds the long-line window
+// arrays (members_w, wtab) when win_enabled(n).
+size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes, bool windows = false) {
   const int64_t slots = n * static_cast<int64_t>(ntree);
   const int64_t ntiles = ntree * rs_tiles_per_tree(n);
   const int64_t hlen = 256 * ntiles;
   const int64_t nparts = (hlen + kScanChunk - 1) / kScanChunk;
   return align256(4 * total_nodes) * 2 + align256(4 * slots) * 5 + align256(8 * slots) + align256(4 * hlen) +
          align256(4 * (nparts + 1)) + 512 + align256(4 * static_cast<int64_t>(ntree)) + 256 +
-         (n <= kMembers16MaxN ? align256(2 * slots) : 0);
+         (n <= kMembers16MaxN ? align256(2 * slots) : 0) +
+         (windows && win_enabled(n) ? align256(2 * slots) + align256(4 * total_nodes * win_bounds(n)) : 0);
 }
 
 // per_tree: the one-problem layout (segments of n slots per tree); false for
 // the batched build (one segment of all slots).
-BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes, bool per_tree = true) {
+BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes, bool per_tree = true,
+                         bool windows = false) {
   BucketWs w;
   char* p = static_cast<char*>(ws);
   const int64_t slots = n * static_cast<int64_t>(ntree);
@@ -548,6 +606,17 @@ BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes
   w.info = reinterpret_cast<uint32_t*>(p);
   p += 256;
   w.members16 = n <= kMembers16MaxN ? reinterpret_cast<uint16_t*>(p) : nullptr;
+  if (w.members16) p += align256(2 * slots);
+  w.members_w = nullptr;
+  w.wtab = nullptr;
+  w.nb = 0;
+  w.keys = nullptr;
+  if (per_tree && windows && win_enabled(n)) {
+    w.nb = win_bounds(n);
+    w.members_w = reinterpret_cast<uint16_t*>(p);
+    p += align256(2 * slots);
+    w.wtab = reinterpret_cast<uint32_t*>(p);
+  }
   // the per-tree sort ends in buffer 0 and writes the global leaf ids into K[1]
   w.mbucket = per_tree ? w.K[1] : w.K[w.passes & 1];
   return w;
@@ -958,7 +1027,10 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   if (rc) return rc;
   hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.V[0], N,
                      static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.members16,
-                     w.pe);
+                     w.pe, w.members_w);
+  if (w.wtab)
+    hipLaunchKernelGGL(win_table_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.members,
+                       w.bstart, w.bend, N, static_cast<uint32_t>(total_nodes), w.nb, win_shift(), w.wtab);
   return rfx_launch_status();
 }
 
