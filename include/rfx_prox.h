@@ -205,6 +205,25 @@ int rfx_prox_accumulate_pair(hipStream_t stream, const int32_t* keys, int64_t n,
                              int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,
                              int32_t* out_all, int32_t* out_oob, void* workspace, size_t workspace_bytes);
 
+/* No-allocation form of rfx_prox_accumulate_pair, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
+ * The leaf buckets live in a static device array of RFX_NOALLOC_SCRATCH_BYTES
+ * per GPU, held from library load until the process ends; the call allocates
+ * nothing and takes its device address once per device (the current device
+ * must be the stream's).  Problems whose bucket workspace
+ * (rfx_prox_workspace_bytes with a BUCKET or GATHER algo) exceeds the array
+ * return RFX_ERR_WORKSPACE; DENSE needs none.  AUTO, DENSE, BUCKET and GATHER
+ * as in rfx_prox_accumulate_pair; RFX_ALGO_PREBUILT is refused
+ * (RFX_ERR_INVALID): every call rebuilds its buckets.  All calls on one GPU
+ * share the array, which keeps the last call's contents: calls that overlap
+ * on one GPU (concurrent streams or host threads) race and must be
+ * serialised by the caller. */
+#define RFX_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int rfx_prox_accumulate_pair_noalloc(hipStream_t stream, const int32_t* keys, int64_t n, int32_t ntree,
+                                     int64_t total_nodes, int32_t layout, int32_t algo, int32_t accumulate,
+                                     int32_t* out_all, int32_t* out_oob);
+
 /* Rows [row_begin, row_end) of the FULL matrix of one variant, written
  * densely as (row_end-row_begin) x n int32 row-major (row r, column j at
  * (r-row_begin)*n + j).  Used to check large problems block by block. */

@@ -43,6 +43,7 @@ ALGO_GATHER = 3
 ALGO_PREBUILT = 0x100  # or-ed into BUCKET/GATHER: buckets already in the workspace
 
 MAX_TOTAL_NODES = 1073741823
+NOALLOC_SCRATCH_BYTES = 64 << 20  # RFX_NOALLOC_SCRATCH_BYTES
 
 MDS_COUNTS = 0
 MDS_REFERENCE = 1
@@ -87,6 +88,7 @@ _SIGS = {
     "rfx_prox_rfgap": (_i32, [_p, _p, _p, _p, _i64, _i32, _i64, _p, _p, _sz]),
     "rfx_prox_accumulate": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _i32, _p, _p, _sz]),
     "rfx_prox_accumulate_pair": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _p, _p, _p, _sz]),
+    "rfx_prox_accumulate_pair_noalloc": (_i32, [_p, _p, _i64, _i32, _i64, _i32, _i32, _i32, _p, _p]),
     "rfx_prox_rows": (_i32, [_p, _p, _i64, _i32, _i32, _i64, _i64, _p]),
     "rfx_prox_pair_stats": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz, C.POINTER(C.c_double)]),
     "rfx_finishprox": (_i32, [_p, _p, _i32, _p, _i64, _p, _i32]),

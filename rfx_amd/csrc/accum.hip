@@ -31,6 +31,7 @@
 // per tree), cuda/rf_proximity_upper_triangle.cu:238-248 (OOB_UPPER fold).
 
 #include <algorithm>
+#include <mutex>
 
 #include "rfx_common.hpp"
 #include "bucket.hpp"
@@ -806,3 +807,52 @@ extern "C" int rfx_prox_rows(hipStream_t st, const int32_t* keys, int64_t n, int
   return rfx_launch_status();
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation form of rfx_prox_accumulate_pair: the leaf-bucket workspace
+// is a file-scope device array, so the call allocates nothing.  Each GPU the
+// library runs on holds its own copy of the array until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only
+// that address reaches kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RFX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// Device address of the current device's scratch array.
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  RFX_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return RFX_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RFX_ERR_HIP;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return RFX_OK;
+}
+}  // namespace
+
+extern "C" int rfx_prox_accumulate_pair_noalloc(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree,
+                                                int64_t total_nodes, int32_t layout, int32_t algo,
+                                                int32_t accumulate, int32_t* out_all, int32_t* out_oob) {
+  // Every call rebuilds the buckets it reads: a prebuilt workspace would be
+  // whatever the last call on this GPU left in the shared array.
+  if (algo & RFX_ALGO_PREBUILT) return RFX_ERR_INVALID;
+  int rc = check_common(keys, n, ntree, total_nodes, layout, algo);
+  if (rc) return rc;
+  if (!out_all || !out_oob) return RFX_ERR_INVALID;
+  if (algo != RFX_ALGO_DENSE && bucket_ws_bytes(n, ntree, total_nodes, true) > RFX_NOALLOC_SCRATCH_BYTES)
+    return RFX_ERR_WORKSPACE;
+  void* ws = nullptr;
+  rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  return rfx_prox_accumulate_pair(st, keys, n, ntree, total_nodes, layout, algo, accumulate, out_all, out_oob, ws,
+                                  RFX_NOALLOC_SCRATCH_BYTES);
+}
