@@ -3,18 +3,23 @@
 
 A step is one pass of the hot path over the resident workload: leaf
 assignment of every sample through every tree, then the ALL and OOB upper
-int32 count matrices (both exact).  The default workload is the covtype-like
-problem: n = 20,000 samples, p = 54 features, 7 classes, T = 500 trees,
-mtry = 7, forest grown on the host by the restated reference grower (growth
-is setup, not timed).
+int32 count matrices (both exact).  The default workload is the metric's
+configuration: the covtype-like problem at n = 200,000 samples, p = 54
+features, 7 classes, T = 1000 trees, mtry = 7, the forest grown on the host
+by the restated reference grower (growth is setup, not timed).  After the
+timed steps it adds the 3-D MDS section: the exact dense MDS of C/T in FP64
+and FP32 at full n, rank-64 factors, and the factor MDS in each storage
+format, with errors against FP64.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--algo auto|dense|bucket|gather]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c5]
+                  [--algo auto|dense|bucket|gather] [--forest-cache PATH]
 
 value = T * n(n+1)/2 tree-pair updates per second (each tree-pair slot counts
 once although it feeds both matrices).  The line carries the roofline of the
 dominant kernel (HIP events on the launch stream) and a CPU baseline: the
 oracle's restatement of the reference's OpenMP path (leaf assignment +
-rf::cpu_proximity's accumulation) timed on a bounded sample of trees.
+rf::cpu_proximity's accumulation) timed on a bounded sample (at n = 200,000,
+on a seeded 20,000-sample subset of the samples through the same forest).
 """
 from __future__ import annotations
 
@@ -76,6 +81,28 @@ def _repeat_for(fn, min_seconds: float) -> tuple[int, float]:
             return passes, dt
 
 
+def host_cpu() -> dict:
+    """Model, sockets and cores of the host CPU (from /proc/cpuinfo)."""
+    info = {"model": None, "sockets": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        phys, cores, model = set(), {}, None
+        cur = None
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                cur = v
+                phys.add(v)
+            elif k == "cpu cores" and cur is not None:
+                cores[cur] = int(v)
+        info.update(model=model, sockets=len(phys) or None, physical_cores=sum(cores.values()) or None)
+    except OSError:
+        pass
+    return info
+
+
 def cpu_baseline(forest, x, sample_trees: int, min_seconds: float = 10.0) -> dict:
     """The oracle port of the reference CPU path (per tree: leaf assignment,
     then rf::cpu_proximity's accumulating OpenMP branch) on sample_trees trees,
@@ -98,8 +125,65 @@ def cpu_baseline(forest, x, sample_trees: int, min_seconds: float = 10.0) -> dic
         "sample": f"{passes} passes over the first {sample_trees} of {forest.ntree} trees at n={n}: oracle leaf "
                   f"assignment (src/rf_testreebag.cpp, find_terminal_node) + rf::cpu_proximity dense OpenMP "
                   f"branch (src/rf_proximity.cpp:99-231) into an n x n float64 matrix; {dt:.2f} s",
+        "host_cpu": host_cpu(),
         "seconds": dt,
     }
+
+
+def cpu_baseline_subset(forest, x, m: int = 20000, seed: int = 0, min_seconds: float = 10.0) -> dict:
+    """The same port where the reference's n x n float64 matrix cannot be held
+    (320 GB at n = 200,000): a seeded subset of m samples, assigned through the
+    same forest (each keeps its bootstrap counts), so the pair density, and
+    with it the work per tree-pair, is that of the full problem.  The rate is
+    reported; the full-size time is extrapolated from it."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    idx = np.sort(np.random.default_rng(seed).choice(forest.n, m, replace=False))
+    sub = forest.samples(idx)
+    xs = np.ascontiguousarray(x[idx])
+    T = forest.ntree
+
+    def one():
+        O.prox_v1(sub, O.leaf_assign(sub, xs))
+
+    passes, dt = _repeat_for(one, min_seconds)
+    rate = passes * T * (m * (m + 1) // 2) / dt
+    full = T * (forest.n * (forest.n + 1) // 2) / rate
+    return {
+        "value": rate,
+        "unit": "tree·pair updates/s",
+        "cores": int(O.num_threads()),
+        "kind": "port",
+        "sample": f"{passes} passes over all {T} trees on a seeded {m}-sample subset (seed {seed}) of the n={forest.n} "
+                  f"samples, assigned through the same forest: oracle leaf assignment + rf::cpu_proximity dense "
+                  f"OpenMP branch (src/rf_proximity.cpp:99-231) into an m x m float64 matrix; {dt:.2f} s.  "
+                  f"Extrapolated to the full problem: {full:.0f} s (the reference itself would need a "
+                  f"{8 * forest.n ** 2 / 1e9:.0f} GB float64 matrix and overflows its int32 indices above n = 46,340)",
+        "extrapolated_full_s": full,
+        "host_cpu": host_cpu(),
+    }
+
+
+def grow_forest(kind: int, n: int, p: int, ncl: int, T: int, mtry: int, cache: str | None):
+    """The workload's dataset and forest: the forest is grown by the restated
+    reference grower, or read from a forest file written by an earlier run."""
+    from rfx_amd.forest import Forest, make_dataset
+
+    t0 = time.perf_counter()
+    x, y = make_dataset(kind, n, p, ncl, 0)
+    if cache and Path(cache).exists():
+        forest = Forest.load(cache)
+        if (forest.n, forest.p, forest.ntree) != (n, p, T):
+            raise SystemExit(f"{cache} holds another forest ({forest.n}, {forest.p}, {forest.ntree})")
+        log(f"forest read from {cache}: {forest.total_nodes} nodes in {time.perf_counter() - t0:.1f} s")
+        return x, y, forest
+    forest = Forest.grow(x, y, T, mtry=mtry, iseed=0)
+    log(f"forest grown: {forest.total_nodes} nodes in {time.perf_counter() - t0:.1f} s")
+    if cache:
+        forest.save(cache)
+    return x, y, forest
 
 
 def main() -> int:
@@ -107,13 +191,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--rank", type=int, default=64, help="factor rank (c3)")
     ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket", "gather"])
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mds", action="store_true", help="c5: also time the 3-D MDS of every instance")
     ap.add_argument("--no-mds", action="store_true", help="c3: counts only, no factor / MDS section")
+    ap.add_argument("--forest-cache", default=None,
+                    help="forest file: read the forest from it if it exists, else grow and write it")
     ap.add_argument("--casewise", action="store_true",
                     help="c1/c2: also time the case-wise weighted FP64 proximity on the same keys")
     ap.add_argument("--rfgap", action="store_true",
@@ -145,10 +231,7 @@ def main() -> int:
         return bench_c5(args)
     NAMES = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket", _lib.ALGO_GATHER: "gather"}
     kind, n, p, ncl, T, mtry, desc = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    x, y = make_dataset(kind, n, p, ncl, 0)
-    forest = Forest.grow(x, y, T, mtry=mtry, iseed=0)
-    log(f"forest grown: {forest.total_nodes} nodes in {time.perf_counter() - t0:.1f} s")
+    x, y, forest = grow_forest(kind, n, p, ncl, T, mtry, args.forest_cache)
 
     dev = torch.device("cuda:0")
     df = DeviceForest(forest, x)
@@ -165,8 +248,10 @@ def main() -> int:
         cand = {}
         # row-segment GATHER and the T*n^2 DENSE compare are only candidates
         # at moderate n (see the AUTO rule in accum.hip)
+        # (beyond that the global-atomic BUCKET form is 30x slower than
+        # GATHER: 2.2 s against 0.07 s per call at n = 200,000, T = 1000)
         small = n <= 65536
-        algos = (_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER) if small else (_lib.ALGO_GATHER, _lib.ALGO_BUCKET)
+        algos = (_lib.ALGO_DENSE, _lib.ALGO_BUCKET, _lib.ALGO_GATHER) if small else (_lib.ALGO_GATHER,)
         reps = 3 if small else 1
         for a in algos:
             acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, a, out_all, out_oob)
@@ -229,9 +314,13 @@ def main() -> int:
     # Algorithmic bytes of the counting launch (one pass over all trees):
     # two int32 upper triangles written + the int32 key array read.
     # the GATHER kernel run_gather (accum.hip) picks: whole lines in LDS up to
-    # n = 20476 and T = 4096, line chunks up to n = 32768, else the gallop walk
+    # n = 20476 and T = 4096, line chunks up to n = 32768, the chunk-boundary
+    # table walk up to 16 boundaries of 16,384 columns (T <= 2048), else the
+    # gallop walk
     gather_kernel = ("gather_rows_kernel<PAIR,UPPER>" if n <= 20476 and T <= 4096 else
-                     "gather_line_kernel<PAIR,UPPER>" if n <= 32768 else "gather_gallop_kernel<PAIR,UPPER>")
+                     "gather_line_kernel<PAIR,UPPER>" if n <= 32768 else
+                     "gather_window_kernel<PAIR,UPPER>" if (n + 16383) // 16384 - 1 <= 16 and T <= 2048 else
+                     "gather_gallop_kernel<PAIR,UPPER>")
     KERNEL = {_lib.ALGO_GATHER: gather_kernel, _lib.ALGO_BUCKET: "bucket_pair_kernel<PAIR,UPPER>",
               _lib.ALGO_DENSE: "dense_tile_kernel<PAIR,UPPER>"}
     acc_bytes = 2 * 4 * P + 4 * n * T
@@ -255,7 +344,7 @@ def main() -> int:
     }
     if not args.no_cpu and n > 50000:
         # the reference CPU path holds an n x n float64 matrix (320 GB at n = 200,000)
-        line["cpu_baseline"] = None
+        line["cpu_baseline"] = cpu_baseline_subset(forest, x)
     elif not args.no_cpu:
         trees = args.cpu_trees or T
         cb = cpu_baseline(forest, x, trees)
@@ -267,9 +356,10 @@ def main() -> int:
         line["rfgap"] = rfgap_section(acc, df, forest, x, n, T, not args.no_cpu)
     if args.workload == "c3" and not args.no_mds:
         del keys
-        out_all = out_oob = None
+        out_oob = None
         torch.cuda.empty_cache()
-        line["mds"] = factor_mds_section(df, forest, args.rank)
+        line["mds"] = factor_mds_section(df, forest, x, y, out_all, args.rank, not args.no_cpu)
+        out_all = None
     print(json.dumps(line), flush=True)
     return 0
 
@@ -349,11 +439,21 @@ def rfgap_section(acc, df, forest, x, n: int, T: int, cpu: bool) -> dict:
     return res
 
 
-def factor_mds_section(df, forest, rank: int) -> dict:
-    """Rank-r factors of P = C/T and the 3-D MDS from them in each storage
-    format, timed, with the error of each against the FP64 operator on the
-    same factors; plus the truncation check on a seeded 20k-sample problem
-    (same generator) against the exact dense FP64 MDS."""
+def factor_mds_section(df, forest, x, y, counts_all, rank: int, cpu: bool) -> dict:
+    """The 3-D MDS half of the metric at the workload's n:
+      * the exact classical MDS of P = C/T from the ALL count triangle, with
+        the operator in FP64 (the FP64 reference on the exact proximity) and
+        in FP32 (the plain FP32 GPU form);
+      * rank-r factors of P, the FP64 MDS of those factors, and the factor
+        MDS from each storage format (FP32, BF16, INT8, FP8-e4m3 with
+        per-column scales): the FP32 VALU operator, and the matrix-core
+        operator on the stored factors followed by FP32 refinement sweeps;
+        errors against the FP64 MDS of the same factors and against the exact
+        dense FP64 MDS;
+      * the same pipelines rerun on a seeded 20,000-sample subset of the
+        samples (its own forest) against its exact dense FP64 MDS;
+      * the CPU port of rf::compute_mds_3d_cpu (renormalisation + full dsyevd)
+        on the n = 2,000 configuration beside the GPU dense MDS."""
     import numpy as np
     import torch
 
@@ -363,14 +463,15 @@ def factor_mds_section(df, forest, rank: int) -> dict:
                                    mds_factors_mixed, quantize_factors)
 
     def errs(X, Xref):
-        X = X.cpu().numpy()
+        X = X.cpu().numpy() if hasattr(X, "cpu") else X
+        Xref = Xref.cpu().numpy() if hasattr(Xref, "cpu") else Xref
         s = np.sign(np.sum(X * Xref, axis=0))
         s[s == 0] = 1
         rel = float(np.linalg.norm(X * s - Xref) / np.linalg.norm(Xref))
         qa, _ = np.linalg.qr(X)
         qb, _ = np.linalg.qr(Xref)
         ang = float(np.arccos(np.clip(np.linalg.svd(qa.T @ qb, compute_uv=False).min(), -1, 1)))
-        return rel, ang
+        return {"rel_err": rel, "max_angle_rad": ang}
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -379,7 +480,21 @@ def factor_mds_section(df, forest, rank: int) -> dict:
         torch.cuda.synchronize()
         return r, time.perf_counter() - t0
 
+    n, T = forest.n, forest.ntree
     out = {"rank": rank}
+    # exact dense MDS of C/T at full n (reads the 80 GB count triangle per block product)
+    (Xd, ed, itd), td = timed(lambda: mds_dense(counts_all, _lib.LAYOUT_UPPER, n, _lib.MDS_COUNTS, 1.0 / T,
+                                               precision=_lib.PREC_FP64))
+    Xexact = Xd.cpu().numpy()
+    out["dense_fp64"] = {"mds_s": round(td, 4), "iters": itd, "eigvals": [float(v) for v in ed[:5]],
+                         "eigengap_l3_l4": float(ed[2] - ed[3])}
+    log(f"dense fp64 mds: {td:.3f} s, {itd} iterations")
+    (X32, e32, it32), t32 = timed(lambda: mds_dense(counts_all, _lib.LAYOUT_UPPER, n, _lib.MDS_COUNTS, 1.0 / T,
+                                                   precision=_lib.PREC_FP32))
+    out["dense_fp32"] = {"mds_s": round(t32, 4), "iters": it32, "vs_dense_fp64": errs(X32, Xexact)}
+    log(f"dense fp32 mds: {t32:.3f} s")
+    del Xd, X32
+
     keys, _ = df.leaf_assign()
     (U, eig), t_fac = timed(lambda: lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2))
     del keys
@@ -390,41 +505,117 @@ def factor_mds_section(df, forest, rank: int) -> dict:
     U32, s32 = quantize_factors(U, _lib.PREC_FP32)
     (X64, e64, it64), t64 = timed(lambda: mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP64))
     Xref = X64.cpu().numpy()
-    out["fp64_s"] = round(t64, 4)
-    out["eigvals_fp64"] = [float(v) for v in e64[:5]]
-    out["eigengap_l3_l4"] = float(e64[2] - e64[3])
+    out["factor_fp64"] = {"mds_s": round(t64, 4), "iters": it64, "eigvals": [float(v) for v in e64[:5]],
+                          "eigengap_l3_l4": float(e64[2] - e64[3]), "vs_dense_fp64": errs(Xref, Xexact)}
+    fp32_form_s = None
     for name, fmt in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16), ("int8", _lib.PREC_INT8),
                       ("fp8", _lib.PREC_FP8)):
         Uq, sc = quantize_factors(U, fmt)
         mds_factors(Uq, sc, fmt, precision=_lib.PREC_FP32)  # warm-up
         (X, ev, it), t = timed(lambda: mds_factors(Uq, sc, fmt, precision=_lib.PREC_FP32))
-        rel, ang = errs(X, Xref)
-        out[name] = {"mds_s": round(t, 4), "iters": it, "rel_err_vs_fp64": rel, "max_angle_rad": ang,
-                     "bytes_per_factor_row": rank * {"fp32": 4, "bf16": 2, "int8": 1, "fp8": 1}[name]}
-        log(f"factor mds {name}: {t:.4f} s rel {rel:.2e}")
-        # bf16 matrix-core operator on the stored factors + FP32 refinement sweeps on the FP32 factors
+        if name == "fp32":
+            fp32_form_s = t
+        rec = {"bytes_per_factor_row": rank * {"fp32": 4, "bf16": 2, "int8": 1, "fp8": 1}[name],
+               "valu_fp32": {"mds_s": round(t, 4), "iters": it, "vs_factor_fp64": errs(X, Xref),
+                             "vs_dense_fp64": errs(X, Xexact)}}
+        log(f"factor mds {name} valu: {t:.4f} s")
         mds_factors_mixed(Uq, sc, fmt, U32)  # warm-up
         (X, ev, itm), t = timed(lambda: mds_factors_mixed(Uq, sc, fmt, U32))
-        rel, ang = errs(X, Xref)
-        out[name]["mfma_refined"] = {"mds_s": round(t, 4), "mfma_iters": itm[0], "fp32_refine_sweeps": itm[1],
-                                     "rel_err_vs_fp64": rel, "max_angle_rad": ang}
-        log(f"factor mds {name} mfma+refine: {t:.4f} s iters {itm} rel {rel:.2e}")
-    # truncation: the exact dense FP64 MDS of a seeded 20k problem vs its factor MDS
-    xs, ys = make_dataset(_lib.DATA_COVTYPE_LIKE, 20000, 54, 7, 7)
-    fs = Forest.grow(xs, ys, forest.ntree, mtry=7, iseed=0)
+        rec["mfma_refined"] = {"mds_s": round(t, 4), "mfma_iters": itm[0], "fp32_refine_sweeps": itm[1],
+                               "speedup_vs_fp32_valu_form": round(fp32_form_s / t, 3),
+                               "speedup_vs_dense_fp32": round(t32 / t, 3),
+                               "vs_factor_fp64": errs(X, Xref), "vs_dense_fp64": errs(X, Xexact)}
+        log(f"factor mds {name} mfma+refine: {t:.4f} s iters {itm}")
+        out[name] = rec
+    del U, U32
+    torch.cuda.empty_cache()
+
+    # the seeded 20k-sample subset: both pipelines rerun on it
+    m = 20000
+    idx = np.sort(np.random.default_rng(7).choice(n, m, replace=False))
+    xs, ys = np.ascontiguousarray(x[idx]), np.ascontiguousarray(y[idx])
+    fs = Forest.grow(xs, ys, T, mtry=7, iseed=0)
     dfs = DeviceForest(fs, xs)
     ks, _ = dfs.leaf_assign()
     acc = Accumulator(fs.n, fs.ntree, fs.total_nodes)
     Cs = acc.accumulate(ks, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_AUTO)
-    Xd, ed, _ = mds_dense(Cs, _lib.LAYOUT_UPPER, fs.n, _lib.MDS_COUNTS, 1.0 / fs.ntree, precision=_lib.PREC_FP64)
+    Xsd, esd, _ = mds_dense(Cs, _lib.LAYOUT_UPPER, m, _lib.MDS_COUNTS, 1.0 / T, precision=_lib.PREC_FP64)
+    Xsd = Xsd.cpu().numpy()
     del Cs
     Us, _ = lowrank_factors(ks, fs, rank=rank, oversample=16, power_iters=2)
-    Uq, sc = quantize_factors(Us, _lib.PREC_BF16)
-    Xs, es, _ = mds_factors(Uq, sc, _lib.PREC_BF16)
-    rel, ang = errs(Xs, Xd.cpu().numpy())
-    out["subset_20k"] = {"dense_fp64_eigs": [float(v) for v in ed[:4]], "factor_bf16_eigs": [float(v) for v in es[:4]],
-                         "rel_err_vs_dense_fp64": rel, "max_angle_rad": ang}
+    Us32, ss32 = quantize_factors(Us, _lib.PREC_FP32)
+    sub = {"n": m, "seed": 7, "dense_fp64_eigs": [float(v) for v in esd[:5]],
+           "eigengap_l3_l4": float(esd[2] - esd[3])}
+    X, _, _ = mds_factors(Us32, ss32, _lib.PREC_FP32, precision=_lib.PREC_FP64)
+    sub["factor_fp64"] = errs(X, Xsd)
+    for name, fmt in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16), ("int8", _lib.PREC_INT8),
+                      ("fp8", _lib.PREC_FP8)):
+        Uq, sc = quantize_factors(Us, fmt)
+        X, _, _ = mds_factors_mixed(Uq, sc, fmt, Us32)
+        sub[name] = errs(X, Xsd)
+    out["subset_20k_vs_dense_fp64"] = sub
+    del ks, Us, Us32, acc
+    torch.cuda.empty_cache()
+    if cpu:
+        out["cpu_mds_baseline"] = cpu_mds_baseline()
     return out
+
+
+def cpu_mds_baseline() -> dict:
+    """rf::compute_mds_3d_cpu's work on the n = 2,000 configuration (the
+    reference chain: OOB counts -> finishprox -> float32 -> nout
+    renormalisation -> D, D^2, double centring -> full dsyevd -> top three),
+    ported to numpy with LAPACK's dsyevd (scipy's OpenBLAS) on the host cores,
+    beside the GPU dense MDS of the same counts (reference chain, FP64 and FP32
+    operators)."""
+    import numpy as np
+    import torch
+    from scipy.linalg import eigh
+
+    from oracle import oracle as O
+    from rfx_amd import _lib
+    from rfx_amd.forest import Forest, make_dataset
+    from rfx_amd.proximity import Accumulator, DeviceForest, mds_dense
+
+    kind, n, p, ncl, T, mtry, _ = WORKLOADS["c1"]
+    x, y = make_dataset(kind, n, p, ncl, 0)
+    f = Forest.grow(x, y, T, mtry=mtry, iseed=0)
+    nout = f.nout(2)
+    P = O.counts(f, O.leaf_assign(f, x), O.OOB_ASYM, 0).astype(np.float64)
+    S = O.finishprox(n, nout, P).reshape(n, n, order="F")
+
+    def cpu_mds():
+        M = O.mds_input_reference(S, nout)
+        D = M.max() - M
+        np.fill_diagonal(D, 0.0)
+        D2 = D * D
+        r = D2.mean(axis=1)
+        B = -0.5 * (D2 - r[:, None] - r[None, :] + r.mean())
+        w, V = eigh(B, driver="evd")
+        return V[:, ::-1][:, :3] * np.sqrt(w[::-1][:3])
+
+    t0 = time.perf_counter()
+    Xc = cpu_mds()
+    tc = time.perf_counter() - t0
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(n, T, f.total_nodes)
+    U = acc.accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER, _lib.ALGO_AUTO)
+    nd = torch.from_numpy(nout).to(keys.device)
+    res = {"n": n, "ntree": T, "cpu_s": round(tc, 4), "cores": int(O.num_threads()), "kind": "port",
+           "lapack": "dsyevd (scipy's bundled OpenBLAS)", "host_cpu": host_cpu()}
+    for name, prec in (("gpu_dense_fp64", _lib.PREC_FP64), ("gpu_dense_fp32", _lib.PREC_FP32)):
+        mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_REFERENCE, 1.0, nd, precision=prec)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, _, it = mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_REFERENCE, 1.0, nd, precision=prec)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t0
+        Xg = X.cpu().numpy()
+        s = np.sign(np.sum(Xg * Xc, axis=0))
+        res[name] = {"mds_s": round(tg, 4), "iters": it, "speedup_vs_cpu": round(tc / tg, 2),
+                     "rel_err_vs_cpu": float(np.linalg.norm(Xg * s - Xc) / np.linalg.norm(Xc))}
+    return res
 
 
 def c5_instances(count: int = 64):

@@ -118,7 +118,11 @@ int orc_leaf_assign(const float* x, int64_t n, int32_t p, int32_t ntree, const i
   return 0;
 }
 
-/* One tree of the reference's accumulating dense branch. */
+/* One tree of the reference's accumulating dense branch.  ncount is indexed
+ * by terminal ordinal, so it holds max(n, nterm) entries: the reference sizes
+ * it n (src/rf_proximity.cpp:108-171), enough for a tree grown on the n
+ * samples it assigns, but a tree assigned only a subset of its samples can
+ * have more leaves than samples. */
 static void v1_one_tree(const int32_t* st, const int32_t* ndx, const int32_t* nin, int64_t n, int32_t nnode,
                         double* prox, int32_t* nod, int32_t* ncount, int32_t* ncn, int32_t* nodexb, int32_t* ndbegin,
                         int32_t* npcase, int casewise) {
@@ -126,8 +130,8 @@ static void v1_one_tree(const int32_t* st, const int32_t* ndx, const int32_t* ni
   for (int32_t k = 0; k < nnode; ++k) nod[k] = -1;
   for (int32_t k = 0; k < nnode; ++k)
     if (st[k] == -1) nod[k] = nterm++;
+  for (int32_t k = 0; k < nterm; ++k) ncount[k] = 0;
   for (int64_t i = 0; i < n; ++i) {
-    ncount[i] = 0;
     ncn[i] = 0;
     nodexb[i] = -1;
   }
@@ -180,20 +184,24 @@ static int prox_v1_all(const int32_t* nodestatus, const int64_t* node_offset, co
     if (nn > maxnn) maxnn = nn;
   }
   int32_t* nod = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
-  int32_t* ws = (int32_t*)malloc(sizeof(int32_t) * (size_t)(4 * n));
+  int32_t* ws = (int32_t*)malloc(sizeof(int32_t) * (size_t)(3 * n));
+  int32_t* ncount = (int32_t*)malloc(sizeof(int32_t) * (size_t)((int64_t)maxnn + 1 > n ? (int64_t)maxnn + 1 : n));
   int32_t* ndbegin = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 2));
-  if (!nod || !ws || !ndbegin) {
+  if (!nod || !ws || !ncount || !ndbegin) {
     free(nod);
     free(ws);
+    free(ncount);
     free(ndbegin);
     return -2;
   }
   for (int32_t t = 0; t < ntree; ++t) {
     v1_one_tree(nodestatus + node_offset[t], nodextr + (int64_t)t * n, nin + (int64_t)t * n, n,
-                (int32_t)(node_offset[t + 1] - node_offset[t]), prox, nod, ws, ws + n, ws + 2 * n, ndbegin, ws + 3 * n, casewise);
+                (int32_t)(node_offset[t + 1] - node_offset[t]), prox, nod, ncount, ws, ws + n, ndbegin, ws + 2 * n,
+                casewise);
   }
   free(nod);
   free(ws);
+  free(ncount);
   free(ndbegin);
   return 0;
 }

@@ -134,3 +134,38 @@ class Forest:
         off = self.node_offset[t0:t1 + 1] - b
         return Forest(self.n, self.p, off, self.nodestatus[b:e], self.bestvar[b:e], self.xbestsplit[b:e],
                       self.treemap[2 * b:2 * e], self.nodeclass[b:e], self.nin[t0:t1])
+
+    def samples(self, idx: np.ndarray) -> "Forest":
+        """The same trees with only the samples idx (ascending): their bootstrap
+        counts are kept, so each keeps its in-bag / out-of-bag status per tree."""
+        idx = np.asarray(idx, np.int64)
+        return Forest(len(idx), self.p, self.node_offset, self.nodestatus, self.bestvar, self.xbestsplit,
+                      self.treemap, self.nodeclass, np.ascontiguousarray(self.nin[:, idx]))
+
+    # Forest file: an uncompressed .npz of the flattened arrays above plus
+    # n, p and a format tag; read back with allow_pickle=False, so loading
+    # executes nothing from the file.
+    FORMAT = "rfx_amd.forest.v1"
+
+    def save(self, path) -> None:
+        with open(path, "wb") as fh:
+            np.savez(fh, format=np.array(self.FORMAT), n=np.int64(self.n), p=np.int64(self.p),
+                     node_offset=self.node_offset, nodestatus=self.nodestatus, bestvar=self.bestvar,
+                     xbestsplit=self.xbestsplit, treemap=self.treemap, nodeclass=self.nodeclass, nin=self.nin)
+
+    @classmethod
+    def load(cls, path) -> "Forest":
+        with np.load(path, allow_pickle=False) as z:
+            if str(z["format"]) != cls.FORMAT:
+                raise ValueError(f"{path}: not a {cls.FORMAT} forest file")
+            n, p = int(z["n"]), int(z["p"])
+            off = np.ascontiguousarray(z["node_offset"], np.int64)
+            f = cls(n, p, off, np.ascontiguousarray(z["nodestatus"], np.int32),
+                    np.ascontiguousarray(z["bestvar"], np.int32), np.ascontiguousarray(z["xbestsplit"], np.float32),
+                    np.ascontiguousarray(z["treemap"], np.int32), np.ascontiguousarray(z["nodeclass"], np.int32),
+                    np.ascontiguousarray(z["nin"], np.int32).reshape(len(off) - 1, n))
+        total = int(off[-1])
+        if (len(f.nodestatus), len(f.bestvar), len(f.xbestsplit), len(f.nodeclass), len(f.treemap)) != \
+                (total, total, total, total, 2 * total):
+            raise ValueError(f"{path}: node arrays do not match node_offset")
+        return f

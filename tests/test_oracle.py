@@ -421,3 +421,14 @@ def test_reference_mds_fresh_inputs():
         M = O.mds_input_rfgap(S, nout) if rfgap else O.mds_input_reference(S, nout)
         X, w = O.classical_mds(M)
         assert O.embedding_errors(X, Xr)["rel_frobenius"] < 1e-8 * max(1.0, w[0] / (w[2] - w[3]))
+
+
+def test_v1_on_a_sample_subset_of_larger_trees():
+    # trees grown on 3,000 samples assigned only 150 of them: more leaves
+    # than samples; the V1 port still equals the bucket-based counts
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 3000, 6, 3, 4, seed=8)
+    idx = np.sort(np.random.default_rng(1).choice(3000, 150, replace=False))
+    s = f.samples(idx)
+    assert min(int((s.nodestatus[s.node_offset[t]:s.node_offset[t + 1]] == -1).sum()) for t in range(4)) > 150
+    nd = O.leaf_assign(s, np.ascontiguousarray(x[idx]))
+    assert np.array_equal(O.prox_v1(s, nd), O.counts(s, nd, O.OOB_ASYM, 0).astype(np.float64))
