@@ -475,3 +475,66 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+
+/* Full rows of the ALL and OOB_UPPER count matrices for the rows listed in
+ * rows[nrows] (any order): all_out[r * n + j] = C[rows[r], j] (with the
+ * diagonal C_ii = number of trees in which i ends on a leaf) and
+ * oob_out[r * n + j] = U[rows[r], j] = P[i, j] + P[j, i] (diagonal 0), the
+ * sums over trees of the reference's same-leaf rule (src/rf_proximity.cpp:
+ * 179-230 for P).  Per tree: a counting sort of the samples by leaf, then
+ * each listed row walks its leaf's members, O(T (n + nrows * leaf)) in all,
+ * so it runs at sizes where the whole matrix does not fit the host. */
+int orc_counts_rows(const int32_t* nodestatus, const int64_t* node_offset, const int32_t* nodextr,
+                    const int32_t* nin, int64_t n, int32_t ntree, const int64_t* rows, int32_t nrows,
+                    int32_t* all_out, int32_t* oob_out) {
+  int32_t maxnn = 0;
+  for (int32_t t = 0; t < ntree; ++t) {
+    int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    if (nn > maxnn) maxnn = nn;
+  }
+  for (int32_t r = 0; r < nrows; ++r)
+    if (rows[r] < 0 || rows[r] >= n) return -1;
+  int32_t* head = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
+  int32_t* memb = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* cnt = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnn + 1));
+  if (!head || !memb || !cnt) {
+    free(head);
+    free(memb);
+    free(cnt);
+    return -2;
+  }
+  memset(all_out, 0, sizeof(int32_t) * (size_t)nrows * (size_t)n);
+  memset(oob_out, 0, sizeof(int32_t) * (size_t)nrows * (size_t)n);
+  for (int32_t t = 0; t < ntree; ++t) {
+    const int32_t* st = nodestatus + node_offset[t];
+    const int32_t nn = (int32_t)(node_offset[t + 1] - node_offset[t]);
+    const int32_t* ndx = nodextr + (int64_t)t * n;
+    const int32_t* bag = nin + (int64_t)t * n;
+    memset(cnt, 0, sizeof(int32_t) * (size_t)(nn + 1));
+    for (int64_t i = 0; i < n; ++i)
+      if (valid_leaf(st, ndx[i], nn)) cnt[ndx[i] + 1]++;
+    head[0] = 0;
+    for (int32_t k = 0; k < nn; ++k) head[k + 1] = head[k] + cnt[k + 1];
+    memset(cnt, 0, sizeof(int32_t) * (size_t)(nn + 1));
+    for (int64_t i = 0; i < n; ++i)
+      if (valid_leaf(st, ndx[i], nn)) memb[head[ndx[i]] + cnt[ndx[i]]++] = (int32_t)i;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int32_t r = 0; r < nrows; ++r) {
+      const int64_t i = rows[r];
+      const int32_t k = ndx[i];
+      if (!valid_leaf(st, k, nn)) continue;
+      const int bi = bag[i] > 0;
+      int32_t* ar = all_out + (int64_t)r * n;
+      int32_t* orow = oob_out + (int64_t)r * n;
+      for (int32_t a = head[k]; a < head[k + 1]; ++a) {
+        const int64_t j = memb[a];
+        ar[j] += 1; /* j == i gives the diagonal */
+        if ((bag[j] > 0) != bi) orow[j] += 1;
+      }
+    }
+  }
+  free(head);
+  free(memb);
+  free(cnt);
+  return 0;
+}

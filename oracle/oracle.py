@@ -54,6 +54,8 @@ def orc() -> C.CDLL:
         L.orc_counts_bucket.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]
         L.orc_counts_brute.argtypes = [_p, _p, _p, _p, _i64, _i32, _i32, _p]
         L.orc_finishprox.argtypes = [_i64, _p, _p, _p]
+        L.orc_counts_rows.argtypes = [_p, _p, _p, _p, _i64, _i32, _p, _i32, _p, _p]
+        L.orc_counts_rows.restype = _i32
         for f in (L.orc_leaf_assign, L.orc_prox_v1, L.orc_prox_v1_casewise, L.orc_prox_rfgap, L.orc_outlier_scores,
                   L.orc_counts_bucket,
                   L.orc_counts_brute, L.orc_finishprox):
@@ -156,6 +158,18 @@ def counts_brute(forest, nodextr: np.ndarray, variant: int) -> np.ndarray:
     _ok(orc().orc_counts_brute(_a(forest.nodestatus), _a(forest.node_offset), _a(np.ascontiguousarray(nodextr)),
                                _a(forest.nin), n, forest.ntree, variant, _a(out)), "orc_counts_brute")
     return out
+
+
+def counts_rows(forest, nodextr: np.ndarray, rows) -> tuple[np.ndarray, np.ndarray]:
+    """Full rows (len(rows) x n) of the ALL and OOB_UPPER count matrices."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    n = forest.n
+    a = np.empty((len(rows), n), np.int32)
+    o = np.empty((len(rows), n), np.int32)
+    nd = np.ascontiguousarray(nodextr, np.int32)
+    _ok(orc().orc_counts_rows(_a(forest.nodestatus), _a(forest.node_offset), _a(nd), _a(forest.nin), n,
+                              forest.ntree, _a(rows), len(rows), _a(a), _a(o)), "orc_counts_rows")
+    return a, o
 
 
 def finishprox(n: int, nout: np.ndarray, prox: np.ndarray) -> np.ndarray:

@@ -180,10 +180,10 @@ def test_oob_depth_cap_and_single_leaf_trees(gpu):
 
 
 def test_full_size_properties_c2_shape(gpu):
-    """n = 20,000, T = 500 (the covtype-like workload): GPU counts against the
-    oracle's on all rows of a seeded block, plus size-independent invariants
-    over the whole matrix (ALL diagonal = T, symmetry of the two algorithms'
-    outputs, total = sum over leaves of |leaf|^2)."""
+    """n = 20,000, T = 500 (the covtype-like workload): the whole ALL and OOB
+    upper matrices of all three algorithms against the oracle's, plus
+    invariants (ALL diagonal = T, total = sum over leaves of |leaf|^2) and the
+    dense row kernel on a block of rows."""
     _say("c2 full-size properties")
     x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, 20000, 54, 7, 500, 0, mtry=7)
     df, keys, _ = run_keys(f, x)
@@ -202,11 +202,11 @@ def test_full_size_properties_c2_shape(gpu):
     total = 2 * int(a_b.sum(dtype=torch.int64).item()) - f.ntree * n
     assert total == sq
     nd = O.leaf_assign(f, x)
+    assert np.array_equal(o_b.cpu().numpy(), O.counts(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER))
+    assert np.array_equal(a_b.cpu().numpy(), O.counts(f, nd, _lib.PROX_ALL, _lib.LAYOUT_UPPER))
     rows = acc.rows(keys, _lib.PROX_OOB_UPPER, 12345, 12345 + 64).cpu().numpy()
-    sub = O.counts(f.subset(slice(0, f.ntree)), nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)
     for k, i in enumerate(range(12345, 12345 + 64)):
         b = i * n - i * (i - 1) // 2
-        assert np.array_equal(sub[b:b + n - i], rows[k, i:])
         assert np.array_equal(o_b[b:b + n - i].cpu().numpy(), rows[k, i:])
 
 

@@ -432,3 +432,17 @@ def test_v1_on_a_sample_subset_of_larger_trees():
     assert min(int((s.nodestatus[s.node_offset[t]:s.node_offset[t + 1]] == -1).sum()) for t in range(4)) > 150
     nd = O.leaf_assign(s, np.ascontiguousarray(x[idx]))
     assert np.array_equal(O.prox_v1(s, nd), O.counts(s, nd, O.OOB_ASYM, 0).astype(np.float64))
+
+
+def test_counts_rows_equal_full_matrices():
+    # the row-restricted oracle against the full ALL and OOB_UPPER matrices
+    # (themselves pinned by the brute force above), rows in any order
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 700, 6, 3, 9, seed=4)
+    nd = O.leaf_assign(f, x)
+    n = f.n
+    C = O.counts(f, nd, O.ALL, 0).reshape(n, n, order="F")
+    U = O.counts(f, nd, O.OOB_UPPER, 0).reshape(n, n, order="F")
+    U = np.triu(U) + np.triu(U, 1).T
+    rows = np.array([699, 0, 350, 3, 350])
+    a, o = O.counts_rows(f, nd, rows)
+    assert np.array_equal(a, C[rows]) and np.array_equal(o, U[rows])
