@@ -74,7 +74,11 @@ extern "C" {
  * leaf buckets of these keys (rfx_prox_build_buckets), so only the counting
  * kernel runs.  A call without it builds in the workspace only what its own
  * kernel reads and leaves the rest of the buckets unspecified: build again
- * with rfx_prox_build_buckets before the next RFX_ALGO_PREBUILT call. */
+ * with rfx_prox_build_buckets before the next RFX_ALGO_PREBUILT call.  The
+ * library records, per workspace address, the shape of its last build and
+ * whether it was complete; a PREBUILT call whose workspace was last built
+ * partially, for another (n, T, total_nodes), or not at all returns
+ * RFX_ERR_INVALID.  It cannot see whether the keys changed since. */
 #define RFX_ALGO_PREBUILT 0x100
 
 /* ========================================================================
@@ -349,6 +353,15 @@ int rfx_lowrank_factors(hipStream_t stream, const int32_t* keys, int64_t n, int3
  * (cuda/rf_proximity_lowrank.cu:1412-1553). */
 int rfx_quantize_factors(hipStream_t stream, const float* U, int64_t n, int32_t rank, int32_t format, void* Uq,
                          float* scales);
+
+/* P = U U^T: the dense FP64 n x n column-major proximity of factors stored
+ * in `format` (FP32 factors need no scales), each entry the exact FP64 sum
+ * of the dequantised products in column order.  Replaces
+ * LowRankProximityMatrix::reconstruct_full_matrix
+ * (include/rf_proximity_lowrank.hpp:142, cuda/rf_proximity_lowrank.cu:
+ * 1307-1367, P = A B^T with A = B here).  rank <= 128, n <= 4,194,240. */
+int rfx_reconstruct_factors(hipStream_t stream, const void* Uq, const float* scales, int32_t format, int64_t n,
+                            int32_t rank, double* out);
 
 /* A9: classical 3-D MDS from factors: E_ij = (1 - u_i.u_j)^2 off the diagonal,
  * 0 on it, double-centred with FP64 row means; top three eigenpairs by the

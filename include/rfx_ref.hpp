@@ -16,6 +16,8 @@
 #ifndef RFX_REF_HPP
 #define RFX_REF_HPP
 
+#include <hip/hip_fp16.h>
+
 #include <cstdint>
 #include <vector>
 
@@ -27,26 +29,52 @@ namespace ref {
 using integer_t = int32_t;
 using dp_t = double;
 
+// The process-wide switch the reference's per-tree proximity reads from its
+// global configuration (rf::g_config.use_casewise, include/rf_config.hpp:39,
+// read at src/rf_proximity.cpp:34 and cuda/rf_proximity.cu:345, 411; set by
+// RandomForest::fit from its config, src/rf_random_forest.cpp:442).  A caller
+// that sets rf::g_config.use_casewise sets rfx::ref::g_config.use_casewise.
+struct Config {
+  bool use_casewise = false;  // weights nin(j) / (sum of nin over the leaf) instead of 1
+};
+extern Config g_config;
+
 // Per-tree OOB proximity, accumulated into the caller's column-major n x n
-// float64 prox: prox[i + j*n] += 1 for i out-of-bag, j in-bag, same terminal
-// node (nodestatus -1).  Replaces rf::cpu_proximity / rf::gpu_proximity
-// (include/rf_proximity.hpp:10-21, src/rf_proximity.cpp:99-231,
-// non-case-wise).  nod/ncount/ncn/nodexb/ndbegin/npcase are the reference's
-// caller-owned scratch arrays; they are accepted and not touched.
+// float64 prox.  Replaces rf::cpu_proximity (include/rf_proximity.hpp:10-21,
+// src/rf_proximity.cpp:99-231): for i out-of-bag and j in-bag in the same
+// terminal node (nodestatus -1), prox[i + j*n] += 1, or with
+// g_config.use_casewise += nin(j) / (sum of nin over the node's in-bag cases)
+// (src/rf_proximity.cpp:186-223).  This is the reference's accumulating
+// dense branch at every n; its n > 5000 block-sparse branch
+// (src/rf_proximity.cpp:37-98), which overwrites prox with the last tree
+// alone, is not reproduced.  nod/ncount/ncn/nodexb/ndbegin/npcase are the
+// reference's caller-owned scratch arrays; they are accepted and not touched.
 void cpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
                    integer_t nnode, dp_t* prox, integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
                    integer_t* ndbegin, integer_t* npcase);
+// Replaces rf::gpu_proximity (cuda/rf_proximity.cu:257-527) with its
+// dispatch: when nsample > 5000 and nnode > 10 it applies the rule of
+// cuda_proximity_kernel (cuda/rf_proximity.cu:32-67): for i out-of-bag,
+// prox[i + j*n] += 1 for every j in the same terminal node, in-bag or not,
+// i itself included, with no case-wise weights (that kernel ignores them);
+// otherwise the rule of cpu_proximity above (cuda_batch_proximity_kernel,
+// cuda/rf_proximity.cu:72-127, case-wise per g_config).  The reference
+// launches the first kernel on a one-dimensional grid although it indexes
+// rows by the second grid dimension, so it only ever fills row 0; here every
+// out-of-bag row gets the kernel's rule.  Accumulates into prox, which the
+// reference's caller does with a zeroed per-tree matrix
+// (cuda/rf_growtree.cu:4096-4118).
 void gpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
                    integer_t nnode, dp_t* prox, integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
                    integer_t* ndbegin, integer_t* npcase);
 
 // One tree's OOB fold U[i<=j] = P_ij + P_ji into a device packed upper
-// triangle of n(n+1)/2 IEEE half values (bits as uint16_t), overwritten.
+// triangle of n(n+1)/2 IEEE half values, overwritten (exact: counts <= 2048).
 // Replaces rf::cuda::gpu_proximity_upper_triangle_fp16
 // (include/rf_proximity_upper_triangle.hpp:42-55,
 // cuda/rf_proximity_upper_triangle.cu:149-306, zeroed at 948).
 void gpu_proximity_upper_triangle_fp16(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin,
-                                       integer_t nsample, integer_t nnode, uint16_t* prox_upper_fp16,
+                                       integer_t nsample, integer_t nnode, __half* prox_upper_fp16,
                                        integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
                                        integer_t* ndbegin, integer_t* npcase);
 
@@ -107,11 +135,24 @@ class LowRankProximityMatrix {
   // One tree (host arrays: nodestatus[nnode], nodextr[n], nin[n]).
   void add_tree_contribution(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin,
                              integer_t nnode);
+  // The reference's per-tree entry for a precomputed FP16 OOB triangle
+  // (include/rf_proximity_lowrank.hpp:106).  A packed count triangle carries
+  // no leaf structure, from which this build's matrix-free factors of C / T
+  // are formed, so the call throws std::runtime_error naming
+  // add_tree_contribution, the entry that takes the tree itself.
+  void add_tree_contribution_incremental_upper_triangle_fp16(const __half* tree_proximity_upper_fp16,
+                                                             integer_t nsample);
   // Builds the factors once all trees are in (final_call true).
   void finalize_accumulation(integer_t n_trees_processed, bool final_call = true);
   void get_factors(dp_t** A_dev, dp_t** B_dev, integer_t* rank);
   // n x k row-major MDS coordinates from the factors (k <= 3).
   std::vector<double> compute_mds_from_factors(integer_t k = 3);
+  // P = A B^T (= U U^T) into the device float64 n x n column-major
+  // output_prox, from the factors stored in `level` (FP32 uses the FP32
+  // factors; another level than the object's storage quantises a copy).
+  // Replaces reconstruct_full_matrix (include/rf_proximity_lowrank.hpp:142,
+  // cuda/rf_proximity_lowrank.cu:1307-1367).
+  void reconstruct_full_matrix(dp_t* output_prox, integer_t nsample, FactorFormat level = FactorFormat::FP32);
   integer_t get_rank() const { return rank_; }
   integer_t get_n_samples() const { return n_; }
   std::vector<double> top_eigenvalues() const { return eig_; }
@@ -139,6 +180,11 @@ std::vector<double> compute_mds_from_factors_host(void* lowrank_proximity_ptr, i
 inline std::vector<double> compute_mds_3d_from_factors_host(void* lowrank_proximity_ptr) {
   return compute_mds_from_factors_host(lowrank_proximity_ptr, 3);
 }
+// include/rf_lowrank_helpers.hpp:29-34 (cuda/rf_lowrank_helpers.cu:73-130):
+// the full float64 n x n column-major P = U U^T (FP32 factors) in a host
+// array allocated with new[] that the caller delete[]s; false on failure and,
+// as the reference, for nsamples > 50,000.
+bool reconstruct_proximity_matrix_host(void* lowrank_proximity_ptr, dp_t** output_prox, integer_t nsamples);
 
 }  // namespace ref
 }  // namespace rfx

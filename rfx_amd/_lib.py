@@ -97,6 +97,7 @@ _SIGS = {
     "rfx_lowrank_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32, _i32]),
     "rfx_lowrank_factors": (_i32, [_p, _p, _i64, _i32, _i64, _p, _i32, _i32, _i32, _u64, _p, _sz, _p, _p]),
     "rfx_quantize_factors": (_i32, [_p, _p, _i64, _i32, _i32, _p, _p]),
+    "rfx_reconstruct_factors": (_i32, [_p, _p, _p, _i32, _i64, _i32, _p]),
     "rfx_mds_factors_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "rfx_mds_factors": (_i32, [_p, _p, _p, _i32, _i64, _i32, _i32, _i32, C.c_double, _p, _sz, _p, _p, _p]),
     "rfx_mds_factors_mixed_workspace_bytes": (_sz, [_i64, _i32]),

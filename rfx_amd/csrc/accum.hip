@@ -38,6 +38,38 @@
 #include "gather.hpp"
 #include "dense.hpp"
 
+// The workspace build records read by RFX_ALGO_PREBUILT calls (bucket.hpp):
+// a small table, most recent builds kept, guarded for concurrent callers.
+namespace {
+constexpr int kBuildRecords = 64;
+std::mutex g_build_mu;
+const void* g_build_key[kBuildRecords] = {};
+BucketBuildRecord g_build_rec[kBuildRecords];
+int g_build_next = 0;
+}  // namespace
+
+void note_bucket_build(const void* ws, const BucketBuildRecord& r) {
+  std::lock_guard<std::mutex> lock(g_build_mu);
+  for (int k = 0; k < kBuildRecords; ++k)
+    if (g_build_key[k] == ws) {
+      g_build_rec[k] = r;
+      return;
+    }
+  g_build_key[g_build_next] = ws;
+  g_build_rec[g_build_next] = r;
+  g_build_next = (g_build_next + 1) % kBuildRecords;
+}
+
+bool last_bucket_build(const void* ws, BucketBuildRecord* r) {
+  std::lock_guard<std::mutex> lock(g_build_mu);
+  for (int k = 0; k < kBuildRecords; ++k)
+    if (g_build_key[k] == ws) {
+      *r = g_build_rec[k];
+      return true;
+    }
+  return false;
+}
+
 namespace {
 
 
@@ -583,11 +615,22 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
   return RFX_ERR_UNSUPPORTED;
 }
 
+// A RFX_ALGO_PREBUILT call needs the workspace's last build to be a full
+// one (rfx_prox_build_buckets, or a BUCKET / long-line GATHER count call) of
+// the same shape, with the window arrays where the GATHER kernel reads them.
+bool prebuilt_usable(const void* ws, int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo) {
+  BucketBuildRecord r;
+  if (!last_bucket_build(ws, &r)) return false;
+  if (!r.full || r.n != n || r.ntree != ntree || r.total_nodes != total_nodes) return false;
+  return !(algo == RFX_ALGO_GATHER && win_enabled(n) && !r.windows);
+}
+
 template <int MODE>
 int dispatch(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes, int32_t layout,
              int32_t algo, int32_t accumulate, int32_t* out, int32_t* out2, void* ws) {
   const bool prebuilt = (algo & RFX_ALGO_PREBUILT) != 0;
   algo &= ~RFX_ALGO_PREBUILT;
+  if (prebuilt && !prebuilt_usable(ws, n, ntree, total_nodes, algo)) return RFX_ERR_INVALID;
   if (algo == RFX_ALGO_DENSE) {
     return layout == RFX_LAYOUT_UPPER ? run_dense<MODE, RFX_LAYOUT_UPPER>(st, keys, n, ntree, accumulate, out, out2)
                                       : run_dense<MODE, RFX_LAYOUT_FULL>(st, keys, n, ntree, accumulate, out, out2);

@@ -41,6 +41,20 @@
 
 #include "rfx_common.hpp"
 
+// Host record of the last bucket build in each workspace, keyed by the
+// workspace's base address (defined in accum.hip): RFX_ALGO_PREBUILT calls
+// refuse a workspace whose last build skipped arrays, lacks the long-line
+// window arrays their kernel reads, or was made for another shape.
+struct BucketBuildRecord {
+  int64_t n = 0;
+  int32_t ntree = 0;
+  int64_t total_nodes = 0;
+  bool full = false;     // every array of the per-tree layout was written
+  bool windows = false;  // members_w and wtab were written too
+};
+void note_bucket_build(const void* ws, const BucketBuildRecord& r);
+bool last_bucket_build(const void* ws, BucketBuildRecord* r);
+
 namespace {
 
 constexpr int64_t kMembers16MaxN = 32768;
@@ -1021,7 +1035,9 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
                        st, keys, n, static_cast<uint32_t>(total_nodes), w.K[0], rows_upper ? nullptr : w.mbucket,
                        lean ? nullptr : w.V[0], lean ? nullptr : w.members, w.members16, w.pe,
                        rows_upper ? nullptr : w.bstart, w.bend, w.nvalid);
-    return rfx_launch_status();
+    const int rc = rfx_launch_status();
+    note_bucket_build(w.bstart, BucketBuildRecord{n, ntree, total_nodes, rc == RFX_OK && !lean, false});
+    return rc;
   }
   int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
@@ -1031,7 +1047,9 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   if (w.wtab)
     hipLaunchKernelGGL(win_table_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.members,
                        w.bstart, w.bend, N, static_cast<uint32_t>(total_nodes), w.nb, win_shift(), w.wtab);
-  return rfx_launch_status();
+  rc = rfx_launch_status();
+  note_bucket_build(w.bstart, BucketBuildRecord{n, ntree, total_nodes, rc == RFX_OK, rc == RFX_OK && w.wtab});
+  return rc;
 }
 
 // Bucket build over a batch of problems whose key arrays lie end to end

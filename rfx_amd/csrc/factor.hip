@@ -774,11 +774,90 @@ int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U
   return RFX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// P = U U^T as a dense FP64 n x n column-major matrix (the factors' own
+// proximity): 64 x 64 output tiles, the two row blocks of U dequantised to
+// FP32 in LDS, each thread a 4 x 4 register tile accumulated in FP64 (the
+// FP32 products are exact in FP64, so the result is the exact FP64 sum of
+// the stored factors' products, in column order).
+// ---------------------------------------------------------------------------
+constexpr int kRcTile = 64;
+constexpr int kRcMaxRank = 128;
+
+template <int FMT>
+__global__ __launch_bounds__(256) void reconstruct_kernel(const typename Store<FMT>::T* __restrict__ U,
+                                                          const float* __restrict__ scale, int64_t n, int r,
+                                                          double* __restrict__ out) {
+  __shared__ float Ui[kRcMaxRank][kRcTile + 1];
+  __shared__ float Uj[kRcMaxRank][kRcTile + 1];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kRcTile, j0 = static_cast<int64_t>(blockIdx.y) * kRcTile;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kRcTile * r; e += 256) {
+    const int row = e / r, col = e % r;  // consecutive threads read one factor row
+    const int64_t i = i0 + row, j = j0 + row;
+    Ui[col][row] = i < n ? Store<FMT>::load(U, i * r + col, scale, col) : 0.f;
+    Uj[col][row] = j < n ? Store<FMT>::load(U, j * r + col, scale, col) : 0.f;
+  }
+  __syncthreads();
+  const int ti = tid & 15, tj = tid >> 4;  // rows ti + 16a, columns tj + 16b
+  double acc[4][4] = {};
+  for (int c = 0; c < r; ++c) {
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = Ui[c][ti + 16 * q];
+      b[q] = Uj[c][tj + 16 * q];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * b[y];
+  }
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    const int64_t j = j0 + tj + 16 * y;
+    if (j >= n) continue;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int64_t i = i0 + ti + 16 * x;
+      if (i < n) out[i + j * n] = acc[x][y];
+    }
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
+extern "C" int rfx_reconstruct_factors(hipStream_t st, const void* Uq, const float* scales, int32_t format, int64_t n,
+                                       int32_t rank, double* out) {
+  if (!Uq || !out || n <= 0 || rank < 1 || rank > kRcMaxRank || fmt_bytes(format) == 0) return RFX_ERR_INVALID;
+  if (format != RFX_PREC_FP32 && !scales) return RFX_ERR_INVALID;
+  const int64_t nb = (n + kRcTile - 1) / kRcTile;
+  if (nb > 65535) return RFX_ERR_UNSUPPORTED;
+  const dim3 grid(static_cast<unsigned>(nb), static_cast<unsigned>(nb));
+  switch (format) {
+    case RFX_PREC_FP32:
+      hipLaunchKernelGGL(reconstruct_kernel<RFX_PREC_FP32>, grid, dim3(256), 0, st, static_cast<const float*>(Uq),
+                         scales, n, rank, out);
+      break;
+    case RFX_PREC_BF16:
+      hipLaunchKernelGGL(reconstruct_kernel<RFX_PREC_BF16>, grid, dim3(256), 0, st, static_cast<const uint16_t*>(Uq),
+                         scales, n, rank, out);
+      break;
+    case RFX_PREC_INT8:
+      hipLaunchKernelGGL(reconstruct_kernel<RFX_PREC_INT8>, grid, dim3(256), 0, st, static_cast<const int8_t*>(Uq),
+                         scales, n, rank, out);
+      break;
+    default:
+      hipLaunchKernelGGL(reconstruct_kernel<RFX_PREC_FP8>, grid, dim3(256), 0, st, static_cast<const uint8_t*>(Uq),
+                         scales, n, rank, out);
+      break;
+  }
+  return rfx_launch_status();
+}
+
 extern "C" size_t rfx_lowrank_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t rank,
                                               int32_t oversample) {
   if (n <= 0 || ntree <= 0 || total_nodes <= 0 || rank <= 0 || oversample < 0) return 0;

@@ -23,13 +23,16 @@ void need(bool ok, const char* what) {
 }
 
 // prox: n x n float64 Fortran-ordered (column-major, as the reference), updated in place.
-void gpu_proximity(I32 nodestatus, I32 nodextr, I32 nin, py::array_t<double> prox) {
+// nnode defaults to the length of nodestatus (the reference's callers pass maxnode).
+void per_tree_proximity(bool gpu, I32 nodestatus, I32 nodextr, I32 nin, py::array_t<double> prox, integer_t nnode) {
   const integer_t n = static_cast<integer_t>(nodextr.size());
   need(nin.size() == n && prox.ndim() == 2 && prox.shape(0) == n && prox.shape(1) == n, "shape mismatch");
   need((prox.flags() & py::array::f_style) != 0 && prox.writeable(), "prox must be a writable Fortran-ordered array");
-  rfx::ref::gpu_proximity(nodestatus.data(), nodextr.data(), nin.data(), n,
-                          static_cast<integer_t>(nodestatus.size()), prox.mutable_data(), nullptr, nullptr, nullptr,
-                          nullptr, nullptr, nullptr);
+  if (nnode <= 0) nnode = static_cast<integer_t>(nodestatus.size());
+  need(nnode <= nodestatus.size(), "nnode exceeds nodestatus");
+  auto f = gpu ? &rfx::ref::gpu_proximity : &rfx::ref::cpu_proximity;
+  f(nodestatus.data(), nodextr.data(), nin.data(), n, nnode, prox.mutable_data(), nullptr, nullptr, nullptr, nullptr,
+    nullptr, nullptr);
 }
 
 py::array_t<uint16_t> upper_triangle_fp16(I32 nodestatus, I32 nodextr, I32 nin) {
@@ -42,7 +45,7 @@ py::array_t<uint16_t> upper_triangle_fp16(I32 nodestatus, I32 nodextr, I32 nin) 
   try {
     rfx::ref::gpu_proximity_upper_triangle_fp16(nodestatus.data(), nodextr.data(), nin.data(), n,
                                                 static_cast<integer_t>(nodestatus.size()),
-                                                static_cast<uint16_t*>(d), nullptr, nullptr, nullptr, nullptr,
+                                                static_cast<__half*>(d), nullptr, nullptr, nullptr, nullptr,
                                                 nullptr, nullptr);
     if (hipMemcpy(out.mutable_data(), d, 2 * len, hipMemcpyDeviceToHost) != hipSuccess)
       throw std::runtime_error("hipMemcpy failed");
@@ -131,6 +134,39 @@ class LowRank {
     return out;
   }
   std::vector<double> eigenvalues() const { return m_.top_eigenvalues(); }
+  // reconstruct_proximity_matrix_host: n x n float64 (Fortran order)
+  F64F reconstruct() {
+    double* P = nullptr;
+    if (!rfx::ref::reconstruct_proximity_matrix_host(&m_, &P, m_.get_n_samples()))
+      throw std::runtime_error("reconstruct_proximity_matrix_host failed");
+    const py::ssize_t n = m_.get_n_samples();
+    F64F out({n, n});
+    std::copy(P, P + n * n, out.mutable_data());
+    delete[] P;
+    return out;
+  }
+  // reconstruct_full_matrix in another storage level, into device memory, copied back
+  F64F reconstruct_level(int32_t level) {
+    const py::ssize_t n = m_.get_n_samples();
+    void* d = nullptr;
+    if (hipMalloc(&d, 8 * static_cast<size_t>(n) * n) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    F64F out({n, n});
+    try {
+      m_.reconstruct_full_matrix(static_cast<double*>(d), static_cast<integer_t>(n),
+                                 static_cast<rfx::ref::FactorFormat>(level));
+      if (hipMemcpy(out.mutable_data(), d, 8 * static_cast<size_t>(n) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("hipMemcpy failed");
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    (void)hipFree(d);
+    return out;
+  }
+  void add_fp16_triangle(py::array_t<uint16_t> tri) {
+    m_.add_tree_contribution_incremental_upper_triangle_fp16(reinterpret_cast<const __half*>(tri.data()),
+                                                             m_.get_n_samples());
+  }
 
  private:
   rfx::ref::LowRankProximityMatrix m_;
@@ -140,7 +176,16 @@ class LowRank {
 
 PYBIND11_MODULE(_rfxref, m) {
   m.doc() = "Reference-signature C++ entry points of the MI355X proximity path";
-  m.def("gpu_proximity", &gpu_proximity, "per-tree OOB proximity accumulated into prox (n x n, column-major)");
+  m.def("gpu_proximity",
+        [](I32 s, I32 x, I32 b, py::array_t<double> p, integer_t nnode) { per_tree_proximity(true, s, x, b, p, nnode); },
+        py::arg("nodestatus"), py::arg("nodextr"), py::arg("nin"), py::arg("prox"), py::arg("nnode") = 0,
+        "rf::gpu_proximity semantics, accumulated into prox (n x n, column-major)");
+  m.def("cpu_proximity",
+        [](I32 s, I32 x, I32 b, py::array_t<double> p, integer_t nnode) { per_tree_proximity(false, s, x, b, p, nnode); },
+        py::arg("nodestatus"), py::arg("nodextr"), py::arg("nin"), py::arg("prox"), py::arg("nnode") = 0,
+        "rf::cpu_proximity semantics, accumulated into prox (n x n, column-major)");
+  m.def("set_casewise", [](bool on) { rfx::ref::g_config.use_casewise = on; }, "rfx::ref::g_config.use_casewise");
+  m.def("get_casewise", []() { return rfx::ref::g_config.use_casewise; });
   m.def("upper_triangle_fp16", &upper_triangle_fp16, "one tree's OOB fold as IEEE half bits, packed upper");
   m.def("finishprox", &finishprox, "rf::cpu_finishprox semantics");
   m.def("proximity_rfgap", &proximity_rfgap, "rf::gpu_proximity_rfgap semantics over all trees");
@@ -153,5 +198,8 @@ PYBIND11_MODULE(_rfxref, m) {
       .def("finalize", &LowRank::finalize)
       .def("factors", &LowRank::factors)
       .def("mds", &LowRank::mds, py::arg("k") = 3)
-      .def("eigenvalues", &LowRank::eigenvalues);
+      .def("eigenvalues", &LowRank::eigenvalues)
+      .def("reconstruct", &LowRank::reconstruct)
+      .def("reconstruct_level", &LowRank::reconstruct_level)
+      .def("add_fp16_triangle", &LowRank::add_fp16_triangle);
 }

@@ -17,6 +17,9 @@
 
 namespace rfx {
 namespace ref {
+
+Config g_config;
+
 namespace {
 
 void hip_ok(hipError_t e, const char* what) {
@@ -87,29 +90,61 @@ void one_tree_counts(const integer_t* nodestatus, const integer_t* nodextr, cons
   hip_ok(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
 }
 
-}  // namespace
-
-void gpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
-                   integer_t nnode, dp_t* prox, integer_t*, integer_t*, integer_t*, integer_t*, integer_t*,
-                   integer_t*) {
-  check_tree_args(nodestatus, nodextr, nin, nsample, nnode);
-  if (!prox) throw std::runtime_error("prox is null");
+// One tree's integer counts of `variant` (FULL), added to the host prox.
+void add_tree_counts(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
+                     integer_t nnode, int32_t variant, dp_t* prox) {
   const size_t nn = static_cast<size_t>(nsample) * nsample;
   DevBuf out(4 * nn);
-  one_tree_counts(nodestatus, nodextr, nin, nsample, nnode, RFX_PROX_OOB_ASYM, RFX_LAYOUT_FULL, out.as<int32_t>());
+  one_tree_counts(nodestatus, nodextr, nin, nsample, nnode, variant, RFX_LAYOUT_FULL, out.as<int32_t>());
   std::vector<int32_t> h(nn);
   download(h.data(), out.p, nn);
   for (size_t e = 0; e < nn; ++e) prox[e] += static_cast<dp_t>(h[e]);
 }
 
+// One tree's case-wise weights (rfx_prox_casewise over a single tree), added
+// to the host prox.
+void add_tree_casewise(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t n,
+                       integer_t nnode, dp_t* prox) {
+  std::vector<int32_t> keys(n);
+  tree_keys(nodestatus, nodextr, nin, n, nnode, 0, keys.data());
+  const size_t nn = static_cast<size_t>(n) * n;
+  DevBuf kd(4 * static_cast<size_t>(n)), nd(4 * static_cast<size_t>(n)), pd(8 * nn);
+  upload(kd.p, keys.data(), keys.size());
+  upload(nd.p, nin, n);
+  upload(pd.p, prox, nn);
+  const size_t wsb = rfx_prox_casewise_workspace_bytes(n, 1, nnode);
+  DevBuf ws(wsb);
+  rfx_ok(rfx_prox_casewise(nullptr, kd.as<int32_t>(), nd.as<int32_t>(), n, 1, nnode, 1, pd.as<double>(), ws.p, wsb),
+         "rfx_prox_casewise");
+  download(prox, pd.p, nn);
+}
+
+}  // namespace
+
 void cpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
+                   integer_t nnode, dp_t* prox, integer_t*, integer_t*, integer_t*, integer_t*, integer_t*,
+                   integer_t*) {
+  check_tree_args(nodestatus, nodextr, nin, nsample, nnode);
+  if (!prox) throw std::runtime_error("prox is null");
+  if (g_config.use_casewise)
+    add_tree_casewise(nodestatus, nodextr, nin, nsample, nnode, prox);
+  else
+    add_tree_counts(nodestatus, nodextr, nin, nsample, nnode, RFX_PROX_OOB_ASYM, prox);
+}
+
+void gpu_proximity(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin, integer_t nsample,
                    integer_t nnode, dp_t* prox, integer_t* nod, integer_t* ncount, integer_t* ncn, integer_t* nodexb,
                    integer_t* ndbegin, integer_t* npcase) {
-  gpu_proximity(nodestatus, nodextr, nin, nsample, nnode, prox, nod, ncount, ncn, nodexb, ndbegin, npcase);
+  check_tree_args(nodestatus, nodextr, nin, nsample, nnode);
+  if (!prox) throw std::runtime_error("prox is null");
+  if (nsample > 5000 && nnode > 10)  // the reference's use_persistent branch
+    add_tree_counts(nodestatus, nodextr, nin, nsample, nnode, RFX_PROX_OOB_ROW, prox);
+  else
+    cpu_proximity(nodestatus, nodextr, nin, nsample, nnode, prox, nod, ncount, ncn, nodexb, ndbegin, npcase);
 }
 
 void gpu_proximity_upper_triangle_fp16(const integer_t* nodestatus, const integer_t* nodextr, const integer_t* nin,
-                                       integer_t nsample, integer_t nnode, uint16_t* prox_upper_fp16, integer_t*,
+                                       integer_t nsample, integer_t nnode, __half* prox_upper_fp16, integer_t*,
                                        integer_t*, integer_t*, integer_t*, integer_t*, integer_t*) {
   check_tree_args(nodestatus, nodextr, nin, nsample, nnode);
   hipPointerAttribute_t attr;
@@ -120,7 +155,8 @@ void gpu_proximity_upper_triangle_fp16(const integer_t* nodestatus, const intege
   DevBuf out(4 * static_cast<size_t>(len));
   one_tree_counts(nodestatus, nodextr, nin, nsample, nnode, RFX_PROX_OOB_UPPER, RFX_LAYOUT_UPPER,
                   out.as<int32_t>());
-  rfx_ok(rfx_counts_convert(nullptr, out.as<int32_t>(), len, RFX_PREC_FP16, prox_upper_fp16), "rfx_counts_convert");
+  rfx_ok(rfx_counts_convert(nullptr, out.as<int32_t>(), len, RFX_PREC_FP16, static_cast<void*>(prox_upper_fp16)),
+         "rfx_counts_convert");
   hip_ok(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
 }
 
@@ -268,6 +304,13 @@ void LowRankProximityMatrix::add_tree_contribution(const integer_t* nodestatus, 
   ready_ = false;
 }
 
+void LowRankProximityMatrix::add_tree_contribution_incremental_upper_triangle_fp16(const __half*, integer_t) {
+  throw std::runtime_error(
+      "LowRankProximityMatrix::add_tree_contribution_incremental_upper_triangle_fp16: a per-tree FP16 proximity "
+      "triangle carries no leaf structure; this build forms the factors of C/T from the trees' leaves -- add each "
+      "tree with add_tree_contribution(nodestatus, nodextr, nin, nnode)");
+}
+
 void LowRankProximityMatrix::finalize_accumulation(integer_t, bool final_call) {
   if (!final_call) return;
   if (trees_ == 0) throw std::runtime_error("LowRankProximityMatrix: no trees added");
@@ -320,6 +363,49 @@ std::vector<double> LowRankProximityMatrix::compute_mds_from_factors(integer_t k
   for (int64_t i = 0; i < n_; ++i)
     for (int32_t c = 0; c < k; ++c) out[i * k + c] = c3[i * 3 + c];
   return out;
+}
+
+void LowRankProximityMatrix::reconstruct_full_matrix(dp_t* output_prox, integer_t nsample, FactorFormat level) {
+  if (!ready_) throw std::runtime_error("LowRankProximityMatrix: factors not finalized");
+  if (!output_prox || nsample != n_) throw std::runtime_error("reconstruct_full_matrix: bad output or nsample");
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, output_prox) != hipSuccess || attr.type != hipMemoryTypeDevice)
+    throw std::runtime_error("reconstruct_full_matrix: output_prox must be device memory");
+  if (level == FactorFormat::FP32) {
+    rfx_ok(rfx_reconstruct_factors(nullptr, U_, nullptr, RFX_PREC_FP32, n_, rank_, output_prox),
+           "rfx_reconstruct_factors");
+  } else if (level == format_) {
+    rfx_ok(rfx_reconstruct_factors(nullptr, Uq_, scales_, static_cast<int32_t>(format_), n_, rank_, output_prox),
+           "rfx_reconstruct_factors");
+  } else {
+    const size_t elt = level == FactorFormat::BF16 ? 2 : 1;
+    DevBuf q(elt * static_cast<size_t>(n_) * rank_), s(4 * static_cast<size_t>(rank_));
+    rfx_ok(rfx_quantize_factors(nullptr, U_, n_, rank_, static_cast<int32_t>(level), q.p, s.as<float>()),
+           "rfx_quantize_factors");
+    rfx_ok(rfx_reconstruct_factors(nullptr, q.p, s.as<float>(), static_cast<int32_t>(level), n_, rank_, output_prox),
+           "rfx_reconstruct_factors");
+  }
+  hip_ok(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+}
+
+bool reconstruct_proximity_matrix_host(void* obj, dp_t** output_prox, integer_t nsamples) {
+  try {
+    auto* m = static_cast<LowRankProximityMatrix*>(obj);
+    if (!m || !output_prox || nsamples != m->get_n_samples() || nsamples > 50000) return false;
+    const size_t nn = static_cast<size_t>(nsamples) * nsamples;
+    DevBuf d(8 * nn);
+    m->reconstruct_full_matrix(d.as<dp_t>(), nsamples, FactorFormat::FP32);
+    auto* h = new (std::nothrow) dp_t[nn];
+    if (!h) return false;
+    if (hipMemcpy(h, d.p, 8 * nn, hipMemcpyDeviceToHost) != hipSuccess) {
+      delete[] h;
+      return false;
+    }
+    *output_prox = h;
+    return true;
+  } catch (const std::exception&) {
+    return false;
+  }
 }
 
 bool get_lowrank_factors_host(void* obj, dp_t** A_host, dp_t** B_host, integer_t* rank, integer_t nsamples) {

@@ -120,3 +120,14 @@ def test_classifier_options_mirror_reference():
         clf.get_proximity_matrix()
     with pytest.raises(TypeError):
         clf.fit(np.zeros((10, 2), np.float32), np.zeros(10, np.int64))
+
+
+def test_prebuilt_on_a_never_built_workspace_is_refused():
+    # the build records are host state: the refusal comes before any launch
+    L = _lib.lib()
+    d = 4096  # a non-null dummy address, never dereferenced on this path
+    n, T, tn = 1000, 4, 4000
+    wsb = L.rfx_prox_workspace_bytes(n, T, tn, _lib.ALGO_GATHER)
+    rc = L.rfx_prox_accumulate(None, d, n, T, tn, _lib.PROX_ALL, _lib.LAYOUT_UPPER,
+                               _lib.ALGO_GATHER | _lib.ALGO_PREBUILT, 0, d, d, wsb)
+    assert rc == -1

@@ -64,3 +64,42 @@ def test_build_size_limit(gpu, n):
     del a_b, o_b
     a_g, o_g = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)
     assert torch.equal(a_g, a_d) and torch.equal(o_g, o_d)
+
+
+@pytest.mark.parametrize("n", [3000, 34000])
+def test_prebuilt_refuses_partial_or_foreign_builds(gpu, n):
+    # RFX_ALGO_PREBUILT runs only on a workspace whose last build was a full
+    # one of the same shape: after rfx_prox_build_buckets it counts; after a
+    # count call that built only part of the buckets (n <= 32,767, UPPER
+    # GATHER) or the case-wise call (no long-line window arrays) it refuses
+    print(f"prebuilt records n={n}", flush=True)
+    x, y = make_dataset(_lib.DATA_COVTYPE_LIKE, n, 54, 7, 4)
+    f = Forest.grow(x, y, 6, mtry=7, iseed=4)
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(n, f.ntree, f.total_nodes)
+    nd = O.leaf_assign(f, x)
+    want = O.counts(f, nd, _lib.PROX_ALL, _lib.LAYOUT_UPPER)
+    acc.build_buckets(keys)
+    a, _ = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER | _lib.ALGO_PREBUILT)
+    assert np.array_equal(a.cpu().numpy(), want)
+    # another shape on the same workspace is refused
+    with pytest.raises(RuntimeError, match="invalid argument"):
+        _lib.check(_lib.lib().rfx_prox_accumulate_pair(
+            torch.cuda.current_stream().cuda_stream, keys.data_ptr(), n, f.ntree - 1, f.total_nodes,
+            _lib.LAYOUT_UPPER, _lib.ALGO_GATHER | _lib.ALGO_PREBUILT, 0, a.data_ptr(), a.data_ptr(),
+            acc.ws.data_ptr(), acc.ws.numel()), "prebuilt")
+    if n <= 32767:
+        acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER)  # partial build
+        with pytest.raises(RuntimeError, match="invalid argument"):
+            acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET | _lib.ALGO_PREBUILT)
+    else:
+        nin = torch.from_numpy(np.ascontiguousarray(f.nin, np.int32)).to(gpu)
+        acc.casewise(keys, nin)  # full buckets, no window arrays
+        with pytest.raises(RuntimeError, match="invalid argument"):
+            acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER | _lib.ALGO_PREBUILT)
+        b = acc.accumulate(keys, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_BUCKET | _lib.ALGO_PREBUILT)
+        assert np.array_equal(b.cpu().numpy(), want)
+    acc.build_buckets(keys)
+    a, _ = acc.accumulate_pair(keys, _lib.LAYOUT_UPPER, _lib.ALGO_GATHER | _lib.ALGO_PREBUILT)
+    assert np.array_equal(a.cpu().numpy(), want)

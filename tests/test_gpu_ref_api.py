@@ -142,3 +142,81 @@ def test_lowrank_object_and_host_helpers(R):
     assert err <= 1.01 * best
     X = m.mds(3)
     assert X.shape == (n, 3) and np.all(np.isfinite(X))
+
+
+@pytest.mark.parametrize("name", ["gauss_n200.npz", "covtype_cat_n240.npz"])
+def test_casewise_switch_of_the_per_tree_proximity(R, name):
+    # rf::g_config.use_casewise: cpu_proximity (and gpu_proximity below its
+    # n > 5000 branch) weight each in-bag partner by nin(j) / (sum of nin
+    # over the leaf); pinned by the reference RandomForest fitted with
+    # use_casewise (golden rf_prox_casewise, after cpu_finishprox)
+    _say(f"ref api case-wise switch {name}")
+    g = load_golden(name)
+    f = g["forest"]
+    n = f.n
+    assert not R.get_casewise()
+    try:
+        R.set_casewise(True)
+        for fn in (R.cpu_proximity, R.gpu_proximity):
+            prox = np.zeros((n, n), np.float64, order="F")
+            for t in range(f.ntree):
+                st, _ = _tree(f, t)
+                fn(st, np.ascontiguousarray(g["nodextr"][t], np.int32), np.ascontiguousarray(f.nin[t], np.int32),
+                   prox)
+            S = R.finishprox(np.ascontiguousarray(g["nout"], np.int32), prox).reshape(-1, order="F")
+            assert np.allclose(S, g["rf_prox_casewise"], rtol=1e-12, atol=1e-15)
+    finally:
+        R.set_casewise(False)
+
+
+def test_gpu_proximity_dispatch_above_5000_samples(R):
+    # n > 5000 with nnode > 10: the rule of the reference's
+    # cuda_proximity_kernel (out-of-bag i, any j of its terminal node, i
+    # itself included; no case-wise weights) for every out-of-bag row;
+    # cpu_proximity keeps the V1 rule at every n
+    _say("ref api gpu_proximity dispatch n=5200")
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 5200, 6, 3, 2, seed=25)
+    nodextr = O.leaf_assign(f, x)
+    n = f.n
+    g4 = np.zeros((n, n), np.float64, order="F")
+    v1 = np.zeros((n, n), np.float64, order="F")
+    for t in range(f.ntree):
+        st, _ = _tree(f, t)
+        args = (st, np.ascontiguousarray(nodextr[t], np.int32), np.ascontiguousarray(f.nin[t], np.int32))
+        R.gpu_proximity(*args, g4)
+        R.cpu_proximity(*args, v1)
+    want4 = O.counts(f, nodextr, _lib.PROX_OOB_ROW, _lib.LAYOUT_FULL).astype(np.float64)
+    want1 = O.counts(f, nodextr, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL).astype(np.float64)
+    assert np.array_equal(g4.reshape(-1, order="F"), want4)
+    assert np.array_equal(v1.reshape(-1, order="F"), want1)
+    # below the threshold gpu_proximity is V1 as well
+    small = f.samples(np.arange(3000))
+    nd = O.leaf_assign(small, np.ascontiguousarray(x[:3000]))
+    p = np.zeros((3000, 3000), np.float64, order="F")
+    for t in range(f.ntree):
+        st, _ = _tree(f, t)
+        R.gpu_proximity(st, np.ascontiguousarray(nd[t], np.int32), np.ascontiguousarray(small.nin[t], np.int32), p)
+    assert np.array_equal(p.reshape(-1, order="F"), O.counts(small, nd, _lib.PROX_OOB_ASYM, _lib.LAYOUT_FULL))
+
+
+def test_lowrank_reconstruct_and_fp16_entry(R):
+    _say("ref api reconstruct_full_matrix")
+    x, y, f = small_forest(_lib.DATA_OVERLAP_MIXTURE, 1200, 8, 3, 20, seed=27)
+    nodextr = O.leaf_assign(f, x)
+    n, rank = f.n, 24
+    m = R.LowRankProximityMatrix(n, rank, _lib.PREC_BF16)
+    for t in range(f.ntree):
+        st, _ = _tree(f, t)
+        m.add_tree(st, np.ascontiguousarray(nodextr[t], np.int32), np.ascontiguousarray(f.nin[t], np.int32))
+    m.finalize()
+    A, _, r = m.factors()
+    # FP32 factors: the exact FP64 sums of their products
+    P = m.reconstruct()
+    assert np.allclose(P, A @ A.T, rtol=1e-12, atol=1e-13)
+    assert np.array_equal(P, P.T)
+    # stored BF16 factors: within the storage rounding (2^-8 relative per entry)
+    Pb = m.reconstruct_level(_lib.PREC_BF16)
+    assert np.abs(Pb - P).max() <= 2 ** -7 * np.abs(A).sum(axis=1).max() ** 2
+    # the FP16-triangle entry refuses loudly (the triangle has no leaves)
+    with pytest.raises(RuntimeError, match="add_tree_contribution"):
+        m.add_fp16_triangle(np.zeros(n * (n + 1) // 2, np.uint16))
