@@ -227,6 +227,14 @@ constexpr size_t kLeafSumBudget = size_t(2) << 30;  // bytes of leaf sums per tr
 // factor MDS operator
 // ---------------------------------------------------------------------------
 constexpr int kPairChunk = 256;
+constexpr int kFacMaxRank = 128;  // factor MDS ranks (the VALU kernels are built for RMAX = 64 and 128)
+// Pair chunk of the apply kernel: 128 for FP64 at RMAX = 128 so that its
+// LDS (RMAX + 1 factor columns of 256 rows + a chunk of Z) stays within
+// the 160 KiB of a workgroup.
+template <typename Real, int RMAX>
+constexpr int fac_pair_chunk() {
+  return (RMAX > 64 && sizeof(Real) == 8) ? 128 : kPairChunk;
+}
 
 struct PairTable {
   const uint8_t* a;  // first index (r = "one")
@@ -241,7 +249,7 @@ __global__ __launch_bounds__(256) void fac_reduce_kernel(const typename Store<FM
                                                          const float* __restrict__ scale, int64_t n, int r,
                                                          PairTable pt, const Real* __restrict__ V, int64_t rows_per_blk,
                                                          Real* __restrict__ partial) {
-  __shared__ float Us[32][65];
+  __shared__ float Us[32][kFacMaxRank + 1];
   __shared__ Real Vs[32][K];
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rows_per_blk;
@@ -291,14 +299,15 @@ __global__ void fac_reduce_partials_kernel(const Real* __restrict__ partial, int
 }
 
 // Y[i][c] = x_i^T Z[:, c] - (1 - |u_i|^2)^2 V[i][c]
-template <int FMT, typename Real, int K>
+template <int FMT, typename Real, int K, int RMAX>
 __global__ __launch_bounds__(256) void fac_apply_kernel(const typename Store<FMT>::T* __restrict__ U,
                                                         const float* __restrict__ scale, int64_t n, int r,
                                                         PairTable pt, const Real* __restrict__ Z,
                                                         const Real* __restrict__ V, Real* __restrict__ Y) {
-  __shared__ float Us[65][256];
-  __shared__ Real Zs[kPairChunk][K];
-  __shared__ uint8_t As[kPairChunk], Bs[kPairChunk];
+  constexpr int PC = fac_pair_chunk<Real, RMAX>();
+  __shared__ float Us[RMAX + 1][256];
+  __shared__ Real Zs[PC][K];
+  __shared__ uint8_t As[PC], Bs[PC];
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * 256;
   const int tid = threadIdx.x;
   const int64_t i = i0 + tid;
@@ -312,8 +321,8 @@ __global__ __launch_bounds__(256) void fac_apply_kernel(const typename Store<FMT
   Real acc[K];
 #pragma unroll
   for (int c = 0; c < K; ++c) acc[c] = Real(0);
-  for (int p0 = 0; p0 < pt.count; p0 += kPairChunk) {
-    const int pc = min(kPairChunk, pt.count - p0);
+  for (int p0 = 0; p0 < pt.count; p0 += PC) {
+    const int pc = min(PC, pt.count - p0);
     __syncthreads();
     for (int e = tid; e < pc * K; e += 256) Zs[e / K][e % K] = Z[static_cast<int64_t>(p0) * K + e];
     for (int e = tid; e < pc; e += 256) {
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(256) void fac_apply_kernel(const typename Store<FMT
 size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 
 constexpr int kMdsK = 16;
-constexpr int kStartDoubles = 64 + 64 * kMdsK;  // column means + r x 16 projection, r <= 64
+constexpr int kStartDoubles = kFacMaxRank + kFacMaxRank * kMdsK;  // column means + r x 16 projection
 
 int64_t fac_rows_per_blk(int64_t n) {
   int64_t rb = (n + 127) / 128;  // about 128 row blocks
@@ -443,8 +452,12 @@ int fac_row_means(hipStream_t st, const void* Uq, const float* scale, int64_t n,
                      fac_rows_per_blk(n), f.partial64);
   hipLaunchKernelGGL((fac_reduce_partials_kernel<double, 1>), dim3(rfx_ceil_div(f.pt.count, 256)), dim3(256), 0, st,
                      f.partial64, static_cast<int>(f.nrb), f.pt, f.Z64);
-  hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
-                     r, f.pt, f.Z64, f.ones, f.rdev);
+  if (r <= 64)
+    hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1, 64>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale,
+                       n, r, f.pt, f.Z64, f.ones, f.rdev);
+  else
+    hipLaunchKernelGGL((fac_apply_kernel<FMT, double, 1, kFacMaxRank>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st,
+                       U, scale, n, r, f.pt, f.Z64, f.ones, f.rdev);
   std::vector<double> rh(n);
   RFX_HIP_TRY(hipMemcpyAsync(rh.data(), f.rdev, 8 * n, hipMemcpyDeviceToHost, st));
   RFX_HIP_TRY(hipStreamSynchronize(st));
@@ -473,15 +486,15 @@ constexpr int kGramTileRows = 32;
 
 // part[blk][m]: sum over the block's rows of u_a u_b (a <= b, packed upper
 // triangle in row order) and of u_a (last r entries), FP64.
-template <int FMT>
+template <int FMT, int RMAX>
 __global__ __launch_bounds__(256) void fac_gram_kernel(const typename Store<FMT>::T* __restrict__ U,
                                                        const float* __restrict__ scale, int64_t n, int r,
                                                        int64_t rows_per_blk, double* __restrict__ part) {
-  __shared__ float us[kGramTileRows][64];
+  __shared__ float us[kGramTileRows][RMAX];
   const int m = r * (r + 1) / 2 + r;
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
-  constexpr int kPer = (64 * 65 / 2 + 64 + 255) / 256;
+  constexpr int kPer = (RMAX * (RMAX + 1) / 2 + RMAX + 255) / 256;
   double acc[kPer];
   int pa[kPer], pb[kPer];
 #pragma unroll
@@ -538,7 +551,7 @@ __global__ void fac_gram_reduce_kernel(const double* __restrict__ part, int nblk
   out[e] = s;
 }
 
-// V[i][c] = sum_a (u_ia - mu_a) Y[a][c]; st = [mu (64) | Y (r x 16)]
+// V[i][c] = sum_a (u_ia - mu_a) Y[a][c]; st = [mu (kFacMaxRank) | Y (r x 16)]
 template <int FMT, typename Real>
 __global__ void fac_start_kernel(const typename Store<FMT>::T* __restrict__ U, const float* __restrict__ scale,
                                  int64_t n, int r, const double* __restrict__ st, Real* __restrict__ V) {
@@ -548,7 +561,7 @@ __global__ void fac_start_kernel(const typename Store<FMT>::T* __restrict__ U, c
   const int c = static_cast<int>(e % kMdsK);
   double s = 0.0;
   for (int a = 0; a < r; ++a)
-    s += (static_cast<double>(Store<FMT>::load(U, i * r + a, scale, a)) - st[a]) * st[64 + a * kMdsK + c];
+    s += (static_cast<double>(Store<FMT>::load(U, i * r + a, scale, a)) - st[a]) * st[kFacMaxRank + a * kMdsK + c];
   V[e] = static_cast<Real>(s);
 }
 
@@ -577,8 +590,12 @@ int fac_start_block(hipStream_t st, const void* Uq, const float* scale, int64_t 
   const ST* U = static_cast<const ST*>(Uq);
   const int m = r * (r + 1) / 2 + r;
   const int64_t rpb = fac_rows_per_blk(n);
-  hipLaunchKernelGGL((fac_gram_kernel<FMT>), dim3(static_cast<unsigned>(f.nrb)), dim3(256), 0, st, U, scale, n, r, rpb,
-                     f.partial64);
+  if (r <= 64)
+    hipLaunchKernelGGL((fac_gram_kernel<FMT, 64>), dim3(static_cast<unsigned>(f.nrb)), dim3(256), 0, st, U, scale, n,
+                       r, rpb, f.partial64);
+  else
+    hipLaunchKernelGGL((fac_gram_kernel<FMT, kFacMaxRank>), dim3(static_cast<unsigned>(f.nrb)), dim3(256), 0, st, U,
+                       scale, n, r, rpb, f.partial64);
   hipLaunchKernelGGL(fac_gram_reduce_kernel, dim3(rfx_ceil_div(m, 256)), dim3(256), 0, st, f.partial64,
                      static_cast<int>(f.nrb), m, f.Z64);
   std::vector<double> G(m);
@@ -619,7 +636,7 @@ int fac_start_block(hipStream_t st, const void* Uq, const float* scale, int64_t 
   if (!(qmax > 0.0) || qmin <= 1e-10 * qmax) return RFX_OK;
   std::vector<double> up(kStartDoubles, 0.0);
   for (int a = 0; a < r; ++a) up[a] = mu[a];
-  for (size_t e = 0; e < Y.size(); ++e) up[64 + e] = Y[e];
+  for (size_t e = 0; e < Y.size(); ++e) up[kFacMaxRank + e] = Y[e];
   RFX_HIP_TRY(hipMemcpyAsync(f.start, up.data(), 8 * kStartDoubles, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL((fac_start_kernel<FMT, Real>), dim3(rfx_ceil_div(n * kMdsK, 256)), dim3(256), 0, st, U, scale, n,
                      r, f.start, V);
@@ -639,8 +656,12 @@ int fac_valu_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n
                      fac_rows_per_blk(n), f.partial);
   hipLaunchKernelGGL((fac_reduce_partials_kernel<Real, kMdsK>), dim3(rfx_ceil_div(f.pt.count * kMdsK, 256)),
                      dim3(256), 0, st, f.partial, static_cast<int>(f.nrb), f.pt, f.Z);
-  hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n,
-                     r, f.pt, f.Z, V, W);
+  if (r <= 64)
+    hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK, 64>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U,
+                       scale, n, r, f.pt, f.Z, V, W);
+  else
+    hipLaunchKernelGGL((fac_apply_kernel<FMT, Real, kMdsK, kFacMaxRank>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0,
+                       st, U, scale, n, r, f.pt, f.Z, V, W);
   return rfx_launch_status();
 }
 
@@ -687,47 +708,71 @@ int fac_mds_dispatch(hipStream_t st, const void* Uq, const float* scale, int fmt
 // mixed path: bf16 MFMA operator on the stored factors, then FP32 refinement
 // sweeps of the VALU operator on the FP32 factors from the converged block
 // ---------------------------------------------------------------------------
-int mf_nt(int r) { return (r + 1 + 15) / 16; }
-
 int64_t mf_rows_per_chunk(int64_t n) {
-  int64_t rpc = (n + 255) / 256;  // about 256 chunks: one 1024-thread workgroup per CU
-  rpc = (rpc + 15) / 16 * 16;
+  int64_t rpc = (n + 255) / 256;  // about 256 chunks per tile subset
+  rpc = (rpc + 31) / 32 * 32;
   return rpc < kMfStage ? kMfStage : rpc;
 }
 
+int64_t mf_nchunks(int64_t n) { return (n + mf_rows_per_chunk(n) - 1) / mf_rows_per_chunk(n); }
+
+// workspace of the matrix-core operator: partials P and Pm, S planes, m and s
 size_t mf_ws_bytes(int64_t n, int r) {
-  const int NT = mf_nt(r);
-  const int64_t nch = (n + mf_rows_per_chunk(n) - 1) / mf_rows_per_chunk(n);
-  return al(4 * static_cast<size_t>(nch) * kMfK * (NT * (NT + 1) / 2) * 256) +
-         al(2 * 2 * static_cast<size_t>(kMfK) * NT * NT * 256);
+  const int WT = mf_wt(r), W = 32 * WT;
+  const int64_t nch = mf_nchunks(n);
+  return al(4 * static_cast<size_t>(nch) * kMfK * mf_ntiles(WT) * 256) +
+         al(4 * static_cast<size_t>(nch) * kMfK * (W + 1)) + al(2 * 2 * static_cast<size_t>(kMfK) * 2 * WT * WT * 512) +
+         al(4 * static_cast<size_t>(kMfK) * (W + 1));
 }
 
-template <int FMT, int NT, bool SPLIT>
-int mf_apply_nt(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W,
-                float* P, uint16_t* Qb) {
+struct MfWs {
+  float* P;
+  float* Pm;
+  uint16_t* Sb;
+  float* mv;
+};
+
+MfWs mf_carve(char* p, int64_t n, int r) {
+  const int WT = mf_wt(r), W = 32 * WT;
+  const int64_t nch = mf_nchunks(n);
+  MfWs m;
+  m.P = reinterpret_cast<float*>(p);
+  p += al(4 * static_cast<size_t>(nch) * kMfK * mf_ntiles(WT) * 256);
+  m.Pm = reinterpret_cast<float*>(p);
+  p += al(4 * static_cast<size_t>(nch) * kMfK * (W + 1));
+  m.Sb = reinterpret_cast<uint16_t*>(p);
+  p += al(2 * 2 * static_cast<size_t>(kMfK) * 2 * WT * WT * 512);
+  m.mv = reinterpret_cast<float*>(p);
+  return m;
+}
+
+template <int FMT, int WT, bool SPLIT>
+int mf_apply_wt(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W,
+                const MfWs& m) {
   using ST = typename Store<FMT>::T;
   const ST* U = static_cast<const ST*>(Uq);
   const int64_t rpc = mf_rows_per_chunk(n);
-  const int64_t nch = (n + rpc - 1) / rpc;
-  constexpr int ntiles = NT * (NT + 1) / 2;
-  hipLaunchKernelGGL((fac_mfma_reduce_kernel<FMT, NT, SPLIT>), dim3(static_cast<unsigned>(nch)), dim3(1024), 0, st, U,
-                     scale, n, r, V, rpc, P);
-  hipLaunchKernelGGL(fac_mfma_finalize_kernel, dim3(rfx_ceil_div(kMfK * ntiles * 256, 64)), dim3(256), 0, st, P,
-                     static_cast<int>(nch), NT, Qb);
-  hipLaunchKernelGGL((fac_mfma_apply_kernel<FMT, NT, SPLIT>), dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, U, scale, n, r,
-                     Qb, V, W);
+  const int64_t nch = mf_nchunks(n);
+  constexpr int ntiles = mf_ntiles(WT);
+  constexpr int RT = WT == 1 ? 4 : 2;  // row tiles per wave: registers for the B operands
+  hipLaunchKernelGGL((fac_mfma_reduce_kernel<FMT, WT, SPLIT>), dim3(static_cast<unsigned>(nch), mf_nsub(WT)),
+                     dim3(1024), 0, st, U, scale, n, r, V, rpc, m.P, m.Pm);
+  const int64_t fin = static_cast<int64_t>(kMfK) * ntiles * 256 + static_cast<int64_t>(kMfK) * (32 * WT + 1);
+  hipLaunchKernelGGL((fac_mfma_finalize_kernel<WT>), dim3(rfx_ceil_div(fin, 256)), dim3(256), 0, st, m.P, m.Pm,
+                     static_cast<int>(nch), m.Sb, m.mv);
+  hipLaunchKernelGGL((fac_mfma_apply_kernel<FMT, WT, SPLIT, RT>), dim3(rfx_ceil_div(n, 64 * RT)), dim3(256), 0, st, U,
+                     scale, n, r, m.Sb, m.mv, V, W);
   return rfx_launch_status();
 }
 
 template <int FMT, bool SPLIT = false>
-int mf_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W, float* P,
-             uint16_t* Qb) {
-  switch (mf_nt(r)) {
-    case 1: return mf_apply_nt<FMT, 1, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 2: return mf_apply_nt<FMT, 2, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 3: return mf_apply_nt<FMT, 3, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 4: return mf_apply_nt<FMT, 4, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
-    case 5: return mf_apply_nt<FMT, 5, SPLIT>(st, Uq, scale, n, r, V, W, P, Qb);
+int mf_apply(hipStream_t st, const void* Uq, const float* scale, int64_t n, int r, const float* V, float* W,
+             const MfWs& m) {
+  switch (mf_wt(r)) {
+    case 1: return mf_apply_wt<FMT, 1, SPLIT>(st, Uq, scale, n, r, V, W, m);
+    case 2: return mf_apply_wt<FMT, 2, SPLIT>(st, Uq, scale, n, r, V, W, m);
+    case 3: return mf_apply_wt<FMT, 3, SPLIT>(st, Uq, scale, n, r, V, W, m);
+    case 4: return mf_apply_wt<FMT, 4, SPLIT>(st, Uq, scale, n, r, V, W, m);
     default: return RFX_ERR_UNSUPPORTED;
   }
 }
@@ -739,19 +784,14 @@ int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U
   FacWs<float> f;
   int rc = fac_ws_setup<float>(st, ws, n, r, f);
   if (rc) return rc;
-  char* p = static_cast<char*>(ws) + fac_mds_ws<float>(n, r);
-  const int NT = mf_nt(r);
-  const int64_t nch = (n + mf_rows_per_chunk(n) - 1) / mf_rows_per_chunk(n);
-  float* P = reinterpret_cast<float*>(p);
-  p += al(4 * static_cast<size_t>(nch) * kMfK * (NT * (NT + 1) / 2) * 256);
-  uint16_t* Qb = reinterpret_cast<uint16_t*>(p);
+  const MfWs mws = mf_carve(static_cast<char*>(ws) + fac_mds_ws<float>(n, r), n, r);
 
   double g = 0.0;
   rc = fac_row_means<FMT, float>(st, Uq, scale, n, r, f, &g);
   if (rc) return rc;
   rfx::SubspaceSolver<float> solver(st, n, kMdsK);
   solver.carve(f.solver);
-  auto apply_mf = [&](const float* V, float* W, int) -> int { return mf_apply<FMT>(st, Uq, scale, n, r, V, W, P, Qb); };
+  auto apply_mf = [&](const float* V, float* W, int) -> int { return mf_apply<FMT>(st, Uq, scale, n, r, V, W, mws); };
   bool informed = false;
   if (r >= kMdsK && n >= 2 * kMdsK) {
     rc = fac_start_block<FMT, float>(st, Uq, scale, n, r, f, solver.block_v(), informed);
@@ -766,7 +806,7 @@ int mixed_run(hipStream_t st, const void* Uq, const float* scale, const float* U
   if (rc) return rc;
   // FP32-accurate matrix-core operator on the FP32 factors (u as bf16 hi + lo)
   auto apply_32 = [&](const float* V, float* W, int) -> int {
-    return mf_apply<RFX_PREC_FP32, true>(st, U32, nullptr, n, r, V, W, P, Qb);
+    return mf_apply<RFX_PREC_FP32, true>(st, U32, nullptr, n, r, V, W, mws);
   };
   rc = solver.run(apply_32, f.rdev, g, 3, refine_max, refine_tol, 0x5eedull, coords, res, 0.0, true);
   if (rc) return rc;
@@ -974,7 +1014,8 @@ extern "C" size_t rfx_mds_factors_workspace_bytes(int64_t n, int32_t rank, int32
 extern "C" int rfx_mds_factors(hipStream_t st, const void* Uq, const float* scales, int32_t format, int64_t n,
                                int32_t rank, int32_t precision, int32_t max_iter, double tol, void* ws,
                                size_t ws_bytes, double* coords, double* eigvals_host, int32_t* iters_host) {
-  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > 64 || fmt_bytes(format) == 0 || max_iter < 1)
+  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > kFacMaxRank || fmt_bytes(format) == 0 ||
+      max_iter < 1)
     return RFX_ERR_INVALID;
   if (precision != RFX_PREC_FP32 && precision != RFX_PREC_FP64) return RFX_ERR_INVALID;
   if (ws_bytes < rfx_mds_factors_workspace_bytes(n, rank, precision)) return RFX_ERR_WORKSPACE;
@@ -991,7 +1032,7 @@ extern "C" int rfx_mds_factors(hipStream_t st, const void* Uq, const float* scal
 }
 
 extern "C" size_t rfx_mds_factors_mixed_workspace_bytes(int64_t n, int32_t rank) {
-  if (n <= 0 || rank <= 0 || rank > 64) return 0;
+  if (n <= 0 || rank <= 0 || rank > kFacMaxRank) return 0;
   return fac_mds_ws<float>(n, rank) + mf_ws_bytes(n, rank);
 }
 
@@ -999,8 +1040,8 @@ extern "C" int rfx_mds_factors_mixed(hipStream_t st, const void* Uq, const float
                                      const float* U32, int64_t n, int32_t rank, int32_t max_iter, double tol,
                                      int32_t refine_max, double refine_tol, void* ws, size_t ws_bytes,
                                      double* coords, double* eigvals_host, int32_t* iters_host) {
-  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > 64 || fmt_bytes(format) == 0 || max_iter < 1 ||
-      refine_max < 0)
+  if (!Uq || !scales || !ws || !coords || n < 4 || rank < 1 || rank > kFacMaxRank || fmt_bytes(format) == 0 ||
+      max_iter < 1 || refine_max < 0)
     return RFX_ERR_INVALID;
   if (ws_bytes < rfx_mds_factors_mixed_workspace_bytes(n, rank)) return RFX_ERR_WORKSPACE;
   rfx::SubspaceResult res;

@@ -43,46 +43,60 @@ __global__ void ss_init_kernel(Real* V, int64_t n, int k, uint64_t seed) {
 
 constexpr int kGramRows = 32;    // rows per LDS tile
 constexpr int kGramBlocks = 240; // row-range partials (about one per CU)
+constexpr int kGramTile = 64;    // output columns of A and of B per workgroup
 
-// partial[b][a*kb + c] = sum over the rows of block b of A[i,a] * B[i,c],
-// rows staged through LDS in tiles, FP64 accumulation.
+// partial[b][a*kb + c] = sum over the rows of row block b (blockIdx.x) of
+// A[i,a] * B[i,c], for the 64 x 64 output tile blockIdx.y (A tiles outer):
+// rows staged through LDS kGramRows at a time, each thread accumulating a
+// 4 x 4 FP64 sub-tile whose columns are 16 apart (consecutive lanes read
+// consecutive LDS words).
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void ss_gram_kernel(const TA* __restrict__ A, int ka, const TB* __restrict__ B,
                                                       int kb, int64_t n, int64_t rows_per_block,
                                                       double* __restrict__ partial) {
-  extern __shared__ double tile[];  // [kGramRows][ka + kb]
+  __shared__ double sa[kGramRows][kGramTile];
+  __shared__ double sb[kGramRows][kGramTile];
+  const int ntc = (kb + kGramTile - 1) / kGramTile;
+  const int a0 = (blockIdx.y / ntc) * kGramTile, c0 = (blockIdx.y % ntc) * kGramTile;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
   const int64_t r1 = min(n, r0 + rows_per_block);
-  const int pairs = ka * kb, w = ka + kb;
-  constexpr int kMaxPerThread = 32;
-  double acc[kMaxPerThread];
+  const int ta = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
 #pragma unroll
-  for (int q = 0; q < kMaxPerThread; ++q) acc[q] = 0.0;
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
   for (int64_t i0 = r0; i0 < r1; i0 += kGramRows) {
     const int rows = static_cast<int>(min<int64_t>(kGramRows, r1 - i0));
     __syncthreads();
-    for (int e = threadIdx.x; e < rows * w; e += 256) {
-      const int q = e / w, c = e % w;
+    for (int e = threadIdx.x; e < kGramRows * kGramTile; e += 256) {
+      const int q = e / kGramTile, c = e % kGramTile;
       const int64_t i = i0 + q;
-      tile[q * w + c] = c < ka ? static_cast<double>(A[i * ka + c]) : static_cast<double>(B[i * kb + (c - ka)]);
+      sa[q][c] = q < rows && a0 + c < ka ? static_cast<double>(A[i * ka + a0 + c]) : 0.0;
+      sb[q][c] = q < rows && c0 + c < kb ? static_cast<double>(B[i * kb + c0 + c]) : 0.0;
     }
     __syncthreads();
+    for (int q = 0; q < rows; ++q) {
+      double va[4], vb[4];
 #pragma unroll
-    for (int s = 0; s < kMaxPerThread; ++s) {
-      const int pc = threadIdx.x + s * 256;
-      if (pc < pairs) {
-        const int a = pc / kb, c = ka + pc % kb;
-        double t = 0.0;
-        for (int q = 0; q < rows; ++q) t += tile[q * w + a] * tile[q * w + c];
-        acc[s] += t;
+      for (int x = 0; x < 4; ++x) {
+        va[x] = sa[q][ta + 16 * x];
+        vb[x] = sb[q][tc + 16 * x];
       }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] += va[x] * vb[y];
     }
   }
+  const int pairs = ka * kb;
 #pragma unroll
-  for (int s = 0; s < kMaxPerThread; ++s) {
-    const int pc = threadIdx.x + s * 256;
-    if (pc < pairs) partial[static_cast<int64_t>(blockIdx.x) * pairs + pc] = acc[s];
-  }
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int a = a0 + ta + 16 * x, c = c0 + tc + 16 * y;
+      if (a < ka && c < kb) partial[static_cast<int64_t>(blockIdx.x) * pairs + a * kb + c] = acc[x][y];
+    }
 }
 
 // out[pc] = sum over blocks of partial[b][pc]: one wavefront per entry,
@@ -275,8 +289,10 @@ class SubspaceSolver {
   SubspaceSolver(hipStream_t st, int64_t n, int k) : st_(st), n_(n), k_(k) {}
 
   static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+  // Entries of the largest Gram: the fused per-iteration sums of run() (k <= 32
+  // only) or a k x k block Gram.
   static int pairs_max(int k) {
-    const int fused = (2 + 2 * k) * 2 * k;
+    const int fused = k <= 32 ? (2 + 2 * k) * 2 * k : 0;
     return fused > k * k ? fused : k * k;
   }
 
@@ -317,12 +333,12 @@ class SubspaceSolver {
   template <typename TA, typename TB>
   int gram(const TA* A, int ka, const TB* B, int kb, std::vector<double>& out) {
     const int pairs = ka * kb;
-    if (pairs > 32 * 256 || (ka + kb) * kGramRows * 8 > 64 * 1024 || pairs > pairs_max(k_))
-      return RFX_ERR_UNSUPPORTED;
+    if (ka < 1 || kb < 1 || pairs > pairs_max(k_)) return RFX_ERR_UNSUPPORTED;
     const int64_t nb = std::min<int64_t>(kGramBlocks, (n_ + kGramRows - 1) / kGramRows);
     const int64_t rpb = (n_ + nb - 1) / nb;
-    hipLaunchKernelGGL((ss_gram_kernel<TA, TB>), dim3(static_cast<unsigned>(nb)), dim3(256),
-                       8 * kGramRows * (ka + kb), st_, A, ka, B, kb, n_, rpb, partial_);
+    const unsigned tiles = static_cast<unsigned>(rfx_ceil_div(ka, kGramTile) * rfx_ceil_div(kb, kGramTile));
+    hipLaunchKernelGGL((ss_gram_kernel<TA, TB>), dim3(static_cast<unsigned>(nb), tiles), dim3(256), 0, st_, A, ka,
+                       B, kb, n_, rpb, partial_);
     hipLaunchKernelGGL(ss_reduce_partials_kernel, dim3(rfx_ceil_div(pairs, 4)), dim3(256), 0, st_, partial_,
                        static_cast<int>(nb), pairs, kk_);
     out.resize(pairs);

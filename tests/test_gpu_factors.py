@@ -93,9 +93,10 @@ def _fp64_factor_mds(Uh: np.ndarray):
     return V[:, o[:3]] * np.sqrt(wv[o[:3]]), wv[o]
 
 
-def test_factor_mds_fp64_matches_closed_form(gpu):
+@pytest.mark.parametrize("rank", [32, 128])
+def test_factor_mds_fp64_matches_closed_form(gpu, rank):
     f, keys, P = _setup(n=1200, T=50, seed=22)
-    U, _ = lowrank_factors(keys, f, rank=32, oversample=16, power_iters=3)
+    U, _ = lowrank_factors(keys, f, rank=rank, oversample=16, power_iters=3)
     Uq, sc = quantize_factors(U, _lib.PREC_FP32)
     X, eig, it = mds_factors(Uq, sc, _lib.PREC_FP32, precision=_lib.PREC_FP64)
     Xref, w = _fp64_factor_mds(U.cpu().numpy().astype(np.float64))
@@ -105,10 +106,11 @@ def test_factor_mds_fp64_matches_closed_form(gpu):
         assert O.embedding_errors(X.cpu().numpy(), Xref)["rel_frobenius"] < 1e-6
 
 
+@pytest.mark.parametrize("rank", [32, 128])
 @pytest.mark.parametrize("fmt", [_lib.PREC_FP32, _lib.PREC_BF16, _lib.PREC_INT8, _lib.PREC_FP8])
-def test_factor_mds_storage_precisions(gpu, fmt):
+def test_factor_mds_storage_precisions(gpu, fmt, rank):
     f, keys, P = _setup(n=1200, T=50, seed=22)
-    U, _ = lowrank_factors(keys, f, rank=32, oversample=16, power_iters=3)
+    U, _ = lowrank_factors(keys, f, rank=rank, oversample=16, power_iters=3)
     Xref, w = _fp64_factor_mds(U.cpu().numpy().astype(np.float64))
     Uq, sc = quantize_factors(U, fmt)
     X, eig, it = mds_factors(Uq, sc, fmt, precision=_lib.PREC_FP32)
@@ -128,7 +130,7 @@ def _fp64_factor_B(Uh: np.ndarray) -> np.ndarray:
     return -0.5 * (D2 - rr[:, None] - rr[None, :] + rr.mean())
 
 
-@pytest.mark.parametrize("rank", [32, 64])
+@pytest.mark.parametrize("rank", [32, 64, 96, 128])
 @pytest.mark.parametrize("fmt", [_lib.PREC_BF16, _lib.PREC_INT8, _lib.PREC_FP8, _lib.PREC_FP32])
 def test_mixed_factor_mds_mfma_then_fp32_refinement(gpu, rank, fmt):
     # The bf16 matrix-core operator (MFMA) on the stored factors gives
