@@ -368,7 +368,7 @@ int rfx_reconstruct_factors(hipStream_t stream, const void* Uq, const float* sca
  * same subspace iteration as rfx_mds_dense, operator arithmetic in
  * `precision` (FP32 or FP64) over factors stored in `format`.  Replaces
  * LowRankProximityMatrix::compute_mds_from_factors
- * (cuda/rf_proximity_lowrank.cu:1935-2531).  rank <= 64.  Synchronises. */
+ * (cuda/rf_proximity_lowrank.cu:1935-2531).  rank <= 128.  Synchronises. */
 size_t rfx_mds_factors_workspace_bytes(int64_t n, int32_t rank, int32_t precision);
 int rfx_mds_factors(hipStream_t stream, const void* Uq, const float* scales, int32_t format, int64_t n, int32_t rank,
                     int32_t precision, int32_t max_iter, double tol, void* workspace, size_t workspace_bytes,
@@ -382,7 +382,7 @@ int rfx_mds_factors(hipStream_t stream, const void* Uq, const float* scales, int
  * from the converged block until refine_tol, with an FP32-accurate operator
  * on U32 (the same GEMMs with u carried as a bf16 pair hi + lo).  FP64 row
  * means, CholeskyQR2, Grams and Rayleigh-Ritz throughout.  iters_host[0] =
- * matrix-core iterations, iters_host[1] = refinement sweeps.  rank <= 64.
+ * matrix-core iterations, iters_host[1] = refinement sweeps.  rank <= 128.
  * Synchronises.
  * Replaces LowRankProximityMatrix::compute_mds_from_factors
  * (cuda/rf_proximity_lowrank.cu:1935-2531). */

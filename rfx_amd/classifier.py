@@ -20,7 +20,7 @@ backed by the HIP path in librfx:
                        classical MDS), computed on the GPU in FP64.
   compute_mds_from_factors / compute_mds_3d_from_factors
                        MDS from the stored factors on the matrix cores,
-                       refined with the FP32 factors (k <= 3, rank <= 64).
+                       refined with the FP32 factors (k <= 3, rank <= 128).
 
 With use_casewise the proximity is the reference's case-wise weighted one
 (src/rf_proximity.cpp:186-223: an out-of-bag case and an in-bag case of a
@@ -224,8 +224,8 @@ class RandomForestClassifier:
         if not 1 <= k <= 3:
             raise ValueError("k must be 1, 2 or 3")
         Uq, sc, fmt, U = self._factors
-        if U.shape[1] > 64:
-            raise ValueError("the factor MDS supports rank <= 64")
+        if U.shape[1] > 128:
+            raise ValueError("the factor MDS supports rank <= 128")
         X, _, _ = mds_factors_mixed(Uq, sc, fmt, U)
         return X.cpu().numpy()[:, :k].copy()
 

@@ -383,18 +383,96 @@ __global__ __launch_bounds__(kGallopThreads) void gather_gallop_kernel(const uin
 // column chunks are aligned to kWinChunk and the members are read as 16-bit
 // offsets within their 32,768-column window (members_w).  For UPPER the
 // first chunk starts at the line's own column.
+//
+// Workgroups take the lines in the order of tree 0's member list (the lines
+// grouped by their tree-0 leaf), so the lines in flight on an XCD are similar
+// and share leaves and boundary rows in its L2; the per-line pe entries and
+// leaves come from line-major copies (rpe, rleaf) for that reason.
+//
+// The LDS counter of column x of chunk [lo, hi) sits at index x - lo + sh,
+// sh in [0, 4) chosen so that index and output address agree mod 16 bytes:
+// the counters are cleared and flushed as aligned 16-byte vectors (a scalar
+// head and tail where the output row is not 16-byte aligned; all scalar when
+// out and out2 differ in alignment).
 // ---------------------------------------------------------------------------
+constexpr int kWinAccWords = static_cast<int>(kWinChunk) + 8;  // counters (+ shift, rounding) and the diagonal word
+constexpr int kWinDiagWord = static_cast<int>(kWinChunk) + 4;
+
+template <int MODE>
+__device__ __forceinline__ void win_emit(uint32_t v, int32_t* __restrict__ out, int32_t* __restrict__ out2,
+                                         int64_t o, int32_t accumulate) {
+  if (MODE == kMode_PAIR) {
+    const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
+    out[o] = accumulate ? out[o] + va : va;
+    out2[o] = accumulate ? out2[o] + vb : vb;
+  } else {
+    out[o] = accumulate ? out[o] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+  }
+}
+
+// Counters acc[L0 + j], j < count, to out[o0 + j] (and out2 for PAIR); the
+// vector body needs (L0 + j) % 4 == 0 exactly where &out[o0 + j] is 16-byte
+// aligned, which the caller's shift provides (vec false: all scalar).
+template <int MODE, int THREADS>
+__device__ __forceinline__ void win_flush(const uint32_t* __restrict__ acc, int L0, int count, bool vec,
+                                          int32_t* __restrict__ out, int32_t* __restrict__ out2, int64_t o0,
+                                          int32_t accumulate) {
+  const int tid = threadIdx.x;
+  const int head = vec ? min(count, static_cast<int>((4 - ((reinterpret_cast<uintptr_t>(out + o0) >> 2) & 3)) & 3))
+                       : count;
+  const int nvec = (count - head) >> 2;
+  const int tail0 = head + 4 * nvec;
+  const int nscalar = head + (count - tail0);
+  for (int k = tid; k < nscalar; k += THREADS) {
+    const int j = k < head ? k : tail0 + (k - head);
+    win_emit<MODE>(acc[L0 + j], out, out2, o0 + j, accumulate);
+  }
+  const uint4* a4 = reinterpret_cast<const uint4*>(acc + L0 + head);
+  int4* p4 = reinterpret_cast<int4*>(out + o0 + head);
+  int4* q4 = MODE == kMode_PAIR ? reinterpret_cast<int4*>(out2 + o0 + head) : nullptr;
+  for (int k = tid; k < nvec; k += THREADS) {
+    const uint4 v = a4[k];
+    if (MODE == kMode_PAIR) {
+      int4 va = make_int4(static_cast<int32_t>(v.x & 0xffffu), static_cast<int32_t>(v.y & 0xffffu),
+                          static_cast<int32_t>(v.z & 0xffffu), static_cast<int32_t>(v.w & 0xffffu));
+      int4 vb = make_int4(static_cast<int32_t>(v.x >> 16), static_cast<int32_t>(v.y >> 16),
+                          static_cast<int32_t>(v.z >> 16), static_cast<int32_t>(v.w >> 16));
+      if (accumulate) {
+        const int4 pa = p4[k], pb = q4[k];
+        va = make_int4(va.x + pa.x, va.y + pa.y, va.z + pa.z, va.w + pa.w);
+        vb = make_int4(vb.x + pb.x, vb.y + pb.y, vb.z + pb.z, vb.w + pb.w);
+      }
+      p4[k] = va;
+      q4[k] = vb;
+    } else {
+      int4 va = make_int4(static_cast<int32_t>(v.x), static_cast<int32_t>(v.y), static_cast<int32_t>(v.z),
+                          static_cast<int32_t>(v.w));
+      if (accumulate) {
+        const int4 pa = p4[k];
+        va = make_int4(va.x + pa.x, va.y + pa.y, va.z + pa.z, va.w + pa.w);
+      }
+      p4[k] = va;
+    }
+  }
+}
+
+// Two workgroups per CU (eight waves per SIMD): at most 64 registers.
 template <int MODE, int LAYOUT, int G>
-__global__ __launch_bounds__(kGallopThreads) void gather_window_kernel(
-    const uint2* __restrict__ pe, const int32_t* __restrict__ keys, const uint32_t* __restrict__ wtab, int nb,
-    const uint32_t* __restrict__ bstart, int64_t n, int32_t ntree, const uint16_t* __restrict__ members,
-    int32_t accumulate, int32_t* __restrict__ out, int32_t* __restrict__ out2) {
-  extern __shared__ uint32_t win_acc[];
-  __shared__ uint32_t diag_valid;
-  const int64_t line = xcd_row(blockIdx.x);
-  if (line >= n) return;
+__global__ __launch_bounds__(kGallopThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void gather_window_kernel(
+    const uint32_t* __restrict__ order, const uint2* __restrict__ rpe, const uint32_t* __restrict__ rleaf,
+    const uint32_t* __restrict__ wtab, int nb, const uint32_t* __restrict__ bstart, int64_t n, int32_t ntree,
+    const uint16_t* __restrict__ members, int32_t accumulate, int32_t* __restrict__ out,
+    int32_t* __restrict__ out2) {
+  extern __shared__ uint4 win_acc4[];  // kWinAccWords words, 16-byte aligned (no static LDS)
+  uint32_t* const win_acc = reinterpret_cast<uint32_t*>(win_acc4);
+  const int64_t slot = xcd_row(blockIdx.x);
+  if (slot >= n) return;
+  const int64_t line = static_cast<int64_t>(order[slot] >> 1);  // tree 0's slot order
   const int tid = threadIdx.x, lane = tid & 63;
+  const int nch = nb + 1;
   const int c0 = LAYOUT == RFX_LAYOUT_UPPER ? static_cast<int>(line / kWinChunk) : 0;
+  // per tree: cursor, end of the counted run, and the run's positions at the
+  // chunk boundaries (wb[g][k]: start of chunk c0 + k + 1)
   uint32_t cur[G], end[G], wb[G][kWinMaxBounds];
   uint32_t bagm = 0, valid = 0;
 #pragma unroll
@@ -402,12 +480,11 @@ __global__ __launch_bounds__(kGallopThreads) void gather_window_kernel(
     const int32_t t = g * kGallopThreads + tid;
     cur[g] = 0;
     end[g] = 0;
-    uint32_t e = 0, bag = 0, pos = 0;
-    int32_t key = -1;
+    uint32_t e = 0, bag = 0, pos = 0, leaf = 0;
     if (t < ntree) {
-      const int64_t s = static_cast<int64_t>(t) * n + line;
-      const uint2 q = pe[s];
-      key = keys[s];
+      const int64_t s = line * ntree + t;
+      const uint2 q = rpe[s];
+      leaf = rleaf[s];
       e = q.y & 0x7fffffffu;
       bag = q.y >> 31;
       pos = q.x;
@@ -415,9 +492,7 @@ __global__ __launch_bounds__(kGallopThreads) void gather_window_kernel(
     if (e > 0) {
       ++valid;
       bagm |= bag << g;
-      const uint32_t leaf = static_cast<uint32_t>(key) >> 1;
       const uint32_t* tb = wtab + static_cast<int64_t>(leaf) * nb;
-      // boundary k of the list is the start of chunk c0 + k + 1
 #pragma unroll
       for (int k = 0; k < kWinMaxBounds; ++k) wb[g][k] = c0 + k < nb ? tb[c0 + k] : e;
       cur[g] = LAYOUT == RFX_LAYOUT_UPPER ? pos + 1 : bstart[leaf];
@@ -427,62 +502,56 @@ __global__ __launch_bounds__(kGallopThreads) void gather_window_kernel(
       for (int k = 0; k < kWinMaxBounds; ++k) wb[g][k] = 0u;
     }
   }
-  if (tid == 0) diag_valid = 0u;
+  if (tid == 0) win_acc[kWinDiagWord] = 0u;
   __syncthreads();
   if (MODE == kMode_ALL || MODE == kMode_PAIR) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) valid += __shfl_down(valid, d, 64);
-    if (lane == 0 && valid) atomicAdd(&diag_valid, valid);
+    if (lane == 0 && valid) atomicAdd(&win_acc[kWinDiagWord], valid);
   }
-  const int nch = nb + 1;
+  const bool vec = ((reinterpret_cast<uintptr_t>(out) ^ reinterpret_cast<uintptr_t>(out2 ? out2 : out)) & 15) == 0;
   for (int c = c0; c < nch; ++c) {
     const int64_t lo = static_cast<int64_t>(c) * kWinChunk;
     const int64_t hi = min(n, lo + kWinChunk);
     const int64_t from = LAYOUT == RFX_LAYOUT_UPPER ? max(lo, line) : lo;  // first column written
     const int64_t wbase = lo & ~(kWinSpan - 1);
+    const int64_t o0 = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, from, n) : from + line * n;
+    const int count = static_cast<int>(hi - from);
+    // counter of column x at x - lo + sh: index and &out[o] agree mod 16 bytes
+    const int sh = vec ? static_cast<int>(((reinterpret_cast<uintptr_t>(out + o0) >> 2) - (from - lo)) & 3) : 0;
+    const int L0 = static_cast<int>(from - lo) + sh;
     __syncthreads();
-    for (int64_t q = from - lo + tid; q < hi - lo; q += kGallopThreads) win_acc[q] = 0u;
+    for (int q = (L0 >> 2) + tid; q < ((L0 + count + 3) >> 2); q += kGallopThreads)
+      win_acc4[q] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const uint32_t e2 = min(wb[g][0], end[g]);
+      const uint32_t stop = min(wb[g][0], end[g]);
       const uint32_t s0 = cur[g];
-      cur[g] = max(cur[g], e2);
+      cur[g] = max(cur[g], stop);
 #pragma unroll
       for (int k = 0; k + 1 < kWinMaxBounds; ++k) wb[g][k] = wb[g][k + 1];
       wb[g][kWinMaxBounds - 1] = end[g];
-      line_visits<MODE, LAYOUT>(s0, e2 > s0 ? e2 - s0 : 0u, (bagm >> g) & 1u, lane, members, win_acc, lo - wbase,
-                                line - wbase);
+      line_visits<MODE, LAYOUT>(s0, stop > s0 ? stop - s0 : 0u, (bagm >> g) & 1u, lane, members, win_acc,
+                                lo - wbase - sh, line - wbase);
     }
     __syncthreads();
     if ((MODE == kMode_ALL || MODE == kMode_PAIR) && tid == 0 && line >= lo && line < hi)
-      win_acc[line - lo] = diag_valid;
+      win_acc[line - lo + sh] = win_acc[kWinDiagWord];
     __syncthreads();
-    const int64_t base = LAYOUT == RFX_LAYOUT_UPPER ? rfx_upper_index(line, from, n) : lo + line * n;
-    const int64_t off = from - lo;
-    for (int64_t q = off + tid; q < hi - lo; q += kGallopThreads) {
-      const uint32_t v = win_acc[q];
-      const int64_t o = base + (q - off);
-      if (MODE == kMode_PAIR) {
-        const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-        out[o] = accumulate ? out[o] + va : va;
-        out2[o] = accumulate ? out2[o] + vb : vb;
-      } else {
-        out[o] = accumulate ? out[o] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
-      }
-    }
+    win_flush<MODE, kGallopThreads>(win_acc, L0, count, vec, out, out2, o0, accumulate);
   }
 }
-// counters of one chunk + the static diag_valid word
-static_assert(4 * kWinChunk + 4 <= kLdsLimitBytes, "window chunk exceeds the LDS");
+// counters of one chunk with their shift and rounding, and the diagonal word
+static_assert(4 * kWinAccWords <= kLdsLimitBytes, "window chunk exceeds the LDS");
 
 template <int MODE, int LAYOUT, int G>
 int launch_window(hipStream_t st, const BucketWs& w, int64_t n, int32_t ntree, int32_t accumulate, int32_t* out,
                   int32_t* out2) {
-  const size_t lds = 4 * static_cast<size_t>(kWinChunk);
+  const size_t lds = 4 * static_cast<size_t>(kWinAccWords);
   hipLaunchKernelGGL((gather_window_kernel<MODE, LAYOUT, G>), dim3(static_cast<unsigned>(xcd_rows_padded(n))),
-                     dim3(kGallopThreads), lds, st, w.pe, w.keys, w.wtab, w.nb, w.bstart, n, ntree, w.members_w,
-                     accumulate, out, out2);
+                     dim3(kGallopThreads), lds, st, w.members, w.rpe, w.rleaf, w.wtab, w.nb, w.bstart, n, ntree,
+                     w.members_w, accumulate, out, out2);
   return rfx_launch_status();
 }
 
@@ -569,8 +638,7 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
 template <int MODE, int LAYOUT>
 int run_gather(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, int64_t total_nodes,
                int32_t accumulate, int32_t* out, int32_t* out2, void* ws, bool prebuilt) {
-  BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes, true, true);
-  w.keys = keys;
+  const BucketWs w = bucket_ws_carve(ws, n, ntree, total_nodes, true, true);
   // the line kernels (n <= kGatherLineMaxN) read no 32-bit members: a lean
   // build; for UPPER they read only pe and the 16-bit members
   const bool lean = n <= kGatherLineMaxN && w.members16 != nullptr;
@@ -603,7 +671,7 @@ int launch_gather_kernels(hipStream_t st, const BucketWs& w, int64_t n, int32_t 
     return rfx_launch_status();
   }
   constexpr int TH = kGallopThreads;
-  if (w.wtab && w.members_w && w.keys && w.nb == win_bounds(n)) {
+  if (w.wtab && w.members_w && w.rpe && w.nb == win_bounds(n)) {
     if (ntree <= TH) return launch_window<MODE, LAYOUT, 1>(st, w, n, ntree, accumulate, out, out2);
     if (ntree <= 2 * TH) return launch_window<MODE, LAYOUT, 2>(st, w, n, ntree, accumulate, out, out2);
   }

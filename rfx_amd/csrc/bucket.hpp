@@ -490,6 +490,37 @@ __global__ void win_table_kernel(const uint32_t* __restrict__ mbucket, const uin
     for (int c = cj + 1; c <= nb; ++c) row[c - 1] = be;
 }
 
+// pe and the leaf ids transposed to line-major order (64 x 64 tiles through
+// LDS): rpe[line * ntree + t] = pe[t * n + line], rleaf likewise from keys.
+__global__ __launch_bounds__(256) void win_rows_kernel(const uint2* __restrict__ pe, const int32_t* __restrict__ keys,
+                                                       int64_t n, int32_t ntree, uint2* __restrict__ rpe,
+                                                       uint32_t* __restrict__ rleaf) {
+  __shared__ uint2 tp[64][65];
+  __shared__ uint32_t tl[64][65];
+  const int64_t l0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int t0 = static_cast<int>(blockIdx.y) * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int tt = e >> 6, ll = e & 63;  // consecutive threads: consecutive lines
+    const int64_t line = l0 + ll;
+    const int t = t0 + tt;
+    if (line < n && t < ntree) {
+      const int64_t s = static_cast<int64_t>(t) * n + line;
+      tp[tt][ll] = pe[s];
+      tl[tt][ll] = static_cast<uint32_t>(keys[s]) >> 1;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int ll = e >> 6, tt = e & 63;  // consecutive threads: consecutive trees
+    const int64_t line = l0 + ll;
+    const int t = t0 + tt;
+    if (line < n && t < ntree) {
+      rpe[line * ntree + t] = tp[tt][ll];
+      rleaf[line * ntree + t] = tl[tt][ll];
+    }
+  }
+}
+
 // Segmented form for a batch of problems laid end to end: slots
 // [seg_start[b], seg_start[b+1]) belong to problem b with seg_n[b] samples,
 // so the sample index is (s - seg_start[b]) % seg_n[b].
@@ -538,8 +569,9 @@ struct BucketWs {
   uint16_t* members16;  // nullptr unless n <= kMembers16MaxN
   uint16_t* members_w;  // nullptr unless the long-line window arrays are carved
   uint32_t* wtab;       // chunk-boundary table, nb entries per node (or nullptr)
+  uint2* rpe;           // pe by line: rpe[line * ntree + t] (with the window arrays)
+  uint32_t* rleaf;      // leaf of (line, tree) by line (with the window arrays)
   int nb;               // chunk boundaries per node in wtab
-  const int32_t* keys;  // the keys the buckets were built from (set by the count calls)
   uint32_t* mbucket;  // sorted keys (one of the two key buffers)
   uint2* pe;
   uint32_t* nvalid;
@@ -578,7 +610,9 @@ size_t bucket_ws_bytes(int64_t n, int32_t ntree, int64_t total_nodes, bool windo
   return align256(4 * total_nodes) * 2 + align256(4 * slots) * 5 + align256(8 * slots) + align256(4 * hlen) +
          align256(4 * (nparts + 1)) + 512 + align256(4 * static_cast<int64_t>(ntree)) + 256 +
          (n <= kMembers16MaxN ? align256(2 * slots) : 0) +
-         (windows && win_enabled(n) ? align256(2 * slots) + align256(4 * total_nodes * win_bounds(n)) : 0);
+         (windows && win_enabled(n) ? align256(2 * slots) + align256(4 * total_nodes * win_bounds(n)) +
+                                          align256(8 * slots) + align256(4 * slots)
+                                    : 0);
 }
 
 // per_tree: the one-problem layout (segments of n slots per tree); false for
@@ -623,13 +657,18 @@ BucketWs bucket_ws_carve(void* ws, int64_t n, int32_t ntree, int64_t total_nodes
   if (w.members16) p += align256(2 * slots);
   w.members_w = nullptr;
   w.wtab = nullptr;
+  w.rpe = nullptr;
+  w.rleaf = nullptr;
   w.nb = 0;
-  w.keys = nullptr;
   if (per_tree && windows && win_enabled(n)) {
     w.nb = win_bounds(n);
     w.members_w = reinterpret_cast<uint16_t*>(p);
     p += align256(2 * slots);
     w.wtab = reinterpret_cast<uint32_t*>(p);
+    p += align256(4 * total_nodes * w.nb);
+    w.rpe = reinterpret_cast<uint2*>(p);
+    p += align256(8 * slots);
+    w.rleaf = reinterpret_cast<uint32_t*>(p);
   }
   // the per-tree sort ends in buffer 0 and writes the global leaf ids into K[1]
   w.mbucket = per_tree ? w.K[1] : w.K[w.passes & 1];
@@ -1047,6 +1086,9 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   if (w.wtab)
     hipLaunchKernelGGL(win_table_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.members,
                        w.bstart, w.bend, N, static_cast<uint32_t>(total_nodes), w.nb, win_shift(), w.wtab);
+  if (w.rpe)
+    hipLaunchKernelGGL(win_rows_kernel, dim3(rfx_ceil_div(n, 64), rfx_ceil_div(ntree, 64)), dim3(256), 0, st, w.pe,
+                       keys, n, ntree, w.rpe, w.rleaf);
   rc = rfx_launch_status();
   note_bucket_build(w.bstart, BucketBuildRecord{n, ntree, total_nodes, rc == RFX_OK, rc == RFX_OK && w.wtab});
   return rc;

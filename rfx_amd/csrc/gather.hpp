@@ -118,9 +118,11 @@ __device__ __forceinline__ void flat_visit_blocks(uint32_t s, uint32_t len, uint
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t wv[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
+      // members blo .. bhi - 1 of the block are in the run (P <= 8)
+      const uint32_t vm = ((1u << bhi[u]) - 1u) & ~((1u << blo[u]) - 1u);
 #pragma unroll
       for (uint32_t k = 0; k < P; ++k) {
-        if (k < blo[u] || k >= bhi[u]) continue;
+        if (!((vm >> k) & 1u)) continue;
         const uint32_t m = sizeof(M) == 2 ? ((wv[k / 2] >> (16 * (k % 2))) & 0xffffu) : wv[k];
         const int64_t q = m >> 1;
         const uint32_t bq = m & 1u;

@@ -348,7 +348,7 @@ void LowRankProximityMatrix::get_factors(dp_t** A_dev, dp_t** B_dev, integer_t* 
 std::vector<double> LowRankProximityMatrix::compute_mds_from_factors(integer_t k) {
   if (!ready_) throw std::runtime_error("LowRankProximityMatrix: factors not finalized");
   if (k < 1 || k > 3) throw std::runtime_error("compute_mds_from_factors supports k <= 3");
-  if (rank_ > 64) throw std::runtime_error("factor MDS supports rank <= 64");
+  if (rank_ > 128) throw std::runtime_error("factor MDS supports rank <= 128");
   // matrix cores on the stored factors, refined with the FP32 factors
   const size_t wsb = rfx_mds_factors_mixed_workspace_bytes(n_, rank_);
   DevBuf ws(wsb), cd(8 * 3 * static_cast<size_t>(n_));
