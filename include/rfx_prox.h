@@ -43,6 +43,7 @@ extern "C" {
 
 /* ---- grower flags -------------------------------------------------------- */
 #define RFX_GROW_NO_FMA 1 /* Gini pno sum without fused multiply-add        */
+#define RFX_GROW_PLAIN 2  /* per-node sorts instead of presorted columns     */
 
 /* ---- proximity variants -------------------------------------------------
  * ALL       C[i,j] = #{t : leaf_t(i) = leaf_t(j)}                (symmetric)

@@ -26,6 +26,7 @@ DATA_SUBSPACE_MIXTURE = 2
 DATA_OVERLAP_MIXTURE = 3
 
 GROW_NO_FMA = 1
+GROW_PLAIN = 2
 
 PROX_ALL = 0
 PROX_OOB_ASYM = 1

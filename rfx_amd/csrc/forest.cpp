@@ -10,10 +10,15 @@
 // their parents, so the reference traversal reaches node k exactly through
 // the splits this partition applies), and the split search sees the node's
 // samples through a (value, index) sort, whose order is total, so the float
-// Gini arithmetic runs in the same sequence as the reference's.
+// Gini arithmetic runs in the same sequence as the reference's.  Two growers
+// produce these trees: grow_tree sorts each node's samples per drawn
+// variable; grow_tree_presorted (the default) keeps every column of the tree
+// sorted for the whole tree and is what makes n = 1,000,000 forests feasible.
 //
 // Everything here is plain host C++ (no HIP): the forest is an input of the
 // path, grown once and exported; it is not part of the timed GPU work.
+
+#include <immintrin.h>
 
 #include <algorithm>
 #include <atomic>
@@ -276,6 +281,628 @@ void grow_tree(const GrowParams& gp, const int32_t* nin, uint32_t seed, GrowScra
 }
 
 // ---------------------------------------------------------------------------
+// Presorted grower: the same trees as grow_tree, without a sort per node.
+//
+// grow_tree sorts the node's samples by (value, index) once per drawn
+// variable, so a node of m samples costs mtry sorts of m gathered values.
+// On data whose trees peel one sample off a large impure node per split
+// (the 16-component mixture with 24 noise columns and minndsize 5 has chains
+// over a thousand nodes deep), the sum of node sizes reaches n x 100 and the
+// sorts dominate.  Here every feature column of the tree's in-bag samples is
+// kept sorted by (value, index) for the whole tree, each node owning the same
+// range [b, e) in every column: the global (value, index) order is filtered
+// to the in-bag samples once per tree, and after a split every column's range
+// is stably partitioned by the side each sample went to.  A stable partition
+// of a (value, index)-sorted range leaves both halves (value, index)-sorted,
+// and the split variable's own range is already partitioned (its left part is
+// the prefix of values <= the split value), so the scans see the same
+// sequences, and do the same float operations, as grow_tree's.
+//
+// A column record holds the value and (local in-bag id << 8) | class, so the
+// fast path needs nclass <= 256, labels in [0, nclass), fewer than 2^24
+// in-bag samples and no NaN; rfx_forest_grow uses grow_tree otherwise.
+// ---------------------------------------------------------------------------
+struct ColRec {
+  float v;
+  uint32_t lc;  // (local id << 8) | class
+};
+
+struct PresortedData {
+  // Global order of every column: (value, sample) pairs sorted by value then
+  // sample, column-major p x n.
+  std::vector<ColRec> order;
+  bool avx2 = false;
+  bool lanes = false;  // AVX2 and a column array whose indices fit int32
+};
+
+struct PeelEvent {
+  int32_t b, e;    // the split node's range
+  int32_t s;       // records on the small side
+  bool few_left;   // the small side is the left child
+  int32_t lid_at;  // offset of the small side's local ids in the lid pool
+};
+
+struct PeelLog {
+  std::vector<PeelEvent> events;
+  std::vector<int32_t> lids;
+  std::vector<std::vector<int32_t>> pend;  // per column, event ids, outermost first
+  std::vector<std::pair<int32_t, int32_t>> rem;  // (position, destination)
+  std::vector<int32_t> gpos;
+  std::vector<ColRec> saved;
+
+  void reset(int32_t p) {
+    events.clear();
+    lids.clear();
+    pend.resize(p);
+    for (auto& v : pend) v.clear();
+  }
+  static int32_t big_b(const PeelEvent& ev) { return ev.few_left ? ev.b + ev.s : ev.b; }
+  static int32_t big_e(const PeelEvent& ev) { return ev.few_left ? ev.e : ev.e - ev.s; }
+};
+
+// Pass-1 output of the split scan: numerators and positions.
+struct ScanBuf {
+  std::vector<float> ln, rn;
+  std::vector<int32_t> at;
+};
+
+struct FastScratch {
+  std::vector<int32_t> lid;   // n: local in-bag id, -1 out of bag
+  std::vector<int32_t> sid;   // m0: sample of each local id
+  std::vector<ColRec> col;    // p x m0, column-major
+  std::vector<ColRec> tmp;    // m0
+  std::vector<uint8_t> left;  // m0
+  std::vector<int32_t> start, end;
+  std::vector<int32_t> wl, wr;  // class counts as integers
+  std::vector<float> cc;
+  ScanBuf sb;
+  std::vector<int32_t> vars;
+  std::vector<int32_t> lane_wl, lane_wr;  // 8 x nclass: class counts per lane
+  std::vector<int32_t> clean_col, cc_at;
+  std::vector<float> cc_pool;
+  PeelLog peels;
+};
+
+// Smallest node scanned eight columns at a time.
+constexpr int32_t kLaneScanMin = 256;
+
+void presort_columns(const GrowParams& gp, int nthreads, PresortedData& d) {
+  const int64_t n = gp.n;
+  const int32_t p = gp.p;
+  d.order.resize(static_cast<size_t>(p) * n);
+  d.avx2 = __builtin_cpu_supports("avx2");
+  d.lanes = d.avx2 && static_cast<int64_t>(p) * n < (int64_t{1} << 31);
+  std::atomic<int32_t> next{0};
+  auto worker = [&]() {
+    std::vector<std::pair<float, int32_t>> s(n);
+    for (;;) {
+      const int32_t v = next.fetch_add(1);
+      if (v >= p) break;
+      for (int64_t i = 0; i < n; ++i) s[i] = {gp.x[i * p + v], static_cast<int32_t>(i)};
+      std::sort(s.begin(), s.end());
+      ColRec* o = d.order.data() + static_cast<size_t>(v) * n;
+      for (int64_t i = 0; i < n; ++i) o[i] = {s[i].first, static_cast<uint32_t>(s[i].second)};
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 1; i < nthreads && i < p; ++i) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+}
+
+// Split scan of one (value, index)-sorted column range: grow_tree's float
+// arithmetic in the same order, in two passes.
+//
+// Pass 1 runs the serial part: for every position that is not tied with the
+// next, the class moves to the left side and the Gini numerators advance,
+//   rln = rln + (1 + 2 wl[c]),  rrn = rrn + (1 - 2 wr[c]),
+// and the pair (rln, rrn) is stored with the position.  The increments are
+// exact integers in float (|.| <= 2m + 1 < 2^24), so the counts are kept as
+// integers.  The left and right sizes rld, rrd are the integers cnt and m - cnt
+// (pdo is the node size, exact in float), so min(rrd, rld) > 1e-5 always holds.
+// Pass 2 evaluates crit = rln / cnt + rrn / (m - cnt) for all stored
+// positions, eight at a time (IEEE division and addition, the same roundings
+// as the scalar code), and keeps the first position of the largest value;
+// the scalar loop's "crit > best" updates end on exactly that position.
+struct ScanBest {
+  float crit;
+  int32_t var;
+  float split;
+  int32_t nl;
+};
+
+inline int32_t scan_prefix(const ColRec* c, int32_t m, float pno, int32_t nclass, const float* cc, int32_t* wl,
+                           int32_t* wr, float* ln, float* rn, int32_t* at) {
+  for (int32_t j = 0; j < nclass; ++j) {
+    wl[j] = 0;
+    wr[j] = static_cast<int32_t>(cc[j]);
+  }
+  int32_t k = 0;
+  float rln = 0.0f, rrn = pno;
+  float v1 = c[0].v;
+  for (int32_t ii = 0; ii + 1 < m; ++ii) {
+    const float v2 = c[ii + 1].v;
+    if (v1 < v2) {
+      const uint32_t cls = c[ii].lc & 0xffu;
+      const int32_t a = wl[cls], bb = wr[cls];
+      wl[cls] = a + 1;
+      wr[cls] = bb - 1;
+      rln = rln + static_cast<float>(1 + 2 * a);
+      rrn = rrn + static_cast<float>(1 - 2 * bb);
+      ln[k] = rln;
+      rn[k] = rrn;
+      at[k] = ii;
+      ++k;
+    }
+    v1 = v2;
+  }
+  return k;
+}
+
+// First index of the largest crit over positions [0, K); returns -1 if none
+// exceeds `floor`.
+int32_t scan_best_scalar(const float* ln, const float* rn, int32_t K, int32_t m, float floor, float* out) {
+  float best = floor;
+  int32_t arg = -1;
+  for (int32_t k = 0; k < K; ++k) {
+    const float crit = (ln[k] / static_cast<float>(k + 1)) + (rn[k] / static_cast<float>(m - k - 1));
+    if (crit > best) {
+      best = crit;
+      arg = k;
+    }
+  }
+  *out = best;
+  return arg;
+}
+
+__attribute__((target("avx2"))) int32_t scan_best_avx2(const float* ln, const float* rn, int32_t K, int32_t m,
+                                                         float floor, float* out) {
+  const int32_t K8 = K & ~7;
+  __m256 vbest = _mm256_set1_ps(-3.0e38f);
+  __m256i varg = _mm256_set1_epi32(-1);
+  __m256i vk = _mm256_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7);
+  const __m256i one = _mm256_set1_epi32(1), eight = _mm256_set1_epi32(8), vm = _mm256_set1_epi32(m);
+  for (int32_t k = 0; k < K8; k += 8) {
+    const __m256 l = _mm256_loadu_ps(ln + k), r = _mm256_loadu_ps(rn + k);
+    const __m256 dl = _mm256_cvtepi32_ps(_mm256_add_epi32(vk, one));
+    const __m256 dr = _mm256_cvtepi32_ps(_mm256_sub_epi32(_mm256_sub_epi32(vm, vk), one));
+    const __m256 crit = _mm256_add_ps(_mm256_div_ps(l, dl), _mm256_div_ps(r, dr));
+    const __m256 gt = _mm256_cmp_ps(crit, vbest, _CMP_GT_OQ);
+    vbest = _mm256_blendv_ps(vbest, crit, gt);
+    varg = _mm256_blendv_epi8(varg, vk, _mm256_castps_si256(gt));
+    vk = _mm256_add_epi32(vk, eight);
+  }
+  alignas(32) float bl[8];
+  alignas(32) int32_t al[8];
+  _mm256_store_ps(bl, vbest);
+  _mm256_store_si256(reinterpret_cast<__m256i*>(al), varg);
+  // Lane maxima hold their lane's first maximum; the overall first maximum
+  // is the smallest index among the lanes with the largest value.
+  float best = floor;
+  int32_t arg = -1;
+  for (int q = 0; q < 8; ++q) {
+    if (al[q] < 0) continue;
+    if (bl[q] > best || (bl[q] == best && arg >= 0 && al[q] < arg)) {
+      best = bl[q];
+      arg = al[q];
+    }
+  }
+  for (int32_t k = K8; k < K; ++k) {
+    const float crit = (ln[k] / static_cast<float>(k + 1)) + (rn[k] / static_cast<float>(m - k - 1));
+    if (crit > best) {
+      best = crit;
+      arg = k;
+    }
+  }
+  *out = best;
+  return arg;
+}
+
+inline void scan_column(const ColRec* c, int32_t m, int32_t mvar, float pno, int32_t nclass, const float* cc,
+                        int32_t* wl, int32_t* wr, ScanBuf& sb, bool avx2, ScanBest& best) {
+  const int32_t K = scan_prefix(c, m, pno, nclass, cc, wl, wr, sb.ln.data(), sb.rn.data(), sb.at.data());
+  float crit;
+  const int32_t arg = avx2 ? scan_best_avx2(sb.ln.data(), sb.rn.data(), K, m, best.crit, &crit)
+                           : scan_best_scalar(sb.ln.data(), sb.rn.data(), K, m, best.crit, &crit);
+  if (arg >= 0) {
+    const int32_t ii = sb.at[arg];
+    best.crit = crit;
+    best.var = mvar;
+    best.split = c[ii].v;
+    best.nl = ii + 1;
+  }
+}
+
+// Up to eight columns scanned together, one per AVX2 lane, for nodes where one
+// class dominates.  Each lane runs the scalar scan's float sequence on its
+// own column: rln and rrn advance by the exact integer increments of the
+// lane's class, and crit = rln / cnt + rrn / (m - cnt) is formed with the same
+// IEEE single divisions and addition.  The counts of the lane's current
+// class are kept in registers and written back to the lane's table when its
+// class changes (rare when one class dominates).  Each lane keeps the first
+// position of its largest crit; the caller merges lanes in draw order with
+// the scalar rule, so the result is that of the columns scanned one by one.
+struct LaneBest {
+  float crit[8];
+  int32_t at[8];
+};
+
+__attribute__((target("avx2"))) inline void gather_lanes(const ColRec* col, __m128i ilo, __m128i ihi, __m256& v,
+                                                          __m256i& c) {
+  const long long* base = reinterpret_cast<const long long*>(col);
+  const __m256 lo = _mm256_castsi256_ps(_mm256_i32gather_epi64(base, ilo, 8));
+  const __m256 hi = _mm256_castsi256_ps(_mm256_i32gather_epi64(base, ihi, 8));
+  // lo = v0 l0 v1 l1 | v2 l2 v3 l3, hi = v4 l4 v5 l5 | v6 l6 v7 l7
+  const __m256d vv = _mm256_castps_pd(_mm256_shuffle_ps(lo, hi, _MM_SHUFFLE(2, 0, 2, 0)));
+  const __m256d ll = _mm256_castps_pd(_mm256_shuffle_ps(lo, hi, _MM_SHUFFLE(3, 1, 3, 1)));
+  v = _mm256_castpd_ps(_mm256_permute4x64_pd(vv, _MM_SHUFFLE(3, 1, 2, 0)));
+  c = _mm256_and_si256(_mm256_castpd_si256(_mm256_permute4x64_pd(ll, _MM_SHUFFLE(3, 1, 2, 0))),
+                       _mm256_set1_epi32(0xff));
+}
+
+// base[q]: index in col of lane q's first record; every lane reads m records.
+__attribute__((target("avx2"))) void scan_lanes_avx2(const ColRec* col, const int32_t* base, int32_t m, float pno,
+                                                     int32_t nclass, const float* cc, int32_t* wl, int32_t* wr,
+                                                     LaneBest& out) {
+  alignas(32) int32_t a_[8], b_[8], cur_[8];
+  __m128i ilo = _mm_loadu_si128(reinterpret_cast<const __m128i*>(base));
+  __m128i ihi = _mm_loadu_si128(reinterpret_cast<const __m128i*>(base + 4));
+  const __m128i one4 = _mm_set1_epi32(1);
+  __m256 v1;
+  __m256i c;
+  gather_lanes(col, ilo, ihi, v1, c);
+  _mm256_store_si256(reinterpret_cast<__m256i*>(cur_), c);
+  for (int q = 0; q < 8; ++q) {
+    for (int32_t j = 0; j < nclass; ++j) {
+      wl[q * nclass + j] = 0;
+      wr[q * nclass + j] = static_cast<int32_t>(cc[j]);
+    }
+    a_[q] = 0;
+    b_[q] = static_cast<int32_t>(cc[cur_[q]]);
+  }
+  __m256i a = _mm256_load_si256(reinterpret_cast<const __m256i*>(a_));
+  __m256i bb = _mm256_load_si256(reinterpret_cast<const __m256i*>(b_));
+  __m256i cur = c;
+  __m256i cnt = _mm256_setzero_si256();
+  __m256 rln = _mm256_setzero_ps(), rrn = _mm256_set1_ps(pno);
+  __m256 best = _mm256_set1_ps(-3.0e38f);
+  __m256i bestat = _mm256_set1_epi32(-1);
+  const __m256i one = _mm256_set1_epi32(1), vm = _mm256_set1_epi32(m);
+  for (int32_t ii = 0; ii + 1 < m; ++ii) {
+    if ((ii & 7) == 0) {
+      // The gathers walk eight streams; fetch each a few lines ahead.
+      for (int q = 0; q < 8; ++q) _mm_prefetch(reinterpret_cast<const char*>(col + base[q] + ii + 64), _MM_HINT_T0);
+    }
+    ilo = _mm_add_epi32(ilo, one4);
+    ihi = _mm_add_epi32(ihi, one4);
+    __m256 v2;
+    __m256i c2;
+    gather_lanes(col, ilo, ihi, v2, c2);
+    const __m256 moved = _mm256_cmp_ps(v1, v2, _CMP_LT_OQ);
+    const __m256i sw = _mm256_andnot_si256(_mm256_cmpeq_epi32(c, cur), _mm256_castps_si256(moved));
+    if (__builtin_expect(!_mm256_testz_si256(sw, sw), 0)) {
+      _mm256_store_si256(reinterpret_cast<__m256i*>(a_), a);
+      _mm256_store_si256(reinterpret_cast<__m256i*>(b_), bb);
+      _mm256_store_si256(reinterpret_cast<__m256i*>(cur_), cur);
+      alignas(32) int32_t c_[8], sw_[8];
+      _mm256_store_si256(reinterpret_cast<__m256i*>(c_), c);
+      _mm256_store_si256(reinterpret_cast<__m256i*>(sw_), sw);
+      for (int q = 0; q < 8; ++q) {
+        if (!sw_[q]) continue;
+        wl[q * nclass + cur_[q]] = a_[q];
+        wr[q * nclass + cur_[q]] = b_[q];
+        cur_[q] = c_[q];
+        a_[q] = wl[q * nclass + c_[q]];
+        b_[q] = wr[q * nclass + c_[q]];
+      }
+      a = _mm256_load_si256(reinterpret_cast<const __m256i*>(a_));
+      bb = _mm256_load_si256(reinterpret_cast<const __m256i*>(b_));
+      cur = _mm256_load_si256(reinterpret_cast<const __m256i*>(cur_));
+    }
+    const __m256i mv = _mm256_castps_si256(moved);
+    const __m256 t1 = _mm256_cvtepi32_ps(_mm256_add_epi32(_mm256_add_epi32(a, a), one));
+    const __m256 t2 = _mm256_cvtepi32_ps(_mm256_sub_epi32(one, _mm256_add_epi32(bb, bb)));
+    rln = _mm256_blendv_ps(rln, _mm256_add_ps(rln, t1), moved);
+    rrn = _mm256_blendv_ps(rrn, _mm256_add_ps(rrn, t2), moved);
+    a = _mm256_sub_epi32(a, mv);
+    bb = _mm256_add_epi32(bb, mv);
+    cnt = _mm256_sub_epi32(cnt, mv);
+    const __m256 dl = _mm256_cvtepi32_ps(cnt), dr = _mm256_cvtepi32_ps(_mm256_sub_epi32(vm, cnt));
+    const __m256 crit = _mm256_add_ps(_mm256_div_ps(rln, dl), _mm256_div_ps(rrn, dr));
+    const __m256 better = _mm256_and_ps(moved, _mm256_cmp_ps(crit, best, _CMP_GT_OQ));
+    best = _mm256_blendv_ps(best, crit, better);
+    bestat = _mm256_blendv_epi8(bestat, _mm256_set1_epi32(ii), _mm256_castps_si256(better));
+    v1 = v2;
+    c = c2;
+  }
+  _mm256_storeu_ps(out.crit, best);
+  _mm256_storeu_si256(reinterpret_cast<__m256i*>(out.at), bestat);
+}
+
+// Stable partition of one column range by the side of every record.
+inline void partition_flags(ColRec* c, int32_t m, const uint8_t* left, ColRec* tmp) {
+  int32_t nl = 0, nr = 0;
+  for (int32_t q = 0; q < m; ++q) {
+    const ColRec x = c[q];
+    const int32_t f = left[x.lc >> 8];
+    c[nl] = x;
+    tmp[nr] = x;
+    nl += f;
+    nr += 1 - f;
+  }
+  std::copy(tmp, tmp + nr, c + nl);
+}
+
+// A split that sends only a few records to one side ("peel": on the
+// 16-component data, chains of tens of thousands of nodes each lose one
+// sample) is applied to the split column at once -- its range is already in
+// order -- but to every other column only when that column is next read.
+// A column's pending peels are nested: each one's node is the larger child of
+// the previous one.  They are applied in one pass over the outermost range:
+// the peeled records are found by binary search on (value, local id), which
+// is the range's order before any of them was applied (local ids ascend with
+// the sample index), and the records between them move by whole segments to
+// their final places.  Each record ends where applying the peels one by one,
+// as stable partitions, would have put it.
+inline int32_t find_record(const ColRec* c, int32_t lo, int32_t hi, float v, uint32_t l) {
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    const ColRec r = c[mid];
+    if (r.v < v || (!(v < r.v) && (r.lc >> 8) < l))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Apply column v's pending peels.  c = the column, xv = &x[v] (row stride p).
+void apply_peels(PeelLog& pl, int32_t v, ColRec* c, const float* xv, int32_t p, const int32_t* sid) {
+  std::vector<int32_t>& ids = pl.pend[v];
+  if (ids.empty()) return;
+  const PeelEvent& outer = pl.events[ids.front()];
+  const PeelEvent& inner = pl.events[ids.back()];
+  const int32_t B = outer.b, E = outer.e;
+  pl.rem.clear();
+  for (int32_t id : ids) {
+    const PeelEvent& ev = pl.events[id];
+    pl.gpos.resize(ev.s);
+    for (int32_t k = 0; k < ev.s; ++k) {
+      const uint32_t l = static_cast<uint32_t>(pl.lids[ev.lid_at + k]);
+      pl.gpos[k] = find_record(c, B, E, xv[static_cast<int64_t>(sid[l]) * p], l);
+    }
+    std::sort(pl.gpos.begin(), pl.gpos.end());
+    const int32_t d0 = ev.few_left ? ev.b : ev.e - ev.s;
+    for (int32_t k = 0; k < ev.s; ++k) pl.rem.push_back({pl.gpos[k], d0 + k});
+  }
+  std::sort(pl.rem.begin(), pl.rem.end());
+  const int32_t R = static_cast<int32_t>(pl.rem.size());
+  pl.saved.resize(R);
+  for (int32_t k = 0; k < R; ++k) pl.saved[k] = c[pl.rem[k].first];
+  // Segment j holds the records between removed positions j - 1 and j; its
+  // records move by L - j, L = records peeled to the left.
+  const int32_t L = PeelLog::big_b(inner) - B;
+  auto seg = [&](int32_t j, int32_t& lo, int32_t& hi) {
+    lo = j == 0 ? B : pl.rem[j - 1].first + 1;
+    hi = j == R ? E : pl.rem[j].first;
+  };
+  for (int32_t j = L - 1; j >= 0; --j) {
+    int32_t lo, hi;
+    seg(j, lo, hi);
+    if (hi > lo) std::memmove(c + lo + (L - j), c + lo, sizeof(ColRec) * static_cast<size_t>(hi - lo));
+  }
+  for (int32_t j = L + 1; j <= R; ++j) {
+    int32_t lo, hi;
+    seg(j, lo, hi);
+    if (hi > lo) std::memmove(c + lo - (j - L), c + lo, sizeof(ColRec) * static_cast<size_t>(hi - lo));
+  }
+  for (int32_t k = 0; k < R; ++k) c[pl.rem[k].second] = pl.saved[k];
+  ids.clear();
+}
+
+void grow_tree_presorted(const GrowParams& gp, const PresortedData& d, const int32_t* nin, uint32_t seed,
+                         FastScratch& s, Tree& t) {
+  const int32_t maxnode = gp.maxnode;
+  const int32_t nclass = gp.nclass;
+  const int32_t p = gp.p;
+  const int64_t n = gp.n;
+  Mt19937 rng(seed);
+
+  std::vector<int32_t> status(maxnode, 0), bestvar(maxnode, 0), treemap(2 * static_cast<size_t>(maxnode), 0),
+      nodeclass(maxnode, 0);
+  std::vector<float> xsplit(maxnode, 0.0f);
+  s.start.assign(maxnode, 0);
+  s.end.assign(maxnode, 0);
+  // A column that is in order over each node's range when the node is
+  // reached (its parent's split column), and class counts handed down by
+  // peel splits (offset into cc_pool, or -1).
+  s.clean_col.assign(maxnode, 0);
+  s.cc_at.assign(maxnode, -1);
+  s.cc_pool.clear();
+  s.peels.reset(p);
+
+  // Unique in-bag samples, numbered in ascending sample order (jinbag).
+  s.lid.resize(n);
+  s.sid.clear();
+  for (int64_t i = 0; i < n; ++i) {
+    s.lid[i] = nin[i] > 0 ? static_cast<int32_t>(s.sid.size()) : -1;
+    if (nin[i] > 0) s.sid.push_back(static_cast<int32_t>(i));
+  }
+  const int32_t m0 = static_cast<int32_t>(s.sid.size());
+  s.col.resize(static_cast<size_t>(p) * m0);
+  s.tmp.resize(m0);
+  s.left.resize(m0);
+  for (int32_t v = 0; v < p; ++v) {
+    const ColRec* o = d.order.data() + static_cast<size_t>(v) * n;
+    ColRec* c = s.col.data() + static_cast<size_t>(v) * m0;
+    int32_t q = 0;
+    for (int64_t r = 0; r < n; ++r) {
+      const uint32_t i = o[r].lc;
+      const int32_t l = s.lid[i];
+      if (l >= 0) c[q++] = {o[r].v, (static_cast<uint32_t>(l) << 8) | static_cast<uint32_t>(gp.cl[i])};
+    }
+  }
+  s.cc.assign(nclass, 0.0f);
+  s.wl.assign(nclass, 0);
+  s.wr.assign(nclass, 0);
+  if (d.lanes) {
+    s.lane_wl.resize(static_cast<size_t>(nclass) * 8);
+    s.lane_wr.resize(static_cast<size_t>(nclass) * 8);
+  }
+  s.sb.ln.resize(m0);
+  s.sb.rn.resize(m0);
+  s.sb.at.resize(m0);
+  auto column = [&](int32_t v) { return s.col.data() + static_cast<size_t>(v) * m0; };
+  auto make_clean = [&](int32_t v) { apply_peels(s.peels, v, column(v), gp.x + v, p, s.sid.data()); };
+
+  status[0] = 2;
+  s.start[0] = 0;
+  s.end[0] = m0;
+  int32_t nnode = 1;
+
+  for (int32_t k = 0; k < nnode; ++k) {
+    if (status[k] != 2) continue;
+    const int32_t b = s.start[k], e = s.end[k];
+    const int32_t m = e - b;
+
+    if (s.cc_at[k] >= 0) {
+      std::copy(s.cc_pool.begin() + s.cc_at[k], s.cc_pool.begin() + s.cc_at[k] + nclass, s.cc.begin());
+    } else {
+      std::fill(s.cc.begin(), s.cc.end(), 0.0f);
+      const ColRec* c0 = column(s.clean_col[k]) + b;
+      for (int32_t q = 0; q < m; ++q) s.cc[c0[q].lc & 0xffu] += 1.0f;
+    }
+    int32_t nonzero = 0, majority = 0;
+    float maxc = 0.0f;
+    for (int32_t j = 0; j < nclass; ++j) {
+      if (s.cc[j] > 0.0f) {
+        ++nonzero;
+        if (s.cc[j] > maxc) {
+          maxc = s.cc[j];
+          majority = j;
+        }
+      }
+    }
+    nodeclass[k] = majority;
+    if (m < gp.minndsize || nonzero <= 1) {
+      status[k] = -1;
+      continue;
+    }
+
+    float pno = 0.0f;
+    for (int32_t j = 0; j < nclass; ++j) {
+      if (gp.fma_gini)
+        pno = std::fmaf(s.cc[j], s.cc[j], pno);
+      else
+        pno = pno + s.cc[j] * s.cc[j];
+    }
+
+    ScanBest best{-2.0f, 0, 0.0f, 0};
+    // The drawn variables do not depend on the scans, so they are drawn first.
+    s.vars.resize(gp.mtry);
+    for (int32_t mv = 0; mv < gp.mtry; ++mv) {
+      float u = rng.next_float();
+      int32_t mvar = static_cast<int32_t>(u * static_cast<float>(p));
+      if (mvar >= p) mvar = p - 1;
+      s.vars[mv] = mvar;
+      make_clean(mvar);
+    }
+    if (d.lanes && m >= kLaneScanMin && maxc >= 0.9f * static_cast<float>(m)) {
+      // Columns in groups of eight lanes, merged in draw order.
+      for (int32_t g0 = 0; g0 < gp.mtry; g0 += 8) {
+        const int32_t gn = std::min(8, gp.mtry - g0);
+        alignas(16) int32_t base[8];
+        for (int q = 0; q < 8; ++q) base[q] = s.vars[g0 + (q < gn ? q : 0)] * m0 + b;
+        LaneBest lb;
+        scan_lanes_avx2(s.col.data(), base, m, pno, nclass, s.cc.data(), s.lane_wl.data(), s.lane_wr.data(), lb);
+        for (int32_t q = 0; q < gn; ++q) {
+          if (lb.at[q] >= 0 && lb.crit[q] > best.crit) {
+            best.crit = lb.crit[q];
+            best.var = s.vars[g0 + q];
+            best.split = s.col[static_cast<size_t>(base[q]) + lb.at[q]].v;
+            best.nl = lb.at[q] + 1;
+          }
+        }
+      }
+    } else {
+      for (int32_t mv = 0; mv < gp.mtry; ++mv)
+        scan_column(column(s.vars[mv]) + b, m, s.vars[mv], pno, nclass, s.cc.data(), s.wl.data(), s.wr.data(),
+                    s.sb, d.avx2, best);
+    }
+
+    if (best.crit > -1.0f && nnode + 2 <= maxnode) {
+      const int32_t lc = nnode, rc = nnode + 1;
+      status[k] = 1;
+      bestvar[k] = best.var;
+      xsplit[k] = best.split;
+      treemap[2 * static_cast<size_t>(k)] = lc;
+      treemap[2 * static_cast<size_t>(k) + 1] = rc;
+      status[lc] = 2;
+      status[rc] = 2;
+      s.clean_col[lc] = best.var;
+      s.clean_col[rc] = best.var;
+      // Sides: the split column's left part is its first best.nl records.
+      const ColRec* cb = column(best.var) + b;
+      const int32_t nl = best.nl, nr = m - nl;
+      const int32_t few = std::min(nl, nr);
+      if (static_cast<int64_t>(few) * 64 <= m) {
+        PeelEvent ev{b, e, few, nl <= nr, static_cast<int32_t>(s.peels.lids.size())};
+        const ColRec* fc = ev.few_left ? cb : cb + nl;
+        // Class counts of both children, from the parent's and the few records'.
+        const int32_t at_few = static_cast<int32_t>(s.cc_pool.size());
+        s.cc_pool.resize(at_few + 2 * static_cast<size_t>(nclass), 0.0f);
+        float* ccf = s.cc_pool.data() + at_few;
+        float* ccb = ccf + nclass;
+        for (int32_t q = 0; q < few; ++q) {
+          s.peels.lids.push_back(static_cast<int32_t>(fc[q].lc >> 8));
+          ccf[fc[q].lc & 0xffu] += 1.0f;
+        }
+        for (int32_t j = 0; j < nclass; ++j) ccb[j] = s.cc[j] - ccf[j];
+        s.cc_at[ev.few_left ? lc : rc] = at_few;
+        s.cc_at[ev.few_left ? rc : lc] = at_few + nclass;
+        const int32_t id = static_cast<int32_t>(s.peels.events.size());
+        s.peels.events.push_back(ev);
+        for (int32_t v = 0; v < p; ++v) {
+          if (v == best.var) continue;
+          std::vector<int32_t>& pv = s.peels.pend[v];
+          if (!pv.empty()) {
+            const PeelEvent& last = s.peels.events[pv.back()];
+            if (b < PeelLog::big_b(last) || e > PeelLog::big_e(last)) make_clean(v);
+          }
+          pv.push_back(id);
+        }
+      } else {
+        for (int32_t v = 0; v < p; ++v) make_clean(v);
+        for (int32_t q = 0; q < m; ++q) s.left[cb[q].lc >> 8] = q < nl ? 1 : 0;
+        for (int32_t v = 0; v < p; ++v) {
+          if (v == best.var) continue;
+          partition_flags(column(v) + b, m, s.left.data(), s.tmp.data());
+        }
+      }
+      s.start[lc] = b;
+      s.end[lc] = b + nl;
+      s.start[rc] = b + nl;
+      s.end[rc] = e;
+      nnode += 2;
+    } else {
+      status[k] = -1;
+    }
+  }
+
+  status.resize(nnode);
+  bestvar.resize(nnode);
+  xsplit.resize(nnode);
+  treemap.resize(2 * static_cast<size_t>(nnode));
+  nodeclass.resize(nnode);
+  t.status = std::move(status);
+  t.bestvar = std::move(bestvar);
+  t.xsplit = std::move(xsplit);
+  t.treemap = std::move(treemap);
+  t.nodeclass = std::move(nodeclass);
+  t.nnode = nnode;
+}
+
+// ---------------------------------------------------------------------------
 // Counter-based generator for the synthetic datasets: SplitMix64 finaliser of
 // (seed, stream, index).  Identical on every host because it is pure integer
 // arithmetic followed by libm calls of the same image.
@@ -501,11 +1128,25 @@ int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int3
   GrowParams gp{x, y, n, p, nclass, maxnode, minndsize, mtry, (flags & RFX_GROW_NO_FMA) == 0};
   int nt = nthreads > 0 ? nthreads : static_cast<int>(std::thread::hardware_concurrency());
   if (nt < 1) nt = 1;
+  // The presorted grower needs byte-sized valid labels and NaN-free features.
+  bool presorted = (flags & RFX_GROW_PLAIN) == 0 && nclass <= 256;
+  for (int64_t i = 0; presorted && i < n; ++i) presorted = y[i] >= 0 && y[i] < nclass;
+  for (int64_t i = 0; presorted && i < n * p; ++i) presorted = !std::isnan(x[i]);
+  PresortedData pd;
+  if (presorted) {
+    try {
+      presort_columns(gp, nt, pd);
+    } catch (...) {
+      delete f;
+      return RFX_ERR_ALLOC;
+    }
+  }
   if (nt > ntree) nt = ntree;
   std::atomic<int32_t> next{0};
   std::atomic<int> failed{0};
   auto worker = [&]() {
     GrowScratch s;
+    FastScratch fs;
     for (;;) {
       int32_t t = next.fetch_add(1);
       if (t >= ntree) break;
@@ -516,7 +1157,10 @@ int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int3
         // (src/rf_bootstrap.cpp:24, include/rf_config.hpp:61) and its grower
         // from the user's iseed (src/rf_random_forest.cpp:515).
         bootstrap_tree(n, static_cast<uint32_t>(boot_seed + t), nin);
-        grow_tree(gp, nin, static_cast<uint32_t>(iseed + t), s, f->trees[t]);
+        if (presorted)
+          grow_tree_presorted(gp, pd, nin, static_cast<uint32_t>(iseed + t), fs, f->trees[t]);
+        else
+          grow_tree(gp, nin, static_cast<uint32_t>(iseed + t), s, f->trees[t]);
       } catch (...) {
         failed = 1;
       }

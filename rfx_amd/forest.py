@@ -57,7 +57,10 @@ class Forest:
 
     @classmethod
     def grow(cls, x: np.ndarray, y: np.ndarray, ntree: int, mtry: int = 0, minndsize: int = 1, maxnode: int = 0,
-             iseed: int = 12345, boot_seed: int = 12345, fma: bool = True, nthreads: int = 0) -> "Forest":
+             iseed: int = 12345, boot_seed: int = 12345, fma: bool = True, nthreads: int = 0,
+             plain: bool = False) -> "Forest":
+        """Grow with the reference CPU grower's semantics.  plain=True uses
+        per-node sorts instead of presorted columns (same trees, slower)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(y, dtype=np.int32)
         n, p = x.shape
@@ -67,7 +70,7 @@ class Forest:
         if mtry <= 0:
             mtry = default_mtry(p)
         h = C.c_void_p()
-        flags = 0 if fma else _lib.GROW_NO_FMA
+        flags = (0 if fma else _lib.GROW_NO_FMA) | (_lib.GROW_PLAIN if plain else 0)
         check(lib().rfx_forest_grow(ptr(x), ptr(y), n, p, nclass, ntree, mtry, minndsize, maxnode, iseed, boot_seed,
                                     flags, nthreads, C.byref(h)), "rfx_forest_grow")
         try:

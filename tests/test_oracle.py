@@ -446,3 +446,41 @@ def test_counts_rows_equal_full_matrices():
     rows = np.array([699, 0, 350, 3, 350])
     a, o = O.counts_rows(f, nd, rows)
     assert np.array_equal(a, C[rows]) and np.array_equal(o, U[rows])
+
+
+def _same_forest(a, b) -> bool:
+    return (all(np.array_equal(getattr(a, k), getattr(b, k))
+                for k in ("node_offset", "nodestatus", "bestvar", "treemap", "nodeclass", "nin"))
+            and np.array_equal(a.xbestsplit.view(np.uint32), b.xbestsplit.view(np.uint32)))
+
+
+@pytest.mark.parametrize("kind,n,p,ncl,T,mtry,minnd", [
+    (_lib.DATA_SUBSPACE_MIXTURE, 20000, 32, 10, 3, 6, 5),
+    (_lib.DATA_COVTYPE_LIKE, 8000, 54, 7, 3, 7, 1),
+])
+def test_presorted_grower_matches_reference_at_scale(kind, n, p, ncl, T, mtry, minnd):
+    """The presorted grower (lane scans, deferred peel partitions) against the
+    compiled reference's cpu_growtree on forests with peel chains and ties."""
+    if not O.ref_available():
+        pytest.skip("reference build oracle/_ref not present")
+    x, y = make_dataset(kind, n, p, ncl, 0)
+    r = O.ref_run_forest(x, y, T, mtry, minndsize=minnd, iseed=0, do_prox=False)
+    ref = Forest.from_arrays(n, p, r["nnode"], r["nodestatus"], r["bestvar"], r["xbestsplit"], r["treemap"], r["nin"],
+                             maxnode=r["maxnode"])
+    got = Forest.grow(x, y, T, mtry=mtry, minndsize=minnd, iseed=0)
+    ref.nodeclass = got.nodeclass  # the reference driver does not export node classes
+    assert _same_forest(got, ref)
+
+
+@pytest.mark.parametrize("kind,n,p,ncl,T,mtry,minnd", [
+    (_lib.DATA_SUBSPACE_MIXTURE, 100000, 32, 10, 4, 6, 5),
+    (_lib.DATA_COVTYPE_LIKE, 20000, 54, 7, 8, 7, 1),
+    (_lib.DATA_GAUSS_MIXTURE, 3000, 20, 3, 8, 4, 1),
+    (_lib.DATA_OVERLAP_MIXTURE, 5000, 10, 4, 8, 3, 2),
+])
+def test_presorted_grower_equals_per_node_sort_grower(kind, n, p, ncl, T, mtry, minnd):
+    """Presorted columns and per-node sorts grow the same trees, node classes included."""
+    x, y = make_dataset(kind, n, p, ncl, 1)
+    a = Forest.grow(x, y, T, mtry=mtry, minndsize=minnd, iseed=5)
+    b = Forest.grow(x, y, T, mtry=mtry, minndsize=minnd, iseed=5, plain=True)
+    assert _same_forest(a, b)
