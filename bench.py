@@ -45,6 +45,10 @@ WORKLOADS = {
     "c3": (1, 200000, 54, 7, 1000, 7, "covtype-like n=200000 p=54 T=1000 mtry=7: leaf assignment + ALL and OOB "
                                       "upper-triangle int32 counts; rank-64 factors; factor 3-D MDS per storage "
                                       "format"),
+    "c4": (2, 1000000, 32, 10, 500, 6, "16-component Gaussian mixture in a random 8-dim subspace + 24 N(0,1) noise "
+                                       "columns, n=1000000 p=32 10 classes T=500 mtry=6 minndsize=5: leaf "
+                                       "assignment, rank-128 factors of C/T from the leaf buckets (matrix-free), "
+                                       "3-D MDS from FP8-e4m3 / BF16 / FP32 factor storage"),
     "c5": (0, 0, 0, 0, 0, 0, "64 independent forests (instance i: seed i, n log-uniform in [1000, 50000], p "
                              "uniform in [8, 64], T uniform in [50, 500], 2-10 classes, Gaussian mixture, "
                              "mtry = floor(sqrt(p))): leaf assignment + OOB upper-triangle int32 counts of all 64 "
@@ -192,7 +196,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--rank", type=int, default=64, help="factor rank (c3)")
+    ap.add_argument("--rank", type=int, default=0, help="factor rank (c3: 64, c4: 128)")
+    ap.add_argument("--trees", type=int, default=0, help="c4: trees grown (0: the workload's 500)")
     ap.add_argument("--algo", default="auto", choices=["auto", "dense", "bucket", "gather"])
     ap.add_argument("--cpu-trees", type=int, default=0, help="trees in the CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -229,6 +234,8 @@ def main() -> int:
         log("copy calibration: 1 GiB read + 1 GiB written")
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "c4":
+        return bench_c4(args)
     NAMES = {_lib.ALGO_DENSE: "dense", _lib.ALGO_BUCKET: "bucket", _lib.ALGO_GATHER: "gather"}
     kind, n, p, ncl, T, mtry, desc = WORKLOADS[args.workload]
     x, y, forest = grow_forest(kind, n, p, ncl, T, mtry, args.forest_cache)
@@ -358,7 +365,7 @@ def main() -> int:
         del keys
         out_oob = None
         torch.cuda.empty_cache()
-        line["mds"] = factor_mds_section(df, forest, x, y, out_all, args.rank, not args.no_cpu)
+        line["mds"] = factor_mds_section(df, forest, x, y, out_all, args.rank or 64, not args.no_cpu)
         out_all = None
     print(json.dumps(line), flush=True)
     return 0
@@ -631,6 +638,133 @@ def c5_instances(count: int = 64):
         ncl = int(r.integers(2, 11))
         out.append((i, n, p, T, ncl))
     return out
+
+
+def bench_c4(args) -> int:
+    """The FP8 scale-up workload: n = 1,000,000 samples, T = 500 trees
+    (minndsize 5).  The proximity is never materialised (its upper triangle
+    would take 2 TB): rank-128 factors of C/T come from the leaf buckets by
+    matrix-free randomized subspace iteration, and the 3-D MDS runs on the
+    factors stored as FP8-e4m3 and BF16 (per-column scales) on the matrix
+    cores with FP32 refinement sweeps, against the FP32 VALU form and the FP64
+    MDS of the same factors.  value = seconds of the FP8 factor MDS with one
+    FP32 refinement sweep (lower is better); the leaf assignment and factor
+    build are reported beside it.  A seeded 20,000-sample subset (its own
+    forest) reruns the pipelines against its exact dense FP64 MDS."""
+    import numpy as np
+    import torch
+
+    from rfx_amd import _lib
+    from rfx_amd.forest import Forest, make_dataset
+    from rfx_amd.proximity import (Accumulator, DeviceForest, lowrank_factors, mds_dense, mds_factors,
+                                   mds_factors_mixed, quantize_factors)
+
+    kind, n, p, ncl, T, mtry, desc = WORKLOADS["c4"]
+    T = args.trees or T
+    rank = args.rank or 128
+    minnd = 5
+    t0 = time.perf_counter()
+    x, y = make_dataset(kind, n, p, ncl, 0)
+    forest = Forest.grow(x, y, T, mtry=mtry, minndsize=minnd, iseed=0,
+                         progress=lambda d, tot: log(f"c4 forest: {d}/{tot} trees, {time.perf_counter() - t0:.0f} s"))
+    grow_s = time.perf_counter() - t0
+    log(f"c4 forest: {forest.total_nodes} nodes in {grow_s:.1f} s")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - s
+
+    def errs(X, Xref):
+        X = X.cpu().numpy() if hasattr(X, "cpu") else X
+        Xref = Xref.cpu().numpy() if hasattr(Xref, "cpu") else Xref
+        sg = np.sign(np.sum(X * Xref, axis=0))
+        sg[sg == 0] = 1
+        rel = float(np.linalg.norm(X * sg - Xref) / np.linalg.norm(Xref))
+        qa, _ = np.linalg.qr(X)
+        qb, _ = np.linalg.qr(Xref)
+        ang = float(np.arccos(np.clip(np.linalg.svd(qa.T @ qb, compute_uv=False).min(), -1, 1)))
+        return {"rel_err": rel, "max_angle_rad": ang}
+
+    dev = torch.device("cuda:0")
+    df = DeviceForest(forest, x)
+    keys = torch.empty((T, n), dtype=torch.int32, device=dev)
+    df.leaf_assign(keys=keys)
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(5)]
+    for a, b in evs:
+        a.record(stream)
+        df.leaf_assign(keys=keys)
+        b.record(stream)
+    torch.cuda.synchronize()
+    leaf_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2)  # warm-up
+    (U, eig), fac_s = timed(lambda: lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2))
+    del keys
+    torch.cuda.empty_cache()
+    log(f"c4 leaf assignment {leaf_ms:.2f} ms, rank-{rank} factors {fac_s:.3f} s")
+    U32, s32 = quantize_factors(U, _lib.PREC_FP32)
+    (X64, e64, it64), t64 = timed(lambda: mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP64))
+    Xref = X64.cpu().numpy()
+    mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP32)  # warm-up
+    (X32, _, it32), t32 = timed(lambda: mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP32))
+    out = {"factor_fp64": {"mds_s": round(t64, 4), "iters": it64, "eigvals": [float(v) for v in e64[:5]],
+                           "eigengap_l3_l4": float(e64[2] - e64[3])},
+           "fp32_valu": {"mds_s": round(t32, 4), "iters": it32, "vs_factor_fp64": errs(X32, Xref)}}
+    log(f"c4 factor mds fp64 {t64:.3f} s, fp32 valu {t32:.4f} s")
+    value = None
+    for name, fmt in (("fp8", _lib.PREC_FP8), ("bf16", _lib.PREC_BF16), ("fp32", _lib.PREC_FP32)):
+        Uq, sc = quantize_factors(U, fmt)
+        rec = {"bytes_per_factor_row": rank * {"fp32": 4, "bf16": 2, "fp8": 1}[name]}
+        for label, rmax in (("one_sweep", 1), ("refined", 60)):
+            mds_factors_mixed(Uq, sc, fmt, U32, refine_max=rmax)  # warm-up
+            (X, _, itm), t = timed(lambda: mds_factors_mixed(Uq, sc, fmt, U32, refine_max=rmax))
+            rec[label] = {"mds_s": round(t, 4), "mfma_iters": itm[0], "fp32_refine_sweeps": itm[1],
+                          "speedup_vs_fp32_valu": round(t32 / t, 3), "vs_factor_fp64": errs(X, Xref)}
+            log(f"c4 factor mds {name} {label}: {t:.4f} s iters {itm}")
+            if name == "fp8" and label == "one_sweep":
+                value = t
+        out[name] = rec
+    del U, U32
+    torch.cuda.empty_cache()
+
+    # seeded 20k subset: exact dense FP64 MDS of its own forest's C/T against the factor pipelines
+    m = 20000
+    idx = np.sort(np.random.default_rng(7).choice(n, m, replace=False))
+    xs, ys = np.ascontiguousarray(x[idx]), np.ascontiguousarray(y[idx])
+    fs = Forest.grow(xs, ys, T, mtry=mtry, minndsize=minnd, iseed=0)
+    dfs = DeviceForest(fs, xs)
+    ks, _ = dfs.leaf_assign()
+    acc = Accumulator(fs.n, fs.ntree, fs.total_nodes)
+    Cs = acc.accumulate(ks, _lib.PROX_ALL, _lib.LAYOUT_UPPER, _lib.ALGO_AUTO)
+    Xsd, esd, _ = mds_dense(Cs, _lib.LAYOUT_UPPER, m, _lib.MDS_COUNTS, 1.0 / T, precision=_lib.PREC_FP64)
+    Xsd = Xsd.cpu().numpy()
+    del Cs
+    Us, _ = lowrank_factors(ks, fs, rank=rank, oversample=16, power_iters=2)
+    Us32, ss32 = quantize_factors(Us, _lib.PREC_FP32)
+    sub = {"n": m, "seed": 7, "dense_fp64_eigs": [float(v) for v in esd[:5]],
+           "eigengap_l3_l4": float(esd[2] - esd[3])}
+    X, _, _ = mds_factors(Us32, ss32, _lib.PREC_FP32, precision=_lib.PREC_FP64)
+    sub["factor_fp64"] = errs(X, Xsd)
+    for name, fmt in (("fp8", _lib.PREC_FP8), ("bf16", _lib.PREC_BF16), ("fp32", _lib.PREC_FP32)):
+        Uq, sc = quantize_factors(Us, fmt)
+        X, _, _ = mds_factors_mixed(Uq, sc, fmt, Us32, refine_max=1)
+        sub[name + "_one_sweep"] = errs(X, Xsd)
+    out["subset_20k_vs_dense_fp64"] = sub
+
+    line = {
+        "metric": "3D-MDS s at n=1M from rank-128 FP8-e4m3 factors (one FP32 refinement sweep)",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": value * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp8-e4m3", "data": "synthetic",
+        "config": {"workload": desc, "n": n, "p": p, "ntree": T, "mtry": mtry, "minndsize": minnd, "rank": rank,
+                   "forest_nodes": forest.total_nodes, "grow_s": round(grow_s, 1), "leaf_assign_ms": round(leaf_ms, 3),
+                   "factor_build_s": round(fac_s, 4), "factor_top_eigs": [round(float(v), 4) for v in eig[:6]]},
+        "mds": out,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def bench_c5(args) -> int:

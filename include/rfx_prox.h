@@ -106,6 +106,14 @@ int rfx_make_dataset(int32_t kind, int64_t n, int32_t p, int32_t nclass, uint64_
 int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int32_t nclass, int32_t ntree,
                     int32_t mtry, int32_t minndsize, int32_t maxnode, int32_t iseed, int32_t boot_seed,
                     int32_t flags, int32_t nthreads, rfx_forest** out);
+/* The same, counting finished trees into *trees_done (atomically, from the
+ * worker threads) so that a caller on another thread can report progress:
+ * the counterpart of the reference's per-tree progress_callback_(itree + 1,
+ * ntree) (src/rf_random_forest.cpp:518-521), polled instead of called back
+ * because the trees grow on several threads. */
+int rfx_forest_grow_progress(const float* x, const int32_t* y, int64_t n, int32_t p, int32_t nclass, int32_t ntree,
+                             int32_t mtry, int32_t minndsize, int32_t maxnode, int32_t iseed, int32_t boot_seed,
+                             int32_t flags, int32_t nthreads, int32_t* trees_done, rfx_forest** out);
 void rfx_forest_free(rfx_forest* f);
 int32_t rfx_forest_ntree(const rfx_forest* f);
 int32_t rfx_forest_nnode(const rfx_forest* f, int32_t t);

@@ -72,6 +72,8 @@ _SIGS = {
     "rfx_make_dataset": (_i32, [_i32, _i64, _i32, _i32, _u64, _p, _p]),
     "rfx_forest_grow": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                                C.POINTER(_p)]),
+    "rfx_forest_grow_progress": (_i32, [_p, _p, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
+                                        C.POINTER(C.c_int32), C.POINTER(_p)]),
     "rfx_forest_free": (None, [_p]),
     "rfx_forest_ntree": (_i32, [_p]),
     "rfx_forest_nnode": (_i32, [_p, _i32]),

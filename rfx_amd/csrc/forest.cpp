@@ -1107,6 +1107,14 @@ int rfx_make_dataset(int32_t kind, int64_t n, int32_t p, int32_t nclass, uint64_
 int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int32_t nclass, int32_t ntree,
                     int32_t mtry, int32_t minndsize, int32_t maxnode, int32_t iseed, int32_t boot_seed,
                     int32_t flags, int32_t nthreads, rfx_forest** out) {
+  return rfx_forest_grow_progress(x, y, n, p, nclass, ntree, mtry, minndsize, maxnode, iseed, boot_seed, flags,
+                                  nthreads, nullptr, out);
+}
+
+int rfx_forest_grow_progress(const float* x, const int32_t* y, int64_t n, int32_t p, int32_t nclass, int32_t ntree,
+                             int32_t mtry, int32_t minndsize, int32_t maxnode, int32_t iseed, int32_t boot_seed,
+                             int32_t flags, int32_t nthreads, int32_t* trees_done, rfx_forest** out) {
+  if (trees_done) __atomic_store_n(trees_done, 0, __ATOMIC_RELAXED);
   if (!x || !y || !out || n <= 0 || n > RFX_MAX_SAMPLES || p <= 0 || p > RFX_MAX_FEATURES || nclass < 2 ||
       ntree <= 0 || mtry <= 0 || minndsize < 1)
     return RFX_ERR_INVALID;
@@ -1161,6 +1169,7 @@ int rfx_forest_grow(const float* x, const int32_t* y, int64_t n, int32_t p, int3
           grow_tree_presorted(gp, pd, nin, static_cast<uint32_t>(iseed + t), fs, f->trees[t]);
         else
           grow_tree(gp, nin, static_cast<uint32_t>(iseed + t), s, f->trees[t]);
+        if (trees_done) __atomic_add_fetch(trees_done, 1, __ATOMIC_RELAXED);
       } catch (...) {
         failed = 1;
       }

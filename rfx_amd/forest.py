@@ -58,9 +58,11 @@ class Forest:
     @classmethod
     def grow(cls, x: np.ndarray, y: np.ndarray, ntree: int, mtry: int = 0, minndsize: int = 1, maxnode: int = 0,
              iseed: int = 12345, boot_seed: int = 12345, fma: bool = True, nthreads: int = 0,
-             plain: bool = False) -> "Forest":
+             plain: bool = False, progress=None, progress_every: float = 20.0) -> "Forest":
         """Grow with the reference CPU grower's semantics.  plain=True uses
-        per-node sorts instead of presorted columns (same trees, slower)."""
+        per-node sorts instead of presorted columns (same trees, slower).
+        progress(trees_done, ntree) is called about every progress_every
+        seconds while the trees grow."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(y, dtype=np.int32)
         n, p = x.shape
@@ -71,8 +73,29 @@ class Forest:
             mtry = default_mtry(p)
         h = C.c_void_p()
         flags = (0 if fma else _lib.GROW_NO_FMA) | (_lib.GROW_PLAIN if plain else 0)
-        check(lib().rfx_forest_grow(ptr(x), ptr(y), n, p, nclass, ntree, mtry, minndsize, maxnode, iseed, boot_seed,
-                                    flags, nthreads, C.byref(h)), "rfx_forest_grow")
+        if progress is None:
+            check(lib().rfx_forest_grow(ptr(x), ptr(y), n, p, nclass, ntree, mtry, minndsize, maxnode, iseed,
+                                        boot_seed, flags, nthreads, C.byref(h)), "rfx_forest_grow")
+        else:
+            # The call runs on a helper thread (ctypes releases the GIL); this
+            # thread reports the finished-tree counter it maintains.
+            import threading
+
+            done = C.c_int32(0)
+            rc = []
+            th = threading.Thread(target=lambda: rc.append(lib().rfx_forest_grow_progress(
+                ptr(x), ptr(y), n, p, nclass, ntree, mtry, minndsize, maxnode, iseed, boot_seed, flags, nthreads,
+                C.byref(done), C.byref(h))))
+            th.start()
+            last = -1
+            while th.is_alive():
+                th.join(progress_every)
+                if done.value != last:
+                    last = done.value
+                    progress(last, ntree)
+            check(rc[0], "rfx_forest_grow_progress")
+            if done.value != last:
+                progress(done.value, ntree)
         try:
             L = lib()
             T = L.rfx_forest_ntree(h)
@@ -89,6 +112,35 @@ class Forest:
         finally:
             lib().rfx_forest_free(h)
         return cls(n, p, off, st, bv, xs, tm, nc, nin)
+
+    @classmethod
+    def grow_chunked(cls, x: np.ndarray, y: np.ndarray, ntree: int, chunk: int = 64, progress=None,
+                     iseed: int = 12345, boot_seed: int = 12345, **kw) -> "Forest":
+        """The forest Forest.grow would return, grown chunk trees at a time
+        (tree t keeps its seeds iseed + t and boot_seed + t), calling
+        progress(trees_done, ntree) after each chunk."""
+        parts = []
+        for t0 in range(0, ntree, chunk):
+            c = min(chunk, ntree - t0)
+            parts.append(cls.grow(x, y, c, iseed=iseed + t0, boot_seed=boot_seed + t0, **kw))
+            if progress:
+                progress(t0 + c, ntree)
+        return cls.concat(parts)
+
+    @classmethod
+    def concat(cls, parts: list["Forest"]) -> "Forest":
+        """The trees of several forests over the same samples, in order."""
+        n, p = parts[0].n, parts[0].p
+        if any(f.n != n or f.p != p for f in parts):
+            raise ValueError("forests over different samples")
+        off = [np.zeros(1, np.int64)]
+        base = 0
+        for f in parts:
+            off.append(f.node_offset[1:] + base)
+            base += f.total_nodes
+        cat = lambda k: np.concatenate([getattr(f, k) for f in parts])
+        return cls(n, p, np.concatenate(off), cat("nodestatus"), cat("bestvar"), cat("xbestsplit"), cat("treemap"),
+                   cat("nodeclass"), np.concatenate([f.nin for f in parts], axis=0))
 
     @classmethod
     def from_arrays(cls, n: int, p: int, nnode: np.ndarray, nodestatus: np.ndarray, bestvar: np.ndarray,

@@ -1,12 +1,14 @@
 """CPU: the forest file (Forest.save / Forest.load) round-trips every array
 bit for bit, refuses files of another format, and the sample-subset view
-keeps each sample's bootstrap counts."""
+keeps each sample's bootstrap counts; a forest grown in chunks of trees is
+the forest of one call."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
-from rfx_amd.forest import Forest
+from rfx_amd import _lib
+from rfx_amd.forest import Forest, make_dataset
 
 from helpers import load_golden
 
@@ -37,3 +39,25 @@ def test_sample_subset_keeps_bootstrap_counts():
     assert s.n == 4 and s.total_nodes == f.total_nodes
     assert np.array_equal(s.nin, f.nin[:, idx])
     assert np.array_equal(s.nout(2), f.nout(2)[idx])
+
+
+def test_chunked_growth_equals_one_call():
+    x, y = make_dataset(_lib.DATA_SUBSPACE_MIXTURE, 3000, 12, 10, 2)
+    a = Forest.grow(x, y, 11, mtry=3, minndsize=5, iseed=4)
+    seen = []
+    b = Forest.grow_chunked(x, y, 11, chunk=4, progress=lambda d, t: seen.append(d), mtry=3, minndsize=5, iseed=4)
+    assert seen == [4, 8, 11]
+    for k in ("node_offset", "nodestatus", "bestvar", "treemap", "nodeclass", "nin"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    assert np.array_equal(a.xbestsplit.view(np.uint32), b.xbestsplit.view(np.uint32))
+
+
+def test_progress_counter_growth_equals_plain_call():
+    x, y = make_dataset(_lib.DATA_SUBSPACE_MIXTURE, 3000, 12, 10, 3)
+    a = Forest.grow(x, y, 9, mtry=3, minndsize=5, iseed=2)
+    seen = []
+    b = Forest.grow(x, y, 9, mtry=3, minndsize=5, iseed=2, progress=lambda d, t: seen.append((d, t)),
+                    progress_every=0.01)
+    assert seen and seen[-1] == (9, 9)
+    for k in ("node_offset", "nodestatus", "bestvar", "treemap", "nodeclass", "nin"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
