@@ -893,23 +893,33 @@ def c5_cpu_baseline(forests, xs, budget_s: float = 15.0) -> dict:
 
 def c5_mds(batch, forests) -> dict:
     """The reference chain's 3-D MDS (OOB counts -> finishprox -> float32 ->
-    renormalisation -> classical MDS) of every instance, FP32 operator."""
+    renormalisation -> classical MDS) of every instance, FP32 operator: the
+    64 instances one after another, and the same 64 run concurrently on
+    eight streams (mds_dense_many), whose coordinates are compared with the
+    sequential ones."""
+    import numpy as np
     import torch
 
     from rfx_amd import _lib
-    from rfx_amd.proximity import mds_dense
+    from rfx_amd.proximity import mds_dense, mds_dense_many
 
+    jobs = [dict(counts=out, layout=_lib.LAYOUT_UPPER, n=f.n, mode=_lib.MDS_REFERENCE, scale=1.0,
+                 nout=torch.from_numpy(f.nout(2)).to(out.device), precision=_lib.PREC_FP32)
+            for f, out in zip(forests, batch.out)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    iters = []
-    for f, out in zip(forests, batch.out):
-        nout = torch.from_numpy(f.nout(2)).to(out.device)
-        _, _, it = mds_dense(out, _lib.LAYOUT_UPPER, f.n, _lib.MDS_REFERENCE, 1.0, nout, precision=_lib.PREC_FP32)
-        iters.append(it)
+    seq = [mds_dense(**j) for j in jobs]
     torch.cuda.synchronize()
-    return {"total_s": round(time.perf_counter() - t0, 4), "mean_iters": float(sum(iters) / len(iters)),
-            "mode": "reference chain, FP32 operator, FP64 Rayleigh-Ritz"}
-
+    t_seq = time.perf_counter() - t0
+    mds_dense_many(jobs[:8], streams=8)  # warm-up of the streams' allocations
+    t0 = time.perf_counter()
+    conc = mds_dense_many(jobs, streams=8)
+    t_conc = time.perf_counter() - t0
+    diff = max(float(torch.linalg.norm(a[0] - b[0]) / torch.linalg.norm(a[0])) for a, b in zip(seq, conc))
+    iters = [r[2] for r in seq]
+    return {"sequential_s": round(t_seq, 4), "concurrent_8_streams_s": round(t_conc, 4),
+            "speedup": round(t_seq / t_conc, 3), "max_rel_diff_concurrent_vs_sequential": diff,
+            "mean_iters": float(np.mean(iters)), "mode": "reference chain, FP32 operator, FP64 Rayleigh-Ritz"}
 
 if __name__ == "__main__":
     sys.exit(main())

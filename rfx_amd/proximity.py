@@ -257,6 +257,47 @@ def mds_dense(counts: torch.Tensor, layout: int, n: int, mode: int = _lib.MDS_CO
     return coords, eig, it.value
 
 
+def mds_dense_many(jobs: list[dict], streams: int = 8) -> list[tuple[torch.Tensor, np.ndarray, int]]:
+    """The 3-D MDS of many independent count matrices (the per-instance MDS of
+    a forest batch), run concurrently: `streams` host threads, each with a
+    HIP stream of its own (default priority), take the jobs in order of
+    decreasing n, so the small problems' host round trips (Gram matrices and
+    Rayleigh-Ritz on the host) and short launches overlap with the large
+    problems' kernels.  Each job is the keyword arguments of mds_dense
+    (counts, layout, n, and optionally mode, scale, nout, precision,
+    max_iter, tol); results come back in job order, as mds_dense returns
+    them, with the device synchronised."""
+    import concurrent.futures as cf
+    import threading
+
+    _require_gpu()
+    if not jobs:
+        return []
+    dev = jobs[0]["counts"].device
+    ready = torch.cuda.Event()
+    ready.record(torch.cuda.current_stream(dev))
+    local = threading.local()
+
+    def run(job):
+        s = getattr(local, "stream", None)
+        if s is None:
+            s = local.stream = torch.cuda.Stream(device=dev)
+            s.wait_event(ready)
+        with torch.cuda.stream(s):
+            r = mds_dense(**job)
+        s.synchronize()
+        return r
+
+    order = sorted(range(len(jobs)), key=lambda k: -int(jobs[k]["n"]))
+    out: list = [None] * len(jobs)
+    with cf.ThreadPoolExecutor(max_workers=max(1, streams)) as ex:
+        futs = {ex.submit(run, jobs[k]): k for k in order}
+        for f in cf.as_completed(futs):
+            out[futs[f]] = f.result()
+    torch.cuda.synchronize(dev)
+    return out
+
+
 _FMT_DTYPE = {_lib.PREC_FP32: torch.float32, _lib.PREC_BF16: torch.int16, _lib.PREC_INT8: torch.int8,
               _lib.PREC_FP8: torch.uint8}
 

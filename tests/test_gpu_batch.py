@@ -236,3 +236,28 @@ def test_batch_own_launch_dense_packed_form(gpu, variant):
         if variant == "pair":
             assert np.array_equal(b.out2[k].cpu().numpy(), O.counts(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)), k
     b.close()
+
+
+def test_batch_mds_concurrent_matches_sequential(gpu):
+    """The per-instance MDS of a batch run concurrently on several streams
+    (mds_dense_many) gives each instance the coordinates of its own
+    sequential mds_dense call."""
+    from rfx_amd.proximity import mds_dense, mds_dense_many
+
+    shapes = [(1500, 8, 3, 40, 11), (900, 12, 4, 30, 12), (2100, 10, 5, 25, 13), (1200, 6, 3, 35, 14),
+              (700, 9, 2, 20, 15)]
+    probs = _problems(shapes)
+    jobs = []
+    for f, x in probs:
+        keys, _ = DeviceForest(f, x).leaf_assign()
+        U = Accumulator(f.n, f.ntree, f.total_nodes).accumulate(keys, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER,
+                                                               _lib.ALGO_GATHER)
+        jobs.append(dict(counts=U, layout=_lib.LAYOUT_UPPER, n=f.n, mode=_lib.MDS_REFERENCE, scale=1.0,
+                         nout=torch.from_numpy(f.nout(2)).to(U.device), precision=_lib.PREC_FP32))
+    seq = [mds_dense(**j) for j in jobs]
+    _say("sequential MDS done; concurrent on 3 streams")
+    conc = mds_dense_many(jobs, streams=3)
+    for (Xs, es, _), (Xc, ec, _) in zip(seq, conc):
+        assert np.allclose(ec[:3], es[:3], rtol=1e-9, atol=0)
+        Xs, Xc = Xs.cpu().numpy(), Xc.cpu().numpy()
+        assert np.linalg.norm(Xc - Xs) <= 1e-9 * np.linalg.norm(Xs)

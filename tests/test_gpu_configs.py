@@ -194,3 +194,36 @@ def test_c5_batch_all_instances(gpu):
         assert int(out.sum(dtype=torch.int64)) == want, k
         if k in small:
             assert np.array_equal(out.cpu().numpy(), O.counts(f, nd, _lib.PROX_OOB_UPPER, _lib.LAYOUT_UPPER)), k
+
+
+def test_c4_shape_rank128_factor_mds(gpu):
+    """The FP8 scale-up workload's data and forest options (16-component
+    subspace mixture, p = 32, 10 classes, mtry 6, minndsize 5: deep peeling
+    trees) at n = 200,000 and 24 trees: leaf assignment of every sample
+    against the oracle, rank-128 factors from the leaf buckets, and the factor
+    MDS from FP8-e4m3 / BF16 / FP32 storage on the matrix cores with FP32
+    refinement against the FP64 MDS of the same factors."""
+    from rfx_amd.proximity import lowrank_factors, mds_factors, mds_factors_mixed, quantize_factors
+
+    _say("c4 shape: grow n=200000 T=24")
+    x, y = make_dataset(_lib.DATA_SUBSPACE_MIXTURE, 200000, 32, 10, 0)
+    f = Forest.grow(x, y, 24, mtry=6, minndsize=5, iseed=0)
+    df = DeviceForest(f, x)
+    keys, nodextr = df.leaf_assign(want_nodextr=True)
+    _say("c4 shape: oracle leaf assignment")
+    assert np.array_equal(nodextr.cpu().numpy(), O.leaf_assign(f, x))
+    _say("c4 shape: rank-128 factors")
+    U, eig = lowrank_factors(keys, f, rank=128, oversample=16, power_iters=2)
+    assert U.shape == (200000, 128) and np.all(np.diff(eig[:128]) <= 1e-9 * eig[0])
+    U32, s32 = quantize_factors(U, _lib.PREC_FP32)
+    Xref, eref, _ = mds_factors(U32, s32, _lib.PREC_FP32, precision=_lib.PREC_FP64)
+    Xref = Xref.cpu().numpy()
+    for name, fmt in (("fp8", _lib.PREC_FP8), ("bf16", _lib.PREC_BF16), ("fp32", _lib.PREC_FP32)):
+        Uq, sc = quantize_factors(U, fmt)
+        X1, _, it1 = mds_factors_mixed(Uq, sc, fmt, U32, refine_max=1)
+        X, e, it = mds_factors_mixed(Uq, sc, fmt, U32)
+        err1 = O.embedding_errors(X1.cpu().numpy(), Xref)["rel_frobenius"]
+        err = O.embedding_errors(X.cpu().numpy(), Xref)["rel_frobenius"]
+        _say(f"c4 shape: {name} one sweep {err1:.2e} {it1}, refined {err:.2e} {it}")
+        assert np.allclose(e[:3], eref[:3], rtol=1e-4)
+        assert err < 1e-3
