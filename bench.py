@@ -837,6 +837,34 @@ def bench_c5(args) -> int:
     ms_step = wall * 1e3 / args.steps
     ev_ms = e0.elapsed_time(e1) / args.steps
     value = work / (ms_step / 1e3)
+
+    # the same step with every forest through the persistent queue (the
+    # autotuner's own launches switched off), same outputs
+    own = batch.own_launch()
+    keep = [o.clone() for o in batch.out[:4]]
+    batch.set_own_launch(np.zeros(len(forests), np.int32))
+    batch.run(use_graph=True)
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert torch.equal(keep[k], batch.out[k]), "queue-only and autotuned counts differ"
+    del keep
+    s = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(use_graph=True)
+    torch.cuda.synchronize()
+    queue_s = (time.perf_counter() - s) / args.steps
+    batch.set_own_launch(own.astype(np.int32))
+    log(f"batched step {ms_step:.2f} ms ({int(own.sum())} own launches), queue only {queue_s * 1e3:.2f} ms")
+    # the DENSE forests' packed 16-bit compare: v_xor_b32, v_pk_min_u16 and
+    # v_pk_add_u16 per word of two trees and output pair, 1.5 VALU operations per
+    # tree-pair, against 256 CUs x 64 lanes x 2.4 GHz
+    dense_pairs = sum(f.ntree * (f.n * (f.n + 1) // 2) for f, a in zip(forests, algos) if a == _lib.ALGO_DENSE)
+    valu_peak = 256 * 64 * 2.4e9
+    valu = {"bound": "valu", "ops_per_tree_pair": 1.5, "dense_tree_pairs": dense_pairs,
+            "achieved_ops_per_s": 1.5 * dense_pairs / (ev_ms / 1e3), "peak_ops_per_s": valu_peak,
+            "frac": round(1.5 * dense_pairs / (ev_ms / 1e3) / valu_peak, 4),
+            "note": "over the whole batched step (the GATHER forests run beside the DENSE ones), so a lower bound "
+                    "of the DENSE kernels' own fraction"}
     P_total = sum(f.n * (f.n + 1) // 2 for f in forests)
     nT_total = sum(f.n * f.ntree for f in forests)
     step_bytes = 4 * P_total + 4 * nT_total
@@ -852,7 +880,9 @@ def bench_c5(args) -> int:
                    "pair_density": float(np.sum(sumsq) / sum(f.ntree * float(f.n) ** 2 for f in forests)),
                    "sequential_s": round(seq_s, 5), "batched_s": round(ms_step / 1e3, 5),
                    "speedup_vs_sequential": round(seq_s / (ms_step / 1e3), 3),
-                   "sequential_value": work / seq_s},
+                   "sequential_value": work / seq_s, "own_launch_forests": int(own.sum()),
+                   "queue_only_s": round(queue_s, 5)},
+        "valu": valu,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole batched step (leaf + segmented sort + persistent dense/gather queue), "

@@ -466,6 +466,10 @@ int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* sum_leaf_sq)
  * rfx_batch_autotune), 0 when it goes through the persistent queue.  No
  * counterpart in the reference, which has no multi-forest batching. */
 int rfx_batch_get_own_launch(const rfx_batch* b, int32_t* flags);
+/* Override those flags (after rfx_batch_autotune): all zeros sends every
+ * problem through the persistent queue, the comparison that justifies the
+ * autotuner's own launches. */
+int rfx_batch_set_own_launch(rfx_batch* b, const int32_t* flags);
 int rfx_batch_get_tuning(const rfx_batch* b, int32_t* lines_per_item, int32_t* blocks_per_cu);
 void rfx_batch_free(rfx_batch* b);
 

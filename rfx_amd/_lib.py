@@ -119,6 +119,7 @@ _SIGS = {
     "rfx_batch_set_algos": (_i32, [_p, _p]),
     "rfx_batch_get_stats": (_i32, [_p, _p, _p]),
     "rfx_batch_get_own_launch": (_i32, [_p, _p]),
+    "rfx_batch_set_own_launch": (_i32, [_p, _p]),
     "rfx_batch_free": (None, [_p]),
     "rfx_mds_dense": (_i32, [_p, _p, _i32, _i64, _i32, C.c_double, _p, _i32, _i32, C.c_double, _p, _sz, _p, _p,
                              _p]),

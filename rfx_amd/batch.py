@@ -89,6 +89,16 @@ class ForestBatch:
         check(lib().rfx_batch_get_own_launch(self._h, f.ctypes.data), "rfx_batch_get_own_launch")
         return f.astype(bool)
 
+    def set_own_launch(self, flags) -> None:
+        """Override the autotuner's per-forest own-launch flags (all False:
+        every forest through the persistent queue)."""
+        import numpy as np
+
+        f = np.ascontiguousarray(flags, np.int32)
+        if f.shape != (len(self.forests),):
+            raise ValueError("one flag per forest")
+        check(lib().rfx_batch_set_own_launch(self._h, f.ctypes.data), "rfx_batch_set_own_launch")
+
     def stats(self):
         """(algorithm per forest, measured sum |leaf|^2 per forest)."""
         import numpy as np

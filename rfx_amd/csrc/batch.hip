@@ -635,6 +635,13 @@ extern "C" int rfx_batch_get_stats(const rfx_batch* b, int32_t* algos, double* s
   return RFX_OK;
 }
 
+extern "C" int rfx_batch_set_own_launch(rfx_batch* b, const int32_t* flags) {
+  if (!b || !flags) return RFX_ERR_INVALID;
+  for (size_t i = 0; i < b->prob.size(); ++i) b->direct[i] = flags[i] ? 1 : 0;
+  order_problems(b);
+  return RFX_OK;
+}
+
 extern "C" int rfx_batch_get_own_launch(const rfx_batch* b, int32_t* flags) {
   if (!b || !flags) return RFX_ERR_INVALID;
   for (size_t i = 0; i < b->prob.size(); ++i) flags[i] = b->direct[i];

@@ -261,3 +261,23 @@ def test_batch_mds_concurrent_matches_sequential(gpu):
         assert np.allclose(ec[:3], es[:3], rtol=1e-9, atol=0)
         Xs, Xc = Xs.cpu().numpy(), Xc.cpu().numpy()
         assert np.linalg.norm(Xc - Xs) <= 1e-9 * np.linalg.norm(Xs)
+
+
+def test_batch_own_launch_override(gpu, problems):
+    """Every forest through the persistent queue, or every forest on its own
+    launch, gives the autotuned batch's counts (graph replays after each
+    change re-capture the sequence)."""
+    probs, dfs = problems
+    exp = _expected(probs, _lib.PROX_OOB_UPPER)
+    b = ForestBatch(dfs, "oob_upper")
+    b.autotune(reps=1)
+    for flags in (np.zeros(len(dfs), np.int32), np.ones(len(dfs), np.int32), b.own_launch().astype(np.int32)):
+        _say(f"batch own-launch flags {flags.tolist()}")
+        b.set_own_launch(flags)
+        assert np.array_equal(b.own_launch(), flags.astype(bool))
+        for _ in range(2):
+            b.run(use_graph=True)
+            torch.cuda.synchronize()
+            for k, e in enumerate(exp):
+                assert np.array_equal(b.out[k].cpu().numpy(), e), f"problem {k}"
+    b.close()
