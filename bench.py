@@ -344,6 +344,7 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": desc, "n": n, "p": p, "ntree": T, "mtry": mtry, "algo": algo_name,
+                   "features": "all numeric (the one-hot columns are 0/1 floats, maxcat 1: no categorical splits)",
                    "layout": "upper", "leaf_assign_ms": round(leaf_ms, 4), "accumulate_ms": round(acc_ms, 4),
                    "bucket_build_call_ms": round(build_ms, 4), "count_kernel_ms": round(kern_ms, 4),
                    "sum_leaf_sq": sq, "pair_density": sq / (T * float(n) * n)},
