@@ -26,6 +26,7 @@
 //              nout_i = 0 undivided)
 
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "rfx_common.hpp"
@@ -174,6 +175,164 @@ __global__ __launch_bounds__(kRows) void mds_apply_kernel(const int32_t* __restr
   }
 }
 
+// The FP32 operator on the matrix cores (v_mfma_f32_16x16x4_f32), for the
+// count forms (COUNTS, REFERENCE).  A workgroup owns 128 rows (4 waves x 2
+// strips of 16) and a split of the columns, 64 columns per step: the step's
+// 128 x 64 counts are staged in LDS, read along the packed triangle's
+// contiguous direction (along a row right of the block rows, along a column
+// left of them and in the FULL column-major layout), V's 64 rows beside them.
+// Each wave then forms E_ij = (mx - M_ij)^2 straight into A operands (one
+// element per lane and instruction, A[row l16][k g]) and multiplies by V on
+// the matrix cores, so each E element feeds 16 multiply-adds in one
+// instruction slot.  M is formed in FP64 as the VALU operator forms it, with
+// per-row reciprocals of nout for REFERENCE (precomputed: 1 / nout or 1).
+// D[i][c] sits in lane c + 16 (i / 4), element i % 4 (checked with a lane
+// probe on gfx950).  Out-of-range elements load element 0 (always valid) and
+// are zeroed by selects, so the loads issue without branches.
+typedef float md_f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMfRowsWg = 128;  // rows per workgroup
+constexpr int kMfStrips = kMfRowsWg / 64;  // 16-row strips per wave
+constexpr int kMfStep = 64;     // columns per step
+// padded LDS stride of the staged counts (int32): 68 = 4 mod 64, so the 64
+// lanes' reads of one product (rows l16, columns g) hit 64 different banks
+constexpr int kMfCStride = kMfStep + 4;
+constexpr int kMfBatch = 8;     // loads in flight per thread before their LDS stores
+
+template <int MODE>
+__device__ __forceinline__ float mf_e_value(const ProxForm& f, int32_t c, double ri, double rj) {
+  double m;
+  if (MODE == RFX_MDS_COUNTS) {
+    m = static_cast<double>(c) * f.scale;
+  } else {
+    const double a = static_cast<double>(c) / 2.0;
+    const double S = 0.5 * (a * ri + a * rj);
+    const double Sf = static_cast<double>(static_cast<float>(S));
+    m = 0.5 * (Sf * ri + Sf * rj);
+  }
+  const double d = f.mx - m;
+  return static_cast<float>(d * d);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __restrict__ counts, int64_t n, ProxForm f,
+                                                             const double* __restrict__ rinv,
+                                                             const float* __restrict__ V, int64_t jlen,
+                                                             float* __restrict__ Ypart) {
+  __shared__ int32_t Cs[kMfRowsWg][kMfCStride];  // counts of the step, [row][column]
+  __shared__ float Vs[kMfStep][16];               // V rows of the step: lanes (c, g) read 64 banks
+  __shared__ double Rs[kMfStep];                  // their reciprocals (REFERENCE)
+  __shared__ int64_t rowbase[kMfRowsWg];          // packed-triangle offset of each block row
+  __shared__ int64_t colbase[kMfStep];            // ... and of each step column (left of the rows)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kMfRowsWg;
+  const int64_t jb = static_cast<int64_t>(blockIdx.y) * jlen;
+  const int64_t je = min(n, jb + jlen);
+  const bool full = f.layout == RFX_LAYOUT_FULL;
+  if (tid < kMfRowsWg) rowbase[tid] = rfx_upper_row(i0 + tid, n);
+  md_f32x4 acc[kMfStrips];
+  double ri[kMfStrips];
+#pragma unroll
+  for (int s = 0; s < kMfStrips; ++s) {
+    acc[s] = md_f32x4{0.f, 0.f, 0.f, 0.f};
+    const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + l16;
+    ri[s] = (MODE == RFX_MDS_REFERENCE && i < n) ? rinv[i] : 1.0;
+  }
+  for (int64_t j0 = jb; j0 < je; j0 += kMfStep) {
+    const bool leftside = full || j0 + kMfStep <= i0;
+    __syncthreads();  // the previous step's operands are consumed (and rowbase written)
+    if (leftside && !full) {
+      if (tid < kMfStep) colbase[tid] = rfx_upper_row(j0 + tid, n);
+      __syncthreads();
+    }
+    if (leftside) {
+      // left of the block rows (or column-major): thread t reads row t % 128 of
+      // columns t / 128 + 2k, consecutive threads one column's consecutive rows
+      const int r = tid & (kMfRowsWg - 1);
+      const int64_t ii = i0 + r;
+#pragma unroll 1
+      for (int k0 = 0; k0 < kMfStep / 2; k0 += kMfBatch) {
+        int32_t v[kMfBatch];
+#pragma unroll
+        for (int k = 0; k < kMfBatch; ++k) {
+          const int q = (tid >> 7) + 2 * (k0 + k);
+          const int64_t jj = j0 + q;
+          const bool ok = ii < n && jj < je;
+          const int64_t idx = full ? ii + jj * n : colbase[q] + (ii - jj);
+          const int32_t c = counts[ok ? idx : 0];
+          v[k] = ok ? c : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kMfBatch; ++k) Cs[r][(tid >> 7) + 2 * (k0 + k)] = v[k];
+      }
+    } else {
+      // right of (or across) the block rows: thread t reads column t % 64 of
+      // rows t / 64 + 4k, consecutive threads one row's consecutive columns
+      const bool diag = j0 < i0 + kMfRowsWg;
+      const int q = tid & 63;
+      const int64_t jj = j0 + q;
+#pragma unroll 1
+      for (int k0 = 0; k0 < kMfRowsWg / 4; k0 += kMfBatch) {
+        int32_t v[kMfBatch];
+#pragma unroll
+        for (int k = 0; k < kMfBatch; ++k) {
+          const int r = (tid >> 6) + 4 * (k0 + k);
+          const int64_t ii = i0 + r;
+          bool ok = ii < n && jj < je;
+          int64_t idx = rowbase[r] + (jj - ii);
+          if (diag) {
+            ok = ok && ii != jj;
+            idx = ii < jj ? idx : rfx_upper_row(jj, n) + (ii - jj);
+          }
+          const int32_t c = counts[ok ? idx : 0];
+          v[k] = ok ? c : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kMfBatch; ++k) Cs[(tid >> 6) + 4 * (k0 + k)][q] = v[k];
+      }
+    }
+    for (int e = tid; e < kMfStep * 16; e += 256) {
+      const int q = e >> 4, c = e & 15;
+      const int64_t jj = j0 + q;
+      Vs[q][c] = jj < je ? V[jj * 16 + c] : 0.f;
+    }
+    if (MODE == RFX_MDS_REFERENCE && tid < kMfStep) Rs[tid] = j0 + tid < je ? rinv[j0 + tid] : 1.0;
+    __syncthreads();
+    // 16 products of K = 4: k slot g of product m is column 4m + g of the step
+#pragma unroll 2
+    for (int m = 0; m < kMfStep / 4; ++m) {
+      const int q = 4 * m + g;
+      const int64_t jj = j0 + q;
+      const float b = Vs[q][l16];
+      const double rj = MODE == RFX_MDS_REFERENCE ? Rs[q] : 1.0;
+#pragma unroll
+      for (int s = 0; s < kMfStrips; ++s) {
+        const int r = wave * (16 * kMfStrips) + s * 16 + l16;
+        const int64_t ii = i0 + r;
+        const float ev = mf_e_value<MODE>(f, Cs[r][q], ri[s], rj);
+        const float a = (ii < n && jj < je && ii != jj) ? ev : 0.f;
+        acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[s], 0, 0, 0);
+      }
+    }
+  }
+  // D[i][c]: lane c + 16 (i / 4), element i % 4
+#pragma unroll
+  for (int s = 0; s < kMfStrips; ++s) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + 4 * g + e;
+      if (i < n) Ypart[(static_cast<int64_t>(blockIdx.y) * n + i) * 16 + l16] = acc[s][e];
+    }
+  }
+}
+
+// Per-row reciprocals of nout for the REFERENCE form: 1 / nout_i, or 1 when
+// nout_i = 0 (such rows are not divided).
+__global__ void mds_rinv_kernel(const int32_t* __restrict__ nout, int64_t n, double* __restrict__ rinv) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) rinv[i] = nout && nout[i] > 0 ? 1.0 / static_cast<double>(nout[i]) : 1.0;
+}
+
 template <typename Real, typename Out>
 __global__ void mds_sum_splits_kernel(const Real* __restrict__ Ypart, int splits, int64_t nk, Out* __restrict__ Y) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -207,7 +366,7 @@ int splits_for(int64_t n) {
 template <typename Real>
 size_t dense_ws(int64_t n) {
   const int sp = splits_for(n);
-  return al(8 * n) + al(8 * 4) + al(sizeof(Real) * sp * n * kBlockK) + al(8 * sp * n) +
+  return al(8 * n) + al(8 * 4) + al(sizeof(Real) * sp * n * kBlockK) + al(8 * sp * n) + al(8 * n) +
          rfx::SubspaceSolver<Real>::workspace_bytes(n, kBlockK);
 }
 
@@ -225,6 +384,8 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
   p += al(sizeof(Real) * sp * n * kBlockK);
   double* rpart = reinterpret_cast<double*>(p);
   p += al(8 * sp * n);
+  double* rinv = reinterpret_cast<double*>(p);
+  p += al(8 * n);
   ProxForm f{mode, layout, scale, 0.0, nout, mat, n};
 
   hipLaunchKernelGGL(mds_zero_kernel, dim3(1), dim3(64), 0, st, mx_bits);
@@ -252,8 +413,26 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
 
   rfx::SubspaceSolver<Real> solver(st, n, kBlockK);
   solver.carve(p);
+  // The FP32 operator of the count forms runs on the matrix cores.
+  const bool mfma = std::is_same<Real, float>::value && (mode == RFX_MDS_COUNTS || mode == RFX_MDS_REFERENCE);
+  if (mfma && mode == RFX_MDS_REFERENCE)
+    hipLaunchKernelGGL(mds_rinv_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, nout, n, rinv);
+  const dim3 mgrid(rfx_ceil_div(n, kMfRowsWg), static_cast<unsigned>(sp));
   auto apply = [&](const Real* V, Real* W, int k) -> int {
     (void)k;
+    if constexpr (std::is_same<Real, float>::value) {
+      if (mfma) {
+        if (mode == RFX_MDS_COUNTS)
+          hipLaunchKernelGGL((mds_apply_mfma_kernel<RFX_MDS_COUNTS>), mgrid, dim3(256), 0, st, counts, n, f, rinv, V,
+                             jlen, Ypart);
+        else
+          hipLaunchKernelGGL((mds_apply_mfma_kernel<RFX_MDS_REFERENCE>), mgrid, dim3(256), 0, st, counts, n, f, rinv,
+                             V, jlen, Ypart);
+        hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
+                           Ypart, sp, n * kBlockK, W);
+        return rfx_launch_status();
+      }
+    }
     hipLaunchKernelGGL((mds_apply_kernel<Real, kBlockK, false>), grid, dim3(kRows), 0, st, counts, n, f, V, jlen,
                        Ypart);
     hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
