@@ -175,8 +175,9 @@ __global__ __launch_bounds__(kRows) void mds_apply_kernel(const int32_t* __restr
   }
 }
 
-// The FP32 operator on the matrix cores (v_mfma_f32_16x16x4_f32), for the
-// count forms (COUNTS, REFERENCE).  A workgroup owns 128 rows (4 waves x 2
+// The operator on the matrix cores (v_mfma_f32_16x16x4f32 for the FP32
+// operator, v_mfma_f64_16x16x4f64 for FP64), for the count forms (COUNTS,
+// REFERENCE).  A workgroup owns 128 rows (4 waves x 2
 // strips of 16) and a split of the columns, 64 columns per step: the step's
 // 128 x 64 counts are staged in LDS, read along the packed triangle's
 // contiguous direction (along a row right of the block rows, along a column
@@ -186,8 +187,8 @@ __global__ __launch_bounds__(kRows) void mds_apply_kernel(const int32_t* __restr
 // the matrix cores, so each E element feeds 16 multiply-adds in one
 // instruction slot.  M is formed in FP64 as the VALU operator forms it, with
 // per-row reciprocals of nout for REFERENCE (precomputed: 1 / nout or 1).
-// D[i][c] sits in lane c + 16 (i / 4), element i % 4 (checked with a lane
-// probe on gfx950).  Out-of-range elements load element 0 (always valid) and
+// The output row of a (lane, element) slot differs between the two products
+// (MdsMfma::row; both checked with lane probes on gfx950).  Out-of-range elements load element 0 (always valid) and
 // are zeroed by selects, so the loads issue without branches.
 typedef float md_f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMfRowsWg = 128;  // rows per workgroup
@@ -198,8 +199,8 @@ constexpr int kMfStep = 64;     // columns per step
 constexpr int kMfCStride = kMfStep + 4;
 constexpr int kMfBatch = 8;     // loads in flight per thread before their LDS stores
 
-template <int MODE>
-__device__ __forceinline__ float mf_e_value(const ProxForm& f, int32_t c, double ri, double rj) {
+template <int MODE, typename Real>
+__device__ __forceinline__ Real mf_e_value(const ProxForm& f, int32_t c, double ri, double rj) {
   double m;
   if (MODE == RFX_MDS_COUNTS) {
     m = static_cast<double>(c) * f.scale;
@@ -210,16 +211,42 @@ __device__ __forceinline__ float mf_e_value(const ProxForm& f, int32_t c, double
     m = 0.5 * (Sf * ri + Sf * rj);
   }
   const double d = f.mx - m;
-  return static_cast<float>(d * d);
+  return static_cast<Real>(d * d);
 }
 
-template <int MODE>
+typedef double md_f64x4 __attribute__((ext_vector_type(4)));
+// One 16 x 16 x 4 product: FP32 (v_mfma_f32_16x16x4f32) or FP64
+// (v_mfma_f64_16x16x4f64); A[i][k] and B[k][j] sit in lanes i + 16k and
+// j + 16k for both.
+template <typename Real>
+struct MdsMfma;
+template <>
+struct MdsMfma<float> {
+  typedef md_f32x4 Acc;
+  static __device__ __forceinline__ Acc mma(float a, float b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // D[i][j] in lane j + 16 (i / 4), element i % 4
+  static __device__ __forceinline__ int row(int g, int e) { return 4 * g + e; }
+};
+template <>
+struct MdsMfma<double> {
+  typedef md_f64x4 Acc;
+  static __device__ __forceinline__ Acc mma(double a, double b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // D[i][j] in lane j + 16 (i % 4), element i / 4
+  static __device__ __forceinline__ int row(int g, int e) { return 4 * e + g; }
+};
+
+template <typename Real, int MODE>
 __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __restrict__ counts, int64_t n, ProxForm f,
                                                              const double* __restrict__ rinv,
-                                                             const float* __restrict__ V, int64_t jlen,
-                                                             float* __restrict__ Ypart) {
+                                                             const Real* __restrict__ V, int64_t jlen,
+                                                             Real* __restrict__ Ypart) {
+  using M = MdsMfma<Real>;
   __shared__ int32_t Cs[kMfRowsWg][kMfCStride];  // counts of the step, [row][column]
-  __shared__ float Vs[kMfStep][16];               // V rows of the step: lanes (c, g) read 64 banks
+  __shared__ Real Vs[kMfStep][16];                // V rows of the step: lanes (c, g) read distinct banks
   __shared__ double Rs[kMfStep];                  // their reciprocals (REFERENCE)
   __shared__ int64_t rowbase[kMfRowsWg];          // packed-triangle offset of each block row
   __shared__ int64_t colbase[kMfStep];            // ... and of each step column (left of the rows)
@@ -230,11 +257,11 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
   const int64_t je = min(n, jb + jlen);
   const bool full = f.layout == RFX_LAYOUT_FULL;
   if (tid < kMfRowsWg) rowbase[tid] = rfx_upper_row(i0 + tid, n);
-  md_f32x4 acc[kMfStrips];
+  typename M::Acc acc[kMfStrips];
   double ri[kMfStrips];
 #pragma unroll
   for (int s = 0; s < kMfStrips; ++s) {
-    acc[s] = md_f32x4{0.f, 0.f, 0.f, 0.f};
+    acc[s] = typename M::Acc{Real(0), Real(0), Real(0), Real(0)};
     const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + l16;
     ri[s] = (MODE == RFX_MDS_REFERENCE && i < n) ? rinv[i] : 1.0;
   }
@@ -294,7 +321,7 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
     for (int e = tid; e < kMfStep * 16; e += 256) {
       const int q = e >> 4, c = e & 15;
       const int64_t jj = j0 + q;
-      Vs[q][c] = jj < je ? V[jj * 16 + c] : 0.f;
+      Vs[q][c] = jj < je ? V[jj * 16 + c] : Real(0);
     }
     if (MODE == RFX_MDS_REFERENCE && tid < kMfStep) Rs[tid] = j0 + tid < je ? rinv[j0 + tid] : 1.0;
     __syncthreads();
@@ -303,24 +330,23 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
     for (int m = 0; m < kMfStep / 4; ++m) {
       const int q = 4 * m + g;
       const int64_t jj = j0 + q;
-      const float b = Vs[q][l16];
+      const Real b = Vs[q][l16];
       const double rj = MODE == RFX_MDS_REFERENCE ? Rs[q] : 1.0;
 #pragma unroll
       for (int s = 0; s < kMfStrips; ++s) {
         const int r = wave * (16 * kMfStrips) + s * 16 + l16;
         const int64_t ii = i0 + r;
-        const float ev = mf_e_value<MODE>(f, Cs[r][q], ri[s], rj);
-        const float a = (ii < n && jj < je && ii != jj) ? ev : 0.f;
-        acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[s], 0, 0, 0);
+        const Real ev = mf_e_value<MODE, Real>(f, Cs[r][q], ri[s], rj);
+        const Real a = (ii < n && jj < je && ii != jj) ? ev : Real(0);
+        acc[s] = M::mma(a, b, acc[s]);
       }
     }
   }
-  // D[i][c]: lane c + 16 (i / 4), element i % 4
 #pragma unroll
   for (int s = 0; s < kMfStrips; ++s) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + 4 * g + e;
+      const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + M::row(g, e);
       if (i < n) Ypart[(static_cast<int64_t>(blockIdx.y) * n + i) * 16 + l16] = acc[s][e];
     }
   }
@@ -413,25 +439,23 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
 
   rfx::SubspaceSolver<Real> solver(st, n, kBlockK);
   solver.carve(p);
-  // The FP32 operator of the count forms runs on the matrix cores.
-  const bool mfma = std::is_same<Real, float>::value && (mode == RFX_MDS_COUNTS || mode == RFX_MDS_REFERENCE);
-  if (mfma && mode == RFX_MDS_REFERENCE)
+  // The operator of the count forms runs on the matrix cores.
+  const bool mfma = mode == RFX_MDS_COUNTS || mode == RFX_MDS_REFERENCE;
+  if (mode == RFX_MDS_REFERENCE)
     hipLaunchKernelGGL(mds_rinv_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, nout, n, rinv);
   const dim3 mgrid(rfx_ceil_div(n, kMfRowsWg), static_cast<unsigned>(sp));
   auto apply = [&](const Real* V, Real* W, int k) -> int {
     (void)k;
-    if constexpr (std::is_same<Real, float>::value) {
-      if (mfma) {
-        if (mode == RFX_MDS_COUNTS)
-          hipLaunchKernelGGL((mds_apply_mfma_kernel<RFX_MDS_COUNTS>), mgrid, dim3(256), 0, st, counts, n, f, rinv, V,
-                             jlen, Ypart);
-        else
-          hipLaunchKernelGGL((mds_apply_mfma_kernel<RFX_MDS_REFERENCE>), mgrid, dim3(256), 0, st, counts, n, f, rinv,
-                             V, jlen, Ypart);
-        hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
-                           Ypart, sp, n * kBlockK, W);
-        return rfx_launch_status();
-      }
+    if (mfma) {
+      if (mode == RFX_MDS_COUNTS)
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS>), mgrid, dim3(256), 0, st, counts, n, f, rinv,
+                           V, jlen, Ypart);
+      else
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE>), mgrid, dim3(256), 0, st, counts, n, f,
+                           rinv, V, jlen, Ypart);
+      hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
+                         Ypart, sp, n * kBlockK, W);
+      return rfx_launch_status();
     }
     hipLaunchKernelGGL((mds_apply_kernel<Real, kBlockK, false>), grid, dim3(kRows), 0, st, counts, n, f, V, jlen,
                        Ypart);
