@@ -154,6 +154,39 @@ int rfx_leaf_assign(hipStream_t stream, const float* x_col, int64_t n, int32_t p
                     const int64_t* node_off, int32_t ntree, const uint32_t* inbag, const int32_t* cat,
                     int32_t* nodextr, int32_t* keys);
 
+/* Chain index of packed trees (host): the heavy paths of at least min_len
+ * internal nodes, for rfx_leaf_assign_ws.  Trees grown to small nodes on
+ * large noisy data hold chains of splits that each peel one sample off a
+ * feature's extreme (at n = 1,000,000, mean walks of thousands of levels);
+ * on such a path the exit of a sample is found by binary searches instead of
+ * a walk of one dependent node gather per level.  Two calls: out == NULL sets
+ * *bytes to the size needed (0 when no tree has such a path: pass chains =
+ * NULL then); else out (host memory of *bytes) receives the index, which is
+ * position independent and goes to the device as is.
+ *   node_offset  int64 [T+1];  packed  rfx_pack_nodes output */
+int rfx_chain_index(const int64_t* node_offset, int32_t ntree, const uint32_t* packed, int32_t min_len, void* out,
+                    size_t* bytes);
+
+/* Workspace bytes rfx_leaf_assign_ws takes for (n, T): a 256-byte header and
+ * a queue of max(1024, min(n*T, max(65536, min(n*T/16, 2^26)))) 16-byte walk
+ * records. */
+size_t rfx_leaf_workspace_bytes(int64_t n, int32_t ntree);
+
+/* rfx_leaf_assign for forests with chains (same outputs, bit for bit, as
+ * rfx_leaf_assign; same reference interfaces).  With all features numeric
+ * and a chain index (device copy of rfx_chain_index's output), the walks
+ * still going after 128 lockstep levels are queued in ws and finished by a
+ * kernel that refills idle lanes and jumps along the indexed paths; trees
+ * go in chunks whose walks the queue can hold twice over, and a full queue
+ * only sends the refused walks back to the first pass.  chains == NULL or a
+ * categorical forest: the one-pass kernel.  ws: device memory of ws_bytes
+ * (16-byte aligned, at least 256 + 16 * 1024 bytes, else
+ * RFX_ERR_WORKSPACE); its contents need no initialisation.  Calls that share
+ * ws race. */
+int rfx_leaf_assign_ws(hipStream_t stream, const float* x_col, int64_t n, int32_t p, const uint32_t* nodes,
+                       const int64_t* node_off, int32_t ntree, const uint32_t* inbag, const int32_t* cat,
+                       int32_t* nodextr, int32_t* keys, const void* chains, void* ws, size_t ws_bytes);
+
 /* Workspace bytes rfx_prox_accumulate needs for (n, T, total_nodes, algo). */
 size_t rfx_prox_workspace_bytes(int64_t n, int32_t ntree, int64_t total_nodes, int32_t algo);
 

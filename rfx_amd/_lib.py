@@ -83,6 +83,9 @@ _SIGS = {
     "rfx_pack_inbag_bits": (_i32, [_p, _i64, _i32, _p]),
     # GPU side
     "rfx_leaf_assign": (_i32, [_p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _p]),
+    "rfx_leaf_workspace_bytes": (_sz, [_i64, _i32]),
+    "rfx_leaf_assign_ws": (_i32, [_p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _p, _p, _p, _sz]),
+    "rfx_chain_index": (_i32, [_p, _i32, _p, _i32, _p, _p]),
     "rfx_prox_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),
     "rfx_prox_build_buckets": (_i32, [_p, _p, _i64, _i32, _i64, _p, _sz]),
     "rfx_prox_casewise_workspace_bytes": (_sz, [_i64, _i32, _i64]),

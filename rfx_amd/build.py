@@ -21,7 +21,7 @@ LIB = PKG / "librfx.so"
 PYMOD_SRC = CSRC / "ref_py.cpp"  # pybind11 module of the reference-signature entry points
 
 HIP_SOURCES = ["leaf.hip", "accum.hip", "casewise.hip", "finish.hip", "mds.hip", "factor.hip", "batch.hip", "outlier.hip"]
-CXX_SOURCES = ["forest.cpp", "shim.cpp"]
+CXX_SOURCES = ["forest.cpp", "shim.cpp", "chains.cpp"]
 # Per-file extra flags: finishprox must not contract a*b+c into an FMA.
 HIP_EXTRA = {"finish.hip": ["-ffp-contract=off"]}
 

@@ -170,6 +170,20 @@ class Forest:
                                    ptr(self.xbestsplit), ptr(self.treemap), self.p, ptr(out)), "rfx_pack_nodes")
         return out
 
+    def chain_index(self, min_len: int = 32, packed: np.ndarray | None = None) -> np.ndarray | None:
+        """Chain index (rfx_chain_index) of the heavy paths of at least min_len
+        internal nodes, as bytes; None when no tree has one."""
+        packed = self.packed_nodes() if packed is None else np.ascontiguousarray(packed, np.uint32)
+        need = C.c_size_t(0)
+        check(lib().rfx_chain_index(ptr(self.node_offset), self.ntree, ptr(packed), min_len, None, C.byref(need)),
+              "rfx_chain_index")
+        if need.value == 0:
+            return None
+        out = np.empty(need.value, np.uint8)
+        check(lib().rfx_chain_index(ptr(self.node_offset), self.ntree, ptr(packed), min_len, ptr(out), C.byref(need)),
+              "rfx_chain_index")
+        return out
+
     def inbag_bits(self) -> np.ndarray:
         W = (self.n + 31) // 32
         out = np.empty(self.ntree * W, np.uint32)

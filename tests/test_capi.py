@@ -32,6 +32,24 @@ def test_library_exports_every_declared_symbol():
 def test_invalid_arguments_return_status():
     L = _lib.lib()
     assert L.rfx_leaf_assign(None, None, 0, 1, None, None, 1, None, None, None, None) == -1
+    # the queued form wants a workspace of at least 256 + 16 * 1024 bytes,
+    # 16-byte aligned; it is sized by rfx_leaf_workspace_bytes
+    d = 256
+    assert L.rfx_leaf_assign_ws(None, d, 10, 2, d, d, 1, d, None, None, d, d, None, 0) == -4
+    assert L.rfx_leaf_assign_ws(None, d, 10, 2, d, d, 1, d, None, None, d, d, d, 256 + 16 * 1023) == -4
+    assert L.rfx_leaf_assign_ws(None, d, 10, 2, d, d, 1, d, None, None, d, d, d + 8, 1 << 20) == -1
+    # the chain index builder: bad arguments, and no index for a one-node tree
+    nb = C.c_size_t(0)
+    off = np.array([0, 1], np.int64)
+    leaf = np.array([0, 0xFFFFFFFF], np.uint32)
+    assert L.rfx_chain_index(None, 1, leaf.ctypes.data, 32, None, C.byref(nb)) == -1
+    assert L.rfx_chain_index(off.ctypes.data, 1, leaf.ctypes.data, 1, None, C.byref(nb)) == -1
+    assert L.rfx_chain_index(off.ctypes.data, 1, leaf.ctypes.data, 32, None, C.byref(nb)) == 0 and nb.value == 0
+    assert L.rfx_leaf_workspace_bytes(0, 5) == 0
+    assert L.rfx_leaf_workspace_bytes(6, 2) == 256 + 16 * 1024
+    assert L.rfx_leaf_workspace_bytes(100, 30) == 256 + 16 * 3000
+    assert L.rfx_leaf_workspace_bytes(200000, 1000) == 256 + 16 * (200000 * 1000 // 16)
+    assert L.rfx_leaf_workspace_bytes(4000000, 5000) == 256 + 16 * (1 << 26)
     assert L.rfx_prox_accumulate(None, None, 0, 1, 1, 0, 0, 0, 0, None, None, 0) == -1
     assert L.rfx_prox_accumulate(None, None, 10, 1, 1, 9, 0, 1, 0, None, None, 0) == -1
     assert L.rfx_finishprox(None, None, 0, None, 10, None, 0) == -1
