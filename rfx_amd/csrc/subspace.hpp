@@ -242,6 +242,76 @@ __global__ void ss_rotate_kernel(const Real* __restrict__ V, int k, const double
   out[e] = static_cast<Real>(s);
 }
 
+// The same product for wide blocks (k and kout of 32 or more, kout up to
+// 160): a 64-row tile of V and a 16-row slab of M staged in LDS, 4 rows x 10
+// columns per thread.  Each output is the same FP64 fma chain over a = 0 ..
+// k-1 as ss_rotate_kernel's, so the results are identical; the simple
+// kernel re-reads M (166 KB at k = kout = 144, more than L1 holds) from L2
+// for every multiply-add and takes 13 ms at n = 1,000,000.
+constexpr int kRotRows = 64;
+constexpr int kRotK = 16;
+constexpr int kRotJ = 10;
+template <typename Real>
+__global__ __launch_bounds__(256) void ss_rotate_tiled_kernel(const Real* __restrict__ V, int k,
+                                                              const double* __restrict__ M, int kout, int64_t n,
+                                                              Real* __restrict__ out) {
+  __shared__ double vs[kRotK][kRotRows + 1];
+  __shared__ double ms[kRotK][16 * kRotJ];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRotRows;
+  double acc[4][kRotJ];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int j = 0; j < kRotJ; ++j) acc[x][j] = 0.0;
+  for (int a0 = 0; a0 < k; a0 += kRotK) {
+    const int ka = min(kRotK, k - a0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRotRows * kRotK; e += 256) {
+      const int q = e / kRotK, a = e % kRotK;
+      const int64_t i = r0 + q;
+      vs[a][q] = (i < n && a < ka) ? static_cast<double>(V[i * k + a0 + a]) : 0.0;
+    }
+    for (int e = threadIdx.x; e < kRotK * 16 * kRotJ; e += 256) {
+      const int a = e / (16 * kRotJ), c = e % (16 * kRotJ);
+      ms[a][c] = (a < ka && c < kout) ? M[static_cast<int64_t>(a0 + a) * kout + c] : 0.0;
+    }
+    __syncthreads();
+    for (int a = 0; a < ka; ++a) {
+      double v[4], m[kRotJ];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) v[x] = vs[a][tr * 4 + x];
+#pragma unroll
+      for (int j = 0; j < kRotJ; ++j) m[j] = ms[a][tc + 16 * j];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int j = 0; j < kRotJ; ++j) acc[x][j] = fma(v[x], m[j], acc[x][j]);
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int64_t i = r0 + tr * 4 + x;
+    if (i >= n) continue;
+#pragma unroll
+    for (int j = 0; j < kRotJ; ++j) {
+      const int c = tc + 16 * j;
+      if (c < kout) out[i * kout + c] = static_cast<Real>(acc[x][j]);
+    }
+  }
+}
+
+// out (n x kout) = V (n x k) M (k x kout), M in FP64 on the device.
+template <typename Real>
+inline void ss_rotate(hipStream_t st, const Real* V, int k, const double* M, int kout, int64_t n, Real* out) {
+  if (k >= 32 && kout >= 32 && kout <= 16 * kRotJ)
+    hipLaunchKernelGGL(ss_rotate_tiled_kernel<Real>, dim3(static_cast<unsigned>(rfx_ceil_div(n, kRotRows))),
+                       dim3(256), 0, st, V, k, M, kout, n, out);
+  else
+    hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(static_cast<unsigned>(rfx_ceil_div(n * kout, 256))), dim3(256),
+                       0, st, V, k, M, kout, n, out);
+}
+
 // Next block from the centred product without forming it:
 //   V[i, c] = sum_a W[i,a] M[a,c] + r_i ar[c] + ao[c] + sum_a Vold[i,a] Ms[a,c]
 // with prm = [M (k*k), ar (k), ao (k), Ms (k*k)]; one thread per (i, c).
@@ -365,8 +435,7 @@ class SubspaceSolver {
       std::vector<double> Rinv = inv_upper_from_lower(G, k_);
       rc = upload(M_, Rinv.data(), static_cast<size_t>(k_) * k_);
       if (rc) return rc;
-      hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(rfx_ceil_div(n_ * k_, 256)), dim3(256), 0, st_, V, k_, M_, k_,
-                         n_, T_);
+      ss_rotate<Real>(st_, V, k_, M_, k_, n_, T_);
       RFX_HIP_TRY(hipMemcpyAsync(V, T_, sizeof(Real) * n_ * k_, hipMemcpyDeviceToDevice, st_));
     }
     return rfx_launch_status();
@@ -498,8 +567,7 @@ class SubspaceSolver {
         if (rc) return rc;
         rc = upload(scale_, sc.data(), want);
         if (rc) return rc;
-        hipLaunchKernelGGL(ss_rotate_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, V_, k, M_,
-                           want, n_, T_);
+        ss_rotate<Real>(st_, V_, k, M_, want, n_, T_);
         hipLaunchKernelGGL(ss_coords_kernel<Real>, dim3(rfx_ceil_div(n_ * want, 256)), dim3(256), 0, st_, T_, want,
                            scale_, want, n_, coords);
         RFX_HIP_TRY(hipStreamSynchronize(st_));
