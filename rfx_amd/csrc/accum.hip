@@ -277,10 +277,10 @@ __device__ __forceinline__ void line_whole_body(const uint2 (&q)[KT], const uint
     const uint32_t v = acc[c];
     if (MODE == kMode_PAIR) {
       const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-      out[base + c] = accumulate ? out[base + c] + va : va;
-      out2[base + c] = accumulate ? out2[base + c] + vb : vb;
+      rfx_emit_count(out + base + c, va, accumulate);
+      rfx_emit_count(out2 + base + c, vb, accumulate);
     } else {
-      out[base + c] = accumulate ? out[base + c] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+      rfx_emit_count(out + base + c, static_cast<int32_t>(v), accumulate);
     }
   }
 }
@@ -403,11 +403,18 @@ __device__ __forceinline__ void win_emit(uint32_t v, int32_t* __restrict__ out, 
                                          int64_t o, int32_t accumulate) {
   if (MODE == kMode_PAIR) {
     const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-    out[o] = accumulate ? out[o] + va : va;
-    out2[o] = accumulate ? out2[o] + vb : vb;
+    rfx_emit_count(out + o, va, accumulate);
+    rfx_emit_count(out2 + o, vb, accumulate);
   } else {
-    out[o] = accumulate ? out[o] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+    rfx_emit_count(out + o, static_cast<int32_t>(v), accumulate);
   }
+}
+
+typedef int32_t rfx_v4i __attribute__((ext_vector_type(4)));
+
+// 16-byte form of rfx_emit_count's overwrite: a nontemporal vector store.
+__device__ __forceinline__ void win_store4(int4* p, int4 v) {
+  __builtin_nontemporal_store(rfx_v4i{v.x, v.y, v.z, v.w}, reinterpret_cast<rfx_v4i*>(p));
 }
 
 // Counters acc[L0 + j], j < count, to out[o0 + j] (and out2 for PAIR); the
@@ -442,8 +449,8 @@ __device__ __forceinline__ void win_flush(const uint32_t* __restrict__ acc, int 
         va = make_int4(va.x + pa.x, va.y + pa.y, va.z + pa.z, va.w + pa.w);
         vb = make_int4(vb.x + pb.x, vb.y + pb.y, vb.z + pb.z, vb.w + pb.w);
       }
-      p4[k] = va;
-      q4[k] = vb;
+      win_store4(p4 + k, va);
+      win_store4(q4 + k, vb);
     } else {
       int4 va = make_int4(static_cast<int32_t>(v.x), static_cast<int32_t>(v.y), static_cast<int32_t>(v.z),
                           static_cast<int32_t>(v.w));
@@ -451,7 +458,7 @@ __device__ __forceinline__ void win_flush(const uint32_t* __restrict__ acc, int 
         const int4 pa = p4[k];
         va = make_int4(va.x + pa.x, va.y + pa.y, va.z + pa.z, va.w + pa.w);
       }
-      p4[k] = va;
+      win_store4(p4 + k, va);
     }
   }
 }

@@ -133,11 +133,10 @@ __device__ __forceinline__ void dense_tile_body(const int32_t* __restrict__ keys
       if (MODE == kMode_PAIR) {
         const int32_t va = static_cast<int32_t>(cnt[r][c] & 0xffffu);
         const int32_t vb = static_cast<int32_t>(cnt[r][c] >> 16);
-        out[o] = accumulate ? out[o] + va : va;
-        out2[o] = accumulate ? out2[o] + vb : vb;
+        rfx_emit_count(out + o, va, accumulate);
+        rfx_emit_count(out2 + o, vb, accumulate);
       } else {
-        const int32_t v = static_cast<int32_t>(cnt[r][c]);
-        out[o] = accumulate ? out[o] + v : v;
+        rfx_emit_count(out + o, static_cast<int32_t>(cnt[r][c]), accumulate);
       }
     }
   }

@@ -228,10 +228,10 @@ __device__ __forceinline__ void gather_line_body(const uint2* __restrict__ pe, c
     const uint32_t v = acc[q];
     if (MODE == kMode_PAIR) {
       const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-      out[base + q] = accumulate ? out[base + q] + va : va;
-      out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+      rfx_emit_count(out + base + q, va, accumulate);
+      rfx_emit_count(out2 + base + q, vb, accumulate);
     } else {
-      out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+      rfx_emit_count(out + base + q, static_cast<int32_t>(v), accumulate);
     }
   }
 }
@@ -302,10 +302,10 @@ __device__ __forceinline__ void gather_gallop_body(const uint2* __restrict__ pe,
       const uint32_t v = acc[q];
       if (MODE == kMode_PAIR) {
         const int32_t va = static_cast<int32_t>(v & 0xffffu), vb = static_cast<int32_t>(v >> 16);
-        out[base + q] = accumulate ? out[base + q] + va : va;
-        out2[base + q] = accumulate ? out2[base + q] + vb : vb;
+        rfx_emit_count(out + base + q, va, accumulate);
+        rfx_emit_count(out2 + base + q, vb, accumulate);
       } else {
-        out[base + q] = accumulate ? out[base + q] + static_cast<int32_t>(v) : static_cast<int32_t>(v);
+        rfx_emit_count(out + base + q, static_cast<int32_t>(v), accumulate);
       }
     }
   }

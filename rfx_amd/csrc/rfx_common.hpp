@@ -19,6 +19,17 @@ inline int rfx_launch_status() { return hipGetLastError() == hipSuccess ? RFX_OK
 // Offset of row i in the packed row-major upper triangle with diagonal.
 __host__ __device__ inline int64_t rfx_upper_row(int64_t i, int64_t n) { return i * n - (i * (i - 1)) / 2; }
 
+// A count written to the output: added to what is there when accumulating,
+// else stored nontemporally -- the kernels write each output word once and
+// never read it back, so it need not displace the member runs that other
+// rows still read from L2 (the C3 counting kernel: 62.4 -> 55.0 ms).
+__device__ __forceinline__ void rfx_emit_count(int32_t* p, int32_t v, int32_t accumulate) {
+  if (accumulate)
+    *p += v;
+  else
+    __builtin_nontemporal_store(v, p);
+}
+
 __host__ __device__ inline int64_t rfx_upper_index(int64_t i, int64_t j, int64_t n) {
   return rfx_upper_row(i, n) + (j - i);
 }
