@@ -701,6 +701,19 @@ def bench_c4(args) -> int:
         b.record(stream)
     torch.cuda.synchronize()
     leaf_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    # the one-pass kernel (no chain jumps) once, for comparison: same keys
+    keys1 = torch.empty_like(keys)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    df.leaf_assign(keys=keys1, queue=False)
+    b.record(stream)
+    torch.cuda.synchronize()
+    onepass_ms = a.elapsed_time(b)
+    same_keys = bool(torch.equal(keys1, keys))
+    del keys1
+    if not same_keys:
+        raise SystemExit("c4: the chain-jump leaf assignment differs from the one-pass kernel")
+    log(f"c4 leaf assignment one-pass {onepass_ms:.1f} ms (chain index: {df.chains is not None}), keys identical")
     lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2)  # warm-up
     (U, eig), fac_s = timed(lambda: lowrank_factors(keys, forest, rank=rank, oversample=16, power_iters=2))
     del keys
@@ -761,6 +774,7 @@ def bench_c4(args) -> int:
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp8-e4m3", "data": "synthetic",
         "config": {"workload": desc, "n": n, "p": p, "ntree": T, "mtry": mtry, "minndsize": minnd, "rank": rank,
                    "forest_nodes": forest.total_nodes, "grow_s": round(grow_s, 1), "leaf_assign_ms": round(leaf_ms, 3),
+                   "leaf_assign_onepass_ms": round(onepass_ms, 3), "leaf_chain_index": df.chains is not None,
                    "factor_build_s": round(fac_s, 4), "factor_top_eigs": [round(float(v), 4) for v in eig[:6]]},
         "mds": out,
     }
