@@ -165,7 +165,9 @@ __global__ __launch_bounds__(kBlock) void leaf_assign_numeric_kernel(const float
 // At n = 200,000 (about 54,000 nodes per tree, 1.5 MiB chunks) 4 wavefronts
 // of 2 trees take 14.1 ms per 1,000 trees, 4 x 4 trees 13.8 ms, 8 x 2 14.2 ms,
 // 4 x 1 19.5 ms, one wavefront of 16 trees 17.4 ms and 8 x 8 trees 23.7 ms:
-// the node records of a tree group must stay in L2.
+// the node records of a tree group must stay in L2.  With one chunk per XCD
+// (the default): 4 x 2 trees 12.5 ms, 2 x 4 12.6, 4 x 4 13.1, 8 x 2 13.6,
+// 4 x 1 16.7, 2 x 2 17.6.
 #ifndef RFX_LEAF_STAGE
 #define RFX_LEAF_STAGE 1
 #endif
