@@ -36,7 +36,18 @@ namespace {
 
 constexpr int kRows = 256;  // rows per workgroup = threads
 constexpr int kCols = 32;   // columns per LDS tile
-constexpr int kBlockK = 16; // subspace block size
+constexpr int kBlockK = 16; // subspace block size (VALU operator; Ritz values reported)
+// Block of the count forms on the matrix cores: 32 columns.  At n = 200,000
+// (covtype-like, T = 1000) the top eigenvalues sit close together
+// (lambda3 / lambda4 = 1.007), so the iteration count follows
+// lambda(k+1) / lambda3; each count, formed into E once, feeds one product per
+// 16 columns, so a wider block costs little more per application.
+#ifndef RFX_MDS_MFMA_GROUPS
+#define RFX_MDS_MFMA_GROUPS 2
+#endif
+constexpr int kMfGroups = RFX_MDS_MFMA_GROUPS;
+constexpr int kMfK = 16 * kMfGroups;
+constexpr int kDenseMaxK = kMfK > kBlockK ? kMfK : kBlockK;
 
 struct ProxForm {
   int32_t mode;    // RFX_MDS_COUNTS / _REFERENCE / _MATRIX / _RFGAP
@@ -239,14 +250,18 @@ struct MdsMfma<double> {
   static __device__ __forceinline__ int row(int g, int e) { return 4 * e + g; }
 };
 
-template <typename Real, int MODE>
+// KG groups of 16 block columns: each E element, formed once, feeds KG
+// products.
+template <typename Real, int MODE, int KG>
 __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __restrict__ counts, int64_t n, ProxForm f,
                                                              const double* __restrict__ rinv,
                                                              const Real* __restrict__ V, int64_t jlen,
                                                              Real* __restrict__ Ypart) {
   using M = MdsMfma<Real>;
+  constexpr int K = 16 * KG;
   __shared__ int32_t Cs[kMfRowsWg][kMfCStride];  // counts of the step, [row][column]
-  __shared__ Real Vs[kMfStep][16];                // V rows of the step: lanes (c, g) read distinct banks
+  __shared__ Real Vs[KG][kMfStep][16];            // V rows of the step in 16-column planes: the 64 lanes
+                                                  // (c, g) of a product read 64 consecutive words
   __shared__ double Rs[kMfStep];                  // their reciprocals (REFERENCE)
   __shared__ int64_t rowbase[kMfRowsWg];          // packed-triangle offset of each block row
   __shared__ int64_t colbase[kMfStep];            // ... and of each step column (left of the rows)
@@ -257,11 +272,12 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
   const int64_t je = min(n, jb + jlen);
   const bool full = f.layout == RFX_LAYOUT_FULL;
   if (tid < kMfRowsWg) rowbase[tid] = rfx_upper_row(i0 + tid, n);
-  typename M::Acc acc[kMfStrips];
+  typename M::Acc acc[kMfStrips][KG];
   double ri[kMfStrips];
 #pragma unroll
   for (int s = 0; s < kMfStrips; ++s) {
-    acc[s] = typename M::Acc{Real(0), Real(0), Real(0), Real(0)};
+#pragma unroll
+    for (int kg = 0; kg < KG; ++kg) acc[s][kg] = typename M::Acc{Real(0), Real(0), Real(0), Real(0)};
     const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + l16;
     ri[s] = (MODE == RFX_MDS_REFERENCE && i < n) ? rinv[i] : 1.0;
   }
@@ -318,10 +334,10 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
         for (int k = 0; k < kMfBatch; ++k) Cs[(tid >> 6) + 4 * (k0 + k)][q] = v[k];
       }
     }
-    for (int e = tid; e < kMfStep * 16; e += 256) {
-      const int q = e >> 4, c = e & 15;
+    for (int e = tid; e < kMfStep * K; e += 256) {
+      const int q = e / K, c = e % K;
       const int64_t jj = j0 + q;
-      Vs[q][c] = jj < je ? V[jj * 16 + c] : Real(0);
+      Vs[c >> 4][q][c & 15] = jj < je ? V[jj * K + c] : Real(0);
     }
     if (MODE == RFX_MDS_REFERENCE && tid < kMfStep) Rs[tid] = j0 + tid < je ? rinv[j0 + tid] : 1.0;
     __syncthreads();
@@ -330,7 +346,9 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
     for (int m = 0; m < kMfStep / 4; ++m) {
       const int q = 4 * m + g;
       const int64_t jj = j0 + q;
-      const Real b = Vs[q][l16];
+      Real b[KG];
+#pragma unroll
+      for (int kg = 0; kg < KG; ++kg) b[kg] = Vs[kg][q][l16];
       const double rj = MODE == RFX_MDS_REFERENCE ? Rs[q] : 1.0;
 #pragma unroll
       for (int s = 0; s < kMfStrips; ++s) {
@@ -338,7 +356,8 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
         const int64_t ii = i0 + r;
         const Real ev = mf_e_value<MODE, Real>(f, Cs[r][q], ri[s], rj);
         const Real a = (ii < n && jj < je && ii != jj) ? ev : Real(0);
-        acc[s] = M::mma(a, b, acc[s]);
+#pragma unroll
+        for (int kg = 0; kg < KG; ++kg) acc[s][kg] = M::mma(a, b[kg], acc[s][kg]);
       }
     }
   }
@@ -347,7 +366,9 @@ __global__ __launch_bounds__(256) void mds_apply_mfma_kernel(const int32_t* __re
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t i = i0 + wave * (16 * kMfStrips) + s * 16 + M::row(g, e);
-      if (i < n) Ypart[(static_cast<int64_t>(blockIdx.y) * n + i) * 16 + l16] = acc[s][e];
+#pragma unroll
+      for (int kg = 0; kg < KG; ++kg)
+        if (i < n) Ypart[(static_cast<int64_t>(blockIdx.y) * n + i) * K + 16 * kg + l16] = acc[s][kg][e];
     }
   }
 }
@@ -392,8 +413,8 @@ int splits_for(int64_t n) {
 template <typename Real>
 size_t dense_ws(int64_t n) {
   const int sp = splits_for(n);
-  return al(8 * n) + al(8 * 4) + al(sizeof(Real) * sp * n * kBlockK) + al(8 * sp * n) + al(8 * n) +
-         rfx::SubspaceSolver<Real>::workspace_bytes(n, kBlockK);
+  return al(8 * n) + al(8 * 4) + al(sizeof(Real) * sp * n * kDenseMaxK) + al(8 * sp * n) + al(8 * n) +
+         rfx::SubspaceSolver<Real>::workspace_bytes(n, kDenseMaxK);
 }
 
 template <typename Real>
@@ -407,7 +428,7 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
   p += al(8 * 4);
   const int sp = splits_for(n);
   Real* Ypart = reinterpret_cast<Real*>(p);
-  p += al(sizeof(Real) * sp * n * kBlockK);
+  p += al(sizeof(Real) * sp * n * kDenseMaxK);
   double* rpart = reinterpret_cast<double*>(p);
   p += al(8 * sp * n);
   double* rinv = reinterpret_cast<double*>(p);
@@ -437,10 +458,11 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
   for (int64_t i = 0; i < n; ++i) g += rh[i];
   g /= static_cast<double>(n);
 
-  rfx::SubspaceSolver<Real> solver(st, n, kBlockK);
-  solver.carve(p);
   // The operator of the count forms runs on the matrix cores.
   const bool mfma = mode == RFX_MDS_COUNTS || mode == RFX_MDS_REFERENCE;
+  const int K = mfma ? kMfK : kBlockK;
+  rfx::SubspaceSolver<Real> solver(st, n, K);
+  solver.carve(p);
   if (mode == RFX_MDS_REFERENCE)
     hipLaunchKernelGGL(mds_rinv_kernel, dim3(rfx_ceil_div(n, 256)), dim3(256), 0, st, nout, n, rinv);
   const dim3 mgrid(rfx_ceil_div(n, kMfRowsWg), static_cast<unsigned>(sp));
@@ -448,13 +470,13 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
     (void)k;
     if (mfma) {
       if (mode == RFX_MDS_COUNTS)
-        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS>), mgrid, dim3(256), 0, st, counts, n, f, rinv,
-                           V, jlen, Ypart);
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS, kMfGroups>), mgrid, dim3(256), 0, st, counts,
+                           n, f, rinv, V, jlen, Ypart);
       else
-        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE>), mgrid, dim3(256), 0, st, counts, n, f,
-                           rinv, V, jlen, Ypart);
-      hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kBlockK, 256)), dim3(256), 0, st,
-                         Ypart, sp, n * kBlockK, W);
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE, kMfGroups>), mgrid, dim3(256), 0, st,
+                           counts, n, f, rinv, V, jlen, Ypart);
+      hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kMfK, 256)), dim3(256), 0, st,
+                         Ypart, sp, n * kMfK, W);
       return rfx_launch_status();
     }
     hipLaunchKernelGGL((mds_apply_kernel<Real, kBlockK, false>), grid, dim3(kRows), 0, st, counts, n, f, V, jlen,
