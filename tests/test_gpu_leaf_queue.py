@@ -102,13 +102,14 @@ def test_queue_on_grown_forest(gpu):
         assert np.array_equal(k1.cpu().numpy(), keys_from_nodextr(f, O.leaf_assign(f, x)))
 
 
-@pytest.mark.parametrize("n", [4000, 70000])
-def test_jumps_on_drifting_chains(gpu, n):
+@pytest.mark.parametrize("n,p", [(4000, 7), (70000, 7), (6000, 80)])
+def test_jumps_on_drifting_chains(gpu, n, p):
     # chains with random features, both heavy sides, thresholds out of order,
     # nested chains and samples on thresholds (the host restatement of the
-    # jumps is pinned against the oracle in test_chain_index.py)
-    print(f"leaf queue drifting chains n={n}", flush=True)
-    x, f = random_chain_forest(n, 7, 9)
+    # jumps is pinned against the oracle in test_chain_index.py); p = 80 takes
+    # the drain kernel that reads features from global memory (p > 64)
+    print(f"leaf queue drifting chains n={n} p={p}", flush=True)
+    x, f = random_chain_forest(n, p, 9)
     nd = O.leaf_assign(f, x)
     df = DeviceForest(f, x, chain_min_len=32)
     assert df.chains is not None
