@@ -37,17 +37,18 @@ namespace {
 constexpr int kRows = 256;  // rows per workgroup = threads
 constexpr int kCols = 32;   // columns per LDS tile
 constexpr int kBlockK = 16; // subspace block size (VALU operator; Ritz values reported)
-// Block of the count forms on the matrix cores: 32 columns.  At n = 200,000
-// (covtype-like, T = 1000) the top eigenvalues sit close together
-// (lambda3 / lambda4 = 1.007), so the iteration count follows
-// lambda(k+1) / lambda3; each count, formed into E once, feeds one product per
-// 16 columns, so a wider block costs little more per application.
-#ifndef RFX_MDS_MFMA_GROUPS
-#define RFX_MDS_MFMA_GROUPS 2
-#endif
-constexpr int kMfGroups = RFX_MDS_MFMA_GROUPS;
-constexpr int kMfK = 16 * kMfGroups;
-constexpr int kDenseMaxK = kMfK > kBlockK ? kMfK : kBlockK;
+// Block of the count forms on the matrix cores: 32 columns from n =
+// kMfWideMinN.  At n = 200,000 (covtype-like, T = 1000) the top eigenvalues
+// sit close together (lambda3 / lambda4 = 1.007), so the iteration count
+// follows lambda(k+1) / lambda3; each count, formed into E once, feeds one
+// product per 16 columns, so a wider block costs 1.3-1.5 x per application
+// for 0.56 x the iterations (dense MDS 10.1 -> 7.5 s FP64, 4.7 -> 3.8 s
+// FP32).  Below, the per-iteration work outside the operator (the 32-wide
+// Rayleigh-Ritz on the host, the Gram) outweighs it: the 64 instances of
+// 1,000 to 50,000 samples took 10.2 s at 32 columns against 5.6 s at 16.
+constexpr int kMfWideK = 32;
+constexpr int64_t kMfWideMinN = 100000;
+constexpr int kDenseMaxK = kMfWideK > kBlockK ? kMfWideK : kBlockK;
 
 struct ProxForm {
   int32_t mode;    // RFX_MDS_COUNTS / _REFERENCE / _MATRIX / _RFGAP
@@ -460,7 +461,7 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
 
   // The operator of the count forms runs on the matrix cores.
   const bool mfma = mode == RFX_MDS_COUNTS || mode == RFX_MDS_REFERENCE;
-  const int K = mfma ? kMfK : kBlockK;
+  const int K = mfma && n >= kMfWideMinN ? kMfWideK : kBlockK;
   rfx::SubspaceSolver<Real> solver(st, n, K);
   solver.carve(p);
   if (mode == RFX_MDS_REFERENCE)
@@ -469,14 +470,20 @@ int run_dense(hipStream_t st, const int32_t* counts, const double* mat, int32_t 
   auto apply = [&](const Real* V, Real* W, int k) -> int {
     (void)k;
     if (mfma) {
-      if (mode == RFX_MDS_COUNTS)
-        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS, kMfGroups>), mgrid, dim3(256), 0, st, counts,
-                           n, f, rinv, V, jlen, Ypart);
-      else
-        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE, kMfGroups>), mgrid, dim3(256), 0, st,
+      if (mode == RFX_MDS_COUNTS && K == kMfWideK)
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS, kMfWideK / 16>), mgrid, dim3(256), 0, st,
                            counts, n, f, rinv, V, jlen, Ypart);
-      hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * kMfK, 256)), dim3(256), 0, st,
-                         Ypart, sp, n * kMfK, W);
+      else if (mode == RFX_MDS_COUNTS)
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_COUNTS, 1>), mgrid, dim3(256), 0, st, counts, n, f,
+                           rinv, V, jlen, Ypart);
+      else if (K == kMfWideK)
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE, kMfWideK / 16>), mgrid, dim3(256), 0, st,
+                           counts, n, f, rinv, V, jlen, Ypart);
+      else
+        hipLaunchKernelGGL((mds_apply_mfma_kernel<Real, RFX_MDS_REFERENCE, 1>), mgrid, dim3(256), 0, st, counts, n,
+                           f, rinv, V, jlen, Ypart);
+      hipLaunchKernelGGL((mds_sum_splits_kernel<Real, Real>), dim3(rfx_ceil_div(n * K, 256)), dim3(256), 0, st,
+                         Ypart, sp, n * K, W);
       return rfx_launch_status();
     }
     hipLaunchKernelGGL((mds_apply_kernel<Real, kBlockK, false>), grid, dim3(kRows), 0, st, counts, n, f, V, jlen,

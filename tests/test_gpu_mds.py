@@ -79,3 +79,45 @@ def test_mds_full_layout_matches_upper(gpu, precision, tol):
     Xu, eu, _ = mds_dense(U, _lib.LAYOUT_UPPER, n, _lib.MDS_COUNTS, 1.0 / f.ntree, precision=precision)
     Xf, ef, _ = mds_dense(full, _lib.LAYOUT_FULL, n, _lib.MDS_COUNTS, 1.0 / f.ntree, precision=precision)
     assert O.embedding_errors(Xf.cpu().numpy(), Xu.cpu().numpy())["rel_frobenius"] < tol
+
+
+def test_mds_wide_block_at_n_100000(gpu):
+    # n = 100,000 takes the 32-column block on the matrix cores.  Residuals of
+    # the top three eigenpairs of the centred operator B = -1/2 J E J,
+    # E = (max M - M)^2 off the diagonal, M = C / T, formed here in float64
+    # from the FULL count matrix in row chunks (torch on the device)
+    n, T = 100000, 60
+    print(f"mds wide block: grow n={n} T={T}", flush=True)
+    x, y, f = small_forest(_lib.DATA_COVTYPE_LIKE, n, 54, 7, T, 3, mtry=7)
+    df = DeviceForest(f, x)
+    keys, _ = df.leaf_assign()
+    acc = Accumulator(f.n, f.ntree, f.total_nodes)
+    C = acc.accumulate(keys, _lib.PROX_ALL, _lib.LAYOUT_FULL, _lib.ALGO_GATHER)
+    del acc, df, keys
+    torch.cuda.empty_cache()
+    print("mds wide block: FP64 and FP32 MDS", flush=True)
+    X64, e64, it64 = mds_dense(C, _lib.LAYOUT_FULL, n, _lib.MDS_COUNTS, 1.0 / T, precision=_lib.PREC_FP64)
+    X32, e32, it32 = mds_dense(C, _lib.LAYOUT_FULL, n, _lib.MDS_COUNTS, 1.0 / T, precision=_lib.PREC_FP32)
+    print("mds wide block: residuals", flush=True)
+    Cm = C.view(n, n)
+    mx = float(Cm.max()) / T
+    V = torch.cat([X64, X32], dim=1)  # n x 6, float64
+    EV = torch.zeros_like(V)
+    r = torch.zeros(n, dtype=torch.float64, device=C.device)
+    for i0 in range(0, n, 2048):
+        i1 = min(n, i0 + 2048)
+        E = (mx - Cm[i0:i1].to(torch.float64) / T) ** 2
+        E[torch.arange(i1 - i0, device=C.device), torch.arange(i0, i1, device=C.device)] = 0.0
+        r[i0:i1] = E.sum(dim=1) / n
+        EV[i0:i1] = E @ V
+        del E
+    g = float(r.mean())
+    s = V.sum(dim=0)
+    BV = -0.5 * (EV - r[:, None] * s[None, :] - (r @ V)[None, :] + g * s[None, :])
+    lam = torch.tensor(np.concatenate([e64[:3], e32[:3]]), dtype=torch.float64, device=C.device)
+    res = (BV - V * lam[None, :]).norm(dim=0) / (lam.abs() * V.norm(dim=0))
+    res = res.cpu().numpy()
+    print("mds wide block: iterations", it64, it32, "residuals", res, flush=True)
+    assert np.all(res[:3] < 1e-6), res
+    assert np.all(res[3:] < 1e-3), res
+    assert np.allclose(e32[:3], e64[:3], rtol=1e-5, atol=0)
