@@ -178,7 +178,12 @@ namespace {
 // 18.9 ms, with the prefetch 15.1 ms; batches of 16 with the prefetch bring
 // the whole rank-128 factorization from 0.166 s to 0.139 s.
 constexpr int kLeafSumUnroll = 16;
-constexpr int kLeafSumGroups = 3;
+constexpr int kLeafSumTail = 8;
+// G column groups of 64 per pass (ceil(L / 64), at most 3).  The members left
+// after the full batches (fewer than 16) go in at most two batches of 8 whose
+// ids past the leaf repeat its last id and whose rows add +0.0f, which leaves
+// every sum exactly as the sequential loop makes it.
+template <int G>
 __global__ __launch_bounds__(256) void leafsum_kernel(const uint32_t* __restrict__ bstart,
                                                       const uint32_t* __restrict__ bend,
                                                       const uint32_t* __restrict__ members, int64_t lo, int64_t hi,
@@ -189,13 +194,13 @@ __global__ __launch_bounds__(256) void leafsum_kernel(const uint32_t* __restrict
   const int lane = static_cast<int>(threadIdx.x & 63);
   const uint32_t q0 = bstart[b], q1 = bend[b];
   float* out = S + (b - lo) * L;
-  for (int c0 = 0; c0 < L; c0 += 64 * kLeafSumGroups) {
+  for (int c0 = 0; c0 < L; c0 += 64 * G) {
     // lanes past the row read its last column (an in-row address, so the
     // loads need no branch between them) and never store
-    int col[kLeafSumGroups];
-    float s[kLeafSumGroups];
+    int col[G];
+    float s[G];
 #pragma unroll
-    for (int g = 0; g < kLeafSumGroups; ++g) {
+    for (int g = 0; g < G; ++g) {
       col[g] = min(c0 + 64 * g + lane, L - 1);
       s[g] = 0.0f;
     }
@@ -206,12 +211,12 @@ __global__ __launch_bounds__(256) void leafsum_kernel(const uint32_t* __restrict
       for (int u = 0; u < kLeafSumUnroll; ++u) mem[u] = members[q + u];
     }
     for (; q + kLeafSumUnroll <= q1; q += kLeafSumUnroll) {
-      float v[kLeafSumUnroll][kLeafSumGroups];
+      float v[kLeafSumUnroll][G];
 #pragma unroll
       for (int u = 0; u < kLeafSumUnroll; ++u) {
         const float* row = V + static_cast<int64_t>(mem[u] >> 1) * L;
 #pragma unroll
-        for (int g = 0; g < kLeafSumGroups; ++g) v[u][g] = row[col[g]];
+        for (int g = 0; g < G; ++g) v[u][g] = row[col[g]];
       }
       if (q + 2 * kLeafSumUnroll <= q1) {  // the next batch's ids, while the rows load
 #pragma unroll
@@ -220,15 +225,27 @@ __global__ __launch_bounds__(256) void leafsum_kernel(const uint32_t* __restrict
 #pragma unroll
       for (int u = 0; u < kLeafSumUnroll; ++u)
 #pragma unroll
-        for (int g = 0; g < kLeafSumGroups; ++g) s[g] += v[u][g];
+        for (int g = 0; g < G; ++g) s[g] += v[u][g];
     }
-    for (; q < q1; ++q) {
-      const float* row = V + static_cast<int64_t>(members[q] >> 1) * L;
+    for (; q < q1; q += kLeafSumTail) {
+      const uint32_t cnt = min(static_cast<uint32_t>(kLeafSumTail), q1 - q);
+      uint32_t mt[kLeafSumTail];
 #pragma unroll
-      for (int g = 0; g < kLeafSumGroups; ++g) s[g] += row[col[g]];
+      for (int u = 0; u < kLeafSumTail; ++u) mt[u] = members[q + min(static_cast<uint32_t>(u), cnt - 1)];
+      float v[kLeafSumTail][G];
+#pragma unroll
+      for (int u = 0; u < kLeafSumTail; ++u) {
+        const float* row = V + static_cast<int64_t>(mt[u] >> 1) * L;
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[u][g] = row[col[g]];
+      }
+#pragma unroll
+      for (int u = 0; u < kLeafSumTail; ++u)
+#pragma unroll
+        for (int g = 0; g < G; ++g) s[g] += static_cast<uint32_t>(u) < cnt ? v[u][g] : 0.0f;
     }
 #pragma unroll
-    for (int g = 0; g < kLeafSumGroups; ++g)
+    for (int g = 0; g < G; ++g)
       if (c0 + 64 * g + lane < L) out[c0 + 64 * g + lane] = s[g];
   }
 }
@@ -991,8 +1008,13 @@ extern "C" int rfx_lowrank_factors(hipStream_t st, const int32_t* keys, int64_t 
     for (size_t q = 0; q + 1 < plan.t0.size(); ++q) {
       const int32_t t0 = plan.t0[q], t1 = plan.t0[q + 1];
       const int64_t lo = node_offset_host[t0], hi = node_offset_host[t1];
-      hipLaunchKernelGGL(leafsum_kernel, dim3(static_cast<unsigned>(rfx_ceil_div(hi - lo, 4))), dim3(256), 0, st,
-                         w.bstart, w.bend, w.members, lo, hi, Vin, L, S);
+      const dim3 lsgrid(static_cast<unsigned>(rfx_ceil_div(hi - lo, 4)));
+      if (L <= 64)
+        hipLaunchKernelGGL(leafsum_kernel<1>, lsgrid, dim3(256), 0, st, w.bstart, w.bend, w.members, lo, hi, Vin, L, S);
+      else if (L <= 128)
+        hipLaunchKernelGGL(leafsum_kernel<2>, lsgrid, dim3(256), 0, st, w.bstart, w.bend, w.members, lo, hi, Vin, L, S);
+      else
+        hipLaunchKernelGGL(leafsum_kernel<3>, lsgrid, dim3(256), 0, st, w.bstart, w.bend, w.members, lo, hi, Vin, L, S);
       hipLaunchKernelGGL(leafgather_kernel, dim3(rfx_ceil_div(n * L, 256)), dim3(256), 0, st, keys, n, t0, t1, lo, S,
                          L, static_cast<int32_t>(q > 0), Yout);
     }
