@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kStageLb * W) void leaf_assign_staged_kernel(const 
 // feature row is copied into the lane's LDS column at refill, and a step is
 // one node gather plus one LDS read.
 //
-// With CHAINS, a walk that stands on a node of a long heavy path (the chain
+// A walk that stands on a node of a long heavy path (the chain
 // index's labels) jumps to its exit from the path: the whole wavefront
 // serves one such lane at a time, one lane per (feature, side) record list
 // of the path, each a binary search for the first record whose exit test the
@@ -283,7 +283,7 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 
 // Trees [tree_base, tree_base + items' tree ids) of the chain index: queue
 // items carry tree ids relative to the launch, the chain index is global.
-template <bool DRAIN_LDS, bool CHAINS>
+template <bool DRAIN_LDS>
 __global__ __launch_bounds__(kDrainBlock) void leaf_drain_kernel(const float* __restrict__ x, int64_t n, int32_t p,
                                                                  const uint2* __restrict__ nodes,
                                                                  const int64_t* __restrict__ node_off, LeafQueue q,
@@ -295,8 +295,7 @@ __global__ __launch_bounds__(kDrainBlock) void leaf_drain_kernel(const float* __
   const unsigned long long offered = *q.tail;
   const unsigned long long count = offered < q.capacity ? offered : q.capacity;
   if (count == 0) return;
-  ChainView cv;
-  if (CHAINS) cv = chain_view(chains);
+  const ChainView cv = chain_view(chains);
   const int lane = __lane_id();
   bool active = false, drained = false, bag = false;
   uint32_t t = 0, i = 0, kt = 0;
@@ -326,7 +325,7 @@ __global__ __launch_bounds__(kDrainBlock) void leaf_drain_kernel(const float* __
           bag = (it.w >> 31) != 0;
           base = node_off[t];
           nd = nodes[base + kt];
-          if (CHAINS) {
+          {
             nnode = static_cast<uint32_t>(node_off[t + 1] - base);
             const int32_t tg = tree_base + static_cast<int32_t>(t);
             lab0 = tg < cv.ntree ? cv.tree_lab[tg] : -1;
@@ -343,7 +342,7 @@ __global__ __launch_bounds__(kDrainBlock) void leaf_drain_kernel(const float* __
       uint32_t lab = rfx_chain::kNoLabel;
       if (active) {
         if (nd.y != 0xffffffffu && (bag || steps < kMaxOobDescents)) {
-          if (CHAINS && lab0 >= 0) lab = cv.labels[lab0 + kt];
+          if (lab0 >= 0) lab = cv.labels[lab0 + kt];
           if (lab == rfx_chain::kNoLabel) {
             const uint32_t f = nd.y & 1023u, l = nd.y >> 10;
             const float v = DRAIN_LDS ? xrow[f * kDrainBlock + lane] : x[static_cast<int64_t>(f) * n + i];
@@ -359,7 +358,7 @@ __global__ __launch_bounds__(kDrainBlock) void leaf_drain_kernel(const float* __
           active = false;
         }
       }
-      if (CHAINS) {
+      {
         unsigned long long jm = __ballot(lab != rfx_chain::kNoLabel);
         while (jm != 0) {
           const int src = __ffsll(jm) - 1;
@@ -451,11 +450,11 @@ int launch_staged(hipStream_t stream, const float* x_col, int64_t n, int32_t p, 
     // enough one-wavefront workgroups for every CU's SIMDs, the rest exit at once
     const unsigned drain_blocks = 256 * 16;
     if (p <= kDrainLdsMaxP)
-      hipLaunchKernelGGL((leaf_drain_kernel<true, true>), dim3(drain_blocks), dim3(kDrainBlock),
+      hipLaunchKernelGGL((leaf_drain_kernel<true>), dim3(drain_blocks), dim3(kDrainBlock),
                          static_cast<unsigned>(static_cast<size_t>(p) * kDrainBlock * 4), stream, x_col, n, p, nd,
                          off, *q, ndx, ky, chains, static_cast<int32_t>(t0));
     else
-      hipLaunchKernelGGL((leaf_drain_kernel<false, true>), dim3(drain_blocks), dim3(kDrainBlock), 0, stream, x_col,
+      hipLaunchKernelGGL((leaf_drain_kernel<false>), dim3(drain_blocks), dim3(kDrainBlock), 0, stream, x_col,
                          n, p, nd, off, *q, ndx, ky, chains, static_cast<int32_t>(t0));
     const int rc = rfx_launch_status();
     if (rc != RFX_OK) return rc;
