@@ -215,7 +215,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the proximity path runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(multi-GPU sharding is not run)", file=sys.stderr)
         return 2
 
     import numpy as np
