@@ -62,3 +62,20 @@ def test_short_paths_give_no_index():
     ci = f.chain_index(min_len=400)
     assert ci is not None
     assert np.array_equal(chain_jump_walk(f, x, ci), O.leaf_assign(f, x))
+
+
+def test_path_cap_per_tree():
+    # min_len 2 on a tree grown on 60,000 samples: more heavy paths than the
+    # 1,023 a tree's labels can name; the index keeps the first 1,023 and the
+    # rest are walked
+    from rfx_amd import _lib
+    from rfx_amd.forest import Forest, make_dataset
+
+    x, y = make_dataset(_lib.DATA_COVTYPE_LIKE, 60000, 54, 7, 4)
+    f = Forest.grow(x, y, 1, iseed=4)
+    idx = np.arange(0, 60000, 150)
+    g = f.samples(idx)
+    xs = np.ascontiguousarray(x[idx])
+    ci = g.chain_index(min_len=2)
+    assert int(ci[8:12].view(np.int32)[0]) == 1023
+    assert chain_jump_walk(g, xs, ci, expect=O.leaf_assign(g, xs))
