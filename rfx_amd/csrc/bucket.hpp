@@ -272,7 +272,19 @@ __global__ void rs_tree_passes_kernel(uint32_t* __restrict__ info) {
 
 // Step 3: key = leaf id - tmin[t] or the sentinel, into the buffer the first
 // pass reads.
-__global__ void rs_prep_tree_kernel(const int32_t* __restrict__ keys, int64_t n, int64_t slots,
+// Tree of slot s = t * n + i without a 64-bit integer division (a long
+// instruction sequence here): the double quotient s * (1 / n) is within one of
+// t for s < 2^52, and one compare either way corrects it.
+__device__ __forceinline__ int64_t slot_tree(int64_t s, int64_t n, double inv_n) {
+  int64_t t = static_cast<int64_t>(static_cast<double>(s) * inv_n);
+  if (t * n > s)
+    --t;
+  else if ((t + 1) * n <= s)
+    ++t;
+  return t;
+}
+
+__global__ void rs_prep_tree_kernel(const int32_t* __restrict__ keys, int64_t n, int64_t slots, double inv_n,
                                     const uint32_t* __restrict__ tmin, const uint32_t* __restrict__ info,
                                     uint32_t* __restrict__ K0, uint32_t* __restrict__ K1, uint32_t* __restrict__ V0,
                                     uint32_t* __restrict__ V1) {
@@ -282,7 +294,7 @@ __global__ void rs_prep_tree_kernel(const int32_t* __restrict__ keys, int64_t n,
   uint32_t* K = odd ? K1 : K0;
   uint32_t* V = odd ? V1 : V0;
   const int32_t k = keys[s];
-  K[s] = k >= 0 ? (static_cast<uint32_t>(k) >> 1) - tmin[s / n] : info[1];
+  K[s] = k >= 0 ? (static_cast<uint32_t>(k) >> 1) - tmin[slot_tree(s, n, inv_n)] : info[1];
   V[s] = (static_cast<uint32_t>(s) << 1) | (k >= 0 ? (static_cast<uint32_t>(k) & 1u) : 0u);
 }
 
@@ -432,36 +444,40 @@ __global__ void rs_bounds_kernel(const uint32_t* __restrict__ K, int64_t N, uint
 // Leaf ranges for the per-tree layout: sorted local keys in K (buffer 0)
 // back to global leaf ids (mbucket, total_nodes for an off-leaf slot), the
 // ranges of every leaf, nvalid = N (off-leaf slots end each tree's segment).
-__global__ void rs_bounds_tree_kernel(const uint32_t* __restrict__ K, int64_t N, int64_t n, uint32_t total_nodes,
-                                      const uint32_t* __restrict__ tmin, const uint32_t* __restrict__ info,
-                                      uint32_t* __restrict__ mbucket, uint32_t* __restrict__ bstart,
-                                      uint32_t* __restrict__ bend, uint32_t* __restrict__ nvalid) {
+__global__ void rs_bounds_tree_kernel(const uint32_t* __restrict__ K, int64_t N, int64_t n, double inv_n,
+                                      uint32_t total_nodes, const uint32_t* __restrict__ tmin,
+                                      const uint32_t* __restrict__ info, uint32_t* __restrict__ mbucket,
+                                      uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend,
+                                      uint32_t* __restrict__ nvalid) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t sentinel = info[1];
-  const auto global = [&](int64_t q) -> uint32_t {
+  // slot p's tree; its neighbours are in the same tree except across a
+  // segment boundary
+  const int64_t t = slot_tree(p, n, inv_n);
+  const auto global = [&](int64_t q, int64_t tq) -> uint32_t {
     const uint32_t k = K[q];
-    return k == sentinel ? total_nodes : k + tmin[q / n];
+    return k == sentinel ? total_nodes : k + tmin[tq];
   };
-  const uint32_t g = global(p);
+  const uint32_t g = global(p, t);
   mbucket[p] = g;
   if (p == 0) *nvalid = static_cast<uint32_t>(N);
   if (g == total_nodes) return;
-  if (p == 0 || global(p - 1) != g) bstart[g] = static_cast<uint32_t>(p);
-  if (p + 1 == N || global(p + 1) != g) bend[g] = static_cast<uint32_t>(p + 1);
+  if (p == 0 || global(p - 1, p == t * n ? t - 1 : t) != g) bstart[g] = static_cast<uint32_t>(p);
+  if (p + 1 == N || global(p + 1, p + 1 == (t + 1) * n ? t + 1 : t) != g) bend[g] = static_cast<uint32_t>(p + 1);
 }
 
 // members[p] = (i << 1) | bag; pe[s] = (p, end of its leaf | bag << 31),
 // (0, 0) off-leaf.
 __global__ void rs_members_kernel(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int64_t N, uint32_t n,
-                                  uint32_t total_nodes, const uint32_t* __restrict__ bend,
+                                  double inv_n, uint32_t total_nodes, const uint32_t* __restrict__ bend,
                                   uint32_t* __restrict__ members, uint16_t* __restrict__ members16,
                                   uint2* __restrict__ pe, uint16_t* __restrict__ members_w = nullptr) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const uint32_t v = V[p];
   const uint32_t s = v >> 1, bag = v & 1u;
-  const uint32_t i = s % n;
+  const uint32_t i = s - static_cast<uint32_t>(slot_tree(s, n, inv_n)) * n;
   members[p] = (i << 1) | bag;
   if (members16) members16[p] = static_cast<uint16_t>((i << 1) | bag);
   if (members_w) members_w[p] = static_cast<uint16_t>(((i & (kWinSpan - 1)) << 1) | bag);
@@ -738,12 +754,14 @@ int bucket_sort_trees(hipStream_t st, const int32_t* keys, int64_t n, int32_t nt
   hipLaunchKernelGGL(rs_tree_range_kernel, dim3(static_cast<unsigned>(ntree)), dim3(256), 0, st, keys, n, w.tmin,
                      w.info);
   hipLaunchKernelGGL(rs_tree_passes_kernel, dim3(1), dim3(1), 0, st, w.info);
-  hipLaunchKernelGGL(rs_prep_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, n, N, w.tmin, w.info,
+  hipLaunchKernelGGL(rs_prep_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, keys, n, N,
+                     1.0 / static_cast<double>(n), w.tmin, w.info,
                      w.K[0], w.K[1], w.V[0], w.V[1]);
   const RsTiles g{n, rs_tiles_per_tree(n), w.ntiles};
   rc = rs_passes(st, w, g, reinterpret_cast<const int32_t*>(w.info + 2));
   if (rc) return rc;
-  hipLaunchKernelGGL(rs_bounds_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[0], N, n, tn, w.tmin,
+  hipLaunchKernelGGL(rs_bounds_tree_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.K[0], N, n,
+                     1.0 / static_cast<double>(n), tn, w.tmin,
                      w.info, w.mbucket, w.bstart, w.bend, w.nvalid);
   return rfx_launch_status();
 }
@@ -1081,8 +1099,8 @@ int bucket_build(hipStream_t st, const int32_t* keys, int64_t n, int32_t ntree, 
   int rc = bucket_sort_trees(st, keys, n, ntree, total_nodes, w);
   if (rc) return rc;
   hipLaunchKernelGGL(rs_members_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.V[0], N,
-                     static_cast<uint32_t>(n), static_cast<uint32_t>(total_nodes), w.bend, w.members, w.members16,
-                     w.pe, w.members_w);
+                     static_cast<uint32_t>(n), 1.0 / static_cast<double>(n), static_cast<uint32_t>(total_nodes), w.bend,
+                     w.members, w.members16, w.pe, w.members_w);
   if (w.wtab)
     hipLaunchKernelGGL(win_table_kernel, dim3(rfx_ceil_div(N, 256)), dim3(256), 0, st, w.mbucket, w.members,
                        w.bstart, w.bend, N, static_cast<uint32_t>(total_nodes), w.nb, win_shift(), w.wtab);
