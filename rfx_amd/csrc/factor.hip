@@ -250,7 +250,12 @@ __global__ __launch_bounds__(256) void leafsum_kernel(const uint32_t* __restrict
   }
 }
 
-// Y[i][c] (+)= sum over trees t in [t0, t1) of S[leaf_t(i) - lo][c]
+// Y[i][c] (+)= sum over trees t in [t0, t1) of S[leaf_t(i) - lo][c].  Keys
+// are taken kGatherBatch trees at a time and their leaf rows loaded together
+// (an off-leaf key reads row 0 and is selected away), then added in tree
+// order: a loop of one key and one dependent row load per tree waited twice
+// per tree.
+constexpr int kGatherBatch = 8;
 __global__ void leafgather_kernel(const int32_t* __restrict__ keys, int64_t n, int32_t t0, int32_t t1, int64_t lo,
                                   const float* __restrict__ S, int L, int32_t accumulate, float* __restrict__ Y) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -258,7 +263,21 @@ __global__ void leafgather_kernel(const int32_t* __restrict__ keys, int64_t n, i
   const int64_t i = e / L;
   const int c = static_cast<int>(e % L);
   float s = accumulate ? Y[e] : 0.0f;
-  for (int32_t t = t0; t < t1; ++t) {
+  int32_t t = t0;
+  for (; t + kGatherBatch <= t1; t += kGatherBatch) {
+    int32_t k[kGatherBatch];
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) k[u] = keys[static_cast<int64_t>(t + u) * n + i];
+    float v[kGatherBatch];
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) {
+      const int64_t r = k[u] >= 0 ? (static_cast<int64_t>(k[u]) >> 1) - lo : 0;
+      v[u] = S[r * L + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) s = k[u] >= 0 ? s + v[u] : s;
+  }
+  for (; t < t1; ++t) {
     const int32_t k = keys[static_cast<int64_t>(t) * n + i];
     if (k >= 0) s += S[((static_cast<int64_t>(k) >> 1) - lo) * L + c];
   }
