@@ -242,6 +242,8 @@ __global__ __launch_bounds__(kStageLb * W) void leaf_assign_staged_kernel(const 
 constexpr int kDrainBlock = 64;
 constexpr int kDrainSteps = 16;
 constexpr int32_t kDrainLdsMaxP = 64;
+static_assert(static_cast<int64_t>(kDrainLdsMaxP) * kDrainBlock * 4 <= kLdsLimitBytes,
+              "drain kernel feature rows exceed the LDS");
 #ifndef RFX_LEAF_CAP_STEPS
 #define RFX_LEAF_CAP_STEPS 128
 #endif
